@@ -1,0 +1,154 @@
+/*
+ * tt.h -- C ABI of the MI355X trace transform (sinogram + circus path).
+ *
+ * Plain C: pointers and sizes only, no C++ or torch types.  Every entry
+ * point returns a tt_status; no exception crosses the ABI.  Buffers are
+ * caller-owned.  Device-pointer entry points (d_*) take memory already in
+ * HBM and run on the context's stream; *_host entry points take host memory
+ * and block until the results are back on the host.
+ *
+ * Layouts follow the reference's Eigen column-major MatrixXf:
+ *   padded image     n x n,  element (row, col) at row + col*n
+ *   sinogram (one T) n x A,  element (col, angle) at col + angle*n, i.e. one
+ *                    angle's n values are contiguous
+ *   sinograms        nt consecutive sinograms: [nt][A][n]
+ *   circus functions [n_sino][np][A]
+ *   index outputs    [A][n] (index angle*n + col)
+ *
+ * Functional codes are the reference's enum ordinals:
+ *   TFunctional Radon,T1..T7 = 0..7      (c++/src/sinogram.hpp:22-31)
+ *   PFunctional Hermite=0, P1=1, P2=2, P3=3  (c++/src/circus.hpp:27-32)
+ */
+#ifndef TT_H
+#define TT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TT_API __attribute__((visibility("default")))
+
+typedef enum tt_status {
+    TT_OK = 0,
+    TT_ERR_INVALID = 1,     /* bad argument; tt_last_error() says which     */
+    TT_ERR_NOMEM = 2,       /* device or host allocation failed             */
+    TT_ERR_DEVICE = 3,      /* HIP runtime error                            */
+    TT_ERR_UNSUPPORTED = 4  /* valid request this build does not implement  */
+} tt_status;
+
+enum {
+    TT_T_RADON = 0, TT_T1 = 1, TT_T2 = 2, TT_T3 = 3,
+    TT_T4 = 4, TT_T5 = 5, TT_T6 = 6, TT_T7 = 7
+};
+enum { TT_P_HERMITE = 0, TT_P1 = 1, TT_P2 = 2, TT_P3 = 3 };
+
+/* One P-functional: the reference's PFunctionalWrapper with its optional
+ * Hermite arguments made explicit (c++/src/circus.hpp:34-57).  order and
+ * center are read only when code == TT_P_HERMITE; center < 0 means "unset",
+ * which the reference leaves undefined and this ABI rejects. */
+typedef struct tt_pfunc {
+    int32_t code;
+    int32_t order;
+    int32_t center;
+} tt_pfunc;
+
+/* Optional index outputs of tt_sinograms, each [A][n] int32 in device
+ * memory, or NULL.  These are the integer decisions the reference makes per
+ * column; they are exported so parity can be checked bit for bit. */
+typedef struct tt_sino_aux {
+    int32_t *median_x;    /* findWeightedMedian(x)            (T1,T2,T7)    */
+    int32_t *median_sqrt; /* findWeightedMedian(sqrt(x))      (T3,T4,T5,T6) */
+    int32_t *t6_rank;     /* T6: position k* in the sorted weighted tail    */
+    int32_t *t6_src;      /* T6: tail offset r of that element (idx[k*])    */
+    int32_t *t7_rank;     /* T7: position k* in the sorted tail             */
+} tt_sino_aux;
+
+typedef struct tt_context tt_context;
+
+/* ---- context ----------------------------------------------------------- */
+
+/* One context per host thread and device: owns a HIP stream, the device
+ * workspaces (trig and T3-T5 tables, transposed image, Hermite weights) and
+ * the cached graph of the batched path.  No reference counterpart: the
+ * reference keeps no state between getSinograms calls. */
+TT_API tt_status tt_context_create(int device, tt_context **out);
+TT_API tt_status tt_context_destroy(tt_context *ctx);
+/* Run subsequent work on a caller stream (hipStream_t as void*); NULL
+ * restores the context's own stream. */
+TT_API tt_status tt_context_set_stream(tt_context *ctx, void *stream);
+TT_API tt_status tt_context_synchronize(tt_context *ctx);
+/* Message of the last failure on this host thread ("" if none). */
+TT_API const char *tt_last_error(void);
+
+/* ---- geometry ---------------------------------------------------------- */
+
+/* Replaces the size computation of pad (c++/src/auxiliary.cpp:257-275):
+ * side n of the padded square and the offset (dfx, dfy) of the image in it. */
+TT_API tt_status tt_pad_geometry(int rows, int cols, int *n, int *dfx,
+                                 int *dfy);
+
+/* Replaces pad (c++/src/auxiliary.cpp:257-275) on the device: rows x cols
+ * column-major image in, n x n zero-padded column-major image out. */
+TT_API tt_status tt_pad(tt_context *ctx, const float *d_image, int rows,
+                        int cols, float *d_padded);
+
+/* Number of angles for a step, as c++/src/sinogram.cpp:77:
+ * floor(360 / step).  Fractional steps are a superset of the reference's
+ * unsigned steps. */
+TT_API int tt_angle_count(float angle_step_deg);
+
+/* ---- sinograms (c++/src/sinogram.hpp:57-59, getSinograms) -------------- */
+
+/* d_image: n x n padded image.  Angles k*step for k < n_angles (the
+ * reference: step unsigned, n_angles = floor(360/step)).  tcodes: nt codes in
+ * 0..7 (host memory).  d_sino: nt x A x n.  aux may be NULL. */
+TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
+                              int n_angles, float angle_step_deg,
+                              const int32_t *tcodes, int nt, float *d_sino,
+                              const tt_sino_aux *aux);
+
+/* Host-memory variant of tt_sinograms (aux unsupported). */
+TT_API tt_status tt_sinograms_host(tt_context *ctx, const float *image, int n,
+                                   int n_angles, float angle_step_deg,
+                                   const int32_t *tcodes, int nt, float *sino);
+
+/* ---- circus functions (c++/src/circus.hpp:69-71, getCircusFunctions) --- */
+
+/* d_sino: n_sino sinograms, each n x n_cols (n_cols = number of angles).
+ * pf: np P-functionals (host memory).  d_circus: n_sino x np x n_cols.
+ * d_p2_rank (optional, n_sino x np x n_cols): P2's weighted-median position
+ * in the sorted column. */
+TT_API tt_status tt_circus(tt_context *ctx, const float *d_sino, int n,
+                           int n_cols, int n_sino, const tt_pfunc *pf, int np,
+                           float *d_circus, int32_t *d_p2_rank);
+
+/* Host-memory variant of tt_circus for one sinogram. */
+TT_API tt_status tt_circus_host(tt_context *ctx, const float *sino, int n,
+                                int n_cols, const tt_pfunc *pf, int np,
+                                float *circus);
+
+/* ---- kernel timing ------------------------------------------------------ */
+
+/* No reference counterpart (the reference times whole getTransform calls,
+ * c++/src/demo.cpp:222-248).  When enabled, the context brackets each of its
+ * kernel launches with HIP events on the stream the kernel runs on;
+ * tt_profile_read synchronizes, returns the summed kernel time (ms) and
+ * launch count of one kernel kind since the last read, and resets them. */
+enum {
+    TT_KERNEL_SINO_T05 = 0,   /* T0..T5 line kernel                         */
+    TT_KERNEL_SINO_SORT = 1,  /* T6/T7 line kernel                          */
+    TT_KERNEL_CIRCUS = 2,     /* P-functional kernel                        */
+    TT_KERNEL_AUX = 3,        /* transpose/pad helpers                      */
+    TT_KERNEL_KINDS = 4
+};
+TT_API tt_status tt_profile_enable(tt_context *ctx, int enable);
+TT_API tt_status tt_profile_read(tt_context *ctx, int kind, double *total_ms,
+                                 int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

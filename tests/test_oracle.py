@@ -1,0 +1,382 @@
+"""CPU tests that pin the oracle (oracle/tt_oracle.c) before it is trusted.
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a swapped operand, a wrong sign or index) fails: closed forms, library
+routines (glibc qsort_r, numpy FFT), brute force on tiny inputs, and
+hand-derived cases for Eigen's packet summation order.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+
+# ----------------------------------------------------------------- geometry
+
+@pytest.mark.parametrize("side,n,df", [(1, 7, 3), (64, 91, 14), (255, 361, 53),
+                                       (256, 363, 54), (512, 725, 107),
+                                       (1024, 1449, 213)])
+def test_pad_sizes(oracle, side, n, df):
+    # c++/src/auxiliary.cpp:257-275: n = 2*ceil(hypot(c-2-ox, r-2-oy)) + 3
+    assert oracle.pad_geometry(side, side) == (n, df, df)
+    ox = math.floor((side + 1) / 2) - 1
+    assert n == 2 * (math.ceil(math.hypot(side - 2 - ox, side - 2 - ox)) + 1) + 1
+
+
+def test_pad_nonsquare_places_image(oracle):
+    rows, cols = 5, 9
+    img = np.arange(rows * cols, dtype=np.float32) + 1  # column-major
+    n, dx, dy = oracle.pad_geometry(rows, cols)
+    out = oracle.pad(img, rows, cols).reshape(n, n)  # [col, row]
+    assert out.sum() == img.sum()
+    assert out[dx, dy] == img[0] and out[dx + cols - 1, dy + rows - 1] == img[-1]
+    # the image is centred: both margins differ by at most one pixel
+    assert abs(dx - (n - cols - dx)) <= 1 and abs(dy - (n - rows - dy)) <= 1
+
+
+# ------------------------------------------------------------------- rotate
+
+def _rot(oracle, img, n, a_step, step=1.0):
+    c, s = oracle.rotation_trig(a_step, step)
+    return oracle.rotate(img, n, c, s).reshape(n, n), c, s
+
+
+def test_rotation_trig_is_cosf_sinf_of_negated_float_angle(oracle):
+    c, s = oracle.rotation_trig(30, 1.0)
+    ang = np.float32(30.0 * math.pi / 180.0)
+    assert c == np.float32(math.cos(-float(ang))) or abs(c - math.cos(-float(ang))) < 1e-7
+    assert s < 0  # sin(-30 deg)
+    c0, s0 = oracle.rotation_trig(0, 1.0)
+    assert c0 == 1.0 and s0 == 0.0
+
+
+def test_rotate_angle0_is_identity_but_last_row_and_column(oracle):
+    n = 11
+    img = np.random.default_rng(0).random(n * n, dtype=np.float32)
+    out, _, _ = _rot(oracle, img, n, 0)
+    ref = img.reshape(n, n).copy()  # [col, row]
+    ref[n - 1, :] = 0  # last column (x < n-1 strict)
+    ref[:, n - 1] = 0  # last row
+    assert np.array_equal(out, ref)
+
+
+def test_rotate_constant_and_centre(oracle):
+    n = 15
+    img = np.full(n * n, 0.75, np.float32)
+    for a in (17, 45, 90, 200):
+        out, _, _ = _rot(oracle, img, n, a)
+        o = (n - 1) // 2
+        assert out[o, o] == 0.75  # the origin maps to itself
+        vals = out[out != 0]
+        assert np.allclose(vals, 0.75, rtol=0, atol=1e-6)
+
+
+def _rotate_f64(img_cm, n, c, s):
+    """Independent FP64 bilinear rotation (vectorised numpy).  Also returns
+    a mask of pixels whose source coordinate is not within 1e-4 of the
+    strict bounds (where float and double may legitimately disagree)."""
+    o = (n - 1) / 2.0
+    col, row = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    px, py = col - o, row - o
+    x = px * float(c) + py * float(s) + o
+    y = -px * float(s) + py * float(c) + o
+    inb = (x >= 0) & (x < n - 1) & (y >= 0) & (y < n - 1)
+    X = np.where(inb, np.floor(x), 0).astype(int)
+    Y = np.where(inb, np.floor(y), 0).astype(int)
+    fx, fy = x - X, y - Y
+    src = img_cm.reshape(n, n).astype(np.float64)  # [col, row]
+    v = (src[X, Y] * (1 - fx) * (1 - fy) + src[X + 1, Y] * fx * (1 - fy)
+         + src[X, Y + 1] * (1 - fx) * fy + src[X + 1, Y + 1] * fx * fy)
+    eps = 1e-4
+    safe = ((np.abs(x) > eps) & (np.abs(x - (n - 1)) > eps)
+            & (np.abs(y) > eps) & (np.abs(y - (n - 1)) > eps))
+    return np.where(inb, v, 0.0), safe
+
+
+@pytest.mark.parametrize("n", [5, 9, 21])
+def test_rotate_matches_fp64_bruteforce(oracle, n):
+    rng = np.random.default_rng(n)
+    img = rng.random(n * n, dtype=np.float32)
+    for a in (0, 1, 29, 45, 90, 137, 180, 271, 359):
+        out, c, s = _rot(oracle, img, n, a)
+        ref, safe = _rotate_f64(img, n, c, s)
+        assert safe.mean() > 0.3
+        assert np.abs(out - ref)[safe].max() < 1e-5, a
+
+
+def test_rotate_reproduces_affine_ramp(oracle):
+    # Bilinear interpolation reproduces affine functions: rotated(row, col)
+    # = f(x, y) with f(row, col) = 0.25*row + 0.5*col + 1 (dyadic, exact).
+    n = 33
+    col, row = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    img = (0.25 * row + 0.5 * col + 1).astype(np.float32).ravel()
+    for a in (10, 33, 60, 150):
+        out, c, s = _rot(oracle, img, n, a)
+        o = (n - 1) / 2.0
+        x = (col - o) * float(c) + (row - o) * float(s) + o
+        y = -(col - o) * float(s) + (row - o) * float(c) + o
+        inb = (x >= 0) & (x < n - 1) & (y >= 0) & (y < n - 1)
+        expect = 0.25 * y + 0.5 * x + 1
+        assert np.allclose(out[inb], expect[inb], rtol=0, atol=2e-5)
+        assert np.all(out[~inb] == 0)
+
+
+def test_rotate_90_is_a_permutation_up_to_cos_rounding(oracle):
+    # theta = 90: x ~ n-1-row, y ~ col, i.e. rotated(row, col) ~ img(col, n-1-row)
+    n = 13
+    img = np.random.default_rng(1).random(n * n, dtype=np.float32)
+    out, c, s = _rot(oracle, img, n, 90)
+    assert s == -1.0 and abs(c) < 1e-7
+    src = img.reshape(n, n)  # [col, row]
+    for col in range(1, n - 2):
+        for row in range(2, n - 1):
+            assert abs(out[col, row] - src[n - 1 - row, col]) < 1e-5
+
+
+# --------------------------------------------- Eigen sum and weighted median
+
+def test_eigen_sum_packet_order_cases(oracle):
+    # Hand-derived from Eigen's packet reduction (acc0/acc1, predux
+    # (a0+a2)+(a1+a3)); sequential order gives different results.
+    x = np.array([1e8, 1, -1e8, 1, 0, 0, 0, 0], np.float32)
+    assert oracle.eigen_sum(x) == 2.0
+    assert np.float32(oracle.lib().tto_seq_sum(x, 8)) == 1.0
+    # n = 5: predux of x[0..3] then + x[4]
+    x = np.array([1e8, 1, -1e8, 1, 3], np.float32)
+    assert oracle.eigen_sum(x) == 5.0
+    # n = 12: acc0 = x[0..3] + x[4..7], then + x[8..11] (tail packet)
+    x = np.zeros(12, np.float32)
+    x[0], x[4], x[8], x[2] = 1e8, 1, -1e8, 1
+    # acc0[0] = (1e8 + 1) + -1e8 = 0 (the 1 is absorbed), acc0[2] = 1
+    assert oracle.eigen_sum(x) == 1.0
+    # n = 19: packets over 16, then 3 sequential tail adds
+    x = np.zeros(19, np.float32)
+    x[16], x[17], x[18] = 1e8, 1, -1e8
+    assert oracle.eigen_sum(x) == 0.0
+    # n = 3: purely sequential
+    x = np.array([1e8, 1, -1e8], np.float32)
+    assert oracle.eigen_sum(x) == 0.0
+    # integers: every order is exact
+    x = np.arange(1, 1000, dtype=np.float32)
+    assert oracle.eigen_sum(x) == 999 * 1000 / 2
+
+
+def test_weighted_median_closed_forms(oracle):
+    for n in (1, 2, 7, 8, 9, 363):
+        assert oracle.weighted_median(np.zeros(n, np.float32)) == 0
+        assert oracle.weighted_median(np.ones(n, np.float32)) == math.ceil(n / 2) - 1
+    x = np.zeros(50, np.float32)
+    x[31] = 2.5
+    assert oracle.weighted_median(x) == 31
+    # no crossing: Eigen sum 3 exceeds twice every sequential prefix -> n-1
+    x = np.array([-1e8, 1, 1, 1, 1e8, 0, 0, 0], np.float32)
+    assert oracle.weighted_median(x) == 7
+
+
+def test_stable_argsort_matches_glibc_qsort_r(oracle):
+    libc = C.CDLL("libc.so.6")
+    CMP = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p)
+
+    def cmp(a, b, arg):
+        keys = C.cast(arg, C.POINTER(C.c_float))
+        ia = C.cast(a, C.POINTER(C.c_int))[0]
+        ib = C.cast(b, C.POINTER(C.c_int))[0]
+        ka, kb = keys[ia], keys[ib]
+        return -1 if ka < kb else (1 if ka > kb else 0)
+
+    cb = CMP(cmp)
+    libc.qsort_r.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, CMP, C.c_void_p]
+    rng = np.random.default_rng(7)
+    for n in (1, 2, 5, 64, 300, 1000):
+        key = rng.integers(0, 4, n).astype(np.float32)  # tie-heavy
+        key[rng.random(n) < 0.3] = 0.0
+        idx = np.arange(n, dtype=np.int32)
+        libc.qsort_r(idx.ctypes.data, n, 4, cb, key.ctypes.data)
+        assert np.array_equal(oracle.stable_argsort(key), idx)
+
+
+# ------------------------------------------------------------ T-functionals
+
+def test_T0_T1_T2_box_closed_form(oracle):
+    n, a, b = 40, 7, 22
+    x = np.zeros(n, np.float32)
+    x[a:b + 1] = 1
+    L = b - a + 1
+    m = a + math.ceil(L / 2) - 1
+    k = b - m
+    assert oracle.tfunctional(0, x)[0] == L
+    v1, i1 = oracle.tfunctional(1, x)
+    v2, i2 = oracle.tfunctional(2, x)
+    assert i1["m"] == m and i2["m"] == m
+    assert v1 == k * (k + 1) / 2
+    assert v2 == k * (k + 1) * (2 * k + 1) / 6
+
+
+def test_T345_two_spikes_closed_form(oracle):
+    n = 60
+    for r in (1, 5, 17):
+        x = np.zeros(n, np.float32)
+        x[10], x[10 + r] = 9.0, 4.0  # sqrt 3 >= sqrt 2 -> sqrt-median at 10
+        expect = {3: r * 4.0, 4: 4.0, 5: math.sqrt(r) * 4.0}
+        for code, e in expect.items():
+            v, idx = oracle.tfunctional(code, x)
+            assert idx["sm"] == 10
+            assert abs(v - e) <= 2e-6 * e
+
+
+def test_t345_tables(oracle):
+    re, im = oracle.t345_table(3, 50)
+    r = np.arange(1, 50)
+    assert np.allclose(re[1:], r * np.cos(5 * np.log(r)), rtol=1e-6, atol=1e-6)
+    assert np.allclose(im[1:], r * np.sin(5 * np.log(r)), rtol=1e-6, atol=1e-6)
+    re, im = oracle.t345_table(5, 50)
+    assert np.allclose(np.hypot(re[1:], im[1:]), np.sqrt(r), rtol=1e-6)
+
+
+def _brute_T6(oracle, x):
+    xs = np.sqrt(x.astype(np.float32))
+    sm = oracle.weighted_median(xs)
+    L = x.size - sm
+    w = [np.float32(np.float32(r) * x[sm + r]) for r in range(L)]
+    order = sorted(range(L), key=lambda r: w[r])  # Python's sort is stable
+    ds = np.sqrt(np.array([x[sm + r] for r in order], np.float32))
+    k = oracle.weighted_median(ds)
+    return w[order[k]], sm, k, order[k]
+
+
+def _brute_T7(oracle, x):
+    m = oracle.weighted_median(x)
+    tail = sorted(x[m:].tolist())
+    k = oracle.weighted_median(np.sqrt(np.array(tail, np.float32)))
+    return np.float32(tail[k]), m, k
+
+
+def test_T6_T7_bruteforce_short_columns(oracle):
+    rng = np.random.default_rng(3)
+    cases = [np.zeros(5, np.float32), np.ones(6, np.float32)]
+    for n in range(1, 9):
+        for _ in range(40):
+            x = rng.integers(0, 3, n).astype(np.float32)  # ties and zeros
+            x[rng.random(n) < 0.3] *= 0.5
+            cases.append(x)
+    for x in cases:
+        v6, i6 = oracle.tfunctional(6, x)
+        e6, sm, k, src = _brute_T6(oracle, x)
+        assert (v6, i6["sm"], i6["rank"], i6["src"]) == (e6, sm, k, src)
+        v7, i7 = oracle.tfunctional(7, x)
+        e7, m, k7 = _brute_T7(oracle, x)
+        assert (v7, i7["m"], i7["rank"]) == (e7, m, k7)
+
+
+# ------------------------------------------------------------ P-functionals
+
+def test_P1_monotone_and_zigzag(oracle):
+    x = np.linspace(0, 3, 17, dtype=np.float32)
+    assert abs(oracle.pfunctional(1, x)[0] - 3.0) < 1e-6
+    z = np.array([0, 1, 0, 1, 0], np.float32)
+    assert oracle.pfunctional(1, z)[0] == 4.0
+
+
+def test_P2_bruteforce(oracle):
+    rng = np.random.default_rng(5)
+    for n in (1, 3, 8, 33, 363):
+        x = rng.random(n, dtype=np.float32)
+        v, idx = oracle.pfunctional(2, x)
+        srt = np.sort(x)
+        m = oracle.weighted_median(srt)
+        assert v == srt[m] and idx["rank"] == m and x[idx["src"]] == v
+
+
+def test_P3_closed_forms(oracle):
+    # The reference stores linspace as float, so the trapz spacings carry a
+    # relative rounding error up to ~1e-5 at n = 363: tolerance 2e-5.
+    for n in (7, 91, 363):
+        c = 0.8
+        v, _ = oracle.pfunctional(3, np.full(n, c, np.float32))
+        assert abs(v - c ** 4 / (n - 1)) <= 2e-5 * c ** 4 / (n - 1)
+        x = np.zeros(n, np.float32)
+        x[n // 3] = 5.0
+        v, _ = oracle.pfunctional(3, x)
+        e = 2 * (5.0 / n) ** 4
+        assert abs(v - e) <= 2e-5 * e
+
+
+@pytest.mark.parametrize("n", [7, 64, 65, 91, 363, 1449])
+def test_dft_f64_matches_numpy(oracle, n):
+    x = np.random.default_rng(n).random(n)
+    assert np.allclose(oracle.dft_f64(x), np.fft.fft(x), rtol=1e-12, atol=1e-10)
+
+
+def test_hermite_basis(oracle):
+    # h_n(0) closed forms and orthonormality on a fine symmetric grid
+    assert abs(oracle.hermite_function(0, 0.0) - math.pi ** -0.25) < 1e-7
+    assert abs(oracle.hermite_function(2, 0.0) + 2 / math.sqrt(8 * math.sqrt(math.pi))) < 1e-7
+    assert oracle.hermite_function(3, 0.0) == 0.0
+    z = np.linspace(-10, 10, 4001)
+    dz = z[1] - z[0]
+    H = np.array([[oracle.hermite_function(k, float(t)) for t in z] for k in range(6)],
+                 np.float64)
+    G = H @ H.T * dz
+    assert np.allclose(G, np.eye(6), atol=1e-4)
+
+
+def test_hermite_pfunctional_grid(oracle):
+    # symmetric centre: z_p = p*10/c - 10 below, (p-c)*10/(n-1-c) above
+    n, c = 21, 10
+    x = np.zeros(n, np.float32)
+    x[c] = 1.0
+    v, _ = oracle.pfunctional(0, x, order=0, center=c)
+    assert abs(v - math.pi ** -0.25) < 1e-6
+    x[:] = 0
+    x[0] = 1.0
+    v, _ = oracle.pfunctional(0, x, order=2, center=c)
+    assert abs(v - oracle.hermite_function(2, -10.0)) == 0
+
+
+def test_zscore(oracle):
+    x = np.random.default_rng(2).random(360, dtype=np.float32)
+    z = oracle.zscore(x)
+    assert abs(z.mean()) < 1e-6 and abs(z.std(ddof=1) - 1) < 1e-6
+
+
+# ------------------------------------------------------------------ drivers
+
+def test_sinogram_driver_matches_column_functionals(oracle):
+    side = 9
+    n, _, _ = oracle.pad_geometry(side, side)
+    img = oracle.pad(np.random.default_rng(4).random(side * side, dtype=np.float32),
+                     side, side)
+    codes = [0, 1, 2, 3, 4, 5, 6, 7]
+    sino, aux = oracle.sinograms(img, n, 12, 30.0, codes, nthreads=2)
+    for a in (0, 5, 11):
+        c, s = oracle.rotation_trig(a, 30.0)
+        rot = oracle.rotate(img, n, c, s).reshape(n, n)
+        for col in range(n):
+            for t, code in enumerate(codes):
+                v, idx = oracle.tfunctional(code, rot[col])
+                assert sino[t, a, col] == v
+                if code in (1, 2):
+                    assert aux["med_x"][a, col] == idx["m"]
+                if code in (3, 4, 5):
+                    assert aux["med_s"][a, col] == idx["sm"]
+
+
+def test_fp64_oracle_agrees_with_fp32(oracle):
+    side = 24
+    n, _, _ = oracle.pad_geometry(side, side)
+    img = oracle.pad(np.random.default_rng(8).random(side * side, dtype=np.float32),
+                     side, side)
+    codes = [0, 1, 2, 3, 4, 5, 6, 7]
+    s32, aux = oracle.sinograms(img, n, 36, 10.0, codes, nthreads=4)
+    s64 = oracle.sinograms_f64(img, n, 36, 10.0, codes, aux, nthreads=4)
+    for t in range(len(codes)):
+        scale = np.abs(s64[t]).max()
+        assert np.abs(s32[t] - s64[t]).max() <= 1e-5 * scale, codes[t]
+    pf = [(1,), (2,), (3,), (0, 2, (n - 1) // 2)]
+    for t in range(len(codes)):
+        c32, paux = oracle.circus(s32[t], n, 36, pf)
+        c64 = oracle.circus_f64(s64[t], n, 36, pf, paux["p2_src"])
+        for p in range(len(pf)):
+            scale = np.abs(c64[p]).max()
+            assert np.abs(c32[p] - c64[p]).max() <= 1e-4 * scale + 1e-30, (t, p)

@@ -1,0 +1,97 @@
+"""ctypes binding of ``libtt_amd.so`` (the C ABI declared in include/tt.h).
+
+There is no fallback: if the library is missing or no GPU is visible, every
+compute entry point raises.  ``load()`` only needs the shared object and does
+not touch the GPU, so CPU-only tests can check the exported symbols.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtt_amd.so")
+
+TT_OK, TT_ERR_INVALID, TT_ERR_NOMEM, TT_ERR_DEVICE, TT_ERR_UNSUPPORTED = range(5)
+TT_KERNEL_SINO_T05, TT_KERNEL_SINO_SORT, TT_KERNEL_CIRCUS, TT_KERNEL_AUX = range(4)
+
+# Every symbol include/tt.h declares; tests check the library exports them.
+EXPORTED = (
+    "tt_context_create", "tt_context_destroy", "tt_context_set_stream",
+    "tt_context_synchronize", "tt_last_error", "tt_pad_geometry", "tt_pad",
+    "tt_angle_count", "tt_sinograms", "tt_sinograms_host", "tt_circus",
+    "tt_circus_host", "tt_profile_enable", "tt_profile_read",
+)
+
+
+class TTError(RuntimeError):
+    """A tt_* entry point returned a status other than TT_OK."""
+
+    def __init__(self, fn: str, status: int, msg: str):
+        super().__init__(f"{fn} failed (status {status}): {msg}")
+        self.status = status
+
+
+class tt_pfunc(C.Structure):
+    _fields_ = [("code", C.c_int32), ("order", C.c_int32), ("center", C.c_int32)]
+
+
+class tt_sino_aux(C.Structure):
+    _fields_ = [("median_x", C.c_void_p), ("median_sqrt", C.c_void_p),
+                ("t6_rank", C.c_void_p), ("t6_src", C.c_void_p),
+                ("t7_rank", C.c_void_p)]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load libtt_amd.so (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python -m tracetransform_amd.build` (there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, f32 = C.c_void_p, C.c_int32, C.c_float
+        ip = C.POINTER(C.c_int)
+        L.tt_last_error.restype = C.c_char_p
+        L.tt_last_error.argtypes = []
+        L.tt_context_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.tt_context_destroy.argtypes = [vp]
+        L.tt_context_set_stream.argtypes = [vp, vp]
+        L.tt_context_synchronize.argtypes = [vp]
+        L.tt_pad_geometry.argtypes = [C.c_int, C.c_int, ip, ip, ip]
+        L.tt_pad.argtypes = [vp, vp, C.c_int, C.c_int, vp]
+        L.tt_angle_count.argtypes = [f32]
+        L.tt_angle_count.restype = C.c_int
+        L.tt_sinograms.argtypes = [vp, vp, C.c_int, C.c_int, f32,
+                                   C.POINTER(i32), C.c_int, vp,
+                                   C.POINTER(tt_sino_aux)]
+        L.tt_sinograms_host.argtypes = [vp, vp, C.c_int, C.c_int, f32,
+                                        C.POINTER(i32), C.c_int, vp]
+        L.tt_circus.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int,
+                                C.POINTER(tt_pfunc), C.c_int, vp, vp]
+        L.tt_circus_host.argtypes = [vp, vp, C.c_int, C.c_int,
+                                     C.POINTER(tt_pfunc), C.c_int, vp]
+        L.tt_profile_enable.argtypes = [vp, C.c_int]
+        L.tt_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int64)]
+        for name in EXPORTED:
+            if name not in ("tt_last_error", "tt_angle_count"):
+                getattr(L, name).restype = C.c_int
+        _lib = L
+        return L
+
+
+def check(fn: str, status: int) -> None:
+    if status != TT_OK:
+        msg = load().tt_last_error().decode(errors="replace")
+        if status == TT_ERR_INVALID:
+            raise ValueError(f"{fn}: {msg}")
+        raise TTError(fn, status, msg)

@@ -1,0 +1,189 @@
+"""Device-level entry points over torch tensors (HBM-resident buffers).
+
+Every call runs the HIP kernels of ``libtt_amd.so`` on the current torch
+stream of the tensor's device.  Memory layouts are the reference's
+column-major ones, spelled as C-ordered torch shapes:
+
+  padded image  [n, n]   memory (col, row): tensor[col, row] = image(row, col)
+  sinograms     [nt, A, n]  (one angle's n values contiguous)
+  circus        [n_sino, np, A]
+  index outputs [A, n]
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+
+import torch
+
+from . import _lib
+from .functionals import pfuncs_of, tcodes_of
+
+_tls = threading.local()
+
+
+class Context:
+    """Owns one tt_context (stream binding, device tables, workspaces)."""
+
+    def __init__(self, device: int = 0):
+        L = _lib.load()
+        h = C.c_void_p()
+        _lib.check("tt_context_create", L.tt_context_create(int(device), C.byref(h)))
+        self.handle = h
+        self.device = int(device)
+
+    def close(self):
+        if self.handle:
+            _lib.load().tt_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bind_current_stream(self):
+        s = torch.cuda.current_stream(self.device)
+        _lib.check("tt_context_set_stream",
+                   _lib.load().tt_context_set_stream(self.handle, C.c_void_p(s.cuda_stream)))
+
+    def profile(self, enable: bool):
+        _lib.check("tt_profile_enable",
+                   _lib.load().tt_profile_enable(self.handle, int(bool(enable))))
+
+    def profile_read(self, kind: int):
+        """(summed kernel ms, launch count) of one kernel kind since the last
+        read (see TT_KERNEL_* in include/tt.h)."""
+        ms, cnt = C.c_double(), C.c_int64()
+        _lib.check("tt_profile_read",
+                   _lib.load().tt_profile_read(self.handle, int(kind), C.byref(ms),
+                                               C.byref(cnt)))
+        return ms.value, cnt.value
+
+    def synchronize(self):
+        _lib.check("tt_context_synchronize", _lib.load().tt_context_synchronize(self.handle))
+
+
+def get_context(device: int | None = None) -> Context:
+    if device is None:
+        device = torch.cuda.current_device()
+    cache = getattr(_tls, "ctx", None)
+    if cache is None:
+        cache = _tls.ctx = {}
+    if device not in cache:
+        cache[device] = Context(device)
+    return cache[device]
+
+
+def _dptr(t: torch.Tensor) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr())
+
+
+def _require_cuda_f32(name: str, t: torch.Tensor):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError(f"{name} must be a CUDA (HIP) tensor")
+    if t.dtype != torch.float32:
+        raise ValueError(f"{name} must be float32 (got {t.dtype})")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+def angle_count(step_deg: float) -> int:
+    return _lib.load().tt_angle_count(float(step_deg))
+
+
+def pad_geometry(rows: int, cols: int):
+    n, dx, dy = C.c_int(), C.c_int(), C.c_int()
+    _lib.check("tt_pad_geometry",
+               _lib.load().tt_pad_geometry(int(rows), int(cols), C.byref(n),
+                                           C.byref(dx), C.byref(dy)))
+    return n.value, dx.value, dy.value
+
+
+def pad(image_cm: torch.Tensor, rows: int, cols: int) -> torch.Tensor:
+    """Zero-pad a column-major rows x cols image ([cols, rows] tensor) to the
+    reference's n x n square (c++/src/auxiliary.cpp:257-275)."""
+    _require_cuda_f32("image", image_cm)
+    if image_cm.numel() != rows * cols:
+        raise ValueError("image size does not match rows x cols")
+    n, _, _ = pad_geometry(rows, cols)
+    out = torch.empty((n, n), dtype=torch.float32, device=image_cm.device)
+    ctx = get_context(image_cm.device.index)
+    ctx.bind_current_stream()
+    _lib.check("tt_pad", _lib.load().tt_pad(ctx.handle, _dptr(image_cm), rows, cols,
+                                            _dptr(out)))
+    return out
+
+
+def sinograms(image_cm: torch.Tensor, n_angles: int, step_deg: float, tfunctionals,
+              out: torch.Tensor | None = None, with_aux: bool = False):
+    """Sinograms of a padded n x n image ([n, n] column-major tensor).
+
+    Returns ``sino`` [nt, A, n], or ``(sino, aux)`` with int32 [A, n] index
+    tensors ``median_x`` and ``median_sqrt`` when ``with_aux``."""
+    _require_cuda_f32("image", image_cm)
+    if image_cm.dim() != 2 or image_cm.shape[0] != image_cm.shape[1]:
+        raise ValueError("image must be a square padded [n, n] tensor")
+    n = image_cm.shape[0]
+    codes = tcodes_of(tfunctionals)
+    nt = len(codes)
+    dev = image_cm.device
+    if out is None:
+        out = torch.empty((nt, n_angles, n), dtype=torch.float32, device=dev)
+    else:
+        _require_cuda_f32("out", out)
+        if tuple(out.shape) != (nt, n_angles, n):
+            raise ValueError(f"out must have shape {(nt, n_angles, n)}")
+    aux = None
+    aux_s = None
+    if with_aux:
+        aux = {k: torch.full((n_angles, n), -1, dtype=torch.int32, device=dev)
+               for k in ("median_x", "median_sqrt")}
+        aux_s = _lib.tt_sino_aux(aux["median_x"].data_ptr(),
+                                 aux["median_sqrt"].data_ptr(), None, None, None)
+    ctx = get_context(dev.index)
+    ctx.bind_current_stream()
+    carr = (C.c_int32 * max(nt, 1))(*codes)
+    _lib.check("tt_sinograms",
+               _lib.load().tt_sinograms(ctx.handle, _dptr(image_cm), n, int(n_angles),
+                                        float(step_deg), carr, nt,
+                                        _dptr(out) if nt else None,
+                                        C.byref(aux_s) if aux_s is not None else None))
+    return (out, aux) if with_aux else out
+
+
+def circus(sino: torch.Tensor, pfunctionals, out: torch.Tensor | None = None,
+           with_p2_rank: bool = False):
+    """P-functionals over sinogram columns.
+
+    ``sino`` is [n_sino, n_cols, n] (or [n_cols, n] for one sinogram).
+    Returns ``circus`` [n_sino, np, n_cols] (and the P2 rank tensor)."""
+    _require_cuda_f32("sino", sino)
+    single = sino.dim() == 2
+    s3 = sino.unsqueeze(0) if single else sino
+    if s3.dim() != 3:
+        raise ValueError("sino must be [n_sino, n_cols, n] or [n_cols, n]")
+    n_sino, n_cols, n = s3.shape
+    pf = pfuncs_of(pfunctionals)
+    npf = len(pf)
+    dev = sino.device
+    if out is None:
+        out = torch.empty((n_sino, npf, n_cols), dtype=torch.float32, device=dev)
+    else:
+        _require_cuda_f32("out", out)
+        if tuple(out.shape) != (n_sino, npf, n_cols):
+            raise ValueError(f"out must have shape {(n_sino, npf, n_cols)}")
+    rank = None
+    if with_p2_rank:
+        rank = torch.full((n_sino, npf, n_cols), -1, dtype=torch.int32, device=dev)
+    arr = (_lib.tt_pfunc * max(npf, 1))(*[_lib.tt_pfunc(*t) for t in pf])
+    ctx = get_context(dev.index)
+    ctx.bind_current_stream()
+    _lib.check("tt_circus",
+               _lib.load().tt_circus(ctx.handle, _dptr(s3), n, n_cols, n_sino, arr, npf,
+                                     _dptr(out), _dptr(rank) if rank is not None else None))
+    if single:
+        out = out[0]
+        rank = rank[0] if rank is not None else None
+    return (out, rank) if with_p2_rank else out

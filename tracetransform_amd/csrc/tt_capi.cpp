@@ -1,0 +1,637 @@
+// tt_capi.cpp -- C-ABI entry points of the MI355X trace transform.
+//
+// Host responsibilities: argument checks (the reference asserts or throws;
+// this ABI returns TT_ERR_INVALID with a message), per-context device
+// workspaces, and the small host-computed tables whose rounding must match
+// the reference's libm calls bit for bit (rotation trig, T3-T5 tables,
+// Hermite weights).  All GPU work is in tt_kernels.hip.
+#include "../../include/tt.h"
+#include "tt_internal.hpp"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+tt_status fail(tt_status st, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return st;
+}
+
+#define TT_HIP(expr)                                                           \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess)                                                  \
+            return fail(e_ == hipErrorOutOfMemory ? TT_ERR_NOMEM : TT_ERR_DEVICE, \
+                        "%s failed: %s", #expr, hipGetErrorString(e_));        \
+    } while (0)
+
+#define TT_CHECK(cond, ...)                                                    \
+    do {                                                                       \
+        if (!(cond))                                                           \
+            return fail(TT_ERR_INVALID, __VA_ARGS__);                          \
+    } while (0)
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap)
+            return hipSuccess;
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            if (e != hipSuccess)
+                return e;
+            ptr = nullptr;
+            cap = 0;
+        }
+        hipError_t e = hipMalloc(&ptr, bytes);
+        if (e == hipSuccess)
+            cap = bytes;
+        return e;
+    }
+    void release()
+    {
+        if (ptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// ---- host tables (reference rounding) -------------------------------------
+
+// c++/src/auxiliary.cpp:180 deg2rad, c++/src/sinogram.cpp:112 angle,
+// c++/src/auxiliary.cpp:235-236 std::cos/std::sin of the negated float
+// angle (glibc cosf/sinf).
+void make_trig(int A, float step, std::vector<float2> &out)
+{
+    out.resize(A);
+    for (int k = 0; k < A; k++) {
+        const float deg = (float)((double)k * (double)step);
+        const float rad = (float)(deg * M_PI / 180);
+        out[k].x = cosf(-rad);
+        out[k].y = sinf(-rad);
+    }
+}
+
+// c++/src/functionals.cpp:136-198 (GCC branch): double log/cos/sin, stored
+// as float; entry 0 is never read.
+void make_tab345(int n, std::vector<float2> &out)
+{
+    out.assign((size_t)3 * n, float2{0.0f, 0.0f});
+    for (int r = 1; r < n; r++) {
+        const double lr = log((double)r);
+        out[r] = float2{(float)(r * cos(5.0 * lr)), (float)(r * sin(5.0 * lr))};
+        out[(size_t)n + r] = float2{(float)cos(3.0 * lr), (float)sin(3.0 * lr)};
+        const double sq = sqrt((double)r);
+        out[(size_t)2 * n + r] =
+            float2{(float)(sq * cos(4.0 * lr)), (float)(sq * sin(4.0 * lr))};
+    }
+}
+
+void make_dft_tw(int n, std::vector<float2> &out)
+{
+    out.resize(n);
+    for (int k = 0; k < n; k++) {
+        const double a = 2.0 * M_PI * (double)k / (double)n;
+        out[k] = float2{(float)cos(a), (float)sin(a)};
+    }
+}
+
+// c++/src/functionals.cpp:62-84: recursion in double, each level rounded to
+// float on return; normalisation in double with z*z/2 in float.
+float herm_poly(unsigned order, float x)
+{
+    if (order == 0)
+        return 1.0f;
+    if (order == 1)
+        return (float)(2.0 * x);
+    return (float)(2.0 * x * herm_poly(order - 1, x) -
+                   2.0 * (order - 1) * herm_poly(order - 2, x));
+}
+
+unsigned herm_factorial(unsigned n)
+{
+    return (n == 1 || n == 0) ? 1 : herm_factorial(n - 1) * n;
+}
+
+float herm_function(unsigned order, float z)
+{
+    const float zz = z * z / 2;
+    return (float)(herm_poly(order, z) /
+                   (sqrt(pow(2, order) * herm_factorial(order) * sqrt(M_PI)) *
+                    exp((double)zz)));
+}
+
+// c++/src/functionals.cpp:369-386: the z grid over [-10, 10] split at
+// center; the weight of sample p is hermite_function(order, z_p).
+void make_hermite(int n, unsigned order, int center, float *w)
+{
+    const float lo = (float)(10.0 / center);
+    const float hi = (float)(10.0 / (double)(n - 1 - center));
+    for (int p = 0; p < n; p++) {
+        const float z = p < center ? p * lo - 10 : (p - center) * hi;
+        w[p] = herm_function(order, z);
+    }
+}
+
+}  // namespace
+
+// Event pairs bracketing kernel launches while profiling is on.
+struct KernelTimer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<int> kind;        // kind of pair i
+    double ms[TT_KERNEL_KINDS] = {};
+    int64_t count[TT_KERNEL_KINDS] = {};
+
+    hipError_t take(hipEvent_t *e)
+    {
+        if (used == pool.size()) {
+            hipEvent_t ev;
+            hipError_t err = hipEventCreate(&ev);
+            if (err != hipSuccess)
+                return err;
+            pool.push_back(ev);
+        }
+        *e = pool[used++];
+        return hipSuccess;
+    }
+    hipError_t begin(int k, hipStream_t s)
+    {
+        if (!on)
+            return hipSuccess;
+        hipEvent_t e;
+        hipError_t err = take(&e);
+        if (err == hipSuccess)
+            err = hipEventRecord(e, s);
+        kind.push_back(k);
+        return err;
+    }
+    hipError_t end(hipStream_t s)
+    {
+        if (!on)
+            return hipSuccess;
+        hipEvent_t e;
+        hipError_t err = take(&e);
+        return err == hipSuccess ? hipEventRecord(e, s) : err;
+    }
+    hipError_t collect()
+    {
+        for (size_t i = 0; i < kind.size(); i++) {
+            float t = 0.0f;
+            hipError_t err = hipEventSynchronize(pool[2 * i + 1]);
+            if (err == hipSuccess)
+                err = hipEventElapsedTime(&t, pool[2 * i], pool[2 * i + 1]);
+            if (err != hipSuccess)
+                return err;
+            ms[kind[i]] += t;
+            count[kind[i]] += 1;
+        }
+        kind.clear();
+        used = 0;
+        return hipSuccess;
+    }
+    void release()
+    {
+        for (hipEvent_t e : pool)
+            (void)hipEventDestroy(e);
+        pool.clear();
+        kind.clear();
+        used = 0;
+    }
+};
+
+struct tt_context {
+    int device = 0;
+    KernelTimer timer;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    DevBuf imgT, trig, tab345, dft_tw, herm_w, io_a, io_b;
+    int trig_A = -1;
+    float trig_step = 0.0f;
+    int tab_n = -1, dft_n = -1;
+    std::vector<int32_t> herm_key;
+    std::vector<float2> h_trig, h_tab, h_tw;
+    std::vector<float> h_herm;
+
+    tt_status set_device()
+    {
+        TT_HIP(hipSetDevice(device));
+        return TT_OK;
+    }
+
+    tt_status ensure_trig(int A, float step)
+    {
+        if (trig_A == A && trig_step == step)
+            return TT_OK;
+        make_trig(A, step, h_trig);
+        TT_HIP(trig.ensure(sizeof(float2) * A));
+        TT_HIP(hipMemcpyAsync(trig.ptr, h_trig.data(), sizeof(float2) * A,
+                              hipMemcpyHostToDevice, stream));
+        TT_HIP(hipStreamSynchronize(stream));
+        trig_A = A;
+        trig_step = step;
+        return TT_OK;
+    }
+
+    tt_status ensure_tab345(int n)
+    {
+        if (tab_n == n)
+            return TT_OK;
+        make_tab345(n, h_tab);
+        TT_HIP(tab345.ensure(sizeof(float2) * 3 * (size_t)n));
+        TT_HIP(hipMemcpyAsync(tab345.ptr, h_tab.data(), sizeof(float2) * 3 * (size_t)n,
+                              hipMemcpyHostToDevice, stream));
+        TT_HIP(hipStreamSynchronize(stream));
+        tab_n = n;
+        return TT_OK;
+    }
+
+    tt_status ensure_dft(int n)
+    {
+        if (dft_n == n)
+            return TT_OK;
+        make_dft_tw(n, h_tw);
+        TT_HIP(dft_tw.ensure(sizeof(float2) * (size_t)n));
+        TT_HIP(hipMemcpyAsync(dft_tw.ptr, h_tw.data(), sizeof(float2) * (size_t)n,
+                              hipMemcpyHostToDevice, stream));
+        TT_HIP(hipStreamSynchronize(stream));
+        dft_n = n;
+        return TT_OK;
+    }
+
+    tt_status ensure_hermite(int n, const std::vector<int32_t> &key)
+    {
+        if (herm_key == key)
+            return TT_OK;
+        const size_t nh = key.size() / 2;
+        h_herm.assign(nh * (size_t)n, 0.0f);
+        for (size_t h = 0; h < nh; h++)
+            make_hermite(n, (unsigned)key[2 * h], key[2 * h + 1],
+                         h_herm.data() + h * (size_t)n);
+        TT_HIP(herm_w.ensure(sizeof(float) * (nh ? nh : 1) * (size_t)n));
+        if (nh)
+            TT_HIP(hipMemcpyAsync(herm_w.ptr, h_herm.data(),
+                                  sizeof(float) * nh * (size_t)n,
+                                  hipMemcpyHostToDevice, stream));
+        TT_HIP(hipStreamSynchronize(stream));
+        herm_key = key;
+        return TT_OK;
+    }
+};
+
+namespace {
+
+tt_status check_tcodes(const int32_t *tcodes, int nt)
+{
+    TT_CHECK(nt >= 0, "nt must be >= 0 (got %d)", nt);
+    TT_CHECK(nt == 0 || tcodes, "tcodes is NULL");
+    for (int t = 0; t < nt; t++)
+        TT_CHECK(tcodes[t] >= 0 && tcodes[t] <= 7,
+                 "invalid T-functional code %d at position %d (valid: 0..7)",
+                 tcodes[t], t);
+    return TT_OK;
+}
+
+tt_status check_pfuncs(const tt_pfunc *pf, int np, int n)
+{
+    TT_CHECK(np >= 0 && np <= ttk::kMaxP, "np must be in 0..%d (got %d)",
+             ttk::kMaxP, np);
+    TT_CHECK(np == 0 || pf, "pf is NULL");
+    for (int q = 0; q < np; q++) {
+        TT_CHECK(pf[q].code >= 0 && pf[q].code <= 3,
+                 "invalid P-functional code %d at position %d (valid: 0..3)",
+                 pf[q].code, q);
+        if (pf[q].code == TT_P_HERMITE) {
+            TT_CHECK(pf[q].order >= 0, "Hermite order must be >= 0 (got %d)",
+                     pf[q].order);
+            TT_CHECK(pf[q].center >= 0 && pf[q].center < n,
+                     "Hermite center %d unset or outside [0, %d)", pf[q].center,
+                     n);
+        }
+    }
+    return TT_OK;
+}
+
+}  // namespace
+
+#define TT_TRY(expr)                                                           \
+    do {                                                                       \
+        tt_status s_ = (expr);                                                 \
+        if (s_ != TT_OK)                                                       \
+            return s_;                                                         \
+    } while (0)
+
+extern "C" {
+
+TT_API const char *tt_last_error(void) { return g_err.c_str(); }
+
+TT_API tt_status tt_context_create(int device, tt_context **out)
+{
+    TT_CHECK(out, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    TT_HIP(hipGetDeviceCount(&count));
+    TT_CHECK(device >= 0 && device < count, "device %d not in [0, %d)", device,
+             count);
+    tt_context *ctx = new (std::nothrow) tt_context();
+    if (!ctx)
+        return fail(TT_ERR_NOMEM, "context allocation failed");
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(TT_ERR_DEVICE, "context setup failed: %s", hipGetErrorString(e));
+    }
+    ctx->stream = ctx->own;
+    *out = ctx;
+    return TT_OK;
+}
+
+TT_API tt_status tt_context_destroy(tt_context *ctx)
+{
+    if (!ctx)
+        return TT_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    DevBuf *bufs[] = {&ctx->imgT, &ctx->trig, &ctx->tab345, &ctx->dft_tw,
+                      &ctx->herm_w, &ctx->io_a, &ctx->io_b};
+    for (DevBuf *b : bufs)
+        b->release();
+    ctx->timer.release();
+    if (ctx->own)
+        (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+    return TT_OK;
+}
+
+TT_API tt_status tt_context_set_stream(tt_context *ctx, void *stream)
+{
+    TT_CHECK(ctx, "ctx is NULL");
+    hipStream_t next = stream ? (hipStream_t)stream : ctx->own;
+    if (next != ctx->stream) {
+        TT_TRY(ctx->set_device());
+        // Tables and workspaces are shared by all work of the context: drain
+        // the old stream before work is queued on the new one.
+        TT_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->stream = next;
+    }
+    return TT_OK;
+}
+
+TT_API tt_status tt_context_synchronize(tt_context *ctx)
+{
+    TT_CHECK(ctx, "ctx is NULL");
+    TT_TRY(ctx->set_device());
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    return TT_OK;
+}
+
+TT_API tt_status tt_pad_geometry(int rows, int cols, int *n, int *dfx, int *dfy)
+{
+    TT_CHECK(rows > 0 && cols > 0, "image must be non-empty (got %d x %d)", rows,
+             cols);
+    TT_CHECK(n && dfx && dfy, "output pointer is NULL");
+    // c++/src/auxiliary.cpp:259-270
+    const float ox = (float)(floor((cols + 1) / 2.0) - 1);
+    const float oy = (float)(floor((rows + 1) / 2.0) - 1);
+    const float hx = (float)cols - 1 - ox - 1;
+    const float hy = (float)rows - 1 - oy - 1;
+    const int rlast = (int)ceilf(hypotf(hx, hy)) + 1;
+    const int nb = 2 * rlast + 1;
+    const float op = (float)(floor((nb + 1) / 2.0) - 1);
+    *n = nb;
+    *dfx = (int)(op - ox);
+    *dfy = (int)(op - oy);
+    return TT_OK;
+}
+
+TT_API tt_status tt_pad(tt_context *ctx, const float *d_image, int rows,
+                        int cols, float *d_padded)
+{
+    TT_CHECK(ctx && d_image && d_padded, "NULL argument");
+    int n, dfx, dfy;
+    TT_TRY(tt_pad_geometry(rows, cols, &n, &dfx, &dfy));
+    TT_TRY(ctx->set_device());
+    TT_HIP(ctx->timer.begin(TT_KERNEL_AUX, ctx->stream));
+    TT_HIP(ttk::launch_pad(d_image, rows, cols, d_padded, n, dfx, dfy, ctx->stream));
+    TT_HIP(ctx->timer.end(ctx->stream));
+    return TT_OK;
+}
+
+TT_API int tt_angle_count(float angle_step_deg)
+{
+    if (!(angle_step_deg > 0.0f))
+        return 0;
+    return (int)floor(360.0 / (double)angle_step_deg);
+}
+
+TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
+                              int n_angles, float angle_step_deg,
+                              const int32_t *tcodes, int nt, float *d_sino,
+                              const tt_sino_aux *aux)
+{
+    TT_CHECK(ctx, "ctx is NULL");
+    TT_CHECK(d_image && (d_sino || nt == 0), "NULL image or sinogram pointer");
+    TT_CHECK(n >= 3, "padded side n must be >= 3 (got %d)", n);
+    TT_CHECK(n_angles >= 1, "n_angles must be >= 1 (got %d)", n_angles);
+    TT_CHECK(angle_step_deg > 0.0f && isfinite(angle_step_deg),
+             "angle step must be positive and finite");
+    TT_CHECK((long)n * n < (1L << 30), "padded side %d too large", n);
+    TT_TRY(check_tcodes(tcodes, nt));
+    tt_sino_aux none = {};
+    const tt_sino_aux &ax = aux ? *aux : none;
+
+    int slot[8];
+    for (int k = 0; k < 8; k++)
+        slot[k] = -1;
+    std::vector<std::pair<int, int>> dups;  // (from slot, to slot)
+    for (int t = 0; t < nt; t++) {
+        if (slot[tcodes[t]] < 0)
+            slot[tcodes[t]] = t;
+        else
+            dups.push_back({slot[tcodes[t]], t});
+    }
+    if (slot[6] >= 0 || slot[7] >= 0 || ax.t6_rank || ax.t6_src || ax.t7_rank)
+        return fail(TT_ERR_UNSUPPORTED, "T6/T7 are not available in this build");
+
+    TT_TRY(ctx->set_device());
+    hipStream_t st = ctx->stream;
+    const bool need_x = slot[1] >= 0 || slot[2] >= 0 || ax.median_x;
+    const bool do345 = slot[3] >= 0 || slot[4] >= 0 || slot[5] >= 0;
+    const bool need_s = do345 || ax.median_sqrt;
+    const bool any = need_x || need_s || slot[0] >= 0;
+    if (!any)
+        return TT_OK;
+
+    TT_TRY(ctx->ensure_trig(n_angles, angle_step_deg));
+    if (do345)
+        TT_TRY(ctx->ensure_tab345(n));
+    TT_HIP(ctx->imgT.ensure(sizeof(float) * (size_t)n * n));
+    TT_HIP(ctx->timer.begin(TT_KERNEL_AUX, st));
+    TT_HIP(ttk::launch_transpose(d_image, ctx->imgT.as<float>(), n, st));
+    TT_HIP(ctx->timer.end(st));
+
+    ttk::SinoParams p = {};
+    p.img = d_image;
+    p.imgT = ctx->imgT.as<float>();
+    p.trig = ctx->trig.as<float2>();
+    p.tab345 = do345 ? ctx->tab345.as<float2>() : nullptr;
+    p.sino = d_sino;
+    p.med_x = ax.median_x;
+    p.med_s = ax.median_sqrt;
+    p.n = n;
+    p.A = n_angles;
+    p.nb = (n + ttk::kWave - 1) / ttk::kWave;
+    p.items = p.nb * n_angles;
+    for (int k = 0; k < 8; k++)
+        p.slot[k] = k < 6 ? slot[k] : -1;
+    TT_HIP(ctx->timer.begin(TT_KERNEL_SINO_T05, st));
+    TT_HIP(ttk::launch_sino_t05(p, need_x, need_s, do345, st));
+    TT_HIP(ctx->timer.end(st));
+    const size_t plane = sizeof(float) * (size_t)n * n_angles;
+    for (auto &d : dups)
+        TT_HIP(hipMemcpyAsync((char *)d_sino + plane * d.second,
+                              (char *)d_sino + plane * d.first, plane,
+                              hipMemcpyDeviceToDevice, st));
+    return TT_OK;
+}
+
+TT_API tt_status tt_sinograms_host(tt_context *ctx, const float *image, int n,
+                                   int n_angles, float angle_step_deg,
+                                   const int32_t *tcodes, int nt, float *sino)
+{
+    TT_CHECK(ctx && image && (sino || nt == 0), "NULL argument");
+    TT_CHECK(n >= 3 && n_angles >= 1 && nt >= 0, "invalid sizes");
+    TT_TRY(ctx->set_device());
+    const size_t img_b = sizeof(float) * (size_t)n * n;
+    const size_t out_b = sizeof(float) * (size_t)nt * n * n_angles;
+    TT_HIP(ctx->io_a.ensure(img_b));
+    TT_HIP(ctx->io_b.ensure(out_b ? out_b : 4));
+    TT_HIP(hipMemcpyAsync(ctx->io_a.ptr, image, img_b, hipMemcpyHostToDevice,
+                          ctx->stream));
+    TT_TRY(tt_sinograms(ctx, ctx->io_a.as<float>(), n, n_angles, angle_step_deg,
+                        tcodes, nt, ctx->io_b.as<float>(), nullptr));
+    if (out_b)
+        TT_HIP(hipMemcpyAsync(sino, ctx->io_b.ptr, out_b, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    return TT_OK;
+}
+
+TT_API tt_status tt_circus(tt_context *ctx, const float *d_sino, int n,
+                           int n_cols, int n_sino, const tt_pfunc *pf, int np,
+                           float *d_circus, int32_t *d_p2_rank)
+{
+    TT_CHECK(ctx, "ctx is NULL");
+    TT_CHECK(n >= 2, "sinogram height must be >= 2 (got %d)", n);
+    TT_CHECK(n_cols >= 1 && n_sino >= 1, "n_cols and n_sino must be >= 1");
+    TT_CHECK(n <= 8192, "sinogram height %d above the supported 8192", n);
+    TT_TRY(check_pfuncs(pf, np, n));
+    if (np == 0)
+        return TT_OK;
+    TT_CHECK(d_sino && d_circus, "NULL sinogram or circus pointer");
+    TT_TRY(ctx->set_device());
+
+    ttk::CircusParams p = {};
+    p.sino = d_sino;
+    p.out = d_circus;
+    p.p2rank = d_p2_rank;
+    p.n = n;
+    p.ncols = n_cols;
+    p.nsino = n_sino;
+    p.np = np;
+    std::vector<int32_t> hkey;
+    bool p3 = false;
+    for (int q = 0; q < np; q++) {
+        p.code[q] = pf[q].code;
+        p.herm[q] = -1;
+        if (pf[q].code == TT_P_HERMITE) {
+            p.herm[q] = (int)(hkey.size() / 2);
+            hkey.push_back(pf[q].order);
+            hkey.push_back(pf[q].center);
+        }
+        p3 = p3 || pf[q].code == TT_P3;
+    }
+    if (p3) {
+        TT_TRY(ctx->ensure_dft(n));
+        p.dft_tw = ctx->dft_tw.as<float2>();
+    }
+    if (!hkey.empty()) {
+        TT_TRY(ctx->ensure_hermite(n, hkey));
+        p.herm_w = ctx->herm_w.as<float>();
+    }
+    TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS, ctx->stream));
+    TT_HIP(ttk::launch_circus(p, ctx->stream));
+    TT_HIP(ctx->timer.end(ctx->stream));
+    return TT_OK;
+}
+
+TT_API tt_status tt_circus_host(tt_context *ctx, const float *sino, int n,
+                                int n_cols, const tt_pfunc *pf, int np,
+                                float *circus)
+{
+    TT_CHECK(ctx && sino && (circus || np == 0), "NULL argument");
+    TT_CHECK(n >= 2 && n_cols >= 1, "invalid sizes");
+    TT_TRY(ctx->set_device());
+    const size_t in_b = sizeof(float) * (size_t)n * n_cols;
+    const size_t out_b = sizeof(float) * (size_t)np * n_cols;
+    TT_HIP(ctx->io_a.ensure(in_b));
+    TT_HIP(ctx->io_b.ensure(out_b ? out_b : 4));
+    TT_HIP(hipMemcpyAsync(ctx->io_a.ptr, sino, in_b, hipMemcpyHostToDevice,
+                          ctx->stream));
+    TT_TRY(tt_circus(ctx, ctx->io_a.as<float>(), n, n_cols, 1, pf, np,
+                     ctx->io_b.as<float>(), nullptr));
+    if (out_b)
+        TT_HIP(hipMemcpyAsync(circus, ctx->io_b.ptr, out_b, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    return TT_OK;
+}
+
+TT_API tt_status tt_profile_enable(tt_context *ctx, int enable)
+{
+    TT_CHECK(ctx, "ctx is NULL");
+    TT_TRY(ctx->set_device());
+    if (!enable && ctx->timer.on)
+        TT_HIP(ctx->timer.collect());
+    ctx->timer.on = enable != 0;
+    return TT_OK;
+}
+
+TT_API tt_status tt_profile_read(tt_context *ctx, int kind, double *total_ms,
+                                 int64_t *launches)
+{
+    TT_CHECK(ctx, "ctx is NULL");
+    TT_CHECK(kind >= 0 && kind < TT_KERNEL_KINDS, "invalid kernel kind %d", kind);
+    TT_CHECK(total_ms && launches, "output pointer is NULL");
+    TT_TRY(ctx->set_device());
+    TT_HIP(ctx->timer.collect());
+    *total_ms = ctx->timer.ms[kind];
+    *launches = ctx->timer.count[kind];
+    ctx->timer.ms[kind] = 0.0;
+    ctx->timer.count[kind] = 0;
+    return TT_OK;
+}
+
+}  // extern "C"

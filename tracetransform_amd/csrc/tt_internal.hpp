@@ -1,0 +1,72 @@
+// tt_internal.hpp -- launch descriptors shared by the HIP kernels
+// (tt_kernels.hip) and the C-ABI host code (tt_capi.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ttk {
+
+constexpr int kWave = 64;        // CDNA wavefront width
+constexpr int kBlock = 256;      // 4 waves per workgroup
+constexpr int kMaxP = 16;        // P-functionals per tt_circus call
+constexpr int kNumXcd = 8;       // MI355X: 8 XCDs, workgroups round-robin
+
+// Forward T-functional sweep for one padded image (T0..T5).  One wave owns
+// 64 adjacent columns of one angle; each lane streams its column's rows.
+struct SinoParams {
+    const float *img;     // n x n column-major
+    const float *imgT;    // n x n row-major copy (transpose of img)
+    const float2 *trig;   // [A] (cos(-theta), sin(-theta)) as float
+    const float2 *tab345; // [3][n] (re, im) tables of T3, T4, T5
+    float *sino;          // [nt][A][n]
+    int32_t *med_x;       // [A][n] or null
+    int32_t *med_s;       // [A][n] or null
+    int n;
+    int A;
+    int nb;               // column blocks per angle: ceil(n / 64)
+    int items;            // A * nb
+    int slot[8];          // output slot of each T code, -1 if absent
+};
+
+// Sort-based T-functionals T6/T7 for one padded image: one workgroup owns one
+// (angle, column) line at a time.
+struct SortTParams {
+    const float *img;
+    const float2 *trig;
+    float *sino;
+    int32_t *med_x;
+    int32_t *med_s;
+    int32_t *t6_rank;
+    int32_t *t6_src;
+    int32_t *t7_rank;
+    int n;
+    int A;
+    int slot6;
+    int slot7;
+};
+
+// P-functionals over sinogram columns: one workgroup per column.
+struct CircusParams {
+    const float *sino;    // [nsino][ncols][n]
+    float *out;           // [nsino][np][ncols]
+    int32_t *p2rank;      // [nsino][np][ncols] or null
+    const float2 *dft_tw; // [n] (cos, sin)(2 pi k / n)
+    const float *herm_w;  // [nherm][n] Hermite weights
+    int n;
+    int ncols;
+    int nsino;
+    int np;
+    int code[kMaxP];
+    int herm[kMaxP];      // row of herm_w for Hermite entries
+};
+
+hipError_t launch_transpose(const float *in, float *out, int n, hipStream_t s);
+hipError_t launch_pad(const float *in, int rows, int cols, float *out, int n,
+                      int dfx, int dfy, hipStream_t s);
+hipError_t launch_sino_t05(const SinoParams &p, bool need_x, bool need_s,
+                           bool do345, hipStream_t s);
+hipError_t launch_sino_sort(const SortTParams &p, hipStream_t s);
+hipError_t launch_circus(const CircusParams &p, hipStream_t s);
+
+}  // namespace ttk
