@@ -76,8 +76,8 @@ typedef struct tt_context tt_context;
  * reference keeps no state between getSinograms calls. */
 TT_API tt_status tt_context_create(int device, tt_context **out);
 TT_API tt_status tt_context_destroy(tt_context *ctx);
-/* Run subsequent work on a caller stream (hipStream_t as void*); NULL
- * restores the context's own stream. */
+/* Run subsequent work on a caller stream (hipStream_t as void*); NULL is
+ * HIP's default (null) stream.  A new context runs on a private stream. */
 TT_API tt_status tt_context_set_stream(tt_context *ctx, void *stream);
 TT_API tt_status tt_context_synchronize(tt_context *ctx);
 /* Message of the last failure on this host thread ("" if none). */

@@ -12,7 +12,11 @@
  *     indices and sort positions are the parity target for the GPU.
  *   FP64: the same algorithm in double, taking the medians and sort
  *     positions from the FP32 run; GPU values are checked against it within
- *     a stated tolerance.
+ *     a stated tolerance.  geometry 0 computes every sample position in
+ *     double too; geometry 1 keeps the reference's float sample positions
+ *     (rotation coordinates and strict bounds, Hermite z grid, P3 trapz
+ *     abscissae) and computes the values at them in double, isolating value
+ *     arithmetic from the positions that bit-exact medians pin.
  *
  * Pinning status of each function (how a plausible slip would be caught):
  *   tto_pad_geometry/tto_pad  pinned: closed-form size table (256->363,
@@ -815,8 +819,35 @@ void tto_zscore(const float *in, int n, float *out)
 /* FP64 oracle                                                              */
 /* ======================================================================= */
 
-static void rotate_f64(const float *in, int n, float c, float s, double *out)
+/* geometry 0: coordinates and bounds in double.  geometry 1: the
+ * reference's float sample positions (tto_rotate's x, y, bounds and modf
+ * fractions, all exact restatements), with the bilinear values in double. */
+static void rotate_f64(const float *in, int n, float c, float s, int geometry,
+                       double *out)
 {
+    if (geometry) {
+        const float o = (float)((n - 1) / 2.0), ns = -s, lim = (float)(n - 1);
+        for (int col = 0; col < n; col++)
+            for (int row = 0; row < n; row++) {
+                float px = (float)col - o, py = (float)row - o;
+                float x = px * c + py * s;
+                float y = px * ns + py * c;
+                x = x + o;
+                y = y + o;
+                double v = 0.0;
+                if (x >= 0 && x < lim && y >= 0 && y < lim) {
+                    float xi, yi;
+                    double fx = modff(x, &xi), fy = modff(y, &yi);
+                    long X = (long)xi, Y = (long)yi;
+                    v = (double)in[Y + X * n] * (1 - fx) * (1 - fy) +
+                        (double)in[Y + (X + 1) * n] * fx * (1 - fy) +
+                        (double)in[(Y + 1) + X * n] * (1 - fx) * fy +
+                        (double)in[(Y + 1) + (X + 1) * n] * fx * fy;
+                }
+                out[(size_t)row + (size_t)col * n] = v;
+            }
+        return;
+    }
     const double o = (n - 1) / 2.0, cd = c, sd = s, lim = n - 1;
     for (int col = 0; col < n; col++)
         for (int row = 0; row < n; row++) {
@@ -852,7 +883,8 @@ static void t345_table_f64(int which, int n, double *re, double *im)
 int tto_sinograms_f64(const float *img, int n, int n_angles, float step_deg,
                       const int *tcodes, int nt, const int32_t *med_x,
                       const int32_t *med_s, const int32_t *t6_src,
-                      const int32_t *t7_src, double *sino, int nthreads)
+                      const int32_t *t7_src, int geometry, double *sino,
+                      int nthreads)
 {
     if (n <= 0 || n_angles <= 0 || nt < 0 || !valid_tcodes(tcodes, nt))
         return -1;
@@ -874,7 +906,7 @@ int tto_sinograms_f64(const float *img, int n, int n_angles, float step_deg,
         for (int a = 0; a < n_angles; a++) {
             float c, s;
             tto_rotation_trig(tto_deg2rad(tto_angle_deg(a, step_deg)), &c, &s);
-            rotate_f64(img, n, c, s, rot);
+            rotate_f64(img, n, c, s, geometry, rot);
             for (int cc = 0; cc < n; cc++) {
                 const double *x = rot + (size_t)cc * n;
                 size_t aux = (size_t)a * n + cc;
@@ -933,7 +965,8 @@ static double hermite_f64(unsigned order, double z)
 
 int tto_circus_f64(const double *sino, int n, int n_cols, const int *pcodes,
                    const int *orders, const int *centers, int np,
-                   const int32_t *p2_src, double *out, int nthreads)
+                   const int32_t *p2_src, int geometry, double *out,
+                   int nthreads)
 {
     if (n <= 0 || n_cols <= 0 || np < 0)
         return -1;
@@ -965,6 +998,8 @@ int tto_circus_f64(const double *sino, int n, int n_cols, const int *pcodes,
                         double mm = mr * mr + mi * mi;
                         double md = mm * mm;
                         double l = -1 + q * 2.0 / (n - 1);
+                        if (geometry)
+                            l = (float)l;
                         if (q > 0)
                             r += (l - prev_l) * (md + prev_m);
                         prev_l = l;
@@ -976,8 +1011,11 @@ int tto_circus_f64(const double *sino, int n, int n_cols, const int *pcodes,
                 case 0: {
                     int c = centers[p];
                     double lo = 10.0 / c, hi = 10.0 / (double)(n - 1 - c);
+                    float lof = (float)lo, hif = (float)hi;
                     for (int q = 0; q < n; q++) {
                         double z = q < c ? q * lo - 10 : (q - c) * hi;
+                        if (geometry)
+                            z = q < c ? (float)(q * lof - 10) : (float)((q - c) * hif);
                         r += x[q] * hermite_f64((unsigned)orders[p], z);
                     }
                     break;

@@ -73,13 +73,17 @@ int tto_circus(const float *sino, int n, int n_cols, const int *pcodes,
 void tto_zscore(const float *in, int n, float *out);
 
 /* ---- FP64 oracle: same algorithm in double, indices from the FP32 run --- */
+/* geometry 0: sample positions in double; 1: the reference's float sample
+ * positions with values in double. */
 int tto_sinograms_f64(const float *img, int n, int n_angles, float step_deg,
                       const int *tcodes, int nt, const int32_t *med_x,
                       const int32_t *med_s, const int32_t *t6_src,
-                      const int32_t *t7_src, double *sino, int nthreads);
+                      const int32_t *t7_src, int geometry, double *sino,
+                      int nthreads);
 int tto_circus_f64(const double *sino, int n, int n_cols, const int *pcodes,
                    const int *orders, const int *centers, int np,
-                   const int32_t *p2_src, double *out, int nthreads);
+                   const int32_t *p2_src, int geometry, double *out,
+                   int nthreads);
 void tto_dft_f64(const double *x, int n, double *re, double *im);
 
 /* ---- full reference step: sinograms -> circus -> zscore ---------------- */

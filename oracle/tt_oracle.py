@@ -82,9 +82,10 @@ def lib():
                                  C.c_int, _f32p, vp, vp, C.c_int]
         L.tto_zscore.argtypes = [_f32p, C.c_int, _f32p]
         L.tto_sinograms_f64.argtypes = [_f32p, C.c_int, C.c_int, C.c_float, _i32p,
-                                        C.c_int, vp, vp, vp, vp, _f64p, C.c_int]
+                                        C.c_int, vp, vp, vp, vp, C.c_int, _f64p,
+                                        C.c_int]
         L.tto_circus_f64.argtypes = [_f64p, C.c_int, C.c_int, _i32p, _i32p, _i32p,
-                                     C.c_int, vp, _f64p, C.c_int]
+                                     C.c_int, vp, C.c_int, _f64p, C.c_int]
         L.tto_dft_f64.argtypes = [_f64p, C.c_int, _f64p, _f64p]
         L.tto_transform.argtypes = [_f32p, C.c_int, C.c_int, C.c_float, _i32p,
                                     C.c_int, _i32p, _i32p, _i32p, C.c_int, _f32p,
@@ -240,15 +241,17 @@ def sinograms(img, n, n_angles, step_deg, tcodes, nthreads=1):
             {k: v.reshape(n_angles, n) for k, v in aux.items()})
 
 
-def sinograms_f64(img, n, n_angles, step_deg, tcodes, aux, nthreads=1):
+def sinograms_f64(img, n, n_angles, step_deg, tcodes, aux, nthreads=1, geometry=0):
+    """FP64 sinograms with the FP32 run's indices.  geometry=1 keeps the
+    reference's float sample positions (values in double)."""
     tc = np.ascontiguousarray(tcodes, np.int32)
     out = np.empty(tc.size * n * n_angles, np.float64)
     keep = {k: np.ascontiguousarray(aux[k], np.int32).ravel()
             for k in ("med_x", "med_s", "t6_src", "t7_src")}
     rc = lib().tto_sinograms_f64(_f32(img).ravel(), n, n_angles, step_deg, tc,
                                  tc.size, _ptr(keep["med_x"]), _ptr(keep["med_s"]),
-                                 _ptr(keep["t6_src"]), _ptr(keep["t7_src"]), out,
-                                 nthreads)
+                                 _ptr(keep["t6_src"]), _ptr(keep["t7_src"]),
+                                 int(geometry), out, nthreads)
     if rc:
         raise ValueError("tto_sinograms_f64 rejected its arguments")
     return out.reshape(tc.size, n_angles, n)
@@ -277,13 +280,13 @@ def circus(sino_one, n, n_cols, pfuncs, nthreads=1):
              "p2_src": p2s[:np_ * n_cols].reshape(np_, n_cols)})
 
 
-def circus_f64(sino_one, n, n_cols, pfuncs, p2_src, nthreads=1):
+def circus_f64(sino_one, n, n_cols, pfuncs, p2_src, nthreads=1, geometry=0):
     codes, orders, centers = _pargs(pfuncs)
     out = np.empty(max(codes.size, 1) * n_cols, np.float64)
     src = np.ascontiguousarray(p2_src, np.int32).ravel()
     rc = lib().tto_circus_f64(np.ascontiguousarray(sino_one, np.float64).ravel(), n,
                               n_cols, codes, orders, centers, codes.size, _ptr(src),
-                              out, nthreads)
+                              int(geometry), out, nthreads)
     if rc:
         raise ValueError("tto_circus_f64 rejected its arguments")
     return out[:codes.size * n_cols].reshape(codes.size, n_cols)

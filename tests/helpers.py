@@ -21,6 +21,16 @@ def normwise_err(got, ref):
     return 0.0 if scale == 0 and d == 0 else d / max(scale, 1e-300)
 
 
+def rel_err(got, ref, floor=1e-3):
+    """Largest elementwise relative error over values above floor * max|ref|."""
+    ref = np.asarray(ref, np.float64)
+    big = np.abs(ref) > floor * np.abs(ref).max()
+    if not big.any():
+        return 0.0
+    got = np.asarray(got, np.float64)
+    return float((np.abs(got[big] - ref[big]) / np.abs(ref[big])).max())
+
+
 def ulp_diff(got, want):
     """Largest distance in float32 ulps (same-sign finite values)."""
     g = np.asarray(got, np.float32).view(np.int32).astype(np.int64)

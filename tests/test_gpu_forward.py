@@ -1,20 +1,32 @@
 """GPU parity of the forward path (tt_sinograms, tt_circus) against the oracle.
 
-Bar: weighted-median and sort indices bit-exact against the FP32-faithful
-oracle; T0/T1/T2 values bit-exact too (same operation order); T3-T5 within
-2 float ulps of the FP32 oracle (double hypot may round differently); all
-sinograms within a normwise relative error of 1e-5 of the FP64 oracle.
+Bar:
+* weighted-median indices (and P2's rank) bit-exact against the FP32-faithful
+  oracle, and T0 (the Eigen-order float sum that also sets the median
+  threshold) bit-exact too;
+* values against the FP64 oracle in reference geometry (the reference's float
+  sample positions, values in double): normwise relative error <= 1e-5 for
+  every functional, elementwise <= 1e-5 for T1/T2/P1/P3 (non-negative sums,
+  on values above 1e-3 of the maximum) and <= 2e-5 for T0 (a float sum of up
+  to 1449 terms in Eigen's 8-lane order);
+* against the all-double FP64 oracle (sample positions in double too), for
+  images with a zero margin: normwise <= 1e-4 and no worse than the FP32
+  reference's own error (its float sample coordinates move T1/T2 by ~1e-5),
+  P3 <= 3e-5 (the reference stores its trapz abscissae as float).
+The FP32 reference itself misses some of these bars (its sequential float
+integrals); the GPU integrals accumulate in double and do not.
 """
 import numpy as np
 import pytest
 import torch
 
-from helpers import assert_same_indices, normwise_err, padded, ulp_diff
+from helpers import assert_same_indices, normwise_err, padded, rel_err, ulp_diff
 from tracetransform_amd import core, inputs
 
 pytestmark = pytest.mark.gpu
 
 T05 = [0, 1, 2, 3, 4, 5]
+T07 = [0, 1, 2, 3, 4, 5, 6, 7]
 NTHREADS = 16
 
 
@@ -25,7 +37,7 @@ def gpu_sinograms(img_cm, n, A, step, codes):
     return sino.cpu().numpy(), {k: v.cpu().numpy() for k, v in aux.items()}
 
 
-def check_forward(oracle, img, n, A, step, codes, f64=True):
+def check_forward(oracle, img, n, A, step, codes, f64=True, margin=True):
     print(f"[forward] n={n} A={A} step={step} codes={codes}", flush=True)
     g, gaux = gpu_sinograms(img, n, A, step, codes)
     o, oaux = oracle.sinograms(img, n, A, step, codes, nthreads=NTHREADS)
@@ -33,17 +45,38 @@ def check_forward(oracle, img, n, A, step, codes, f64=True):
         assert_same_indices("median_x", gaux["median_x"], oaux["med_x"])
     if any(c in (3, 4, 5, 6) for c in codes):
         assert_same_indices("median_sqrt", gaux["median_sqrt"], oaux["med_s"])
+    if 6 in codes:
+        assert_same_indices("t6_rank", gaux["t6_rank"], oaux["t6_rank"])
+        assert_same_indices("t6_src", gaux["t6_src"], oaux["t6_src"])
+    if 7 in codes:
+        assert_same_indices("t7_rank", gaux["t7_rank"], oaux["t7_rank"])
     for t, c in enumerate(codes):
-        if c in (0, 1, 2):
+        if c in (0, 6, 7):  # float sum in Eigen order / selected elements
             assert np.array_equal(g[t], o[t]), (c, ulp_diff(g[t], o[t]))
-        else:
-            assert ulp_diff(g[t], o[t]) <= 2, (c, ulp_diff(g[t], o[t]))
-    if f64:
-        d = oracle.sinograms_f64(img, n, A, step, codes, oaux, nthreads=NTHREADS)
-        for t, c in enumerate(codes):
-            err = normwise_err(g[t], d[t])
-            print(f"  T{c}: normwise err vs FP64 = {err:.2e}", flush=True)
-            assert err <= 1e-5, (c, err)
+    if not f64:
+        return g, o
+    d1 = oracle.sinograms_f64(img, n, A, step, codes, oaux, nthreads=NTHREADS,
+                              geometry=1)
+    d0 = oracle.sinograms_f64(img, n, A, step, codes, oaux, nthreads=NTHREADS,
+                              geometry=0) if margin else None
+    for t, c in enumerate(codes):
+        err = normwise_err(g[t], d1[t])
+        ref_err = normwise_err(o[t], d1[t])
+        msg = f"  T{c}: normwise vs FP64 = {err:.2e} (FP32 reference {ref_err:.2e})"
+        assert err <= 1e-5, (c, err)
+        if c in (0, 1, 2, 6, 7):
+            rel = rel_err(g[t], d1[t])
+            msg += f", elementwise {rel:.2e}"
+            assert rel <= (2e-5 if c == 0 else 1e-5), (c, rel)
+        if d0 is not None:
+            # all-double positions: the reference's float sample coordinates
+            # alone move T1/T2 by up to ~1e-5 (the FP32 reference shows the
+            # same error); the GPU must be no worse than the reference
+            e0 = normwise_err(g[t], d0[t])
+            r0 = normwise_err(o[t], d0[t])
+            msg += f", vs all-double FP64 {e0:.2e} (reference {r0:.2e})"
+            assert e0 <= max(1e-5, 1.05 * r0) and e0 <= 1e-4, (c, e0, r0)
+        print(msg, flush=True)
     return g, o
 
 
@@ -51,6 +84,32 @@ def test_uniform_64(oracle, gpu):
     side = 64
     img, n = padded(oracle, inputs.uniform_image(side, 11), side, side)
     check_forward(oracle, img, n, 360, 1.0, T05)
+
+
+def test_all_T_uniform_64(oracle, gpu):
+    side = 64
+    img, n = padded(oracle, inputs.uniform_image(side, 12), side, side)
+    check_forward(oracle, img, n, 360, 1.0, T07)
+
+
+def test_all_T_blob_1024_few_angles(oracle, gpu):
+    # the large config's image (1024x1024 blobs, padded to 1449), 8 angles
+    side = 1024
+    img, n = padded(oracle, inputs.blob_image(side, 2), side, side)
+    check_forward(oracle, img, n, 8, 45.0, T07)
+
+
+@pytest.mark.parametrize("rows,cols", [(1, 1), (3, 8), (30, 30)])
+def test_T6_T7_small_and_ties(oracle, gpu, rows, cols):
+    rng = np.random.default_rng(rows + 7 * cols)
+    # quantised values: many equal keys exercise the stable tie order
+    img_cm = (rng.integers(0, 4, rows * cols) / 4.0).astype(np.float32)
+    n, _, _ = oracle.pad_geometry(rows, cols)
+    img = oracle.pad(img_cm, rows, cols)
+    check_forward(oracle, img, n, 24, 15.0, [6, 7, 0], margin=False)
+    zero = np.zeros(n * n, np.float32)
+    g, _ = check_forward(oracle, zero, n, 4, 90.0, [7, 6], f64=False)
+    assert not g.any()
 
 
 def test_config1_uniform_256(oracle, gpu):
@@ -74,7 +133,9 @@ def test_small_and_nonsquare(oracle, gpu, rows, cols):
     img_cm = rng.random(rows * cols, dtype=np.float32)
     n, _, _ = oracle.pad_geometry(rows, cols)
     img = oracle.pad(img_cm, rows, cols)
-    check_forward(oracle, img, n, 72, 5.0, T05)
+    # thin images touch the padded border (the reference's pad measures to
+    # the second-to-last row), so only reference geometry is comparable
+    check_forward(oracle, img, n, 72, 5.0, T05, margin=False)
 
 
 def test_negative_and_zero_images(oracle, gpu):
@@ -98,7 +159,8 @@ def test_fractional_step(oracle, gpu):
 def test_subsets_and_duplicates(oracle, gpu):
     side = 32
     img, n = padded(oracle, inputs.uniform_image(side, 5), side, side)
-    for codes in ([0], [1], [4], [2, 5], [3, 0, 3], [5, 4, 3, 2, 1, 0]):
+    for codes in ([0], [1], [4], [2, 5], [3, 0, 3], [5, 4, 3, 2, 1, 0], [6], [7],
+                  [7, 6, 7]):
         check_forward(oracle, img, n, 36, 10.0, codes, f64=False)
 
 
@@ -111,27 +173,26 @@ def gpu_circus(sino_cols, pf):
     return out.cpu().numpy(), rank.cpu().numpy()
 
 
-def check_circus(oracle, sino_cols, pf, tol_f64):
+def check_circus(oracle, sino_cols, pf):
     A, n = sino_cols.shape
     print(f"[circus] n={n} A={A} pf={pf}", flush=True)
     g, grank = gpu_circus(sino_cols, pf)
     o, oaux = oracle.circus(sino_cols, n, A, pf, nthreads=NTHREADS)
-    d = oracle.circus_f64(sino_cols.astype(np.float64), n, A, pf, oaux["p2_src"],
-                          nthreads=NTHREADS)
+    sd = sino_cols.astype(np.float64)
+    d1 = oracle.circus_f64(sd, n, A, pf, oaux["p2_src"], nthreads=NTHREADS, geometry=1)
+    d0 = oracle.circus_f64(sd, n, A, pf, oaux["p2_src"], nthreads=NTHREADS, geometry=0)
     for q, p in enumerate(pf):
         if p[0] == 2:
             assert_same_indices("p2_rank", grank[q], oaux["p2_rank"][q])
             assert np.array_equal(g[q], o[q])
-        err = normwise_err(g[q], d[q])
-        err32 = normwise_err(g[q], o[q])
-        print(f"  P{p[0]}: normwise err vs FP64 = {err:.2e}, vs FP32 = {err32:.2e}",
-              flush=True)
-        assert err <= tol_f64[p[0]], (p, err)
-
-
-# P3 stores its trapz abscissae as float in the reference, which alone costs
-# up to ~1e-5 relative against exact arithmetic; its bar is 3e-5.
-TOL_P = {0: 1e-5, 1: 1e-5, 2: 1e-5, 3: 3e-5}
+        e1, e0, e32 = (normwise_err(g[q], d1[q]), normwise_err(g[q], d0[q]),
+                       normwise_err(o[q], d1[q]))
+        print(f"  P{p[0]}: normwise vs FP64 {e1:.2e}, vs all-double FP64 {e0:.2e} "
+              f"(FP32 reference {e32:.2e})", flush=True)
+        assert e1 <= 1e-5, (p, e1)
+        assert e0 <= (3e-5 if p[0] == 3 else 1e-5), (p, e0)
+        if p[0] in (1, 3):
+            assert rel_err(g[q], d1[q]) <= 1e-5, p
 
 
 def test_circus_on_oracle_sinograms(oracle, gpu):
@@ -140,21 +201,21 @@ def test_circus_on_oracle_sinograms(oracle, gpu):
     sino, _ = oracle.sinograms(img, n, 360, 1.0, T05, nthreads=NTHREADS)
     pf = [(1,), (2,), (3,), (0, 1, (n - 1) // 2), (0, 4, 100)]
     for t in range(len(T05)):
-        check_circus(oracle, sino[t], pf, TOL_P)
+        check_circus(oracle, sino[t], pf)
 
 
 def test_circus_sizes(oracle, gpu):
     rng = np.random.default_rng(12)
     for n, A in ((7, 3), (91, 180), (725, 40), (1449, 16)):
         sino = rng.random((A, n), dtype=np.float32)
-        check_circus(oracle, sino, [(1,), (2,), (3,), (0, 3, n // 3)], TOL_P)
+        check_circus(oracle, sino, [(1,), (2,), (3,), (0, 3, n // 3)])
 
 
 def test_circus_batched_sinograms(oracle, gpu):
     rng = np.random.default_rng(13)
     sino = rng.random((3, 50, 91), dtype=np.float32)
     t = torch.from_numpy(sino).cuda()
-    out = core.circus(t, [(2,), (1,)]).cpu().numpy()
+    out = core.circus(t, [(2,), (1,)]).cpu().numpy()  # no explicit synchronize
     for s in range(3):
         o, _ = oracle.circus(sino[s], 91, 50, [(2,), (1,)])
         assert np.array_equal(out[s, 0], o[0])
@@ -176,7 +237,8 @@ def test_reference_surface(oracle, gpu):
     assert np.array_equal(sinos[0].cpu().numpy(), o[0].T)
     pf = [tt.PFunctionalWrapper.parse(s) for s in ("1", "2")]
     circ = tt.getCircusFunctions(sinos[1], pf)
-    oc, _ = oracle.circus(o[1], n, 180, [(1,), (2,)])
+    oc, _ = oracle.circus(np.ascontiguousarray(sinos[1].cpu().numpy().T), n, 180,
+                          [(1,), (2,)])
     assert normwise_err(circ[0].cpu().numpy(), oc[0]) < 1e-6
     assert np.array_equal(circ[1].cpu().numpy(), oc[1])
     # Transformer pads on the device

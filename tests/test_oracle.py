@@ -380,3 +380,28 @@ def test_fp64_oracle_agrees_with_fp32(oracle):
         for p in range(len(pf)):
             scale = np.abs(c64[p]).max()
             assert np.abs(c32[p] - c64[p]).max() <= 1e-4 * scale + 1e-30, (t, p)
+
+
+def test_fp64_reference_geometry(oracle):
+    # Reference geometry keeps the float sample positions: on an image that
+    # touches the padded border the all-double oracle disagrees with the FP32
+    # reference at boundary flips, the reference-geometry one does not.
+    rows, cols = 17, 3
+    n, _, _ = oracle.pad_geometry(rows, cols)
+    img = oracle.pad(np.random.default_rng(3).random(rows * cols, dtype=np.float32),
+                     rows, cols)
+    s32, aux = oracle.sinograms(img, n, 72, 5.0, [0], nthreads=2)
+    g1 = oracle.sinograms_f64(img, n, 72, 5.0, [0], aux, nthreads=2, geometry=1)
+    g0 = oracle.sinograms_f64(img, n, 72, 5.0, [0], aux, nthreads=2, geometry=0)
+    assert np.abs(s32[0] - g1[0]).max() <= 1e-6 * np.abs(g1[0]).max()
+    assert np.abs(s32[0] - g0[0]).max() > 1e-3 * np.abs(g0[0]).max()
+    # on a padded square image with a zero margin both agree to float rounding
+    side = 20
+    n, _, _ = oracle.pad_geometry(side, side)
+    img = oracle.pad(np.random.default_rng(4).random(side * side, dtype=np.float32),
+                     side, side)
+    s32, aux = oracle.sinograms(img, n, 36, 10.0, [0, 1, 3], nthreads=2)
+    g1 = oracle.sinograms_f64(img, n, 36, 10.0, [0, 1, 3], aux, nthreads=2, geometry=1)
+    g0 = oracle.sinograms_f64(img, n, 36, 10.0, [0, 1, 3], aux, nthreads=2, geometry=0)
+    for t in range(3):
+        assert np.abs(g1[t] - g0[t]).max() <= 1e-5 * np.abs(g0[t]).max()

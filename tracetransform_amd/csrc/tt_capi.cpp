@@ -2,9 +2,10 @@
 //
 // Host responsibilities: argument checks (the reference asserts or throws;
 // this ABI returns TT_ERR_INVALID with a message), per-context device
-// workspaces, and the small host-computed tables whose rounding must match
-// the reference's libm calls bit for bit (rotation trig, T3-T5 tables,
-// Hermite weights).  All GPU work is in tt_kernels.hip.
+// workspaces, and the small host-computed tables: rotation trig and T3-T5
+// tables whose rounding must match the reference's libm calls bit for bit,
+// and Hermite weights at the reference's float sample positions.  All GPU
+// work is in tt_kernels.hip.
 #include "../../include/tt.h"
 #include "tt_internal.hpp"
 
@@ -114,40 +115,36 @@ void make_dft_tw(int n, std::vector<float2> &out)
     }
 }
 
-// c++/src/functionals.cpp:62-84: recursion in double, each level rounded to
-// float on return; normalisation in double with z*z/2 in float.
-float herm_poly(unsigned order, float x)
+// Hermite function h_n(z) = H_n(z) / (sqrt(2^n n! sqrt(pi)) e^{z^2/2}) in
+// double, with the physicists' recursion H_k = 2z H_{k-1} - 2(k-1) H_{k-2}
+// (c++/src/functionals.cpp:62-84 evaluates the same recursion, rounding each
+// level to float).
+double herm_function(unsigned order, double z)
 {
+    double h0 = 1.0, h1 = 2.0 * z;
     if (order == 0)
-        return 1.0f;
-    if (order == 1)
-        return (float)(2.0 * x);
-    return (float)(2.0 * x * herm_poly(order - 1, x) -
-                   2.0 * (order - 1) * herm_poly(order - 2, x));
-}
-
-unsigned herm_factorial(unsigned n)
-{
-    return (n == 1 || n == 0) ? 1 : herm_factorial(n - 1) * n;
-}
-
-float herm_function(unsigned order, float z)
-{
-    const float zz = z * z / 2;
-    return (float)(herm_poly(order, z) /
-                   (sqrt(pow(2, order) * herm_factorial(order) * sqrt(M_PI)) *
-                    exp((double)zz)));
+        h1 = 1.0;
+    for (unsigned k = 2; k <= order; k++) {
+        const double h2 = 2.0 * z * h1 - 2.0 * (k - 1) * h0;
+        h0 = h1;
+        h1 = h2;
+    }
+    double fact = 1.0;
+    for (unsigned k = 2; k <= order; k++)
+        fact *= k;
+    return h1 / (sqrt(pow(2.0, order) * fact * sqrt(M_PI)) * exp(z * z / 2));
 }
 
 // c++/src/functionals.cpp:369-386: the z grid over [-10, 10] split at
-// center; the weight of sample p is hermite_function(order, z_p).
-void make_hermite(int n, unsigned order, int center, float *w)
+// center, with the reference's float step sizes and float z values (the
+// sample positions); the weight of sample p is h_order(z_p), kept in double.
+void make_hermite(int n, unsigned order, int center, double *w)
 {
     const float lo = (float)(10.0 / center);
     const float hi = (float)(10.0 / (double)(n - 1 - center));
     for (int p = 0; p < n; p++) {
         const float z = p < center ? p * lo - 10 : (p - center) * hi;
-        w[p] = herm_function(order, z);
+        w[p] = herm_function(order, (double)z);
     }
 }
 
@@ -224,13 +221,13 @@ struct tt_context {
     KernelTimer timer;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
-    DevBuf imgT, trig, tab345, dft_tw, herm_w, io_a, io_b;
+    DevBuf imgT, trig, tab345, dft_tw, herm_w, io_a, io_b, med_x, med_s;
     int trig_A = -1;
     float trig_step = 0.0f;
     int tab_n = -1, dft_n = -1;
     std::vector<int32_t> herm_key;
     std::vector<float2> h_trig, h_tab, h_tw;
-    std::vector<float> h_herm;
+    std::vector<double> h_herm;
 
     tt_status set_device()
     {
@@ -287,10 +284,10 @@ struct tt_context {
         for (size_t h = 0; h < nh; h++)
             make_hermite(n, (unsigned)key[2 * h], key[2 * h + 1],
                          h_herm.data() + h * (size_t)n);
-        TT_HIP(herm_w.ensure(sizeof(float) * (nh ? nh : 1) * (size_t)n));
+        TT_HIP(herm_w.ensure(sizeof(double) * (nh ? nh : 1) * (size_t)n));
         if (nh)
             TT_HIP(hipMemcpyAsync(herm_w.ptr, h_herm.data(),
-                                  sizeof(float) * nh * (size_t)n,
+                                  sizeof(double) * nh * (size_t)n,
                                   hipMemcpyHostToDevice, stream));
         TT_HIP(hipStreamSynchronize(stream));
         herm_key = key;
@@ -375,7 +372,8 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->imgT, &ctx->trig, &ctx->tab345, &ctx->dft_tw,
-                      &ctx->herm_w, &ctx->io_a, &ctx->io_b};
+                      &ctx->herm_w, &ctx->io_a, &ctx->io_b, &ctx->med_x,
+                      &ctx->med_s};
     for (DevBuf *b : bufs)
         b->release();
     ctx->timer.release();
@@ -388,7 +386,7 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
 TT_API tt_status tt_context_set_stream(tt_context *ctx, void *stream)
 {
     TT_CHECK(ctx, "ctx is NULL");
-    hipStream_t next = stream ? (hipStream_t)stream : ctx->own;
+    hipStream_t next = (hipStream_t)stream;  // NULL: HIP's default stream
     if (next != ctx->stream) {
         TT_TRY(ctx->set_device());
         // Tables and workspaces are shared by all work of the context: drain
@@ -457,7 +455,8 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
     TT_CHECK(n_angles >= 1, "n_angles must be >= 1 (got %d)", n_angles);
     TT_CHECK(angle_step_deg > 0.0f && isfinite(angle_step_deg),
              "angle step must be positive and finite");
-    TT_CHECK((long)n * n < (1L << 30), "padded side %d too large", n);
+    // the T3-T5 tables are staged in LDS (24 bytes per row, 64 KiB maximum)
+    TT_CHECK(n <= 2730, "padded side %d above the supported 2730", n);
     TT_TRY(check_tcodes(tcodes, nt));
     tt_sino_aux none = {};
     const tt_sino_aux &ax = aux ? *aux : none;
@@ -472,17 +471,27 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
         else
             dups.push_back({slot[tcodes[t]], t});
     }
-    if (slot[6] >= 0 || slot[7] >= 0 || ax.t6_rank || ax.t6_src || ax.t7_rank)
-        return fail(TT_ERR_UNSUPPORTED, "T6/T7 are not available in this build");
-
     TT_TRY(ctx->set_device());
     hipStream_t st = ctx->stream;
-    const bool need_x = slot[1] >= 0 || slot[2] >= 0 || ax.median_x;
+    const bool do6 = slot[6] >= 0, do7 = slot[7] >= 0;
+    const bool need_x = slot[1] >= 0 || slot[2] >= 0 || ax.median_x || do7;
     const bool do345 = slot[3] >= 0 || slot[4] >= 0 || slot[5] >= 0;
-    const bool need_s = do345 || ax.median_sqrt;
+    const bool need_s = do345 || ax.median_sqrt || do6;
     const bool any = need_x || need_s || slot[0] >= 0;
     if (!any)
         return TT_OK;
+    // T6/T7 take their medians from the T0..T5 line kernel; route them
+    // through the workspace when the caller does not ask for them.
+    int32_t *med_x = ax.median_x, *med_s = ax.median_sqrt;
+    const size_t med_b = sizeof(int32_t) * (size_t)n * n_angles;
+    if (do7 && !med_x) {
+        TT_HIP(ctx->med_x.ensure(med_b));
+        med_x = ctx->med_x.as<int32_t>();
+    }
+    if (do6 && !med_s) {
+        TT_HIP(ctx->med_s.ensure(med_b));
+        med_s = ctx->med_s.as<int32_t>();
+    }
 
     TT_TRY(ctx->ensure_trig(n_angles, angle_step_deg));
     if (do345)
@@ -498,8 +507,8 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
     p.trig = ctx->trig.as<float2>();
     p.tab345 = do345 ? ctx->tab345.as<float2>() : nullptr;
     p.sino = d_sino;
-    p.med_x = ax.median_x;
-    p.med_s = ax.median_sqrt;
+    p.med_x = med_x;
+    p.med_s = med_s;
     p.n = n;
     p.A = n_angles;
     p.nb = (n + ttk::kWave - 1) / ttk::kWave;
@@ -509,6 +518,25 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
     TT_HIP(ctx->timer.begin(TT_KERNEL_SINO_T05, st));
     TT_HIP(ttk::launch_sino_t05(p, need_x, need_s, do345, st));
     TT_HIP(ctx->timer.end(st));
+    if (do6 || do7) {
+        ttk::SortTParams q = {};
+        q.img = d_image;
+        q.imgT = ctx->imgT.as<float>();
+        q.trig = ctx->trig.as<float2>();
+        q.sino = d_sino;
+        q.med_x = med_x;
+        q.med_s = med_s;
+        q.t6_rank = ax.t6_rank;
+        q.t6_src = ax.t6_src;
+        q.t7_rank = ax.t7_rank;
+        q.n = n;
+        q.A = n_angles;
+        q.slot6 = slot[6];
+        q.slot7 = slot[7];
+        TT_HIP(ctx->timer.begin(TT_KERNEL_SINO_SORT, st));
+        TT_HIP(ttk::launch_sino_sort(q, st));
+        TT_HIP(ctx->timer.end(st));
+    }
     const size_t plane = sizeof(float) * (size_t)n * n_angles;
     for (auto &d : dups)
         TT_HIP(hipMemcpyAsync((char *)d_sino + plane * d.second,
@@ -546,7 +574,8 @@ TT_API tt_status tt_circus(tt_context *ctx, const float *d_sino, int n,
     TT_CHECK(ctx, "ctx is NULL");
     TT_CHECK(n >= 2, "sinogram height must be >= 2 (got %d)", n);
     TT_CHECK(n_cols >= 1 && n_sino >= 1, "n_cols and n_sino must be >= 1");
-    TT_CHECK(n <= 8192, "sinogram height %d above the supported 8192", n);
+    // column, sort buffer and spectrum staged in LDS (64 KiB maximum)
+    TT_CHECK(n <= 4096, "sinogram height %d above the supported 4096", n);
     TT_TRY(check_pfuncs(pf, np, n));
     if (np == 0)
         return TT_OK;
@@ -579,7 +608,7 @@ TT_API tt_status tt_circus(tt_context *ctx, const float *d_sino, int n,
     }
     if (!hkey.empty()) {
         TT_TRY(ctx->ensure_hermite(n, hkey));
-        p.herm_w = ctx->herm_w.as<float>();
+        p.herm_w = ctx->herm_w.as<double>();
     }
     TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS, ctx->stream));
     TT_HIP(ttk::launch_circus(p, ctx->stream));

@@ -32,7 +32,8 @@ struct SinoParams {
 // Sort-based T-functionals T6/T7 for one padded image: one workgroup owns one
 // (angle, column) line at a time.
 struct SortTParams {
-    const float *img;
+    const float *img;     // n x n column-major
+    const float *imgT;    // row-major copy
     const float2 *trig;
     float *sino;
     int32_t *med_x;
@@ -52,7 +53,7 @@ struct CircusParams {
     float *out;           // [nsino][np][ncols]
     int32_t *p2rank;      // [nsino][np][ncols] or null
     const float2 *dft_tw; // [n] (cos, sin)(2 pi k / n)
-    const float *herm_w;  // [nherm][n] Hermite weights
+    const double *herm_w; // [nherm][n] Hermite weights
     int n;
     int ncols;
     int nsino;

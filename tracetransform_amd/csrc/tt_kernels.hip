@@ -5,7 +5,8 @@
 // (no FMA), so median and sort indices are bit-identical to the reference's
 // arithmetic.  Denormals are kept (no flush) and sqrtf is correctly rounded,
 // as on the reference's x86-64 SSE2 build.  Sums that only feed values (the
-// T3-T5 and P integrals) use explicit fmaf where it helps.
+// T1-T5 and P integrals) use explicit fma and, where it buys accuracy against
+// the FP64 oracle, double accumulators.
 #include "tt_internal.hpp"
 
 #include <math.h>
@@ -90,7 +91,7 @@ __device__ __forceinline__ float rot_sample(const float *__restrict__ src,
 // the sequential float prefix to find findWeightedMedian's first index with
 // 2*prefix >= S (c++/src/functionals.cpp:21-30), and from there accumulates
 // T1/T2 (c++/src/functionals.cpp:104-129) and T3/T4/T5
-// (c++/src/functionals.cpp:200-213) in the reference's sequential order.
+// (c++/src/functionals.cpp:200-213) over the same rows as the reference.
 // ---------------------------------------------------------------------------
 template <bool TR, bool NEED_X, bool NEED_S, bool DO345>
 __device__ __forceinline__ void sino_line(const SinoParams &p, const float *src,
@@ -175,14 +176,19 @@ __device__ __forceinline__ void sino_line(const SinoParams &p, const float *src,
         return;
 
     // ---- pass 2: median walk + integrals -----------------------------
+    // The walk is the reference's float prefix; the integrals only feed
+    // values, so they accumulate in double (full rate on CDNA4) and land
+    // within ~1e-7 of the FP64 oracle instead of inheriting the reference's
+    // sequential float error.
     float Px = 0.0f, Ps = 0.0f;
-    float t1 = 0.0f, t2 = 0.0f;
-    float r3 = 0.0f, i3 = 0.0f, r4 = 0.0f, i4 = 0.0f, r5 = 0.0f, i5 = 0.0f;
+    double t1 = 0.0, t2 = 0.0;
+    double r3 = 0.0, i3 = 0.0, r4 = 0.0, i4 = 0.0, r5 = 0.0, i5 = 0.0;
     int m = -1, sm = -1;
     const bool do3 = p.slot[3] >= 0, do4 = p.slot[4] >= 0, do5 = p.slot[5] >= 0;
     for (int r = 0; r < n; r++) {
         const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s,
                                        (float)r - o);
+        const double vd = (double)v;
         if (NEED_X) {
             if (m < 0) {
                 Px += v;
@@ -190,9 +196,10 @@ __device__ __forceinline__ void sino_line(const SinoParams &p, const float *src,
                     m = r;
             }
             if (m >= 0) {
-                const float rr = (float)(r - m);
-                t1 += v * rr;
-                t2 += v * rr * rr;
+                const double rr = (double)(r - m);
+                const double vr = vd * rr;
+                t1 += vr;
+                t2 = fma(vr, rr, t2);
             }
         }
         if (NEED_S) {
@@ -204,18 +211,18 @@ __device__ __forceinline__ void sino_line(const SinoParams &p, const float *src,
                 const int r1 = r - sm;
                 if (do3) {
                     const float2 w = s_tab[r1];
-                    r3 += w.x * v;
-                    i3 += w.y * v;
+                    r3 = fma((double)w.x, vd, r3);
+                    i3 = fma((double)w.y, vd, i3);
                 }
                 if (do4) {
                     const float2 w = s_tab[n + r1];
-                    r4 += w.x * v;
-                    i4 += w.y * v;
+                    r4 = fma((double)w.x, vd, r4);
+                    i4 = fma((double)w.y, vd, i4);
                 }
                 if (do5) {
                     const float2 w = s_tab[2 * n + r1];
-                    r5 += w.x * v;
-                    i5 += w.y * v;
+                    r5 = fma((double)w.x, vd, r5);
+                    i5 = fma((double)w.y, vd, i5);
                 }
             }
         }
@@ -231,16 +238,16 @@ __device__ __forceinline__ void sino_line(const SinoParams &p, const float *src,
     if (NEED_S && p.med_s)
         p.med_s[line] = sm;
     if (p.slot[1] >= 0)
-        p.sino[p.slot[1] * tstride + line] = t1;
+        p.sino[p.slot[1] * tstride + line] = (float)t1;
     if (p.slot[2] >= 0)
-        p.sino[p.slot[2] * tstride + line] = t2;
+        p.sino[p.slot[2] * tstride + line] = (float)t2;
     if (DO345) {
         if (do3)
-            p.sino[p.slot[3] * tstride + line] = (float)hypot((double)r3, (double)i3);
+            p.sino[p.slot[3] * tstride + line] = (float)hypot(r3, i3);
         if (do4)
-            p.sino[p.slot[4] * tstride + line] = (float)hypot((double)r4, (double)i4);
+            p.sino[p.slot[4] * tstride + line] = (float)hypot(r4, i4);
         if (do5)
-            p.sino[p.slot[5] * tstride + line] = (float)hypot((double)r5, (double)i5);
+            p.sino[p.slot[5] * tstride + line] = (float)hypot(r5, i5);
     }
 }
 
@@ -338,6 +345,23 @@ __device__ __forceinline__ float block_sum(float v, float *s_red)
     return r;
 }
 
+__device__ __forceinline__ double block_sum_d(double v, double *s_red)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, kWave);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+        s_red[w] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < kBlock / kWave; i++)
+        r += s_red[i];
+    return r;
+}
+
 // Eigen packet-order sum of x[0..n) held in LDS, evaluated by one thread.
 __device__ float eigen_sum_lds(const float *x, int n)
 {
@@ -423,6 +447,7 @@ __global__ __launch_bounds__(kBlock) void circus_kernel(CircusParams p)
     float *s_mod = s_sort + m2;    // n
     float *s_red = s_mod + n;      // 8
     __shared__ int s_rank;
+    __shared__ double s_red_d[kBlock / kWave];
 
     const int colg = blockIdx.x;   // column over all sinograms
     const int sg = colg / p.ncols;
@@ -441,11 +466,11 @@ __global__ __launch_bounds__(kBlock) void circus_kernel(CircusParams p)
                 acc += fabsf(s_x[i - 1] - s_x[i]);
             res = block_sum(acc, s_red);
         } else if (code == 0) {
-            const float *w = p.herm_w + (size_t)p.herm[q] * n;
-            float acc = 0.0f;
+            const double *w = p.herm_w + (size_t)p.herm[q] * n;
+            double acc = 0.0;
             for (int i = threadIdx.x; i < n; i += kBlock)
-                acc = fmaf(s_x[i], w[i], acc);
-            res = block_sum(acc, s_red);
+                acc = fma((double)s_x[i], w[i], acc);
+            res = (float)block_sum_d(acc, s_red_d);
         } else if (code == 2) {
             for (int i = threadIdx.x; i < m2; i += kBlock)
                 s_sort[i] = i < n ? s_x[i] : INFINITY;
@@ -494,6 +519,197 @@ __global__ __launch_bounds__(kBlock) void circus_kernel(CircusParams p)
         if (threadIdx.x == 0)
             p.out[((size_t)sg * p.np + q) * p.ncols + col] = res;
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sort-based T-functionals T6 (c++/src/functionals.cpp:226-255) and T7
+// (:262-276).  One workgroup per line, persistent over lines.  The line's
+// column is sampled into LDS with lanes along rows; the medians m and sm come
+// from the T0..T5 kernel, which ran first on the same stream.  Sorting is an
+// LDS bitonic sort over 64-bit keys (orderable float bits << 32 | index), a
+// total order equal to the stable order glibc's merge-sort qsort_r gives
+// compareFloat; T7 sorts values only (equal keys are equal values).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t ord_key(float f)
+{
+    if (f == 0.0f)
+        f = 0.0f;  // -0 and +0 compare equal in compareFloat
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ float ord_val(uint32_t k)
+{
+    const uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __uint_as_float(b);
+}
+
+template <class K>
+__device__ void bitonic_lds(K *s, int m)
+{
+    for (int k = 2; k <= m; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < m; i += kBlock) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const bool up = (i & k) == 0;
+                    const K a = s[i], b = s[ixj];
+                    if ((a > b) == up) {
+                        s[i] = b;
+                        s[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Eigen packet-order sum of x[0..n) in LDS: threads 0..7 run the eight
+// stride-8 chains, thread 0 combines (c++/src/functionals.cpp:22 via Eigen
+// Redux).  Result broadcast through *s_out.  Must be called by the block.
+__device__ void eigen_sum_block(const float *x, int n, float *s_chain, float *s_out)
+{
+    const int a4 = n & ~3, a8 = n & ~7;
+    if (threadIdx.x < 8 && n >= 8) {
+        float r = 0.0f;
+        for (int i = threadIdx.x; i < a8; i += 8)
+            r += x[i];
+        s_chain[threadIdx.x] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float res;
+        if (a4 == 0) {
+            res = n > 0 ? x[0] : 0.0f;
+            for (int i = 1; i < n; i++)
+                res += x[i];
+        } else {
+            float ax[4];
+            for (int j = 0; j < 4; j++)
+                ax[j] = n >= 8 ? s_chain[j] + s_chain[j + 4] : x[j];
+            if (n >= 8 && a4 > a8)
+                for (int j = 0; j < 4; j++)
+                    ax[j] += x[a8 + j];
+            res = (ax[0] + ax[2]) + (ax[1] + ax[3]);
+            for (int i = a4; i < n; i++)
+                res += x[i];
+        }
+        *s_out = res;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n = p.n;
+    int m2 = 1;
+    while (m2 < n)
+        m2 <<= 1;
+    uint64_t *s_key = reinterpret_cast<uint64_t *>(smem_raw);      // m2
+    float *s_col = reinterpret_cast<float *>(s_key + m2);           // n
+    float *s_sq = s_col + n;                                        // n
+    __shared__ float s_chain[8];
+    __shared__ float s_sum;
+    __shared__ int s_k;
+
+    const float o = (float)((n - 1) / 2.0);
+    const float lim = (float)(n - 1);
+    const size_t tstride = (size_t)p.A * n;
+    const int lines = p.A * n;
+    for (int line = blockIdx.x; line < lines; line += gridDim.x) {
+        const int a = line / n;
+        const int col = line - a * n;
+        const float2 cs = p.trig[a];
+        const float px = (float)col - o;
+        const float pxc = px * cs.x, pxns = px * (-cs.y);
+        // lanes step (dx, dy) = (s, c) per row
+        if (fabsf(cs.y) > fabsf(cs.x)) {
+            for (int r = threadIdx.x; r < n; r += kBlock)
+                s_col[r] = rot_sample<true>(p.imgT, n, lim, o, pxc, pxns, cs.x,
+                                            cs.y, (float)r - o);
+        } else {
+            for (int r = threadIdx.x; r < n; r += kBlock)
+                s_col[r] = rot_sample<false>(p.img, n, lim, o, pxc, pxns, cs.x,
+                                             cs.y, (float)r - o);
+        }
+        __syncthreads();
+
+        if (p.slot6 >= 0) {
+            const int sm = p.med_s[line];
+            const int L = n - sm;
+            int mL = 1;
+            while (mL < L)
+                mL <<= 1;
+            for (int r = threadIdx.x; r < mL; r += kBlock) {
+                uint64_t k = ~0ull;
+                if (r < L) {
+                    const float w = (float)r * s_col[sm + r];
+                    k = ((uint64_t)ord_key(w) << 32) | (uint32_t)r;
+                }
+                s_key[r] = k;
+            }
+            __syncthreads();
+            bitonic_lds(s_key, mL);
+            for (int r = threadIdx.x; r < L; r += kBlock)
+                s_sq[r] = sqrtf(s_col[sm + (int)(uint32_t)s_key[r]]);
+            __syncthreads();
+            eigen_sum_block(s_sq, L, s_chain, &s_sum);
+            if (threadIdx.x == 0) {
+                const float S = s_sum;
+                float integral = 0.0f;
+                int k = L - 1;
+                for (int i = 0; i < L; i++) {
+                    integral += s_sq[i];
+                    if (2.0f * integral >= S) {
+                        k = i;
+                        break;
+                    }
+                }
+                const int src = (int)(uint32_t)s_key[k];
+                p.sino[p.slot6 * tstride + line] = (float)src * s_col[sm + src];
+                if (p.t6_rank)
+                    p.t6_rank[line] = k;
+                if (p.t6_src)
+                    p.t6_src[line] = src;
+            }
+            __syncthreads();
+        }
+
+        if (p.slot7 >= 0) {
+            const int m = p.med_x[line];
+            const int L = n - m;
+            int mL = 1;
+            while (mL < L)
+                mL <<= 1;
+            uint32_t *s_k32 = reinterpret_cast<uint32_t *>(s_key);
+            for (int r = threadIdx.x; r < mL; r += kBlock)
+                s_k32[r] = r < L ? ord_key(s_col[m + r]) : 0xFFFFFFFFu;
+            __syncthreads();
+            bitonic_lds(s_k32, mL);
+            for (int r = threadIdx.x; r < L; r += kBlock)
+                s_sq[r] = sqrtf(ord_val(s_k32[r]));
+            __syncthreads();
+            eigen_sum_block(s_sq, L, s_chain, &s_sum);
+            if (threadIdx.x == 0) {
+                const float S = s_sum;
+                float integral = 0.0f;
+                int k = L - 1;
+                for (int i = 0; i < L; i++) {
+                    integral += s_sq[i];
+                    if (2.0f * integral >= S) {
+                        k = i;
+                        break;
+                    }
+                }
+                p.sino[p.slot7 * tstride + line] = ord_val(s_k32[k]);
+                if (p.t7_rank)
+                    p.t7_rank[line] = k;
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -555,9 +771,21 @@ hipError_t launch_circus(const CircusParams &p, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_sino_sort(const SortTParams &, hipStream_t)
+hipError_t launch_sino_sort(const SortTParams &p, hipStream_t s)
 {
-    return hipErrorNotSupported;
+    int m2 = 1;
+    while (m2 < p.n)
+        m2 <<= 1;
+    const size_t shm = sizeof(uint64_t) * (size_t)m2 + sizeof(float) * 2 * (size_t)p.n;
+    const int lines = p.A * p.n;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    int blocks = cus * 8;
+    if (blocks > lines)
+        blocks = lines;
+    sino_sort_kernel<<<blocks, kBlock, shm, s>>>(p);
+    return hipGetLastError();
 }
 
 }  // namespace ttk
