@@ -513,8 +513,8 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
     p.A = n_angles;
     p.nb = (n + ttk::kWave - 1) / ttk::kWave;
     p.items = p.nb * n_angles;
-    for (int k = 0; k < 8; k++)
-        p.slot[k] = k < 6 ? slot[k] : -1;
+    for (int k = 0; k < 6; k++)
+        p.slot.s[k] = slot[k];
     TT_HIP(ctx->timer.begin(TT_KERNEL_SINO_T05, st));
     TT_HIP(ttk::launch_sino_t05(p, need_x, need_s, do345, st));
     TT_HIP(ctx->timer.end(st));
@@ -574,8 +574,8 @@ TT_API tt_status tt_circus(tt_context *ctx, const float *d_sino, int n,
     TT_CHECK(ctx, "ctx is NULL");
     TT_CHECK(n >= 2, "sinogram height must be >= 2 (got %d)", n);
     TT_CHECK(n_cols >= 1 && n_sino >= 1, "n_cols and n_sino must be >= 1");
-    // column, sort buffer and spectrum staged in LDS (64 KiB maximum)
-    TT_CHECK(n <= 4096, "sinogram height %d above the supported 4096", n);
+    // column, sort buffer, spectrum and twiddles staged in LDS (64 KiB max)
+    TT_CHECK(n <= 2730, "sinogram height %d above the supported 2730", n);
     TT_TRY(check_pfuncs(pf, np, n));
     if (np == 0)
         return TT_OK;

@@ -12,6 +12,11 @@ constexpr int kBlock = 256;      // 4 waves per workgroup
 constexpr int kMaxP = 16;        // P-functionals per tt_circus call
 constexpr int kNumXcd = 8;       // MI355X: 8 XCDs, workgroups round-robin
 
+// Output slot of each T code (T0..T5), -1 when not requested.
+struct Slots {
+    int s[6];
+};
+
 // Forward T-functional sweep for one padded image (T0..T5).  One wave owns
 // 64 adjacent columns of one angle; each lane streams its column's rows.
 struct SinoParams {
@@ -26,7 +31,46 @@ struct SinoParams {
     int A;
     int nb;               // column blocks per angle: ceil(n / 64)
     int items;            // A * nb
-    int slot[8];          // output slot of each T code, -1 if absent
+    Slots slot;
+};
+
+// One image of a batch (device array, ordered by first item).
+struct BatchImage {
+    const float *img;     // n x n column-major
+    const float *imgT;    // row-major copy (written by the batched transpose)
+    float *sino;          // [nt][A][n]
+    int32_t *med_x;
+    int32_t *med_s;
+    int n;
+    int nb;               // column blocks per angle
+    int item0;            // first wave item of this image
+    int col0;             // first sinogram column (over all T) of this image
+    float *circus;        // [nt][np][A]
+};
+
+struct BatchSinoParams {
+    const BatchImage *images;
+    int nimg;
+    int items;            // total wave items
+    const float2 *trig;   // [A]
+    const float2 *tab345; // [3][tabn]
+    int tabn;             // largest n of the batch
+    int A;
+    unsigned int *counter;
+    Slots slot;
+};
+
+// P-functionals over every sinogram column of a batch: one workgroup per
+// column; P3 twiddles are computed per image in LDS.
+struct BatchCircusParams {
+    const BatchImage *images;
+    int nimg;
+    int cols;             // total columns: sum over images of nt * A
+    int nt;
+    int A;
+    int nmax;
+    int np;
+    int code[kMaxP];
 };
 
 // Sort-based T-functionals T6/T7 for one padded image: one workgroup owns one
@@ -69,5 +113,15 @@ hipError_t launch_sino_t05(const SinoParams &p, bool need_x, bool need_s,
                            bool do345, hipStream_t s);
 hipError_t launch_sino_sort(const SortTParams &p, hipStream_t s);
 hipError_t launch_circus(const CircusParams &p, hipStream_t s);
+
+// Batched path; `waves_per_block` in {1, 2, 4} and `blocks` (persistent grid)
+// are the autotuned launch shape.
+hipError_t launch_transpose_batched(const BatchImage *d_images, int nimg,
+                                    int nmax, hipStream_t s);
+hipError_t launch_reset_counter(unsigned int *counter, hipStream_t s);
+hipError_t launch_sino_t05_batched(const BatchSinoParams &p, bool need_x,
+                                   bool need_s, bool do345, int waves_per_block,
+                                   int blocks, hipStream_t s);
+hipError_t launch_circus_batched(const BatchCircusParams &p, hipStream_t s);
 
 }  // namespace ttk

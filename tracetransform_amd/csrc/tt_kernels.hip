@@ -93,8 +93,20 @@ __device__ __forceinline__ float rot_sample(const float *__restrict__ src,
 // T1/T2 (c++/src/functionals.cpp:104-129) and T3/T4/T5
 // (c++/src/functionals.cpp:200-213) over the same rows as the reference.
 // ---------------------------------------------------------------------------
+// Where one line's results go.  slot[t] is the output slot of T-code t
+// (-1 if not requested); tabn is the row stride of the T3..T5 tables in LDS.
+struct LineDst {
+    float *sino;
+    int32_t *med_x;
+    int32_t *med_s;
+    int n;
+    int A;
+    int tabn;
+    Slots slot;
+};
+
 template <bool TR, bool NEED_X, bool NEED_S, bool DO345>
-__device__ __forceinline__ void sino_line(const SinoParams &p, const float *src,
+__device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
                                           const float2 *s_tab, int a, int col,
                                           bool active, float c, float s)
 {
@@ -170,8 +182,8 @@ __device__ __forceinline__ void sino_line(const SinoParams &p, const float *src,
 
     const size_t line = (size_t)a * n + col;
     const size_t tstride = (size_t)p.A * n;
-    if (active && p.slot[0] >= 0)
-        p.sino[p.slot[0] * tstride + line] = Sx;
+    if (active && p.slot.s[0] >= 0)
+        p.sino[p.slot.s[0] * tstride + line] = Sx;
     if (!NEED_X && !NEED_S)
         return;
 
@@ -184,7 +196,7 @@ __device__ __forceinline__ void sino_line(const SinoParams &p, const float *src,
     double t1 = 0.0, t2 = 0.0;
     double r3 = 0.0, i3 = 0.0, r4 = 0.0, i4 = 0.0, r5 = 0.0, i5 = 0.0;
     int m = -1, sm = -1;
-    const bool do3 = p.slot[3] >= 0, do4 = p.slot[4] >= 0, do5 = p.slot[5] >= 0;
+    const bool do3 = p.slot.s[3] >= 0, do4 = p.slot.s[4] >= 0, do5 = p.slot.s[5] >= 0;
     for (int r = 0; r < n; r++) {
         const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s,
                                        (float)r - o);
@@ -215,12 +227,12 @@ __device__ __forceinline__ void sino_line(const SinoParams &p, const float *src,
                     i3 = fma((double)w.y, vd, i3);
                 }
                 if (do4) {
-                    const float2 w = s_tab[n + r1];
+                    const float2 w = s_tab[p.tabn + r1];
                     r4 = fma((double)w.x, vd, r4);
                     i4 = fma((double)w.y, vd, i4);
                 }
                 if (do5) {
-                    const float2 w = s_tab[2 * n + r1];
+                    const float2 w = s_tab[2 * p.tabn + r1];
                     r5 = fma((double)w.x, vd, r5);
                     i5 = fma((double)w.y, vd, i5);
                 }
@@ -237,17 +249,17 @@ __device__ __forceinline__ void sino_line(const SinoParams &p, const float *src,
         p.med_x[line] = m;
     if (NEED_S && p.med_s)
         p.med_s[line] = sm;
-    if (p.slot[1] >= 0)
-        p.sino[p.slot[1] * tstride + line] = (float)t1;
-    if (p.slot[2] >= 0)
-        p.sino[p.slot[2] * tstride + line] = (float)t2;
+    if (p.slot.s[1] >= 0)
+        p.sino[p.slot.s[1] * tstride + line] = (float)t1;
+    if (p.slot.s[2] >= 0)
+        p.sino[p.slot.s[2] * tstride + line] = (float)t2;
     if (DO345) {
         if (do3)
-            p.sino[p.slot[3] * tstride + line] = (float)hypot(r3, i3);
+            p.sino[p.slot.s[3] * tstride + line] = (float)hypot(r3, i3);
         if (do4)
-            p.sino[p.slot[4] * tstride + line] = (float)hypot(r4, i4);
+            p.sino[p.slot.s[4] * tstride + line] = (float)hypot(r4, i4);
         if (do5)
-            p.sino[p.slot[5] * tstride + line] = (float)hypot(r5, i5);
+            p.sino[p.slot.s[5] * tstride + line] = (float)hypot(r5, i5);
     }
 }
 
@@ -271,14 +283,100 @@ __global__ __launch_bounds__(kBlock) void sino_t05_kernel(SinoParams p)
     if (!active)
         col = p.n - 1;
     const float2 cs = p.trig[a];
+    const LineDst dst = {p.sino, p.med_x, p.med_s, p.n, p.A, p.n, p.slot};
     // Lanes step (dx, dy) = (c, -s) per column: read the copy whose fast axis
     // is the one the lanes mostly move along.
     if (fabsf(cs.x) > fabsf(cs.y))
-        sino_line<true, NEED_X, NEED_S, DO345>(p, p.imgT, s_tab, a, col, active,
+        sino_line<true, NEED_X, NEED_S, DO345>(dst, p.imgT, s_tab, a, col, active,
                                               cs.x, cs.y);
     else
-        sino_line<false, NEED_X, NEED_S, DO345>(p, p.img, s_tab, a, col, active,
+        sino_line<false, NEED_X, NEED_S, DO345>(dst, p.img, s_tab, a, col, active,
                                                cs.x, cs.y);
+}
+
+// ---------------------------------------------------------------------------
+// Batched T0..T5: one persistent launch over every (image, angle, column
+// block) wave item of a batch of padded images of uneven side.  Waves take
+// items from a device counter (dynamic load balance; items are ordered
+// largest image first) and find their image by binary search over the
+// descriptors' first-item offsets.  The counter is zeroed by
+// reset_counter_kernel ahead of each launch.
+// ---------------------------------------------------------------------------
+template <bool NEED_X, bool NEED_S, bool DO345>
+__global__ __launch_bounds__(kBlock) void sino_t05_batched_kernel(BatchSinoParams p)
+{
+    extern __shared__ float2 s_tab[];
+    if (DO345) {
+        for (int i = threadIdx.x; i < 3 * p.tabn; i += blockDim.x)
+            s_tab[i] = p.tab345[i];
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    for (;;) {
+        unsigned int v = 0;
+        if (lane == 0)
+            v = atomicAdd(p.counter, 1u);
+        const int item = __builtin_amdgcn_readfirstlane(__shfl(v, 0, kWave));
+        if (item >= p.items)
+            break;
+        int lo = 0, hi = p.nimg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (p.images[mid].item0 <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BatchImage im = p.images[lo];
+        const int local = item - im.item0;
+        const int a = local / im.nb;
+        const int cb = local - a * im.nb;
+        int col = cb * kWave + lane;
+        const bool active = col < im.n;
+        if (!active)
+            col = im.n - 1;
+        const float2 cs = p.trig[a];
+        const LineDst dst = {im.sino, im.med_x, im.med_s, im.n, p.A, p.tabn, p.slot};
+        if (fabsf(cs.x) > fabsf(cs.y))
+            sino_line<true, NEED_X, NEED_S, DO345>(dst, im.imgT, s_tab, a, col,
+                                                  active, cs.x, cs.y);
+        else
+            sino_line<false, NEED_X, NEED_S, DO345>(dst, im.img, s_tab, a, col,
+                                                   active, cs.x, cs.y);
+    }
+}
+
+__global__ void reset_counter_kernel(unsigned int *c)
+{
+    if (threadIdx.x == 0)
+        c[0] = 0u;
+}
+
+// Row-major copies of every image of a batch: grid.y walks the images.
+__global__ __launch_bounds__(kBlock) void transpose_batched_kernel(
+    const BatchImage *images)
+{
+    __shared__ float tile[32][33];
+    const BatchImage im = images[blockIdx.y];
+    const int n = im.n;
+    const int tiles = (n + 31) / 32;
+    float *out = const_cast<float *>(im.imgT);
+    for (int t = blockIdx.x; t < tiles * tiles; t += gridDim.x) {
+        const int bx = (t % tiles) * 32, by = (t / tiles) * 32;
+        const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+        for (int k = ty; k < 32; k += 8) {
+            const int col = bx + k, row = by + tx;
+            if (col < n && row < n)
+                tile[k][tx] = im.img[(size_t)col * n + row];
+        }
+        __syncthreads();
+        for (int k = ty; k < 32; k += 8) {
+            const int row = by + k, col = bx + tx;
+            if (col < n && row < n)
+                out[(size_t)row * n + col] = tile[tx][k];
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -362,51 +460,69 @@ __device__ __forceinline__ double block_sum_d(double v, double *s_red)
     return r;
 }
 
-// Eigen packet-order sum of x[0..n) held in LDS, evaluated by one thread.
-__device__ float eigen_sum_lds(const float *x, int n)
-{
-    if (n <= 0)
-        return 0.0f;
-    const int a4 = n & ~3, a8 = n & ~7;
-    if (a4 == 0) {
-        float r = x[0];
-        for (int i = 1; i < n; i++)
-            r += x[i];
-        return r;
-    }
-    float p0[4], p1[4];
-    for (int j = 0; j < 4; j++)
-        p0[j] = x[j];
-    if (a4 > 4) {
-        for (int j = 0; j < 4; j++)
-            p1[j] = x[4 + j];
-        for (int i = 8; i < a8; i += 8)
-            for (int j = 0; j < 4; j++) {
-                p0[j] += x[i + j];
-                p1[j] += x[i + 4 + j];
-            }
-        for (int j = 0; j < 4; j++)
-            p0[j] += p1[j];
-        if (a4 > a8)
-            for (int j = 0; j < 4; j++)
-                p0[j] += x[a8 + j];
-    }
-    float r = (p0[0] + p0[2]) + (p0[1] + p0[3]);
-    for (int i = a4; i < n; i++)
-        r += x[i];
-    return r;
-}
-
-// findWeightedMedian over x[0..n) in LDS (c++/src/functionals.cpp:21-30).
-__device__ int weighted_median_lds(const float *x, int n, float S)
+// findWeightedMedian's walk over x[0..n) in LDS (c++/src/functionals.cpp
+// :21-30): first i with 2 * (sequential float prefix) >= S.  One thread;
+// x must be 16-byte aligned so the rows come in four at a time.
+__device__ int weighted_median_walk(const float *x, int n, float S)
 {
     float integral = 0.0f;
-    for (int i = 0; i < n; i++) {
+    int i = 0;
+    for (; i + 4 <= n; i += 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(x + i);
+        integral += v.x;
+        if (2.0f * integral >= S)
+            return i;
+        integral += v.y;
+        if (2.0f * integral >= S)
+            return i + 1;
+        integral += v.z;
+        if (2.0f * integral >= S)
+            return i + 2;
+        integral += v.w;
+        if (2.0f * integral >= S)
+            return i + 3;
+    }
+    for (; i < n; i++) {
         integral += x[i];
         if (2.0f * integral >= S)
             return i;
     }
     return n - 1;
+}
+
+// Eigen packet-order sum of x[0..n) in LDS: threads 0..7 run the eight
+// stride-8 chains, thread 0 combines (c++/src/functionals.cpp:22 via Eigen
+// Redux).  Result broadcast through *s_out.  Must be called by the block.
+__device__ void eigen_sum_block(const float *x, int n, float *s_chain, float *s_out)
+{
+    const int a4 = n & ~3, a8 = n & ~7;
+    if (threadIdx.x < 8 && n >= 8) {
+        float r = 0.0f;
+        for (int i = threadIdx.x; i < a8; i += 8)
+            r += x[i];
+        s_chain[threadIdx.x] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float res;
+        if (a4 == 0) {
+            res = n > 0 ? x[0] : 0.0f;
+            for (int i = 1; i < n; i++)
+                res += x[i];
+        } else {
+            float ax[4];
+            for (int j = 0; j < 4; j++)
+                ax[j] = n >= 8 ? s_chain[j] + s_chain[j + 4] : x[j];
+            if (n >= 8 && a4 > a8)
+                for (int j = 0; j < 4; j++)
+                    ax[j] += x[a8 + j];
+            res = (ax[0] + ax[2]) + (ax[1] + ax[3]);
+            for (int i = a4; i < n; i++)
+                res += x[i];
+        }
+        *s_out = res;
+    }
+    __syncthreads();
 }
 
 // Ascending bitonic sort of s[0..m) (m a power of two) by the whole block.
@@ -435,91 +551,175 @@ __device__ void bitonic_sort_lds(float *s, int m)
 // :104-177): P1 (functionals.cpp:287-296), P2 (:303-311), P3 (:332-360),
 // Hermite (:369-386).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void circus_kernel(CircusParams p)
+// LDS layout of one circus workgroup for columns of height up to nmax.
+struct CircusLds {
+    float *x;      // nmax4
+    float *sort;   // pow2 >= nmax
+    float *mod;    // nmax4
+    float2 *tw;    // nmax
+};
+
+__device__ __forceinline__ CircusLds circus_lds(float *smem, int nmax)
 {
-    extern __shared__ float smem[];
-    const int n = p.n;
+    const int n4 = (nmax + 3) & ~3;
+    int m2 = 1;
+    while (m2 < nmax)
+        m2 <<= 1;
+    CircusLds l;
+    l.x = smem;
+    l.sort = l.x + n4;
+    l.mod = l.sort + m2;
+    l.tw = reinterpret_cast<float2 *>(l.mod + n4);
+    return l;
+}
+
+// All P-functionals of one sinogram column x[0..n) (already in L.x, twiddles
+// in L.tw when P3 is requested).  Result q goes to out[q * out_stride].
+__device__ void circus_column(const CircusLds &L, int n, int np, const int *code,
+                              const int *herm, const double *herm_w, float *out,
+                              size_t out_stride, int32_t *p2rank)
+{
+    __shared__ float s_red[kBlock / kWave];
+    __shared__ double s_red_d[kBlock / kWave];
+    __shared__ float s_chain[8];
+    __shared__ float s_sum;
+    __shared__ int s_rank;
     int m2 = 1;
     while (m2 < n)
         m2 <<= 1;
-    float *s_x = smem;             // n
-    float *s_sort = s_x + n;       // m2
-    float *s_mod = s_sort + m2;    // n
-    float *s_red = s_mod + n;      // 8
-    __shared__ int s_rank;
-    __shared__ double s_red_d[kBlock / kWave];
-
-    const int colg = blockIdx.x;   // column over all sinograms
-    const int sg = colg / p.ncols;
-    const int col = colg - sg * p.ncols;
-    const float *x = p.sino + (size_t)colg * n;
-    for (int i = threadIdx.x; i < n; i += kBlock)
-        s_x[i] = x[i];
-    __syncthreads();
-
-    for (int q = 0; q < p.np; q++) {
-        const int code = p.code[q];
+    for (int q = 0; q < np; q++) {
+        const int cd = code[q];
         float res = 0.0f;
-        if (code == 1) {
+        if (cd == 1) {
             float acc = 0.0f;
             for (int i = 1 + threadIdx.x; i < n; i += kBlock)
-                acc += fabsf(s_x[i - 1] - s_x[i]);
+                acc += fabsf(L.x[i - 1] - L.x[i]);
             res = block_sum(acc, s_red);
-        } else if (code == 0) {
-            const double *w = p.herm_w + (size_t)p.herm[q] * n;
+        } else if (cd == 0) {
+            const double *w = herm_w + (size_t)herm[q] * n;
             double acc = 0.0;
             for (int i = threadIdx.x; i < n; i += kBlock)
-                acc = fma((double)s_x[i], w[i], acc);
+                acc = fma((double)L.x[i], w[i], acc);
             res = (float)block_sum_d(acc, s_red_d);
-        } else if (code == 2) {
+        } else if (cd == 2) {
             for (int i = threadIdx.x; i < m2; i += kBlock)
-                s_sort[i] = i < n ? s_x[i] : INFINITY;
+                L.sort[i] = i < n ? L.x[i] : INFINITY;
             __syncthreads();
-            bitonic_sort_lds(s_sort, m2);
-            if (threadIdx.x == 0) {
-                const float S = eigen_sum_lds(s_sort, n);
-                s_rank = weighted_median_lds(s_sort, n, S);
-            }
+            bitonic_sort_lds(L.sort, m2);
+            eigen_sum_block(L.sort, n, s_chain, &s_sum);
+            if (threadIdx.x == 0)
+                s_rank = weighted_median_walk(L.sort, n, s_sum);
             __syncthreads();
-            res = s_sort[s_rank];
-            if (p.p2rank && threadIdx.x == 0)
-                p.p2rank[((size_t)sg * p.np + q) * p.ncols + col] = s_rank;
+            res = L.sort[s_rank];
+            if (p2rank && threadIdx.x == 0)
+                p2rank[q * out_stride] = s_rank;
         } else {
-            // P3: half spectrum by direct DFT with a float twiddle table,
-            // then pow(hypot(Re/N, Im/N), 4) in double and trapz.
+            // P3: half spectrum by direct DFT (twiddles from LDS), then
+            // pow(hypot(Re/N, Im/N), 4) in double and the trapz sum.
             const int half = n / 2;
             const float fn = (float)n;
             for (int k = threadIdx.x; k <= half; k += kBlock) {
-                float re = 0.0f, im = 0.0f;
+                float re0 = 0.0f, im0 = 0.0f, re1 = 0.0f, im1 = 0.0f;
                 int idx = 0;
-                for (int j = 0; j < n; j++) {
-                    const float2 tw = p.dft_tw[idx];
-                    re = fmaf(s_x[j], tw.x, re);
-                    im = fmaf(-s_x[j], tw.y, im);
+                int j = 0;
+                for (; j + 2 <= n; j += 2) {
+                    const float2 t0 = L.tw[idx];
                     idx += k;
-                    if (idx >= n)
-                        idx -= n;
+                    idx -= idx >= n ? n : 0;
+                    const float2 t1 = L.tw[idx];
+                    idx += k;
+                    idx -= idx >= n ? n : 0;
+                    re0 = fmaf(L.x[j], t0.x, re0);
+                    im0 = fmaf(-L.x[j], t0.y, im0);
+                    re1 = fmaf(L.x[j + 1], t1.x, re1);
+                    im1 = fmaf(-L.x[j + 1], t1.y, im1);
                 }
+                if (j < n) {
+                    const float2 t0 = L.tw[idx];
+                    re0 = fmaf(L.x[j], t0.x, re0);
+                    im0 = fmaf(-L.x[j], t0.y, im0);
+                }
+                const float re = re0 + re1, im = im0 + im1;
                 const double h = hypot((double)(re / fn), (double)(im / fn));
                 const double h2 = h * h;
-                s_mod[k] = (float)(h2 * h2);
+                L.mod[k] = (float)(h2 * h2);
             }
             __syncthreads();
             for (int k = half + 1 + threadIdx.x; k < n; k += kBlock)
-                s_mod[k] = s_mod[n - k];
+                L.mod[k] = L.mod[n - k];
             __syncthreads();
             float acc = 0.0f;
             for (int i = threadIdx.x; i < n - 1; i += kBlock) {
                 const float l0 = (float)(-1 + i * 2.0 / (n - 1));
                 const float l1 = (float)(-1 + (i + 1) * 2.0 / (n - 1));
-                acc += (l1 - l0) * (s_mod[i + 1] + s_mod[i]);
+                acc += (l1 - l0) * (L.mod[i + 1] + L.mod[i]);
             }
             res = (float)(block_sum(acc, s_red) * 0.5);
         }
         if (threadIdx.x == 0)
-            p.out[((size_t)sg * p.np + q) * p.ncols + col] = res;
+            out[q * out_stride] = res;
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(kBlock) void circus_kernel(CircusParams p)
+{
+    extern __shared__ __align__(16) float smem[];
+    const int n = p.n;
+    const CircusLds L = circus_lds(smem, n);
+    const int colg = blockIdx.x;   // column over all sinograms
+    const int sg = colg / p.ncols;
+    const int col = colg - sg * p.ncols;
+    const float *x = p.sino + (size_t)colg * n;
+    for (int i = threadIdx.x; i < n; i += kBlock)
+        L.x[i] = x[i];
+    if (p.dft_tw)
+        for (int i = threadIdx.x; i < n; i += kBlock)
+            L.tw[i] = p.dft_tw[i];
+    __syncthreads();
+    const size_t o = (size_t)sg * p.np * p.ncols + col;
+    circus_column(L, n, p.np, p.code, p.herm, p.herm_w, p.out + o, p.ncols,
+                  p.p2rank ? p.p2rank + o : nullptr);
+}
+
+// Every sinogram column of a batch; the image is found by binary search over
+// the descriptors' first-column offsets.  Twiddles of P3 are formed per
+// column in LDS from double sincospi (the single-image path reads a host
+// table; both are the float rounding of the exact twiddle to within 1 ulp).
+__global__ __launch_bounds__(kBlock) void circus_batched_kernel(BatchCircusParams p)
+{
+    extern __shared__ __align__(16) float smem[];
+    const CircusLds L = circus_lds(smem, p.nmax);
+    const int colg = blockIdx.x;
+    int lo = 0, hi = p.nimg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (p.images[mid].col0 <= colg)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BatchImage im = p.images[lo];
+    const int n = im.n;
+    const int lc = colg - im.col0;
+    const int t = lc / p.A;
+    const int a = lc - t * p.A;
+    const float *x = im.sino + ((size_t)t * p.A + a) * n;
+    for (int i = threadIdx.x; i < n; i += kBlock)
+        L.x[i] = x[i];
+    bool p3 = false;
+    for (int q = 0; q < p.np; q++)
+        p3 = p3 || p.code[q] == 3;
+    if (p3)
+        for (int i = threadIdx.x; i < n; i += kBlock) {
+            double sn, cn;
+            sincospi(2.0 * (double)i / (double)n, &sn, &cn);
+            L.tw[i] = float2{(float)cn, (float)sn};
+        }
+    __syncthreads();
+    const int herm_none[kMaxP] = {};
+    circus_column(L, n, p.np, p.code, herm_none, nullptr,
+                  im.circus + (size_t)t * p.np * p.A + a, p.A, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -566,41 +766,6 @@ __device__ void bitonic_lds(K *s, int m)
     }
 }
 
-// Eigen packet-order sum of x[0..n) in LDS: threads 0..7 run the eight
-// stride-8 chains, thread 0 combines (c++/src/functionals.cpp:22 via Eigen
-// Redux).  Result broadcast through *s_out.  Must be called by the block.
-__device__ void eigen_sum_block(const float *x, int n, float *s_chain, float *s_out)
-{
-    const int a4 = n & ~3, a8 = n & ~7;
-    if (threadIdx.x < 8 && n >= 8) {
-        float r = 0.0f;
-        for (int i = threadIdx.x; i < a8; i += 8)
-            r += x[i];
-        s_chain[threadIdx.x] = r;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float res;
-        if (a4 == 0) {
-            res = n > 0 ? x[0] : 0.0f;
-            for (int i = 1; i < n; i++)
-                res += x[i];
-        } else {
-            float ax[4];
-            for (int j = 0; j < 4; j++)
-                ax[j] = n >= 8 ? s_chain[j] + s_chain[j + 4] : x[j];
-            if (n >= 8 && a4 > a8)
-                for (int j = 0; j < 4; j++)
-                    ax[j] += x[a8 + j];
-            res = (ax[0] + ax[2]) + (ax[1] + ax[3]);
-            for (int i = a4; i < n; i++)
-                res += x[i];
-        }
-        *s_out = res;
-    }
-    __syncthreads();
-}
-
 __global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -608,12 +773,12 @@ __global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
     int m2 = 1;
     while (m2 < n)
         m2 <<= 1;
+    const int n4 = (n + 3) & ~3;
     uint64_t *s_key = reinterpret_cast<uint64_t *>(smem_raw);      // m2
-    float *s_col = reinterpret_cast<float *>(s_key + m2);           // n
-    float *s_sq = s_col + n;                                        // n
+    float *s_col = reinterpret_cast<float *>(s_key + m2);           // n4
+    float *s_sq = s_col + n4;                                       // n
     __shared__ float s_chain[8];
     __shared__ float s_sum;
-    __shared__ int s_k;
 
     const float o = (float)((n - 1) / 2.0);
     const float lim = (float)(n - 1);
@@ -658,16 +823,7 @@ __global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
             __syncthreads();
             eigen_sum_block(s_sq, L, s_chain, &s_sum);
             if (threadIdx.x == 0) {
-                const float S = s_sum;
-                float integral = 0.0f;
-                int k = L - 1;
-                for (int i = 0; i < L; i++) {
-                    integral += s_sq[i];
-                    if (2.0f * integral >= S) {
-                        k = i;
-                        break;
-                    }
-                }
+                const int k = weighted_median_walk(s_sq, L, s_sum);
                 const int src = (int)(uint32_t)s_key[k];
                 p.sino[p.slot6 * tstride + line] = (float)src * s_col[sm + src];
                 if (p.t6_rank)
@@ -694,16 +850,7 @@ __global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
             __syncthreads();
             eigen_sum_block(s_sq, L, s_chain, &s_sum);
             if (threadIdx.x == 0) {
-                const float S = s_sum;
-                float integral = 0.0f;
-                int k = L - 1;
-                for (int i = 0; i < L; i++) {
-                    integral += s_sq[i];
-                    if (2.0f * integral >= S) {
-                        k = i;
-                        break;
-                    }
-                }
+                const int k = weighted_median_walk(s_sq, L, s_sum);
                 p.sino[p.slot7 * tstride + line] = ord_val(s_k32[k]);
                 if (p.t7_rank)
                     p.t7_rank[line] = k;
@@ -761,13 +908,66 @@ hipError_t launch_sino_t05(const SinoParams &p, bool need_x, bool need_s,
     return hipGetLastError();
 }
 
-hipError_t launch_circus(const CircusParams &p, hipStream_t s)
+static size_t circus_shm(int nmax)
 {
     int m2 = 1;
-    while (m2 < p.n)
+    while (m2 < nmax)
         m2 <<= 1;
-    const size_t shm = sizeof(float) * ((size_t)p.n * 2 + m2 + 8);
-    circus_kernel<<<p.ncols * p.nsino, kBlock, shm, s>>>(p);
+    const size_t n4 = ((size_t)nmax + 3) & ~(size_t)3;
+    return sizeof(float) * (2 * n4 + m2) + sizeof(float2) * (size_t)nmax;
+}
+
+hipError_t launch_circus(const CircusParams &p, hipStream_t s)
+{
+    circus_kernel<<<p.ncols * p.nsino, kBlock, circus_shm(p.n), s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_circus_batched(const BatchCircusParams &p, hipStream_t s)
+{
+    circus_batched_kernel<<<p.cols, kBlock, circus_shm(p.nmax), s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_batched(const BatchImage *d_images, int nimg, int nmax,
+                                    hipStream_t s)
+{
+    const int tiles = (nmax + 31) / 32;
+    int gx = tiles * tiles;
+    if (gx > 1024)
+        gx = 1024;
+    dim3 grid(gx, nimg);
+    transpose_batched_kernel<<<grid, kBlock, 0, s>>>(d_images);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_counter(unsigned int *counter, hipStream_t s)
+{
+    reset_counter_kernel<<<1, kWave, 0, s>>>(counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_sino_t05_batched(const BatchSinoParams &p, bool need_x,
+                                   bool need_s, bool do345, int waves_per_block,
+                                   int blocks, hipStream_t s)
+{
+    const int threads = kWave * waves_per_block;
+    const size_t shm = do345 ? sizeof(float2) * 3 * (size_t)p.tabn : 0;
+#define TT_LAUNCH(X, S, D)                                                     \
+    sino_t05_batched_kernel<X, S, D><<<blocks, threads, shm, s>>>(p)
+    if (need_x && need_s && do345)
+        TT_LAUNCH(true, true, true);
+    else if (need_x && need_s)
+        TT_LAUNCH(true, true, false);
+    else if (need_x)
+        TT_LAUNCH(true, false, false);
+    else if (need_s && do345)
+        TT_LAUNCH(false, true, true);
+    else if (need_s)
+        TT_LAUNCH(false, true, false);
+    else
+        TT_LAUNCH(false, false, false);
+#undef TT_LAUNCH
     return hipGetLastError();
 }
 
@@ -776,7 +976,8 @@ hipError_t launch_sino_sort(const SortTParams &p, hipStream_t s)
     int m2 = 1;
     while (m2 < p.n)
         m2 <<= 1;
-    const size_t shm = sizeof(uint64_t) * (size_t)m2 + sizeof(float) * 2 * (size_t)p.n;
+    const size_t n4 = ((size_t)p.n + 3) & ~(size_t)3;
+    const size_t shm = sizeof(uint64_t) * (size_t)m2 + sizeof(float) * (n4 + (size_t)p.n);
     const int lines = p.A * p.n;
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
