@@ -30,21 +30,41 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
+    # One 1024x1024 image, sum of 32 Gaussian blobs (seed 2), padded to 1449,
+    # 360 angles at 1 deg, T0-T7, P1-P3 + Hermite orders 1-5 (centre
+    # floor((n-1)/2) of the raw sinogram; see DESIGN.md on the orthonormal
+    # step).
+    "large-1024-blob-T0-T7-P1-P3-H1-H5": dict(
+        side=1024, kind="blob", seed=2, step=1.0, tcodes=[0, 1, 2, 3, 4, 5, 6, 7],
+        pfuncs=[(1,), (2,), (3,)] + [(0, k, -1) for k in range(1, 6)]),
     # One 256x256 U[0,1) image (seed 1) padded to 363, 360 angles at 1 deg,
     # T0-T5, P1-P3.
     "plain-256-uniform-T0-T5-P1-P3": dict(
         side=256, kind="uniform", seed=1, step=1.0, tcodes=[0, 1, 2, 3, 4, 5],
         pfuncs=[(1,), (2,), (3,)]),
+    # 256 blob images with sides uniform in 64..512 (seed 3), each padded to
+    # its own n, 180 angles at 2 deg, T0-T5, P1-P3: one batched graph replay
+    # per step, beside the plain per-image path.
+    "batched-256-blob-T0-T5-P1-P3": dict(
+        batch=256, lo=64, hi=512, kind="blob", seed=3, step=2.0,
+        tcodes=[0, 1, 2, 3, 4, 5], pfuncs=[(1,), (2,), (3,)]),
 }
-DEFAULT_WORKLOAD = "plain-256-uniform-T0-T5-P1-P3"
+DEFAULT_WORKLOAD = "large-1024-blob-T0-T7-P1-P3-H1-H5"
 
 
-def make_image(w):
+def make_image(w, side=None, seed=None):
     from tracetransform_amd import inputs
-    side = w["side"]
+    side = w["side"] if side is None else side
+    seed = w["seed"] if seed is None else seed
     if w["kind"] == "uniform":
-        return inputs.uniform_image(side, w["seed"])
-    return inputs.blob_image(side, w["seed"])
+        return inputs.uniform_image(side, seed)
+    return inputs.blob_image(side, seed)
+
+
+def resolve_pfuncs(w, n):
+    """Hermite entries carry centre -1 in the table: use floor((n-1)/2)."""
+    return [(p[0], p[1], (n - 1) // 2) if p[0] == 0 and p[2] < 0 else p
+            for p in w["pfuncs"]]
 
 
 def sino_bytes(n, A, nt):
@@ -62,19 +82,23 @@ def load_traffic(workload):
         return None
 
 
-def cpu_baseline(img_cm, w, n, A, iters):
+def cpu_threads():
+    threads = int(os.environ.get("TT_CPU_THREADS", "16"))
+    return max(1, min(threads, os.cpu_count() or 1))
+
+
+def cpu_baseline(img_cm, w, n, A, iters, side):
     """OpenMP port of the reference pipeline on this host's cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import tt_oracle
-    threads = int(os.environ.get("TT_CPU_THREADS", "16"))
-    threads = max(1, min(threads, os.cpu_count() or 1))
-    side = w["side"]
+    threads = cpu_threads()
     img_cm = tt_oracle.pad(img_cm.ravel(), side, side)
-    tt_oracle.transform(img_cm, n, A, w["step"], w["tcodes"], w["pfuncs"], threads)
+    pf = resolve_pfuncs(w, n)
+    tt_oracle.transform(img_cm, n, A, w["step"], w["tcodes"], pf, threads)
     times = []
     for _ in range(iters):
         t0 = time.perf_counter()
-        tt_oracle.transform(img_cm, n, A, w["step"], w["tcodes"], w["pfuncs"], threads)
+        tt_oracle.transform(img_cm, n, A, w["step"], w["tcodes"], pf, threads)
         times.append(time.perf_counter() - t0)
     times.sort()
     med = times[len(times) // 2]
@@ -84,31 +108,93 @@ def cpu_baseline(img_cm, w, n, A, iters):
                       f"{iters} timed runs of sinograms+circus+zscore, median"}
 
 
-def main():
+def cpu_baseline_batch(imgs_cm, sides, w, A, budget_s=20.0):
+    """Per-image reference pipeline on this host over a prefix of the batch
+    (images in batch order) until ~budget_s of CPU time, one warm-up."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import tt_oracle
+    threads = cpu_threads()
+    done_lines, spent, used = 0, 0.0, 0
+    warm = tt_oracle.pad(imgs_cm[0].ravel(), int(sides[0]), int(sides[0]))
+    n0, _, _ = tt_oracle.pad_geometry(int(sides[0]), int(sides[0]))
+    tt_oracle.transform(warm, n0, A, w["step"], w["tcodes"], w["pfuncs"], threads)
+    for img, side in zip(imgs_cm, sides):
+        side = int(side)
+        n, _, _ = tt_oracle.pad_geometry(side, side)
+        p = tt_oracle.pad(img.ravel(), side, side)
+        t0 = time.perf_counter()
+        tt_oracle.transform(p, n, A, w["step"], w["tcodes"], w["pfuncs"], threads)
+        spent += time.perf_counter() - t0
+        done_lines += A * n
+        used += 1
+        if spent >= budget_s:
+            break
+    return {"value": done_lines / spent, "unit": "lines/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {used} of {len(imgs_cm)} batch images processed one "
+                      f"after another (sinograms+circus+zscore), {spent:.1f} s"}
+
+
+def parse_args():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-iters", type=int, default=5)
+    ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path "
-              "shards by image or angle without a collective in the forward pass, "
-              "see DESIGN.md", file=sys.stderr)
-        sys.exit(2)
+    ap.add_argument("--autotune", type=int, default=1)
+    return ap.parse_args()
 
+
+def read_kernels(ctx, steps):
+    from tracetransform_amd import _lib
+    kinds = (("sino_t05", _lib.TT_KERNEL_SINO_T05), ("sino_sort", _lib.TT_KERNEL_SINO_SORT),
+             ("circus", _lib.TT_KERNEL_CIRCUS), ("aux", _lib.TT_KERNEL_AUX),
+             ("batch_graph", _lib.TT_KERNEL_BATCH))
+    return {name: ctx.profile_read(k) for name, k in kinds}
+
+
+def timed(ctx, step, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    for k in range(5):
+        ctx.profile_read(k)
+    ctx.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.profile(False)
+    return (t1 - t0) * 1e3 / steps, read_kernels(ctx, steps)
+
+
+def roofline_of(kern, bytes_launch):
+    ms_k, cnt = kern["sino_t05"]
+    avg_ms = ms_k / max(cnt, 1)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "kernel": "sino_t05_kernel",
+            "avg_launch_ms": round(avg_ms, 4),
+            "algorithmic_bytes_per_launch": bytes_launch}
+
+
+def run_single(args, w):
     import numpy as np
     import torch
-    from tracetransform_amd import _lib, core
+    from tracetransform_amd import core
 
-    w = WORKLOADS[args.workload]
     img_cm = make_image(w)
     side = w["side"]
     n, _, _ = core.pad_geometry(side, side)
     A = core.angle_count(w["step"])
-    nt, npf = len(w["tcodes"]), len(w["pfuncs"])
+    pf = resolve_pfuncs(w, n)
+    nt, npf = len(w["tcodes"]), len(pf)
     dev = torch.device("cuda", 0)
     d_raw = torch.from_numpy(np.ascontiguousarray(img_cm)).to(dev)
     d_img = core.pad(d_raw, side, side)
@@ -118,54 +204,104 @@ def main():
 
     def step():
         core.sinograms(d_img, A, w["step"], w["tcodes"], out=sino)
-        core.circus(sino, w["pfuncs"], out=circ)
+        core.circus(sino, pf, out=circ)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    for k in range(4):
-        ctx.profile_read(k)
-    ctx.profile(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ctx.profile(False)
-    kern = {name: ctx.profile_read(k) for name, k in
-            (("sino_t05", _lib.TT_KERNEL_SINO_T05), ("sino_sort", _lib.TT_KERNEL_SINO_SORT),
-             ("circus", _lib.TT_KERNEL_CIRCUS), ("aux", _lib.TT_KERNEL_AUX))}
-    ms_step = (t1 - t0) * 1e3 / args.steps
+    ms_step, kern = timed(ctx, step, args.steps, args.warmup)
     lines = A * n
-    value = lines / (ms_step * 1e-3)
-
-    ms_k, cnt = kern["sino_t05"]
-    avg_ms = ms_k / max(cnt, 1)
-    bytes_launch = sino_bytes(n, A, nt)
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": load_traffic(args.workload),
-                "kernel": "sino_t05_kernel", "avg_launch_ms": round(avg_ms, 4),
-                "algorithmic_bytes_per_launch": bytes_launch}
-
+    roof = roofline_of(kern, sino_bytes(n, A, min(nt, 6)))
+    roof["traffic"] = load_traffic(args.workload)
     out = {
-        "metric": "sinogram lines/s", "value": round(value, 1), "unit": "lines/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "metric": "sinogram lines/s", "value": round(lines / (ms_step * 1e-3), 1),
+        "unit": "lines/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_step, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": args.workload, "image": f"{side}x{side} {w['kind']} "
                    f"seed {w['seed']}", "padded_n": n, "angles": A,
                    "angle_step_deg": w["step"], "tfunctionals": w["tcodes"],
-                   "pfunctionals": [p[0] for p in w["pfuncs"]],
-                   "lines_per_step": lines},
-        "roofline": roofline,
+                   "pfunctionals": [list(p) for p in pf], "lines_per_step": lines},
+        "roofline": roof,
         "kernel_ms_per_step": {k: round(v[0] / max(args.steps, 1), 4)
                                for k, v in kern.items() if v[1]},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(img_cm, w, n, A, args.cpu_iters)
+        out["cpu_baseline"] = cpu_baseline(img_cm, w, n, A, args.cpu_iters, side)
+    return out
+
+
+def run_batched(args, w):
+    import numpy as np
+    import torch
+    from tracetransform_amd import core, inputs
+
+    sides = inputs.batch_sides(w["batch"], w["lo"], w["hi"], w["seed"])
+    imgs_cm = [make_image(w, side=int(sd), seed=w["seed"] * 1000 + k)
+               for k, sd in enumerate(sides)]
+    dev = torch.device("cuda", 0)
+    imgs = []
+    for img, sd in zip(imgs_cm, sides):
+        d_raw = torch.from_numpy(np.ascontiguousarray(img)).to(dev)
+        imgs.append(core.pad(d_raw, int(sd), int(sd)))
+    A = core.angle_count(w["step"])
+    ns = [im.shape[0] for im in imgs]
+    lines = A * sum(ns)
+    ctx = core.get_context(0)
+    batch = core.Batch(imgs, A, w["step"], w["tcodes"], w["pfuncs"])
+    batch.run(autotune=bool(args.autotune))  # plans, tunes and captures
+    torch.cuda.synchronize()
+    ms_step, kern = timed(ctx, batch.run, args.steps, args.warmup)
+    shape = batch.launch_shape()
+
+    # plain per-image path: the same kernels launched image by image
+    nt, npf = len(w["tcodes"]), len(w["pfuncs"])
+    sinos = [torch.empty((nt, A, n), dtype=torch.float32, device=dev) for n in ns]
+    circs = [torch.empty((nt, npf, A), dtype=torch.float32, device=dev) for _ in ns]
+
+    def plain():
+        for im, so, co in zip(imgs, sinos, circs):
+            core.sinograms(im, A, w["step"], w["tcodes"], out=so)
+            core.circus(so, w["pfuncs"], out=co)
+
+    ms_plain, kern_plain = timed(ctx, plain, max(2, args.steps // 2), 1)
+    gbytes = sum(sino_bytes(n, A, nt) for n in ns)
+    ms_graph, cnt = kern["batch_graph"]
+    avg_graph = ms_graph / max(cnt, 1)
+    achieved = gbytes / (avg_graph * 1e-3) / 1e9
+    out = {
+        "metric": "sinogram lines/s", "value": round(lines / (ms_step * 1e-3), 1),
+        "unit": "lines/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "images": len(ns),
+                   "sides": f"{int(min(sides))}..{int(max(sides))}",
+                   "padded_n": f"{min(ns)}..{max(ns)}", "angles": A,
+                   "angle_step_deg": w["step"], "tfunctionals": w["tcodes"],
+                   "pfunctionals": [p[0] for p in w["pfuncs"]],
+                   "lines_per_step": lines, "launch_shape": list(shape),
+                   "autotune": bool(args.autotune)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "batched graph (transpose + line + circus kernels)",
+                     "avg_launch_ms": round(avg_graph, 4),
+                     "algorithmic_bytes_per_launch": gbytes},
+        "plain_per_image": {"ms_per_step": round(ms_plain, 4),
+                            "value": round(lines / (ms_plain * 1e-3), 1),
+                            "batched_speedup": round(ms_plain / ms_step, 3)},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_batch(imgs_cm, sides, w, A)
+    return out
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path "
+              "shards by image or angle without a collective in the forward pass, "
+              "see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    w = WORKLOADS[args.workload]
+    out = run_batched(args, w) if "batch" in w else run_single(args, w)
     print(json.dumps(out), flush=True)
 
 

@@ -130,6 +130,60 @@ TT_API tt_status tt_circus_host(tt_context *ctx, const float *sino, int n,
                                 int n_cols, const tt_pfunc *pf, int np,
                                 float *circus);
 
+/* ---- batched forward (no reference counterpart) ------------------------- */
+
+/* One image of a batch; all pointers are device memory. */
+typedef struct tt_image {
+    const float *d_image;  /* n x n padded image, column-major              */
+    int32_t n;             /* padded side, 3..2730                           */
+    float *d_sino;         /* nt x A x n sinograms                           */
+    float *d_circus;       /* nt x np x A circus functions (NULL if np == 0) */
+} tt_image;
+
+/* Sinograms (T0..T5) and circus functions (P1..P3) of every image of a batch
+ * in one persistent launch per stage, the whole pass captured into a HIP graph
+ * on the first call for a batch and replayed when the same batch (pointers,
+ * sizes, functionals) comes again.  The reference processes images one after
+ * another (c++/src/demo.cpp:179); results equal tt_sinograms/tt_circus per
+ * image (P3 to float rounding of its twiddles).  The launch shape is the
+ * default or, with tt_context_set_autotune, the fastest of the candidate
+ * shapes timed on the batch's first call. */
+TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
+                                    int nimg, int n_angles, float angle_step_deg,
+                                    const int32_t *tcodes, int nt,
+                                    const tt_pfunc *pf, int np);
+TT_API tt_status tt_context_set_autotune(tt_context *ctx, int enable);
+/* Launch shape chosen for the current batch: waves per workgroup and
+ * persistent workgroups of the line kernel. */
+TT_API tt_status tt_batched_launch_shape(tt_context *ctx, int *waves_per_block,
+                                         int *blocks);
+
+/* ---- adjoint (no reference counterpart) --------------------------------- */
+
+/* Vector-Jacobian products of the forward path with the weighted medians
+ * and P2's rank held at their forward values (they are piecewise constant in
+ * the input) and the sample positions of the forward rotation.  The forward
+ * quantities are recomputed from d_image; rotated images are never stored.
+ * Supported: T0..T5, P1, P2, P3, Hermite; padded sides up to 1800. */
+
+/* d_dsino (n_sino x n_cols x n) = J_circus^T d_dcircus (n_sino x np x n_cols)
+ * for the sinograms d_sino. */
+TT_API tt_status tt_circus_backward(tt_context *ctx, const float *d_sino, int n,
+                                    int n_cols, int n_sino, const tt_pfunc *pf,
+                                    int np, const float *d_dcircus,
+                                    float *d_dsino);
+/* d_dimage (n x n) = J_sinograms^T d_dsino (nt x A x n). */
+TT_API tt_status tt_sinograms_backward(tt_context *ctx, const float *d_image,
+                                       int n, int n_angles, float angle_step_deg,
+                                       const int32_t *tcodes, int nt,
+                                       const float *d_dsino, float *d_dimage);
+/* d_dimage (n x n) = J^T d_dcircus (nt x np x A) of the whole path
+ * image -> sinograms -> circus functions. */
+TT_API tt_status tt_backward(tt_context *ctx, const float *d_image, int n,
+                             int n_angles, float angle_step_deg,
+                             const int32_t *tcodes, int nt, const tt_pfunc *pf,
+                             int np, const float *d_dcircus, float *d_dimage);
+
 /* ---- kernel timing ------------------------------------------------------ */
 
 /* No reference counterpart (the reference times whole getTransform calls,
@@ -142,7 +196,10 @@ enum {
     TT_KERNEL_SINO_SORT = 1,  /* T6/T7 line kernel                          */
     TT_KERNEL_CIRCUS = 2,     /* P-functional kernel                        */
     TT_KERNEL_AUX = 3,        /* transpose/pad helpers                      */
-    TT_KERNEL_KINDS = 4
+    TT_KERNEL_BATCH = 4,      /* one replay of the batched graph            */
+    TT_KERNEL_GATHER = 5,     /* adjoint: transposed rotation (+ records)   */
+    TT_KERNEL_CIRCUS_VJP = 6, /* adjoint of the P-functionals               */
+    TT_KERNEL_KINDS = 7
 };
 TT_API tt_status tt_profile_enable(tt_context *ctx, int enable);
 TT_API tt_status tt_profile_read(tt_context *ctx, int kind, double *total_ms,

@@ -1053,3 +1053,238 @@ int tto_transform(const float *img, int n, int n_angles, float step_deg,
     free(circ);
     return rc;
 }
+
+/* ======================================================================= */
+/* FP64 adjoint (vector-Jacobian product), medians and P2 ranks held fixed  */
+/* ======================================================================= */
+
+/* dS = J_P^T dC for one sinogram column x[0..n) (reference geometry: float
+ * P3 abscissae, float Hermite z grid).  dc[p] is the upstream gradient of
+ * circus function p at this column; src2[p] the P2 source index. */
+static void circus_vjp_f64(const double *x, int n, const int *pcodes,
+                           const int *orders, const int *centers, int np,
+                           const double *dc, const int32_t *src2, double *ds,
+                           double *re, double *im)
+{
+    for (int j = 0; j < n; j++)
+        ds[j] = 0.0;
+    for (int p = 0; p < np; p++) {
+        const double g = dc[p];
+        if (g == 0.0)
+            continue;
+        switch (pcodes[p]) {
+        case 1: /* P1 = sum_{j>=1} |x[j-1] - x[j]| */
+            for (int j = 0; j < n; j++) {
+                double d = 0.0;
+                if (j >= 1) {
+                    double u = x[j - 1] - x[j];
+                    d -= (u > 0) - (u < 0);
+                }
+                if (j <= n - 2) {
+                    double u = x[j] - x[j + 1];
+                    d += (u > 0) - (u < 0);
+                }
+                ds[j] += g * d;
+            }
+            break;
+        case 2:
+            ds[src2[p]] += g;
+            break;
+        case 3: { /* sum_p w_p |X_q(p)|^4 / N^4 */
+            tto_dft_f64(x, n, re, im);
+            const int half = n / 2;
+            double *ck = (double *)calloc((size_t)half + 1, sizeof(double));
+            for (int q = 0; q < n; q++) {
+                double lq = (float)(-1 + q * 2.0 / (n - 1));
+                double w = 0.0;
+                if (q >= 1)
+                    w += 0.5 * (lq - (double)(float)(-1 + (q - 1) * 2.0 / (n - 1)));
+                if (q <= n - 2)
+                    w += 0.5 * ((double)(float)(-1 + (q + 1) * 2.0 / (n - 1)) - lq);
+                ck[q <= half ? q : n - q] += w;
+            }
+            const double n4 = (double)n * n * (double)n * n;
+            for (int j = 0; j < n; j++) {
+                double acc = 0.0;
+                for (int k = 0; k <= half; k++) {
+                    const double m2 = re[k] * re[k] + im[k] * im[k];
+                    const long jk = ((long)j * k) % n;
+                    const double th = 2.0 * M_PI * (double)jk / n;
+                    acc += ck[k] * 4.0 * m2 / n4 * (re[k] * cos(th) - im[k] * sin(th));
+                }
+                ds[j] += g * acc;
+            }
+            free(ck);
+            break;
+        }
+        case 0: {
+            const int c = centers[p];
+            const float lof = (float)(10.0 / c), hif = (float)(10.0 / (double)(n - 1 - c));
+            for (int q = 0; q < n; q++) {
+                double z = q < c ? (float)(q * lof - 10) : (float)((q - c) * hif);
+                ds[q] += g * hermite_f64((unsigned)orders[p], z);
+            }
+            break;
+        }
+        }
+    }
+}
+
+int tto_sinograms_vjp_f64(const float *img, int n, int n_angles, float step_deg,
+                          const int *tcodes, int nt, const int32_t *med_x,
+                          const int32_t *med_s, const double *dsino, double *dimg,
+                          int nthreads);
+
+/* J^T dC of the whole pipeline (sinograms of T codes 0..5, then circus),
+ * in FP64 with reference geometry, medians (med_x, med_s) and P2 sources
+ * (p2_src [nt][np][A]) fixed at the FP32 run's values.  dcircus is
+ * [nt][np][A]; dimg is n*n (column-major).  T3..T5 use their double tables. */
+int tto_backward_f64(const float *img, int n, int n_angles, float step_deg,
+                     const int *tcodes, int nt, const int *pcodes,
+                     const int *orders, const int *centers, int np,
+                     const int32_t *med_x, const int32_t *med_s,
+                     const int32_t *p2_src, const double *dcircus, double *dimg,
+                     int nthreads)
+{
+    if (n <= 0 || n_angles <= 0 || nt <= 0)
+        return -1;
+    for (int t = 0; t < nt; t++)
+        if (tcodes[t] < 0 || tcodes[t] > 5)
+            return -1;
+    const int A = n_angles;
+    double *sino = (double *)malloc(sizeof(double) * (size_t)nt * n * A);
+    tto_sinograms_f64(img, n, A, step_deg, tcodes, nt, med_x, med_s, NULL, NULL, 1,
+                      sino, nthreads);
+    double *dsino = (double *)calloc((size_t)nt * n * A, sizeof(double));
+    if (nthreads <= 0)
+        nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+    {
+        double *re = (double *)malloc(sizeof(double) * (size_t)n);
+        double *im = (double *)malloc(sizeof(double) * (size_t)n);
+        double *dc = (double *)malloc(sizeof(double) * (size_t)(np > 0 ? np : 1));
+        int32_t *src = (int32_t *)malloc(sizeof(int32_t) * (size_t)(np > 0 ? np : 1));
+#pragma omp for schedule(static)
+        for (int ta = 0; ta < nt * A; ta++) {
+            const int t = ta / A, a = ta % A;
+            for (int p = 0; p < np; p++) {
+                dc[p] = dcircus[((size_t)t * np + p) * A + a];
+                src[p] = p2_src ? p2_src[((size_t)t * np + p) * A + a] : 0;
+            }
+            circus_vjp_f64(sino + (size_t)ta * n, n, pcodes, orders, centers, np, dc,
+                           src, dsino + (size_t)ta * n, re, im);
+        }
+        free(re);
+        free(im);
+        free(dc);
+        free(src);
+    }
+    int rc = tto_sinograms_vjp_f64(img, n, A, step_deg, tcodes, nt, med_x, med_s,
+                                   dsino, dimg, nthreads);
+    free(sino);
+    free(dsino);
+    return rc;
+}
+
+/* J_sinograms^T dsino ([nt][A][n]) in FP64, reference geometry, medians
+ * fixed; dimg is n*n column-major. */
+int tto_sinograms_vjp_f64(const float *img, int n, int n_angles, float step_deg,
+                          const int *tcodes, int nt, const int32_t *med_x,
+                          const int32_t *med_s, const double *dsino, double *dimg,
+                          int nthreads)
+{
+    if (n <= 0 || n_angles <= 0 || nt <= 0)
+        return -1;
+    for (int t = 0; t < nt; t++)
+        if (tcodes[t] < 0 || tcodes[t] > 5)
+            return -1;
+    if (nthreads <= 0)
+        nthreads = 1;
+    const int A = n_angles;
+    double *tre[6] = {0}, *tim[6] = {0};
+    for (int t = 0; t < nt; t++) {
+        int code = tcodes[t];
+        if (code >= 3 && !tre[code]) {
+            tre[code] = (double *)malloc(sizeof(double) * (size_t)n);
+            tim[code] = (double *)malloc(sizeof(double) * (size_t)n);
+            t345_table_f64(code, n, tre[code], tim[code]);
+        }
+    }
+    memset(dimg, 0, sizeof(double) * (size_t)n * n);
+#pragma omp parallel num_threads(nthreads)
+    {
+        double *acc = (double *)calloc((size_t)n * n, sizeof(double));
+        double *rot = (double *)malloc(sizeof(double) * (size_t)n * n);
+        double *G = (double *)malloc(sizeof(double) * (size_t)n);
+#pragma omp for schedule(static)
+        for (int a = 0; a < A; a++) {
+            float c, s;
+            tto_rotation_trig(tto_deg2rad(tto_angle_deg(a, step_deg)), &c, &s);
+            rotate_f64(img, n, c, s, 1, rot);
+            const float o = (float)((n - 1) / 2.0), ns = -s, lim = (float)(n - 1);
+            for (int col = 0; col < n; col++) {
+                const double *x = rot + (size_t)col * n;
+                const size_t line = (size_t)a * n + col;
+                for (int r = 0; r < n; r++)
+                    G[r] = 0.0;
+                for (int t = 0; t < nt; t++) {
+                    const double g = dsino[(size_t)t * n * A + line];
+                    const int code = tcodes[t];
+                    if (g == 0.0)
+                        continue;
+                    if (code == 0) {
+                        for (int r = 0; r < n; r++)
+                            G[r] += g;
+                    } else if (code == 1 || code == 2) {
+                        const int m = med_x[line];
+                        for (int r = m; r < n; r++)
+                            G[r] += g * (code == 1 ? (double)(r - m)
+                                                   : (double)(r - m) * (r - m));
+                    } else {
+                        const int sm = med_s[line];
+                        double zr = 0.0, zi = 0.0;
+                        for (int q = 1; q < n - sm; q++) {
+                            zr += tre[code][q] * x[sm + q];
+                            zi += tim[code][q] * x[sm + q];
+                        }
+                        const double z = hypot(zr, zi);
+                        if (z > 0.0)
+                            for (int q = 1; q < n - sm; q++)
+                                G[sm + q] += g * (zr * tre[code][q] + zi * tim[code][q]) / z;
+                    }
+                }
+                /* transposed bilinear sampling at the reference's float
+                 * coordinates */
+                for (int r = 0; r < n; r++) {
+                    if (G[r] == 0.0)
+                        continue;
+                    float px = (float)col - o, py = (float)r - o;
+                    float xx = px * c + py * s;
+                    float yy = px * ns + py * c;
+                    xx = xx + o;
+                    yy = yy + o;
+                    if (!(xx >= 0 && xx < lim && yy >= 0 && yy < lim))
+                        continue;
+                    float xi, yi;
+                    double fx = modff(xx, &xi), fy = modff(yy, &yi);
+                    long X = (long)xi, Y = (long)yi;
+                    acc[Y + X * n] += G[r] * (1 - fx) * (1 - fy);
+                    acc[Y + (X + 1) * n] += G[r] * fx * (1 - fy);
+                    acc[(Y + 1) + X * n] += G[r] * (1 - fx) * fy;
+                    acc[(Y + 1) + (X + 1) * n] += G[r] * fx * fy;
+                }
+            }
+        }
+#pragma omp critical(tto_backward_reduce)
+        for (size_t i = 0; i < (size_t)n * n; i++)
+            dimg[i] += acc[i];
+        free(acc);
+        free(rot);
+        free(G);
+    }
+    for (int k = 0; k < 6; k++) {
+        free(tre[k]);
+        free(tim[k]);
+    }
+    return 0;
+}

@@ -87,6 +87,13 @@ def lib():
         L.tto_circus_f64.argtypes = [_f64p, C.c_int, C.c_int, _i32p, _i32p, _i32p,
                                      C.c_int, vp, C.c_int, _f64p, C.c_int]
         L.tto_dft_f64.argtypes = [_f64p, C.c_int, _f64p, _f64p]
+        L.tto_backward_f64.argtypes = [_f32p, C.c_int, C.c_int, C.c_float, _i32p,
+                                       C.c_int, _i32p, _i32p, _i32p, C.c_int, vp, vp,
+                                       vp, _f64p, _f64p, C.c_int]
+        L.tto_backward_f64.restype = C.c_int
+        L.tto_sinograms_vjp_f64.argtypes = [_f32p, C.c_int, C.c_int, C.c_float, _i32p,
+                                            C.c_int, vp, vp, _f64p, _f64p, C.c_int]
+        L.tto_sinograms_vjp_f64.restype = C.c_int
         L.tto_transform.argtypes = [_f32p, C.c_int, C.c_int, C.c_float, _i32p,
                                     C.c_int, _i32p, _i32p, _i32p, C.c_int, _f32p,
                                     C.c_int]
@@ -335,3 +342,52 @@ def nearest_orthonormal_sinogram(sino_cols: np.ndarray):
     nos = u @ vt
     return np.ascontiguousarray(nos.T), new_center
 
+
+
+# ------------------------------------------------------------------ adjoint
+
+def forward_f64_fixed(img, n, n_angles, step_deg, tcodes, pfuncs, aux, p2_src,
+                      nthreads=1):
+    """Circus functions [nt, np, A] in FP64, reference geometry, with the
+    FP32 run's medians (aux) and P2 sources held fixed: the function whose
+    derivative backward_f64 returns."""
+    s = sinograms_f64(img, n, n_angles, step_deg, tcodes, aux, nthreads, geometry=1)
+    out = np.empty((len(tcodes), len(pfuncs), n_angles))
+    for t in range(len(tcodes)):
+        out[t] = circus_f64(s[t], n, n_angles, pfuncs, p2_src[t], nthreads, geometry=1)
+    return out
+
+
+def backward_f64(img, n, n_angles, step_deg, tcodes, pfuncs, aux, p2_src, dcircus,
+                 nthreads=1):
+    """J^T dcircus (FP64, reference geometry, medians/P2 sources fixed):
+    gradient of sum(dcircus * circus) with respect to the padded image,
+    returned as a flat column-major n*n array."""
+    tc = np.ascontiguousarray(tcodes, np.int32)
+    codes, orders, centers = _pargs(pfuncs)
+    dimg = np.empty(n * n, np.float64)
+    mx = np.ascontiguousarray(aux["med_x"], np.int32).ravel()
+    ms = np.ascontiguousarray(aux["med_s"], np.int32).ravel()
+    src = np.ascontiguousarray(p2_src, np.int32).ravel()
+    dc = np.ascontiguousarray(dcircus, np.float64).ravel()
+    rc = lib().tto_backward_f64(_f32(img).ravel(), n, n_angles, step_deg, tc, tc.size,
+                                codes, orders, centers, codes.size, _ptr(mx), _ptr(ms),
+                                _ptr(src), dc, dimg, nthreads)
+    if rc:
+        raise ValueError("tto_backward_f64 rejected its arguments")
+    return dimg
+
+
+def sinograms_vjp_f64(img, n, n_angles, step_deg, tcodes, aux, dsino, nthreads=1):
+    """J_sinograms^T dsino ([nt, A, n]) in FP64 with fixed medians."""
+    tc = np.ascontiguousarray(tcodes, np.int32)
+    dimg = np.empty(n * n, np.float64)
+    mx = np.ascontiguousarray(aux["med_x"], np.int32).ravel()
+    ms = np.ascontiguousarray(aux["med_s"], np.int32).ravel()
+    rc = lib().tto_sinograms_vjp_f64(_f32(img).ravel(), n, n_angles, step_deg, tc,
+                                     tc.size, _ptr(mx), _ptr(ms),
+                                     np.ascontiguousarray(dsino, np.float64).ravel(),
+                                     dimg, nthreads)
+    if rc:
+        raise ValueError("tto_sinograms_vjp_f64 rejected its arguments")
+    return dimg

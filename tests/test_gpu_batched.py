@@ -1,0 +1,116 @@
+"""GPU tests of the batched path (tt_forward_batched): one persistent launch
+per stage over a batch of uneven padded images, captured into a HIP graph.
+
+Bar: sinograms equal the single-image path bit for bit (the same line
+function runs on the same inputs); P1/P2 circus functions equal too; P3
+within 1e-6 normwise (its twiddles come from double sincospi in the batch,
+from a host table in the single-image path).  A few images are also checked
+against the oracle's medians via the single-image path's aux outputs.
+Graph replays are checked against the eager path after the inputs change.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import normwise_err, padded
+from tracetransform_amd import core, inputs
+
+pytestmark = pytest.mark.gpu
+
+T05 = [0, 1, 2, 3, 4, 5]
+P123 = [(1,), (2,), (3,)]
+
+
+def make_batch(oracle, sides, seed):
+    imgs = []
+    for k, side in enumerate(sides):
+        img, n = padded(oracle, inputs.blob_image(int(side), seed + k, nblobs=8,
+                                                  sigma=(3, 20)), side, side)
+        imgs.append(torch.from_numpy(img.reshape(n, n)).cuda())
+    return imgs
+
+
+def eager(img, A, step, codes, pf):
+    s = core.sinograms(img, A, step, codes)
+    return s, core.circus(s, pf)
+
+
+def check_against_eager(batch, A, step, codes, pf):
+    sinos, circs = batch.sino, batch.circus
+    torch.cuda.synchronize()
+    for k, img in enumerate(batch.images):
+        s, c = eager(img, A, step, codes, pf)
+        torch.cuda.synchronize()
+        assert torch.equal(sinos[k], s), k
+        if not pf:
+            assert circs[k] is None
+            continue
+        cb, ce = circs[k].cpu().numpy(), c.cpu().numpy()
+        for q, p in enumerate(pf):
+            if p[0] == 3:
+                assert normwise_err(cb[:, q], ce[:, q]) <= 1e-6, (k, q)
+            else:
+                assert np.array_equal(cb[:, q], ce[:, q]), (k, q)
+
+
+def test_batched_matches_single_image(oracle, gpu):
+    sides = inputs.batch_sides(24, 16, 160, 3)
+    imgs = make_batch(oracle, sides, 100)
+    print(f"[batched] {len(imgs)} images, n in {min(i.shape[0] for i in imgs)}.."
+          f"{max(i.shape[0] for i in imgs)}", flush=True)
+    b = core.Batch(imgs, 90, 4.0, T05, P123)
+    b.run(autotune=False)
+    check_against_eager(b, 90, 4.0, T05, P123)
+
+
+def test_batched_replays_follow_input_changes(oracle, gpu):
+    sides = [64, 33, 100, 7, 64]
+    imgs = make_batch(oracle, sides, 200)
+    b = core.Batch(imgs, 36, 10.0, [0, 3, 1], P123)
+    for rep in range(4):
+        b.run(autotune=False)
+        check_against_eager(b, 36, 10.0, [0, 3, 1], P123)
+        # eager work between replays: new image contents in place
+        for im in imgs:
+            im.mul_(0.5).add_(torch.rand_like(im) * (im != 0))
+        torch.cuda.synchronize()
+
+
+def test_batched_autotune_and_subsets(oracle, gpu):
+    sides = inputs.batch_sides(12, 16, 96, 9)
+    imgs = make_batch(oracle, sides, 300)
+    for codes, pf in (([0], []), ([2, 4], [(2,)]), ([5, 0], [(3,), (1,)])):
+        b = core.Batch(imgs, 45, 8.0, codes, pf)
+        b.run(autotune=True)
+        w, nb = b.launch_shape()
+        print(f"[batched] autotuned shape: {w} waves/block, {nb} blocks", flush=True)
+        assert w in (1, 2, 4) and nb > 0
+        check_against_eager(b, 45, 8.0, codes, pf)
+    core.get_context(0)  # leave autotune off for later tests
+    b = core.Batch(imgs[:2], 4, 90.0, [0], [])
+    b.run(autotune=False)
+
+
+def test_batched_medians_match_oracle(oracle, gpu):
+    sides = [40, 17, 61]
+    imgs = make_batch(oracle, sides, 400)
+    b = core.Batch(imgs, 60, 6.0, T05, P123)
+    b.run(autotune=False)
+    torch.cuda.synchronize()
+    for k, img in enumerate(imgs):
+        n = img.shape[0]
+        o, _ = oracle.sinograms(img.cpu().numpy().ravel(), n, 60, 6.0, T05, nthreads=8)
+        g = b.sino[k].cpu().numpy()
+        assert np.array_equal(g[0], o[0])          # T0 is bit-exact
+        for t in range(1, 6):
+            assert normwise_err(g[t], o[t]) <= 2e-5
+
+
+def test_batched_rejects_unsupported(gpu):
+    img = torch.zeros((7, 7), device="cuda")
+    with pytest.raises(Exception):
+        core.Batch([img], 4, 90.0, [6], []).run()
+    with pytest.raises(Exception):
+        core.Batch([img], 4, 90.0, [0], [(0, 1, 3)]).run()
+    with pytest.raises(ValueError):
+        core.Batch([img], 4, 90.0, [0, 0], []).run()

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtt_amd.so")
 
 TT_OK, TT_ERR_INVALID, TT_ERR_NOMEM, TT_ERR_DEVICE, TT_ERR_UNSUPPORTED = range(5)
-TT_KERNEL_SINO_T05, TT_KERNEL_SINO_SORT, TT_KERNEL_CIRCUS, TT_KERNEL_AUX = range(4)
+(TT_KERNEL_SINO_T05, TT_KERNEL_SINO_SORT, TT_KERNEL_CIRCUS, TT_KERNEL_AUX,
+ TT_KERNEL_BATCH, TT_KERNEL_GATHER, TT_KERNEL_CIRCUS_VJP) = range(7)
 
 # Every symbol include/tt.h declares; tests check the library exports them.
 EXPORTED = (
@@ -22,6 +23,8 @@ EXPORTED = (
     "tt_context_synchronize", "tt_last_error", "tt_pad_geometry", "tt_pad",
     "tt_angle_count", "tt_sinograms", "tt_sinograms_host", "tt_circus",
     "tt_circus_host", "tt_profile_enable", "tt_profile_read",
+    "tt_forward_batched", "tt_context_set_autotune", "tt_batched_launch_shape",
+    "tt_circus_backward", "tt_sinograms_backward", "tt_backward",
 )
 
 
@@ -35,6 +38,11 @@ class TTError(RuntimeError):
 
 class tt_pfunc(C.Structure):
     _fields_ = [("code", C.c_int32), ("order", C.c_int32), ("center", C.c_int32)]
+
+
+class tt_image(C.Structure):
+    _fields_ = [("d_image", C.c_void_p), ("n", C.c_int32), ("d_sino", C.c_void_p),
+                ("d_circus", C.c_void_p)]
 
 
 class tt_sino_aux(C.Structure):
@@ -79,6 +87,17 @@ def load():
                                 C.POINTER(tt_pfunc), C.c_int, vp, vp]
         L.tt_circus_host.argtypes = [vp, vp, C.c_int, C.c_int,
                                      C.POINTER(tt_pfunc), C.c_int, vp]
+        L.tt_forward_batched.argtypes = [vp, C.POINTER(tt_image), C.c_int, C.c_int,
+                                         f32, C.POINTER(i32), C.c_int,
+                                         C.POINTER(tt_pfunc), C.c_int]
+        L.tt_context_set_autotune.argtypes = [vp, C.c_int]
+        L.tt_batched_launch_shape.argtypes = [vp, ip, ip]
+        L.tt_circus_backward.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int,
+                                         C.POINTER(tt_pfunc), C.c_int, vp, vp]
+        L.tt_sinograms_backward.argtypes = [vp, vp, C.c_int, C.c_int, f32,
+                                            C.POINTER(i32), C.c_int, vp, vp]
+        L.tt_backward.argtypes = [vp, vp, C.c_int, C.c_int, f32, C.POINTER(i32),
+                                  C.c_int, C.POINTER(tt_pfunc), C.c_int, vp, vp]
         L.tt_profile_enable.argtypes = [vp, C.c_int]
         L.tt_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_int64)]

@@ -195,3 +195,140 @@ def circus(sino: torch.Tensor, pfunctionals, out: torch.Tensor | None = None,
         out = out[0]
         rank = rank[0] if rank is not None else None
     return (out, rank) if with_p2_rank else out
+
+
+class Batch:
+    """A batch of padded images with preallocated outputs for
+    ``tt_forward_batched``.  Reusing the same Batch (same buffers) replays the
+    captured HIP graph; only the image contents may change between calls."""
+
+    def __init__(self, images, n_angles: int, step_deg: float, tfunctionals,
+                 pfunctionals=()):
+        self.images = [im for im in images]
+        for k, im in enumerate(self.images):
+            _require_cuda_f32(f"images[{k}]", im)
+            if im.dim() != 2 or im.shape[0] != im.shape[1]:
+                raise ValueError(f"images[{k}] must be a square padded [n, n] tensor")
+        self.n_angles = int(n_angles)
+        self.step = float(step_deg)
+        self.codes = tcodes_of(tfunctionals)
+        self.pf = pfuncs_of(pfunctionals)
+        dev = self.images[0].device
+        nt, npf, A = len(self.codes), len(self.pf), self.n_angles
+        self.sino = [torch.empty((nt, A, im.shape[0]), dtype=torch.float32, device=dev)
+                     for im in self.images]
+        self.circus = [torch.empty((nt, npf, A), dtype=torch.float32, device=dev)
+                       if npf else None for _ in self.images]
+        arr = (_lib.tt_image * len(self.images))()
+        for k, im in enumerate(self.images):
+            arr[k].d_image = im.data_ptr()
+            arr[k].n = im.shape[0]
+            arr[k].d_sino = self.sino[k].data_ptr()
+            arr[k].d_circus = self.circus[k].data_ptr() if npf else None
+        self._arr = arr
+        self._codes = (C.c_int32 * nt)(*self.codes)
+        self._pf = (_lib.tt_pfunc * max(npf, 1))(*[_lib.tt_pfunc(*t) for t in self.pf])
+        self.device = dev.index
+
+    def run(self, autotune: bool | None = None):
+        ctx = get_context(self.device)
+        L = _lib.load()
+        if autotune is not None:
+            _lib.check("tt_context_set_autotune",
+                       L.tt_context_set_autotune(ctx.handle, int(bool(autotune))))
+        ctx.bind_current_stream()
+        _lib.check("tt_forward_batched",
+                   L.tt_forward_batched(ctx.handle, self._arr, len(self.images),
+                                        self.n_angles, self.step, self._codes,
+                                        len(self.codes), self._pf, len(self.pf)))
+        return self.sino, self.circus
+
+    def launch_shape(self):
+        ctx = get_context(self.device)
+        w, b = C.c_int(), C.c_int()
+        _lib.check("tt_batched_launch_shape",
+                   _lib.load().tt_batched_launch_shape(ctx.handle, C.byref(w), C.byref(b)))
+        return w.value, b.value
+
+
+# ---------------------------------------------------------------- adjoint
+
+def circus_backward(sino: torch.Tensor, pfunctionals, dcircus: torch.Tensor):
+    """J_circus^T dcircus: ``sino`` [n_sino, n_cols, n], ``dcircus``
+    [n_sino, np, n_cols] -> dsino [n_sino, n_cols, n]."""
+    _require_cuda_f32("sino", sino)
+    _require_cuda_f32("dcircus", dcircus)
+    n_sino, n_cols, n = sino.shape
+    pf = pfuncs_of(pfunctionals)
+    dsino = torch.empty_like(sino)
+    arr = (_lib.tt_pfunc * max(len(pf), 1))(*[_lib.tt_pfunc(*t) for t in pf])
+    ctx = get_context(sino.device.index)
+    ctx.bind_current_stream()
+    _lib.check("tt_circus_backward",
+               _lib.load().tt_circus_backward(ctx.handle, _dptr(sino), n, n_cols, n_sino,
+                                              arr, len(pf), _dptr(dcircus), _dptr(dsino)))
+    return dsino
+
+
+def sinograms_backward(image_cm: torch.Tensor, n_angles: int, step_deg: float,
+                       tfunctionals, dsino: torch.Tensor):
+    """J_sinograms^T dsino ([nt, A, n]) -> dimage [n, n] (column-major)."""
+    _require_cuda_f32("image", image_cm)
+    _require_cuda_f32("dsino", dsino)
+    n = image_cm.shape[0]
+    codes = tcodes_of(tfunctionals)
+    dimg = torch.empty_like(image_cm)
+    ctx = get_context(image_cm.device.index)
+    ctx.bind_current_stream()
+    carr = (C.c_int32 * len(codes))(*codes)
+    _lib.check("tt_sinograms_backward",
+               _lib.load().tt_sinograms_backward(ctx.handle, _dptr(image_cm), n,
+                                                 int(n_angles), float(step_deg), carr,
+                                                 len(codes), _dptr(dsino), _dptr(dimg)))
+    return dimg
+
+
+def backward(image_cm: torch.Tensor, n_angles: int, step_deg: float, tfunctionals,
+             pfunctionals, dcircus: torch.Tensor):
+    """J^T dcircus ([nt, np, A]) of image -> circus -> dimage [n, n]."""
+    _require_cuda_f32("image", image_cm)
+    _require_cuda_f32("dcircus", dcircus)
+    n = image_cm.shape[0]
+    codes = tcodes_of(tfunctionals)
+    pf = pfuncs_of(pfunctionals)
+    dimg = torch.empty_like(image_cm)
+    ctx = get_context(image_cm.device.index)
+    ctx.bind_current_stream()
+    carr = (C.c_int32 * len(codes))(*codes)
+    arr = (_lib.tt_pfunc * max(len(pf), 1))(*[_lib.tt_pfunc(*t) for t in pf])
+    _lib.check("tt_backward",
+               _lib.load().tt_backward(ctx.handle, _dptr(image_cm), n, int(n_angles),
+                                       float(step_deg), carr, len(codes), arr, len(pf),
+                                       _dptr(dcircus), _dptr(dimg)))
+    return dimg
+
+
+class TraceTransformFunction(torch.autograd.Function):
+    """Thin autograd wrapper (used for gradient checking): padded image
+    [n, n] (column-major) -> circus functions [nt, np, A]."""
+
+    @staticmethod
+    def forward(ctx, image_cm, n_angles, step_deg, tcodes, pfuncs):
+        sino = sinograms(image_cm, n_angles, step_deg, tcodes)
+        circ = circus(sino, pfuncs)
+        ctx.save_for_backward(image_cm)
+        ctx.args = (n_angles, step_deg, tcodes, pfuncs)
+        return circ
+
+    @staticmethod
+    def backward(ctx, dcirc):
+        (image_cm,) = ctx.saved_tensors
+        n_angles, step_deg, tcodes, pfuncs = ctx.args
+        dimg = backward(image_cm, n_angles, step_deg, tcodes, pfuncs,
+                        dcirc.contiguous().float())
+        return dimg, None, None, None, None
+
+
+def trace_transform(image_cm, n_angles, step_deg, tfunctionals, pfunctionals):
+    return TraceTransformFunction.apply(image_cm, n_angles, step_deg,
+                                        tcodes_of(tfunctionals), pfuncs_of(pfunctionals))

@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -216,12 +217,49 @@ struct KernelTimer {
     }
 };
 
+// A batch ready to replay: descriptors, workspaces and the captured graph.
+// It owns copies of the tables it uses so later single-image calls (which
+// rewrite the context's tables) cannot change what the graph reads.
+struct BatchPlan {
+    std::vector<int64_t> key;
+    DevBuf desc, imgT, trig, tab, counter;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int waves_per_block = 4, blocks = 0;
+    bool valid = false;
+
+    void drop_graph()
+    {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        valid = false;
+    }
+    void release()
+    {
+        drop_graph();
+        desc.release();
+        imgT.release();
+        trig.release();
+        tab.release();
+        counter.release();
+        key.clear();
+    }
+};
+
 struct tt_context {
     int device = 0;
     KernelTimer timer;
+    BatchPlan batch;
+    bool autotune = false;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     DevBuf imgT, trig, tab345, dft_tw, herm_w, io_a, io_b, med_x, med_s;
+    DevBuf bw_sino, bw_dsino, bw_z, bw_rec;
     int trig_A = -1;
     float trig_step = 0.0f;
     int tab_n = -1, dft_n = -1;
@@ -373,10 +411,16 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->imgT, &ctx->trig, &ctx->tab345, &ctx->dft_tw,
                       &ctx->herm_w, &ctx->io_a, &ctx->io_b, &ctx->med_x,
-                      &ctx->med_s};
+                      &ctx->med_s, &ctx->bw_sino, &ctx->bw_dsino, &ctx->bw_z,
+                      &ctx->bw_rec};
     for (DevBuf *b : bufs)
         b->release();
     ctx->timer.release();
+    ctx->batch.release();
+    if (ctx->ev_in)
+        (void)hipEventDestroy(ctx->ev_in);
+    if (ctx->ev_out)
+        (void)hipEventDestroy(ctx->ev_out);
     if (ctx->own)
         (void)hipStreamDestroy(ctx->own);
     delete ctx;
@@ -474,6 +518,9 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
     TT_TRY(ctx->set_device());
     hipStream_t st = ctx->stream;
     const bool do6 = slot[6] >= 0, do7 = slot[7] >= 0;
+    // T6/T7 sort up to 2048 keys per line in a wave's registers
+    TT_CHECK(!(do6 || do7) || n <= 2048,
+             "T6/T7 support padded sides up to 2048 (got %d)", n);
     const bool need_x = slot[1] >= 0 || slot[2] >= 0 || ax.median_x || do7;
     const bool do345 = slot[3] >= 0 || slot[4] >= 0 || slot[5] >= 0;
     const bool need_s = do345 || ax.median_sqrt || do6;
@@ -635,6 +682,447 @@ TT_API tt_status tt_circus_host(tt_context *ctx, const float *sino, int n,
         TT_HIP(hipMemcpyAsync(circus, ctx->io_b.ptr, out_b, hipMemcpyDeviceToHost,
                               ctx->stream));
     TT_HIP(hipStreamSynchronize(ctx->stream));
+    return TT_OK;
+}
+
+namespace {
+
+int device_cus(int device)
+{
+    int cus = 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) !=
+        hipSuccess)
+        cus = 256;
+    return cus;
+}
+
+// Enqueue the batched pass (used for capture and for autotuning).
+hipError_t enqueue_batch(tt_context *ctx, const ttk::BatchSinoParams &sp,
+                         const ttk::BatchCircusParams &cp, int nimg, int nmax,
+                         bool need_x, bool need_s, bool do345, int wpb, int blocks,
+                         bool with_transpose, bool with_circus, hipStream_t st)
+{
+    hipError_t e = hipSuccess;
+    if (with_transpose)
+        e = ttk::launch_transpose_batched(sp.images, nimg, nmax, st);
+    if (e == hipSuccess)
+        e = ttk::launch_reset_counter(sp.counter, st);
+    if (e == hipSuccess)
+        e = ttk::launch_sino_t05_batched(sp, need_x, need_s, do345, wpb, blocks, st);
+    if (e == hipSuccess && with_circus && cp.np > 0)
+        e = ttk::launch_circus_batched(cp, st);
+    return e;
+}
+
+}  // namespace
+
+TT_API tt_status tt_context_set_autotune(tt_context *ctx, int enable)
+{
+    TT_CHECK(ctx, "ctx is NULL");
+    if ((enable != 0) != ctx->autotune)
+        ctx->batch.drop_graph();
+    ctx->autotune = enable != 0;
+    return TT_OK;
+}
+
+TT_API tt_status tt_batched_launch_shape(tt_context *ctx, int *waves_per_block,
+                                         int *blocks)
+{
+    TT_CHECK(ctx && waves_per_block && blocks, "NULL argument");
+    TT_CHECK(ctx->batch.valid, "no batch has been planned on this context");
+    *waves_per_block = ctx->batch.waves_per_block;
+    *blocks = ctx->batch.blocks;
+    return TT_OK;
+}
+
+TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
+                                    int nimg, int n_angles, float angle_step_deg,
+                                    const int32_t *tcodes, int nt,
+                                    const tt_pfunc *pf, int np)
+{
+    TT_CHECK(ctx, "ctx is NULL");
+    TT_CHECK(images && nimg >= 1, "images is NULL or nimg < 1");
+    TT_CHECK(n_angles >= 1, "n_angles must be >= 1 (got %d)", n_angles);
+    TT_CHECK(angle_step_deg > 0.0f && isfinite(angle_step_deg),
+             "angle step must be positive and finite");
+    TT_TRY(check_tcodes(tcodes, nt));
+    TT_CHECK(nt >= 1, "at least one T-functional is required");
+    int slot[6] = {-1, -1, -1, -1, -1, -1};
+    for (int t = 0; t < nt; t++) {
+        if (tcodes[t] > 5)
+            return fail(TT_ERR_UNSUPPORTED,
+                        "the batched path computes T0..T5 (got T%d)", tcodes[t]);
+        TT_CHECK(slot[tcodes[t]] < 0, "duplicate T-functional %d in a batch",
+                 tcodes[t]);
+        slot[tcodes[t]] = t;
+    }
+    TT_CHECK(np >= 0 && np <= ttk::kMaxP, "np must be in 0..%d", ttk::kMaxP);
+    for (int q = 0; q < np; q++) {
+        TT_CHECK(pf, "pf is NULL");
+        if (pf[q].code == TT_P_HERMITE)
+            return fail(TT_ERR_UNSUPPORTED,
+                        "Hermite P-functionals are not available in the batched path");
+        TT_CHECK(pf[q].code >= 1 && pf[q].code <= 3,
+                 "invalid P-functional code %d at position %d", pf[q].code, q);
+    }
+    int nmax = 0;
+    for (int i = 0; i < nimg; i++) {
+        TT_CHECK(images[i].d_image && images[i].d_sino,
+                 "image %d: NULL image or sinogram pointer", i);
+        TT_CHECK(images[i].n >= 3 && images[i].n <= 2730,
+                 "image %d: padded side %d outside 3..2730", i, images[i].n);
+        TT_CHECK(np == 0 || images[i].d_circus, "image %d: NULL circus pointer", i);
+        nmax = images[i].n > nmax ? images[i].n : nmax;
+    }
+    TT_TRY(ctx->set_device());
+
+    std::vector<int64_t> key = {nimg, n_angles, (int64_t)(*(const uint32_t *)&angle_step_deg),
+                                nt, np, ctx->autotune};
+    for (int t = 0; t < nt; t++)
+        key.push_back(tcodes[t]);
+    for (int q = 0; q < np; q++)
+        key.push_back(pf[q].code);
+    for (int i = 0; i < nimg; i++) {
+        key.push_back((int64_t)(intptr_t)images[i].d_image);
+        key.push_back(images[i].n);
+        key.push_back((int64_t)(intptr_t)images[i].d_sino);
+        key.push_back((int64_t)(intptr_t)images[i].d_circus);
+    }
+    BatchPlan &B = ctx->batch;
+    hipStream_t own = ctx->own;
+    const bool do345 = slot[3] >= 0 || slot[4] >= 0 || slot[5] >= 0;
+    const bool need_x = slot[1] >= 0 || slot[2] >= 0;
+    const bool need_s = do345;
+    if (!B.valid || B.key != key) {
+        B.drop_graph();
+        // largest images first: the dynamic counter then hands out the long
+        // items early and the short ones fill the tail
+        std::vector<int> order(nimg);
+        for (int i = 0; i < nimg; i++)
+            order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return images[a].n > images[b].n;
+        });
+        std::vector<ttk::BatchImage> d(nimg);
+        size_t tot = 0;
+        for (int i = 0; i < nimg; i++)
+            tot += (size_t)images[i].n * images[i].n;
+        TT_HIP(B.imgT.ensure(sizeof(float) * tot));
+        size_t off = 0;
+        int items = 0, cols = 0;
+        for (int k = 0; k < nimg; k++) {
+            const tt_image &im = images[order[k]];
+            ttk::BatchImage &b = d[k];
+            b.img = im.d_image;
+            b.imgT = B.imgT.as<float>() + off;
+            off += (size_t)im.n * im.n;
+            b.sino = im.d_sino;
+            b.med_x = nullptr;
+            b.med_s = nullptr;
+            b.n = im.n;
+            b.nb = (im.n + ttk::kWave - 1) / ttk::kWave;
+            b.item0 = items;
+            b.col0 = cols;
+            b.circus = im.d_circus;
+            items += b.nb * n_angles;
+            cols += nt * n_angles;
+        }
+        TT_HIP(B.desc.ensure(sizeof(ttk::BatchImage) * nimg));
+        TT_HIP(hipMemcpy(B.desc.ptr, d.data(), sizeof(ttk::BatchImage) * nimg,
+                         hipMemcpyHostToDevice));
+        std::vector<float2> h;
+        make_trig(n_angles, angle_step_deg, h);
+        TT_HIP(B.trig.ensure(sizeof(float2) * n_angles));
+        TT_HIP(hipMemcpy(B.trig.ptr, h.data(), sizeof(float2) * n_angles,
+                         hipMemcpyHostToDevice));
+        make_tab345(nmax, h);
+        TT_HIP(B.tab.ensure(sizeof(float2) * h.size()));
+        TT_HIP(hipMemcpy(B.tab.ptr, h.data(), sizeof(float2) * h.size(),
+                         hipMemcpyHostToDevice));
+        TT_HIP(B.counter.ensure(sizeof(unsigned int)));
+
+        ttk::BatchSinoParams sp = {};
+        sp.images = B.desc.as<ttk::BatchImage>();
+        sp.nimg = nimg;
+        sp.items = items;
+        sp.trig = B.trig.as<float2>();
+        sp.tab345 = B.tab.as<float2>();
+        sp.tabn = nmax;
+        sp.A = n_angles;
+        sp.counter = B.counter.as<unsigned int>();
+        for (int k = 0; k < 6; k++)
+            sp.slot.s[k] = slot[k];
+        ttk::BatchCircusParams cp = {};
+        cp.images = sp.images;
+        cp.nimg = nimg;
+        cp.cols = cols;
+        cp.nt = nt;
+        cp.A = n_angles;
+        cp.nmax = nmax;
+        cp.np = np;
+        for (int q = 0; q < np; q++)
+            cp.code[q] = pf[q].code;
+
+        // launch shape: default, or the fastest candidate on this batch
+        const int cus = device_cus(ctx->device);
+        int best_w = 4, best_b = cus * 4;
+        if (ctx->autotune) {
+            TT_HIP(ttk::launch_transpose_batched(sp.images, nimg, nmax, own));
+            hipEvent_t e0, e1;
+            TT_HIP(hipEventCreate(&e0));
+            TT_HIP(hipEventCreate(&e1));
+            float best = 1e30f;
+            const int waves[] = {1, 2, 4};
+            const int per_cu[] = {2, 4, 8, 16};
+            for (int w : waves)
+                for (int pc : per_cu) {
+                    const int blocks = cus * pc * 4 / w;
+                    float tsum = 0.0f;
+                    for (int rep = 0; rep < 3; rep++) {
+                        TT_HIP(hipEventRecord(e0, own));
+                        TT_HIP(enqueue_batch(ctx, sp, cp, nimg, nmax, need_x, need_s,
+                                             do345, w, blocks, false, false, own));
+                        TT_HIP(hipEventRecord(e1, own));
+                        TT_HIP(hipEventSynchronize(e1));
+                        float ms = 0.0f;
+                        TT_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        if (rep > 0)
+                            tsum += ms;
+                    }
+                    if (tsum < best) {
+                        best = tsum;
+                        best_w = w;
+                        best_b = blocks;
+                    }
+                }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        B.waves_per_block = best_w;
+        B.blocks = best_b;
+
+        TT_HIP(hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = enqueue_batch(ctx, sp, cp, nimg, nmax, need_x, need_s, do345,
+                                      best_w, best_b, true, true, own);
+        hipGraph_t g = nullptr;
+        hipError_t ee = hipStreamEndCapture(own, &g);
+        TT_HIP(ce);
+        TT_HIP(ee);
+        B.graph = g;
+        TT_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+        B.key = key;
+        B.valid = true;
+    }
+    if (!ctx->ev_in)
+        TT_HIP(hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming));
+    if (!ctx->ev_out)
+        TT_HIP(hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming));
+    if (ctx->stream != own) {
+        TT_HIP(hipEventRecord(ctx->ev_in, ctx->stream));
+        TT_HIP(hipStreamWaitEvent(own, ctx->ev_in, 0));
+    }
+    TT_HIP(ctx->timer.begin(TT_KERNEL_BATCH, own));
+    TT_HIP(hipGraphLaunch(B.exec, own));
+    TT_HIP(ctx->timer.end(own));
+    if (ctx->stream != own) {
+        TT_HIP(hipEventRecord(ctx->ev_out, own));
+        TT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_out, 0));
+    }
+    return TT_OK;
+}
+
+namespace {
+
+// Hermite weights of the P list into the context (rows in list order).
+tt_status prepare_hermite(tt_context *ctx, const tt_pfunc *pf, int np, int n,
+                          int *herm_row)
+{
+    std::vector<int32_t> hkey;
+    for (int q = 0; q < np; q++) {
+        herm_row[q] = -1;
+        if (pf[q].code == TT_P_HERMITE) {
+            herm_row[q] = (int)(hkey.size() / 2);
+            hkey.push_back(pf[q].order);
+            hkey.push_back(pf[q].center);
+        }
+    }
+    if (!hkey.empty())
+        TT_TRY(ctx->ensure_hermite(n, hkey));
+    return TT_OK;
+}
+
+tt_status circus_vjp(tt_context *ctx, const float *d_sino, int n, int n_cols,
+                     int n_sino, const tt_pfunc *pf, int np, const float *d_dcircus,
+                     float *d_dsino)
+{
+    ttk::CircusVjpParams p = {};
+    p.sino = d_sino;
+    p.dcircus = d_dcircus;
+    p.dsino = d_dsino;
+    p.n = n;
+    p.ncols = n_cols;
+    p.nsino = n_sino;
+    p.np = np;
+    for (int q = 0; q < np; q++)
+        p.code[q] = pf[q].code;
+    TT_TRY(prepare_hermite(ctx, pf, np, n, p.herm));
+    p.herm_w = ctx->herm_w.as<double>();
+    TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS_VJP, ctx->stream));
+    TT_HIP(ttk::launch_circus_vjp(p, ctx->stream));
+    TT_HIP(ctx->timer.end(ctx->stream));
+    return TT_OK;
+}
+
+// Forward recompute (sinograms, medians, T3..T5 integrals) into the
+// context's adjoint workspace; returns the sinogram workspace.
+tt_status forward_for_adjoint(tt_context *ctx, const float *d_image, int n, int A,
+                              float step, const int *slot, int nt, float **sino_out)
+{
+    const size_t lines = (size_t)n * A;
+    TT_HIP(ctx->bw_sino.ensure(sizeof(float) * nt * lines));
+    TT_HIP(ctx->bw_z.ensure(sizeof(float2) * 3 * lines));
+    TT_HIP(ctx->med_x.ensure(sizeof(int32_t) * lines));
+    TT_HIP(ctx->med_s.ensure(sizeof(int32_t) * lines));
+    TT_TRY(ctx->ensure_trig(A, step));
+    TT_TRY(ctx->ensure_tab345(n));
+    TT_HIP(ctx->imgT.ensure(sizeof(float) * (size_t)n * n));
+    hipStream_t st = ctx->stream;
+    TT_HIP(ctx->timer.begin(TT_KERNEL_AUX, st));
+    TT_HIP(ttk::launch_transpose(d_image, ctx->imgT.as<float>(), n, st));
+    TT_HIP(ctx->timer.end(st));
+    ttk::SinoParams p = {};
+    p.img = d_image;
+    p.imgT = ctx->imgT.as<float>();
+    p.trig = ctx->trig.as<float2>();
+    p.tab345 = ctx->tab345.as<float2>();
+    p.sino = ctx->bw_sino.as<float>();
+    p.med_x = ctx->med_x.as<int32_t>();
+    p.med_s = ctx->med_s.as<int32_t>();
+    p.z345 = ctx->bw_z.as<float2>();
+    p.n = n;
+    p.A = A;
+    p.nb = (n + ttk::kWave - 1) / ttk::kWave;
+    p.items = p.nb * A;
+    for (int k = 0; k < 6; k++)
+        p.slot.s[k] = slot[k];
+    const bool do345 = slot[3] >= 0 || slot[4] >= 0 || slot[5] >= 0;
+    TT_HIP(ctx->timer.begin(TT_KERNEL_SINO_T05, st));
+    TT_HIP(ttk::launch_sino_t05(p, true, true, do345, st));
+    TT_HIP(ctx->timer.end(st));
+    *sino_out = ctx->bw_sino.as<float>();
+    return TT_OK;
+}
+
+tt_status image_vjp(tt_context *ctx, int n, int A, const int *slot,
+                    const float *d_dsino, float *d_dimage)
+{
+    const size_t lines = (size_t)n * A;
+    TT_HIP(ctx->bw_rec.ensure(sizeof(ttk::LineRecord) * lines));
+    hipStream_t st = ctx->stream;
+    ttk::RecordParams r = {};
+    r.dsino = d_dsino;
+    r.med_x = ctx->med_x.as<int32_t>();
+    r.med_s = ctx->med_s.as<int32_t>();
+    r.z345 = ctx->bw_z.as<float2>();
+    r.rec = ctx->bw_rec.as<ttk::LineRecord>();
+    r.n = n;
+    r.A = A;
+    for (int k = 0; k < 6; k++)
+        r.slot.s[k] = slot[k];
+    ttk::GatherParams g = {};
+    g.rec = r.rec;
+    g.trig = ctx->trig.as<float2>();
+    g.tab345 = ctx->tab345.as<float2>();
+    g.dimg = d_dimage;
+    g.n = n;
+    g.A = A;
+    g.do345 = slot[3] >= 0 || slot[4] >= 0 || slot[5] >= 0;
+    TT_HIP(ctx->timer.begin(TT_KERNEL_GATHER, st));
+    TT_HIP(ttk::launch_records(r, st));
+    TT_HIP(ttk::launch_gather(g, st));
+    TT_HIP(ctx->timer.end(st));
+    return TT_OK;
+}
+
+tt_status adjoint_slots(const int32_t *tcodes, int nt, int *slot)
+{
+    TT_TRY(check_tcodes(tcodes, nt));
+    TT_CHECK(nt >= 1, "at least one T-functional is required");
+    for (int k = 0; k < 6; k++)
+        slot[k] = -1;
+    for (int t = 0; t < nt; t++) {
+        if (tcodes[t] > 5)
+            return fail(TT_ERR_UNSUPPORTED, "the adjoint covers T0..T5 (got T%d)",
+                        tcodes[t]);
+        TT_CHECK(slot[tcodes[t]] < 0, "duplicate T-functional %d", tcodes[t]);
+        slot[tcodes[t]] = t;
+    }
+    return TT_OK;
+}
+
+}  // namespace
+
+TT_API tt_status tt_circus_backward(tt_context *ctx, const float *d_sino, int n,
+                                    int n_cols, int n_sino, const tt_pfunc *pf,
+                                    int np, const float *d_dcircus, float *d_dsino)
+{
+    TT_CHECK(ctx && d_sino && d_dsino, "NULL argument");
+    TT_CHECK(n >= 2 && n <= 1800, "sinogram height %d outside 2..1800", n);
+    TT_CHECK(n_cols >= 1 && n_sino >= 1, "n_cols and n_sino must be >= 1");
+    TT_TRY(check_pfuncs(pf, np, n));
+    TT_CHECK(np == 0 || d_dcircus, "d_dcircus is NULL");
+    TT_TRY(ctx->set_device());
+    if (np == 0)
+        TT_HIP(hipMemsetAsync(d_dsino, 0, sizeof(float) * (size_t)n * n_cols * n_sino,
+                              ctx->stream));
+    else
+        TT_TRY(circus_vjp(ctx, d_sino, n, n_cols, n_sino, pf, np, d_dcircus, d_dsino));
+    return TT_OK;
+}
+
+TT_API tt_status tt_sinograms_backward(tt_context *ctx, const float *d_image,
+                                       int n, int n_angles, float angle_step_deg,
+                                       const int32_t *tcodes, int nt,
+                                       const float *d_dsino, float *d_dimage)
+{
+    TT_CHECK(ctx && d_image && d_dsino && d_dimage, "NULL argument");
+    TT_CHECK(n >= 3 && n <= 1800, "padded side %d outside 3..1800", n);
+    TT_CHECK(n_angles >= 1, "n_angles must be >= 1");
+    TT_CHECK(angle_step_deg > 0.0f && isfinite(angle_step_deg),
+             "angle step must be positive and finite");
+    int slot[6];
+    TT_TRY(adjoint_slots(tcodes, nt, slot));
+    TT_TRY(ctx->set_device());
+    float *sino = nullptr;
+    TT_TRY(forward_for_adjoint(ctx, d_image, n, n_angles, angle_step_deg, slot, nt,
+                               &sino));
+    TT_TRY(image_vjp(ctx, n, n_angles, slot, d_dsino, d_dimage));
+    return TT_OK;
+}
+
+TT_API tt_status tt_backward(tt_context *ctx, const float *d_image, int n,
+                             int n_angles, float angle_step_deg,
+                             const int32_t *tcodes, int nt, const tt_pfunc *pf,
+                             int np, const float *d_dcircus, float *d_dimage)
+{
+    TT_CHECK(ctx && d_image && d_dimage, "NULL argument");
+    TT_CHECK(n >= 3 && n <= 1800, "padded side %d outside 3..1800", n);
+    TT_CHECK(n_angles >= 1, "n_angles must be >= 1");
+    TT_CHECK(angle_step_deg > 0.0f && isfinite(angle_step_deg),
+             "angle step must be positive and finite");
+    int slot[6];
+    TT_TRY(adjoint_slots(tcodes, nt, slot));
+    TT_TRY(check_pfuncs(pf, np, n));
+    TT_CHECK(np >= 1 && d_dcircus, "tt_backward needs P-functionals and d_dcircus");
+    TT_TRY(ctx->set_device());
+    float *sino = nullptr;
+    TT_TRY(forward_for_adjoint(ctx, d_image, n, n_angles, angle_step_deg, slot, nt,
+                               &sino));
+    const size_t lines = (size_t)n * n_angles;
+    TT_HIP(ctx->bw_dsino.ensure(sizeof(float) * nt * lines));
+    TT_TRY(circus_vjp(ctx, sino, n, n_angles, nt, pf, np, d_dcircus,
+                      ctx->bw_dsino.as<float>()));
+    TT_TRY(image_vjp(ctx, n, n_angles, slot, ctx->bw_dsino.as<float>(), d_dimage));
     return TT_OK;
 }
 

@@ -27,6 +27,7 @@ struct SinoParams {
     float *sino;          // [nt][A][n]
     int32_t *med_x;       // [A][n] or null
     int32_t *med_s;       // [A][n] or null
+    float2 *z345;         // [3][A][n] complex T3..T5 integrals, or null
     int n;
     int A;
     int nb;               // column blocks per angle: ceil(n / 64)
@@ -37,6 +38,7 @@ struct SinoParams {
 // One image of a batch (device array, ordered by first item).
 struct BatchImage {
     const float *img;     // n x n column-major
+    // (the batched forward writes neither medians nor T3..T5 integrals)
     const float *imgT;    // row-major copy (written by the batched transpose)
     float *sino;          // [nt][A][n]
     int32_t *med_x;
@@ -105,6 +107,55 @@ struct CircusParams {
     int code[kMaxP];
     int herm[kMaxP];      // row of herm_w for Hermite entries
 };
+
+// Adjoint.  Per-line record of the rotated-domain gradient of one angle's
+// column: G(row) = g0 + [row >= m] (g1 (row-m) + g2 (row-m)^2)
+//                + [row > sm] sum_t (alpha_t W_t.re[row-sm] + beta_t W_t.im[row-sm]).
+struct LineRecord {
+    float g0, g1, g2;
+    int m, sm;
+    float ab[6];          // alpha3, beta3, alpha4, beta4, alpha5, beta5
+    float pad;
+};
+
+struct RecordParams {
+    const float *dsino;   // [nt][A][n] sinogram gradient
+    const int32_t *med_x; // [A][n]
+    const int32_t *med_s; // [A][n]
+    const float2 *z345;   // [3][A][n]
+    LineRecord *rec;      // [A][n]
+    int n;
+    int A;
+    Slots slot;
+};
+
+struct GatherParams {
+    const LineRecord *rec;  // [A][n]
+    const float2 *trig;     // [A]
+    const float2 *tab345;   // [3][n]
+    float *dimg;            // n x n column-major
+    int n;
+    int A;
+    int do345;
+};
+
+// Circus vector-Jacobian product: one workgroup per sinogram column.
+struct CircusVjpParams {
+    const float *sino;    // [nsino][ncols][n]
+    const float *dcircus; // [nsino][np][ncols]
+    float *dsino;         // [nsino][ncols][n]
+    const double *herm_w; // [nherm][n]
+    int n;
+    int ncols;
+    int nsino;
+    int np;
+    int code[kMaxP];
+    int herm[kMaxP];
+};
+
+hipError_t launch_records(const RecordParams &p, hipStream_t s);
+hipError_t launch_gather(const GatherParams &p, hipStream_t s);
+hipError_t launch_circus_vjp(const CircusVjpParams &p, hipStream_t s);
 
 hipError_t launch_transpose(const float *in, float *out, int n, hipStream_t s);
 hipError_t launch_pad(const float *in, int rows, int cols, float *out, int n,

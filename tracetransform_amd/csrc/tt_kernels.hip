@@ -99,6 +99,7 @@ struct LineDst {
     float *sino;
     int32_t *med_x;
     int32_t *med_s;
+    float2 *z345;
     int n;
     int A;
     int tabn;
@@ -260,6 +261,12 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
             p.sino[p.slot.s[4] * tstride + line] = (float)hypot(r4, i4);
         if (do5)
             p.sino[p.slot.s[5] * tstride + line] = (float)hypot(r5, i5);
+        if (p.z345) {
+            const size_t plane = (size_t)p.A * n;
+            p.z345[line] = float2{(float)r3, (float)i3};
+            p.z345[plane + line] = float2{(float)r4, (float)i4};
+            p.z345[2 * plane + line] = float2{(float)r5, (float)i5};
+        }
     }
 }
 
@@ -283,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void sino_t05_kernel(SinoParams p)
     if (!active)
         col = p.n - 1;
     const float2 cs = p.trig[a];
-    const LineDst dst = {p.sino, p.med_x, p.med_s, p.n, p.A, p.n, p.slot};
+    const LineDst dst = {p.sino, p.med_x, p.med_s, p.z345, p.n, p.A, p.n, p.slot};
     // Lanes step (dx, dy) = (c, -s) per column: read the copy whose fast axis
     // is the one the lanes mostly move along.
     if (fabsf(cs.x) > fabsf(cs.y))
@@ -336,7 +343,8 @@ __global__ __launch_bounds__(kBlock) void sino_t05_batched_kernel(BatchSinoParam
         if (!active)
             col = im.n - 1;
         const float2 cs = p.trig[a];
-        const LineDst dst = {im.sino, im.med_x, im.med_s, im.n, p.A, p.tabn, p.slot};
+        const LineDst dst = {im.sino, im.med_x, im.med_s, nullptr, im.n, p.A, p.tabn,
+                             p.slot};
         if (fabsf(cs.x) > fabsf(cs.y))
             sino_line<true, NEED_X, NEED_S, DO345>(dst, im.imgT, s_tab, a, col,
                                                   active, cs.x, cs.y);
@@ -766,98 +774,448 @@ __device__ void bitonic_lds(K *s, int m)
     }
 }
 
+// 64-bit broadcast from one lane.
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src_lane)
+{
+    const int lo = __shfl((int)(uint32_t)v, src_lane, kWave);
+    const int hi = __shfl((int)(uint32_t)(v >> 32), src_lane, kWave);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// 64-bit shuffle (two 32-bit halves).
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m)
+{
+    const int lo = __shfl_xor((int)(uint32_t)v, m, kWave);
+    const int hi = __shfl_xor((int)(uint32_t)(v >> 32), m, kWave);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// Ascending bitonic sort of 64*KPL keys held by one wave in registers,
+// blocked layout: lane l holds positions l*KPL .. l*KPL+KPL-1.  Strides below
+// KPL exchange registers inside a lane; larger strides exchange with lane
+// l ^ (stride / KPL) through shuffles.  Wave-synchronous: no barriers.
+template <int KPL>
+__device__ __forceinline__ void wave_bitonic_sort(uint64_t (&k)[KPL], int lane)
+{
+    constexpr int M = kWave * KPL;
+    for (int size = 2; size <= M; size <<= 1) {
+        // strides that cross lanes (runtime loop; the register index is static)
+        for (int stride = size >> 1; stride >= KPL; stride >>= 1) {
+            const int lx = stride / KPL;
+            const bool lower = (lane & lx) == 0;
+#pragma unroll
+            for (int r = 0; r < KPL; r++) {
+                const int pos = lane * KPL + r;
+                const bool up = (pos & size) == 0;
+                const uint64_t o = shfl_xor64(k[r], lx);
+                const bool keep_min = lower == up;
+                const uint64_t mn = k[r] < o ? k[r] : o;
+                const uint64_t mx = k[r] < o ? o : k[r];
+                k[r] = keep_min ? mn : mx;
+            }
+        }
+        // strides inside a lane
+#pragma unroll
+        for (int stride = KPL / 2; stride > 0; stride >>= 1) {
+            if (stride < size) {
+#pragma unroll
+                for (int r = 0; r < KPL; r++) {
+                    const int r2 = r ^ stride;
+                    if (r2 > r) {
+                        const int pos = lane * KPL + r;
+                        const bool up = (pos & size) == 0;
+                        const uint64_t a = k[r], b = k[r2];
+                        const bool sw = (a > b) == up;
+                        k[r] = sw ? b : a;
+                        k[r2] = sw ? a : b;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Eigen packet-order sum of x[0..n) in a wave's LDS slice: lanes 0..7 run the
+// stride-8 chains, lane 0 combines; the result is returned to every lane.
+__device__ __forceinline__ float eigen_sum_wave(const float *x, int n, int lane)
+{
+    const int a4 = n & ~3, a8 = n & ~7;
+    float r = 0.0f;
+    if (lane < 8 && n >= 8) {
+        int i = lane;
+        for (; i + 24 < a8; i += 32) {
+            const float v0 = x[i], v1 = x[i + 8], v2 = x[i + 16], v3 = x[i + 24];
+            r += v0;
+            r += v1;
+            r += v2;
+            r += v3;
+        }
+        for (; i < a8; i += 8)
+            r += x[i];
+    }
+    float ch[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        ch[j] = __shfl(r, j, kWave);
+    float res = 0.0f;
+    if (lane == 0) {
+        if (a4 == 0) {
+            res = n > 0 ? x[0] : 0.0f;
+            for (int i = 1; i < n; i++)
+                res += x[i];
+        } else {
+            float ax[4];
+            for (int j = 0; j < 4; j++)
+                ax[j] = n >= 8 ? ch[j] + ch[j + 4] : x[j];
+            if (n >= 8 && a4 > a8)
+                for (int j = 0; j < 4; j++)
+                    ax[j] += x[a8 + j];
+            res = (ax[0] + ax[2]) + (ax[1] + ax[3]);
+            for (int i = a4; i < n; i++)
+                res += x[i];
+        }
+    }
+    return __shfl(res, 0, kWave);
+}
+
+// One wave per line (persistent over lines).  T6: keys (orderable(w) << 32 |
+// r) for w = float(r) * x[sm + r]; after the sort each lane re-samples the
+// rows its keys point at to form sqrt(x[sm + idx[k]]).  T7: keys
+// (orderable(x[m + r]) << 32 | r).  The sorted square roots go to the wave's
+// LDS slice for the Eigen-order sum and findWeightedMedian's walk.
+template <int KPL>
 __global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
 {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int M = kWave * KPL;
+    __shared__ __align__(16) float s_sq_all[(kBlock / kWave) * M];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = wave_id();
+    float *s_sq = s_sq_all + wv * M;
     const int n = p.n;
-    int m2 = 1;
-    while (m2 < n)
-        m2 <<= 1;
-    const int n4 = (n + 3) & ~3;
-    uint64_t *s_key = reinterpret_cast<uint64_t *>(smem_raw);      // m2
-    float *s_col = reinterpret_cast<float *>(s_key + m2);           // n4
-    float *s_sq = s_col + n4;                                       // n
-    __shared__ float s_chain[8];
-    __shared__ float s_sum;
-
     const float o = (float)((n - 1) / 2.0);
     const float lim = (float)(n - 1);
     const size_t tstride = (size_t)p.A * n;
     const int lines = p.A * n;
-    for (int line = blockIdx.x; line < lines; line += gridDim.x) {
+    const int waves = gridDim.x * (kBlock / kWave);
+    for (int line = blockIdx.x * (kBlock / kWave) + wv; line < lines; line += waves) {
         const int a = line / n;
         const int col = line - a * n;
         const float2 cs = p.trig[a];
         const float px = (float)col - o;
         const float pxc = px * cs.x, pxns = px * (-cs.y);
-        // lanes step (dx, dy) = (s, c) per row
-        if (fabsf(cs.y) > fabsf(cs.x)) {
-            for (int r = threadIdx.x; r < n; r += kBlock)
-                s_col[r] = rot_sample<true>(p.imgT, n, lim, o, pxc, pxns, cs.x,
-                                            cs.y, (float)r - o);
-        } else {
-            for (int r = threadIdx.x; r < n; r += kBlock)
-                s_col[r] = rot_sample<false>(p.img, n, lim, o, pxc, pxns, cs.x,
-                                             cs.y, (float)r - o);
-        }
-        __syncthreads();
-
-        if (p.slot6 >= 0) {
-            const int sm = p.med_s[line];
-            const int L = n - sm;
-            int mL = 1;
-            while (mL < L)
-                mL <<= 1;
-            for (int r = threadIdx.x; r < mL; r += kBlock) {
-                uint64_t k = ~0ull;
-                if (r < L) {
-                    const float w = (float)r * s_col[sm + r];
-                    k = ((uint64_t)ord_key(w) << 32) | (uint32_t)r;
+        const bool tr = fabsf(cs.y) > fabsf(cs.x);
+        const float *src = tr ? p.imgT : p.img;
+        auto sample = [&](int row) -> float {
+            return tr ? rot_sample<true>(src, n, lim, o, pxc, pxns, cs.x, cs.y,
+                                         (float)row - o)
+                      : rot_sample<false>(src, n, lim, o, pxc, pxns, cs.x, cs.y,
+                                          (float)row - o);
+        };
+        for (int which = 0; which < 2; which++) {
+            const int slotw = which == 0 ? p.slot6 : p.slot7;
+            if (slotw < 0)
+                continue;
+            const int base = which == 0 ? p.med_s[line] : p.med_x[line];
+            const int L = n - base;
+            uint64_t key[KPL];
+#pragma unroll
+            for (int r = 0; r < KPL; r++) {
+                const int i = lane * KPL + r;
+                key[r] = ~0ull;
+                if (i < L) {
+                    const float x = sample(base + i);
+                    const float kv = which == 0 ? (float)i * x : x;
+                    key[r] = ((uint64_t)ord_key(kv) << 32) | (uint32_t)i;
                 }
-                s_key[r] = k;
             }
-            __syncthreads();
-            bitonic_lds(s_key, mL);
-            for (int r = threadIdx.x; r < L; r += kBlock)
-                s_sq[r] = sqrtf(s_col[sm + (int)(uint32_t)s_key[r]]);
-            __syncthreads();
-            eigen_sum_block(s_sq, L, s_chain, &s_sum);
-            if (threadIdx.x == 0) {
-                const int k = weighted_median_walk(s_sq, L, s_sum);
-                const int src = (int)(uint32_t)s_key[k];
-                p.sino[p.slot6 * tstride + line] = (float)src * s_col[sm + src];
-                if (p.t6_rank)
-                    p.t6_rank[line] = k;
-                if (p.t6_src)
-                    p.t6_src[line] = src;
+            wave_bitonic_sort<KPL>(key, lane);
+#pragma unroll
+            for (int r = 0; r < KPL; r++) {
+                const int i = lane * KPL + r;
+                if (i < L) {
+                    const float x = which == 0 ? sample(base + (int)(uint32_t)key[r])
+                                               : ord_val((uint32_t)(key[r] >> 32));
+                    s_sq[i] = sqrtf(x);
+                }
             }
-            __syncthreads();
-        }
-
-        if (p.slot7 >= 0) {
-            const int m = p.med_x[line];
-            const int L = n - m;
-            int mL = 1;
-            while (mL < L)
-                mL <<= 1;
-            uint32_t *s_k32 = reinterpret_cast<uint32_t *>(s_key);
-            for (int r = threadIdx.x; r < mL; r += kBlock)
-                s_k32[r] = r < L ? ord_key(s_col[m + r]) : 0xFFFFFFFFu;
-            __syncthreads();
-            bitonic_lds(s_k32, mL);
-            for (int r = threadIdx.x; r < L; r += kBlock)
-                s_sq[r] = sqrtf(ord_val(s_k32[r]));
-            __syncthreads();
-            eigen_sum_block(s_sq, L, s_chain, &s_sum);
-            if (threadIdx.x == 0) {
-                const int k = weighted_median_walk(s_sq, L, s_sum);
-                p.sino[p.slot7 * tstride + line] = ord_val(s_k32[k]);
-                if (p.t7_rank)
-                    p.t7_rank[line] = k;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const float S = eigen_sum_wave(s_sq, L, lane);
+            int k = 0;
+            if (lane == 0)
+                k = weighted_median_walk(s_sq, L, S);
+            k = __shfl(k, 0, kWave);
+            // the key at sorted position k sits in lane k / KPL, register k % KPL
+            uint64_t kk = 0;
+#pragma unroll
+            for (int r = 0; r < KPL; r++)
+                kk = (r == k % KPL) ? key[r] : kk;
+            kk = shfl64(kk, k / KPL);
+            if (lane == 0) {
+                if (which == 0) {
+                    const int srcr = (int)(uint32_t)kk;
+                    p.sino[slotw * tstride + line] = ord_val((uint32_t)(kk >> 32));
+                    if (p.t6_rank)
+                        p.t6_rank[line] = k;
+                    if (p.t6_src)
+                        p.t6_src[line] = srcr;
+                } else {
+                    p.sino[slotw * tstride + line] = ord_val((uint32_t)(kk >> 32));
+                    if (p.t7_rank)
+                        p.t7_rank[line] = k;
+                }
             }
-            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint kernels.  The medians, the T3..T5 integrals and the sample
+// positions are those of a forward recompute on the same image (held fixed,
+// as the medians are piecewise constant in the image).
+// ---------------------------------------------------------------------------
+
+// One LineRecord per (angle, column): folds the sinogram gradient of every
+// requested T0..T5 into the coefficients of G(row).
+__global__ __launch_bounds__(kBlock) void records_kernel(RecordParams p)
+{
+    const size_t lines = (size_t)p.A * p.n;
+    const size_t plane = lines;
+    for (size_t line = (size_t)blockIdx.x * kBlock + threadIdx.x; line < lines;
+         line += (size_t)gridDim.x * kBlock) {
+        LineRecord r = {};
+        r.m = p.med_x ? p.med_x[line] : 0;
+        r.sm = p.med_s ? p.med_s[line] : 0;
+        if (p.slot.s[0] >= 0)
+            r.g0 = p.dsino[p.slot.s[0] * plane + line];
+        if (p.slot.s[1] >= 0)
+            r.g1 = p.dsino[p.slot.s[1] * plane + line];
+        if (p.slot.s[2] >= 0)
+            r.g2 = p.dsino[p.slot.s[2] * plane + line];
+        for (int k = 0; k < 3; k++) {
+            const int sl = p.slot.s[3 + k];
+            if (sl < 0)
+                continue;
+            const float2 z = p.z345[k * plane + line];
+            const float mag = hypotf(z.x, z.y);
+            if (mag > 0.0f) {
+                const float g = p.dsino[sl * plane + line] / mag;
+                r.ab[2 * k] = g * z.x;
+                r.ab[2 * k + 1] = g * z.y;
+            }
+        }
+        p.rec[line] = r;
+    }
+}
+
+// Transposed bilinear rotation as a gather: thread = one source pixel
+// (row Y, col X).  For every angle it enumerates the rotated samples whose
+// 2x2 footprint can cover the pixel (the rotated pixels within sqrt(2) of its
+// inverse image), recomputes each sample's forward float coordinates with
+// the exact operations of rot_sample, keeps those whose taps include (Y, X),
+// and accumulates G(row) times the tap weight.  Deterministic, no atomics.
+__global__ __launch_bounds__(kBlock) void gather_kernel(GatherParams p)
+{
+    extern __shared__ float2 s_tab[];
+    const int n = p.n;
+    if (p.do345) {
+        for (int i = threadIdx.x; i < 3 * n; i += kBlock)
+            s_tab[i] = p.tab345[i];
+    }
+    __syncthreads();
+    const int pix = blockIdx.x * kBlock + threadIdx.x;
+    if (pix >= n * n)
+        return;
+    const int X = pix / n;      // column
+    const int Y = pix - X * n;  // row
+    const float o = (float)((n - 1) / 2.0);
+    const float lim = (float)(n - 1);
+    const float u = (float)X - o, v = (float)Y - o;
+    const float R = 1.4163f;
+    double acc = 0.0;
+    for (int a = 0; a < p.A; a++) {
+        const float2 cs = p.trig[a];
+        const float c = cs.x, s = cs.y;
+        const float pxc = u * c - v * s;   // inverse rotation of the pixel
+        const float pyc = u * s + v * c;
+        int c0 = (int)ceilf(pxc - R + o), c1 = (int)floorf(pxc + R + o);
+        int r0 = (int)ceilf(pyc - R + o), r1 = (int)floorf(pyc + R + o);
+        c0 = c0 < 0 ? 0 : c0;
+        r0 = r0 < 0 ? 0 : r0;
+        c1 = c1 > n - 1 ? n - 1 : c1;
+        r1 = r1 > n - 1 ? n - 1 : r1;
+        const LineRecord *rec = p.rec + (size_t)a * n;
+        for (int col = c0; col <= c1; col++) {
+            const float px = (float)col - o;
+            const float pxcc = px * c;
+            const float pxns = px * (-s);
+            for (int row = r0; row <= r1; row++) {
+                const float py = (float)row - o;
+                float x = pxcc + py * s;
+                float y = pxns + py * c;
+                x = x + o;
+                y = y + o;
+                if (!((x >= 0.0f) && (x < lim) && (y >= 0.0f) && (y < lim)))
+                    continue;
+                const float xi = __builtin_truncf(x), yi = __builtin_truncf(y);
+                const int Xi = (int)xi, Yi = (int)yi;
+                if ((Xi != X && Xi != X - 1) || (Yi != Y && Yi != Y - 1))
+                    continue;
+                const float fx = x - xi, fy = y - yi;
+                const float wx = Xi == X ? 1.0f - fx : fx;
+                const float wy = Yi == Y ? 1.0f - fy : fy;
+                const LineRecord r = rec[col];
+                float G = r.g0;
+                if (row >= r.m) {
+                    const float d = (float)(row - r.m);
+                    G = fmaf(d, fmaf(r.g2, d, r.g1), G);
+                }
+                if (p.do345 && row > r.sm) {
+                    const int q = row - r.sm;
+                    const float2 w3 = s_tab[q], w4 = s_tab[n + q], w5 = s_tab[2 * n + q];
+                    G = fmaf(r.ab[0], w3.x, G);
+                    G = fmaf(r.ab[1], w3.y, G);
+                    G = fmaf(r.ab[2], w4.x, G);
+                    G = fmaf(r.ab[3], w4.y, G);
+                    G = fmaf(r.ab[4], w5.x, G);
+                    G = fmaf(r.ab[5], w5.y, G);
+                }
+                acc = fma((double)G, (double)wx * (double)wy, acc);
+            }
+        }
+    }
+    p.dimg[pix] = (float)acc;
+}
+
+// dS = J_P^T dC for every sinogram column (P1: sign differences; P2: one-hot
+// at the source of the weighted-median rank of the stable sort; P3: the
+// derivative of sum_p w_p |X_q(p)|^4 / N^4, an inverse-DFT-shaped sum;
+// Hermite: its weights).
+__global__ __launch_bounds__(kBlock) void circus_vjp_kernel(CircusVjpParams p)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n = p.n;
+    const int n4 = (n + 3) & ~3;
+    int m2 = 1;
+    while (m2 < n)
+        m2 <<= 1;
+    uint64_t *s_key = reinterpret_cast<uint64_t *>(smem_raw);     // m2
+    float *s_x = reinterpret_cast<float *>(s_key + m2);            // n4
+    float *s_sq = s_x + n4;                                        // n4
+    float *s_ds = s_sq + n4;                                       // n4
+    float2 *s_X = reinterpret_cast<float2 *>(s_ds + n4);           // n/2+1
+    float *s_ck = reinterpret_cast<float *>(s_X + (n / 2 + 1));    // n/2+1
+    float2 *s_tw = reinterpret_cast<float2 *>(s_ck + ((n / 2 + 2) & ~1));  // n
+    __shared__ float s_chain[8];
+    __shared__ float s_sum;
+    __shared__ int s_src;
+
+    const int colg = blockIdx.x;
+    const int sg = colg / p.ncols;
+    const int col = colg - sg * p.ncols;
+    const float *x = p.sino + (size_t)colg * n;
+    bool p3 = false;
+    for (int q = 0; q < p.np; q++)
+        p3 = p3 || p.code[q] == 3;
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+        s_x[i] = x[i];
+        s_ds[i] = 0.0f;
+        if (p3) {
+            double sn, cn;
+            sincospi(2.0 * (double)i / (double)n, &sn, &cn);
+            s_tw[i] = float2{(float)cn, (float)sn};
+        }
+    }
+    __syncthreads();
+    for (int q = 0; q < p.np; q++) {
+        const float g = p.dcircus[((size_t)sg * p.np + q) * p.ncols + col];
+        const int code = p.code[q];
+        if (code == 1) {
+            for (int j = threadIdx.x; j < n; j += kBlock) {
+                float d = 0.0f;
+                if (j >= 1) {
+                    const float u = s_x[j - 1] - s_x[j];
+                    d -= (float)((u > 0.0f) - (u < 0.0f));
+                }
+                if (j <= n - 2) {
+                    const float u = s_x[j] - s_x[j + 1];
+                    d += (float)((u > 0.0f) - (u < 0.0f));
+                }
+                s_ds[j] += g * d;
+            }
+        } else if (code == 0) {
+            const double *w = p.herm_w + (size_t)p.herm[q] * n;
+            for (int j = threadIdx.x; j < n; j += kBlock)
+                s_ds[j] += (float)((double)g * w[j]);
+        } else if (code == 2) {
+            for (int i = threadIdx.x; i < m2; i += kBlock)
+                s_key[i] = i < n ? (((uint64_t)ord_key(s_x[i]) << 32) | (uint32_t)i)
+                                 : ~0ull;
+            __syncthreads();
+            bitonic_lds(s_key, m2);
+            for (int i = threadIdx.x; i < n; i += kBlock)
+                s_sq[i] = s_x[(int)(uint32_t)s_key[i]];
+            __syncthreads();
+            eigen_sum_block(s_sq, n, s_chain, &s_sum);
+            if (threadIdx.x == 0) {
+                const int m = weighted_median_walk(s_sq, n, s_sum);
+                s_src = (int)(uint32_t)s_key[m];
+            }
+            __syncthreads();
+            if (threadIdx.x == 0)
+                s_ds[s_src] += g;
+        } else {
+            const int half = n / 2;
+            for (int k = threadIdx.x; k <= half; k += kBlock) {
+                double re = 0.0, im = 0.0;
+                int idx = 0;
+                for (int j = 0; j < n; j++) {
+                    const float2 t = s_tw[idx];
+                    idx += k;
+                    idx -= idx >= n ? n : 0;
+                    re = fma((double)s_x[j], (double)t.x, re);
+                    im = fma(-(double)s_x[j], (double)t.y, im);
+                }
+                s_X[k] = float2{(float)re, (float)im};
+                // trapz weights of the float abscissae, folded onto k
+                double w = 0.0;
+                for (int side = 0; side < 2; side++) {
+                    const int qq = side == 0 ? k : n - k;
+                    if (qq >= n || (side == 1 && (qq == k || k == 0)))
+                        continue;
+                    const double l = (float)(-1 + qq * 2.0 / (n - 1));
+                    if (qq >= 1)
+                        w += 0.5 * (l - (double)(float)(-1 + (qq - 1) * 2.0 / (n - 1)));
+                    if (qq <= n - 2)
+                        w += 0.5 * ((double)(float)(-1 + (qq + 1) * 2.0 / (n - 1)) - l);
+                }
+                const double n4d = (double)n * n * (double)n * n;
+                s_ck[k] = (float)(w * 4.0 * (re * re + im * im) / n4d);
+            }
+            __syncthreads();
+            for (int j = threadIdx.x; j < n; j += kBlock) {
+                double acc = 0.0;
+                int idx = 0;
+                for (int k = 0; k <= half; k++) {
+                    const float2 t = s_tw[idx];
+                    idx += j;
+                    idx -= idx >= n ? n : 0;
+                    const float2 X = s_X[k];
+                    acc = fma((double)s_ck[k],
+                              (double)X.x * (double)t.x - (double)X.y * (double)t.y, acc);
+                }
+                s_ds[j] += (float)((double)g * acc);
+            }
+        }
+        __syncthreads();
+    }
+    float *out = p.dsino + (size_t)colg * n;
+    for (int i = threadIdx.x; i < n; i += kBlock)
+        out[i] = s_ds[i];
 }
 
 }  // namespace
@@ -973,19 +1331,66 @@ hipError_t launch_sino_t05_batched(const BatchSinoParams &p, bool need_x,
 
 hipError_t launch_sino_sort(const SortTParams &p, hipStream_t s)
 {
-    int m2 = 1;
-    while (m2 < p.n)
-        m2 <<= 1;
-    const size_t n4 = ((size_t)p.n + 3) & ~(size_t)3;
-    const size_t shm = sizeof(uint64_t) * (size_t)m2 + sizeof(float) * (n4 + (size_t)p.n);
     const int lines = p.A * p.n;
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     int blocks = cus * 8;
-    if (blocks > lines)
-        blocks = lines;
-    sino_sort_kernel<<<blocks, kBlock, shm, s>>>(p);
+    const int need = (lines + kBlock / kWave - 1) / (kBlock / kWave);
+    if (blocks > need)
+        blocks = need;
+#define TT_SORT(K) sino_sort_kernel<K><<<blocks, kBlock, 0, s>>>(p)
+    if (p.n <= 64)
+        TT_SORT(1);
+    else if (p.n <= 128)
+        TT_SORT(2);
+    else if (p.n <= 256)
+        TT_SORT(4);
+    else if (p.n <= 512)
+        TT_SORT(8);
+    else if (p.n <= 1024)
+        TT_SORT(16);
+    else if (p.n <= 2048)
+        TT_SORT(32);
+    else
+        return hipErrorInvalidValue;
+#undef TT_SORT
+    return hipGetLastError();
+}
+
+}  // namespace ttk
+
+namespace ttk {
+
+hipError_t launch_records(const RecordParams &p, hipStream_t s)
+{
+    const size_t lines = (size_t)p.A * p.n;
+    int blocks = (int)((lines + kBlock - 1) / kBlock);
+    if (blocks > 8192)
+        blocks = 8192;
+    records_kernel<<<blocks, kBlock, 0, s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const GatherParams &p, hipStream_t s)
+{
+    const int blocks = (p.n * p.n + kBlock - 1) / kBlock;
+    const size_t shm = p.do345 ? sizeof(float2) * 3 * (size_t)p.n : 0;
+    gather_kernel<<<blocks, kBlock, shm, s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_circus_vjp(const CircusVjpParams &p, hipStream_t s)
+{
+    int m2 = 1;
+    while (m2 < p.n)
+        m2 <<= 1;
+    const size_t n4 = ((size_t)p.n + 3) & ~(size_t)3;
+    const size_t h = (size_t)p.n / 2 + 1;
+    const size_t shm = sizeof(uint64_t) * m2 + sizeof(float) * 3 * n4 +
+                       sizeof(float2) * h + sizeof(float) * ((h + 1) & ~(size_t)1) +
+                       sizeof(float2) * (size_t)p.n;
+    circus_vjp_kernel<<<p.ncols * p.nsino, kBlock, shm, s>>>(p);
     return hipGetLastError();
 }
 
