@@ -144,7 +144,26 @@ def parse_args():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune", type=int, default=1)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB "
+                         "(calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """One timed device copy of a known size, for scaling HBM counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.rand(n, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n,
+            "copy_ms": round(dt * 1e3, 4), "copy_gbs": round(8 * n / dt / 1e9, 1)}
 
 
 def read_kernels(ctx, steps):
@@ -300,6 +319,9 @@ def main():
               "shards by image or angle without a collective in the forward pass, "
               "see DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy_mib:
+        print(json.dumps(calibrate_copy(args.calibrate_copy_mib)), flush=True)
+        return
     w = WORKLOADS[args.workload]
     out = run_batched(args, w) if "batch" in w else run_single(args, w)
     print(json.dumps(out), flush=True)

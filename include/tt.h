@@ -130,6 +130,33 @@ TT_API tt_status tt_circus_host(tt_context *ctx, const float *sino, int n,
                                 int n_cols, const tt_pfunc *pf, int np,
                                 float *circus);
 
+/* ---- steps either side of the path -------------------------------------- */
+
+/* Replaces resize (c++/src/auxiliary.cpp:198-221), used by the orthonormal
+ * mode (c++/src/transform.cpp:35-41): in_rows x in_cols -> rows x cols,
+ * column-major, one-pixel zero border. */
+TT_API tt_status tt_resize(tt_context *ctx, const float *d_in, int in_rows,
+                           int in_cols, float *d_out, int rows, int cols);
+
+/* Replaces zscore (c++/src/auxiliary.cpp:305-318) on `count` consecutive
+ * vectors of length n (e.g. the circus functions [nt][np][A] -> the
+ * signature columns of c++/src/transform.cpp:93-102). */
+TT_API tt_status tt_zscore(tt_context *ctx, const float *d_in, int n, int count,
+                           float *d_out);
+
+/* Replaces nearest_orthonormal_sinogram (c++/src/circus.cpp:68-102): column
+ * medians, alignment to the sinogram centre, then the polar factor
+ * U[:, :A] V^T of the aligned (n + |max| + |min|) x A matrix, computed in
+ * double (rocSOLVER gesvd, rocBLAS gemm) and stored as float.  d_out holds
+ * out_capacity_rows x A floats; *out_rows receives the aligned height and
+ * *center the Hermite centre (c++/src/transform.cpp:78-84).  The aligned
+ * height must be at least A (the reference's U block needs it). */
+TT_API tt_status tt_nearest_orthonormal_sinogram(tt_context *ctx,
+                                                 const float *d_sino, int n,
+                                                 int n_cols, float *d_out,
+                                                 int out_capacity_rows,
+                                                 int *out_rows, int *center);
+
 /* ---- batched forward (no reference counterpart) ------------------------- */
 
 /* One image of a batch; all pointers are device memory. */
@@ -184,6 +211,23 @@ TT_API tt_status tt_backward(tt_context *ctx, const float *d_image, int n,
                              const int32_t *tcodes, int nt, const tt_pfunc *pf,
                              int np, const float *d_dcircus, float *d_dimage);
 
+/* One image of a batched backward; all pointers are device memory. */
+typedef struct tt_image_grad {
+    const float *d_image;    /* n x n padded image, column-major            */
+    int32_t n;               /* padded side, 3..1800                         */
+    const float *d_dcircus;  /* nt x np x A upstream gradient                */
+    float *d_dimage;         /* n x n gradient out                           */
+} tt_image_grad;
+
+/* tt_backward for every image of a batch, each stage one launch over the
+ * batch (forward recompute, circus VJP, line records, transposed-rotation
+ * gather), captured into a HIP graph on the first call for a batch and
+ * replayed for the same batch.  T0..T5, P1..P3. */
+TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *images,
+                                     int nimg, int n_angles, float angle_step_deg,
+                                     const int32_t *tcodes, int nt,
+                                     const tt_pfunc *pf, int np);
+
 /* ---- kernel timing ------------------------------------------------------ */
 
 /* No reference counterpart (the reference times whole getTransform calls,
@@ -199,7 +243,8 @@ enum {
     TT_KERNEL_BATCH = 4,      /* one replay of the batched graph            */
     TT_KERNEL_GATHER = 5,     /* adjoint: transposed rotation (+ records)   */
     TT_KERNEL_CIRCUS_VJP = 6, /* adjoint of the P-functionals               */
-    TT_KERNEL_KINDS = 7
+    TT_KERNEL_BATCH_BACKWARD = 7, /* one replay of the batched backward graph */
+    TT_KERNEL_KINDS = 8
 };
 TT_API tt_status tt_profile_enable(tt_context *ctx, int enable);
 TT_API tt_status tt_profile_read(tt_context *ctx, int kind, double *total_ms,

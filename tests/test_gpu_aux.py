@@ -1,0 +1,69 @@
+"""GPU tests of the steps either side of the path (resize, zscore) and of
+the nearest orthonormal sinogram (NOS) used by the Hermite P-functionals.
+
+Bar: resize bit-exact against the oracle's restatement (same float
+operations); zscore within 1e-6 normwise (the double variance is summed in
+another order); NOS: column offsets, aligned height and Hermite centre exact,
+values within 1e-6 of numpy's FP64 polar factor on well-conditioned inputs
+(the polar factor of a rank-deficient matrix is not unique, so sinograms of
+smooth images are not compared elementwise).
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import normwise_err, padded
+from tracetransform_amd import core, inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x, np.float32)).cuda()
+
+
+@pytest.mark.parametrize("src,dst", [((256, 256), (255, 255)), ((100, 60), (255, 255)),
+                                     ((1024, 1024), (255, 255)), ((40, 40), (97, 31))])
+def test_resize_matches_oracle(oracle, gpu, src, dst):
+    rows, cols = src
+    img = inputs.blob_image(cols, 7, rows=rows).ravel()
+    g = core.resize(dev(img.reshape(cols, rows)), rows, cols, dst[0], dst[1])
+    o = oracle.resize(img, rows, cols, dst[0], dst[1])
+    assert np.array_equal(g.cpu().numpy().ravel(), o)
+
+
+def test_zscore_matches_oracle(oracle, gpu):
+    x = np.random.default_rng(3).random((7, 360), dtype=np.float32) * 5 + 1
+    g = core.zscore(dev(x)).cpu().numpy()
+    for k in range(7):
+        assert normwise_err(g[k], oracle.zscore(x[k])) <= 1e-6
+
+
+def test_nos_matches_fp64_polar_factor(oracle, gpu):
+    rng = np.random.default_rng(5)
+    for n, A in ((400, 360), (255, 120), (91, 90)):
+        sino = (rng.random((A, n)) + 0.05).astype(np.float32)
+        nos, center = core.nearest_orthonormal_sinogram(dev(sino))
+        ref, rc = oracle.nearest_orthonormal_sinogram(sino)
+        print(f"[nos] n={n} A={A} rows={ref.shape[1]} center={rc}", flush=True)
+        assert center == rc and tuple(nos.shape) == ref.shape
+        err = normwise_err(nos.cpu().numpy(), ref)
+        assert err <= 1e-6, err
+        q = nos.double()
+        gram = (q @ q.t()).cpu().numpy()       # columns orthonormal
+        assert np.abs(gram - np.eye(A)).max() <= 1e-5
+
+
+def test_transformer_orthonormal_pipeline(oracle, gpu):
+    import tracetransform_amd as tt
+    img = torch.from_numpy(inputs.uniform_image(64, 9).T.copy())
+    pf = [tt.PFunctionalWrapper.parse(s) for s in ("H1", "H3")]
+    tr = tt.Transformer(img, "x", angle_stepsize=4, orthonormal=True)
+    n = tr.image.shape[0]
+    assert n == oracle.pad_geometry(64, 64)[0]  # ceil(ceil(90)/sqrt 2) = 64
+    sig = tr.getTransform(["0", "3"], pf)
+    assert tuple(sig.shape) == (90, 4) and torch.isfinite(sig).all()
+    assert pf[0].arguments.center is not None
+    # column 0 is zscore(H1 circus of T0 on its NOS): mean 0, std 1
+    col = sig[:, 0].double()
+    assert abs(col.mean().item()) < 1e-5 and abs(col.std().item() - 1) < 1e-5

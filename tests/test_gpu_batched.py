@@ -114,3 +114,24 @@ def test_batched_rejects_unsupported(gpu):
         core.Batch([img], 4, 90.0, [0], [(0, 1, 3)]).run()
     with pytest.raises(ValueError):
         core.Batch([img], 4, 90.0, [0, 0], []).run()
+
+
+def test_batched_backward_matches_single_image(oracle, gpu):
+    sides = [48, 20, 75, 33]
+    imgs = make_batch(oracle, sides, 500)
+    codes, pf, A, step = [0, 3, 4, 5], P123, 72, 5.0
+    bb = core.BatchBackward(imgs, A, step, codes, pf)
+    rng = np.random.default_rng(6)
+    for rep in range(3):
+        for dc in bb.dcircus:
+            dc.copy_(torch.from_numpy(
+                rng.standard_normal(tuple(dc.shape)).astype(np.float32)))
+        out = bb.run()
+        torch.cuda.synchronize()
+        for k, img in enumerate(imgs):
+            ref = core.backward(img, A, step, codes, pf, bb.dcircus[k])
+            torch.cuda.synchronize()
+            assert torch.equal(out[k], ref), (rep, k)
+        for im in imgs:  # eager work between replays
+            im.mul_(0.9)
+        torch.cuda.synchronize()

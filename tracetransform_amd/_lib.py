@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(HERE, "libtt_amd.so")
 
 TT_OK, TT_ERR_INVALID, TT_ERR_NOMEM, TT_ERR_DEVICE, TT_ERR_UNSUPPORTED = range(5)
 (TT_KERNEL_SINO_T05, TT_KERNEL_SINO_SORT, TT_KERNEL_CIRCUS, TT_KERNEL_AUX,
- TT_KERNEL_BATCH, TT_KERNEL_GATHER, TT_KERNEL_CIRCUS_VJP) = range(7)
+ TT_KERNEL_BATCH, TT_KERNEL_GATHER, TT_KERNEL_CIRCUS_VJP,
+ TT_KERNEL_BATCH_BACKWARD) = range(8)
 
 # Every symbol include/tt.h declares; tests check the library exports them.
 EXPORTED = (
@@ -25,6 +26,8 @@ EXPORTED = (
     "tt_circus_host", "tt_profile_enable", "tt_profile_read",
     "tt_forward_batched", "tt_context_set_autotune", "tt_batched_launch_shape",
     "tt_circus_backward", "tt_sinograms_backward", "tt_backward",
+    "tt_resize", "tt_zscore", "tt_nearest_orthonormal_sinogram",
+    "tt_backward_batched",
 )
 
 
@@ -43,6 +46,11 @@ class tt_pfunc(C.Structure):
 class tt_image(C.Structure):
     _fields_ = [("d_image", C.c_void_p), ("n", C.c_int32), ("d_sino", C.c_void_p),
                 ("d_circus", C.c_void_p)]
+
+
+class tt_image_grad(C.Structure):
+    _fields_ = [("d_image", C.c_void_p), ("n", C.c_int32), ("d_dcircus", C.c_void_p),
+                ("d_dimage", C.c_void_p)]
 
 
 class tt_sino_aux(C.Structure):
@@ -98,6 +106,13 @@ def load():
                                             C.POINTER(i32), C.c_int, vp, vp]
         L.tt_backward.argtypes = [vp, vp, C.c_int, C.c_int, f32, C.POINTER(i32),
                                   C.c_int, C.POINTER(tt_pfunc), C.c_int, vp, vp]
+        L.tt_resize.argtypes = [vp, vp, C.c_int, C.c_int, vp, C.c_int, C.c_int]
+        L.tt_zscore.argtypes = [vp, vp, C.c_int, C.c_int, vp]
+        L.tt_nearest_orthonormal_sinogram.argtypes = [vp, vp, C.c_int, C.c_int, vp,
+                                                      C.c_int, ip, ip]
+        L.tt_backward_batched.argtypes = [vp, C.POINTER(tt_image_grad), C.c_int,
+                                          C.c_int, f32, C.POINTER(i32), C.c_int,
+                                          C.POINTER(tt_pfunc), C.c_int]
         L.tt_profile_enable.argtypes = [vp, C.c_int]
         L.tt_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_int64)]

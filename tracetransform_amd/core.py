@@ -332,3 +332,88 @@ class TraceTransformFunction(torch.autograd.Function):
 def trace_transform(image_cm, n_angles, step_deg, tfunctionals, pfunctionals):
     return TraceTransformFunction.apply(image_cm, n_angles, step_deg,
                                         tcodes_of(tfunctionals), pfuncs_of(pfunctionals))
+
+
+# ------------------------------------------------ steps either side of the path
+
+def resize(image_cm: torch.Tensor, in_rows: int, in_cols: int, rows: int, cols: int):
+    """c++/src/auxiliary.cpp:198-221 on a column-major image ([cols, rows]
+    tensor in, [cols', rows'] out)."""
+    _require_cuda_f32("image", image_cm)
+    out = torch.empty((cols, rows), dtype=torch.float32, device=image_cm.device)
+    ctx = get_context(image_cm.device.index)
+    ctx.bind_current_stream()
+    _lib.check("tt_resize", _lib.load().tt_resize(ctx.handle, _dptr(image_cm), in_rows,
+                                                  in_cols, _dptr(out), rows, cols))
+    return out
+
+
+def zscore(x: torch.Tensor):
+    """c++/src/auxiliary.cpp:305-318 over the last dimension."""
+    _require_cuda_f32("x", x)
+    n = x.shape[-1]
+    out = torch.empty_like(x)
+    ctx = get_context(x.device.index)
+    ctx.bind_current_stream()
+    _lib.check("tt_zscore", _lib.load().tt_zscore(ctx.handle, _dptr(x), n,
+                                                  x.numel() // n, _dptr(out)))
+    return out
+
+
+def nearest_orthonormal_sinogram(sino: torch.Tensor):
+    """c++/src/circus.cpp:68-102 for one sinogram ([A, n] tensor, i.e. the
+    n x A column-major matrix).  Returns (nos [A, n'], center)."""
+    _require_cuda_f32("sino", sino)
+    A, n = sino.shape
+    cap = 3 * n
+    out = torch.empty((A, cap), dtype=torch.float32, device=sino.device)
+    rows, center = C.c_int(), C.c_int()
+    ctx = get_context(sino.device.index)
+    ctx.bind_current_stream()
+    _lib.check("tt_nearest_orthonormal_sinogram",
+               _lib.load().tt_nearest_orthonormal_sinogram(
+                   ctx.handle, _dptr(sino), n, A, _dptr(out), cap, C.byref(rows),
+                   C.byref(center)))
+    r = rows.value
+    # the kernel wrote an r x A column-major matrix into the first r*A floats
+    nos = out.view(-1)[: r * A].view(A, r).clone()
+    return nos, center.value
+
+
+class BatchBackward:
+    """A batch for ``tt_backward_batched``: per image an upstream gradient
+    [nt, np, A] in, the image gradient [n, n] out; reusing the object replays
+    the captured graph."""
+
+    def __init__(self, images, n_angles: int, step_deg: float, tfunctionals,
+                 pfunctionals):
+        self.images = list(images)
+        self.n_angles = int(n_angles)
+        self.step = float(step_deg)
+        self.codes = tcodes_of(tfunctionals)
+        self.pf = pfuncs_of(pfunctionals)
+        dev = self.images[0].device
+        nt, npf = len(self.codes), len(self.pf)
+        self.dcircus = [torch.zeros((nt, npf, self.n_angles), dtype=torch.float32,
+                                    device=dev) for _ in self.images]
+        self.dimage = [torch.empty_like(im) for im in self.images]
+        arr = (_lib.tt_image_grad * len(self.images))()
+        for k, im in enumerate(self.images):
+            _require_cuda_f32(f"images[{k}]", im)
+            arr[k].d_image = im.data_ptr()
+            arr[k].n = im.shape[0]
+            arr[k].d_dcircus = self.dcircus[k].data_ptr()
+            arr[k].d_dimage = self.dimage[k].data_ptr()
+        self._arr = arr
+        self._codes = (C.c_int32 * nt)(*self.codes)
+        self._pf = (_lib.tt_pfunc * npf)(*[_lib.tt_pfunc(*t) for t in self.pf])
+        self.device = dev.index
+
+    def run(self):
+        ctx = get_context(self.device)
+        ctx.bind_current_stream()
+        _lib.check("tt_backward_batched",
+                   _lib.load().tt_backward_batched(ctx.handle, self._arr, len(self.images),
+                                                   self.n_angles, self.step, self._codes,
+                                                   len(self.codes), self._pf, len(self.pf)))
+        return self.dimage

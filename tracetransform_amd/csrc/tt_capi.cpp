@@ -9,6 +9,9 @@
 #include "../../include/tt.h"
 #include "tt_internal.hpp"
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -250,16 +253,26 @@ struct BatchPlan {
     }
 };
 
+// Workspaces of a batched backward (per image: sinograms, gradients,
+// medians, T3..T5 integrals, line records), laid out back to back.
+struct BackwardPlan {
+    BatchPlan g;                  // descriptors, tables, graph
+    DevBuf sino, dsino, z, medx, meds, rec;
+};
+
 struct tt_context {
     int device = 0;
     KernelTimer timer;
     BatchPlan batch;
+    BackwardPlan bwd;
     bool autotune = false;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     DevBuf imgT, trig, tab345, dft_tw, herm_w, io_a, io_b, med_x, med_s;
     DevBuf bw_sino, bw_dsino, bw_z, bw_rec;
+    DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info;
+    rocblas_handle blas = nullptr;
     int trig_A = -1;
     float trig_step = 0.0f;
     int tab_n = -1, dft_n = -1;
@@ -412,11 +425,20 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
     DevBuf *bufs[] = {&ctx->imgT, &ctx->trig, &ctx->tab345, &ctx->dft_tw,
                       &ctx->herm_w, &ctx->io_a, &ctx->io_b, &ctx->med_x,
                       &ctx->med_s, &ctx->bw_sino, &ctx->bw_dsino, &ctx->bw_z,
-                      &ctx->bw_rec};
+                      &ctx->bw_rec, &ctx->nos_med, &ctx->nos_a, &ctx->nos_u,
+                      &ctx->nos_v, &ctx->nos_s, &ctx->nos_e, &ctx->nos_c,
+                      &ctx->nos_info};
     for (DevBuf *b : bufs)
         b->release();
     ctx->timer.release();
     ctx->batch.release();
+    ctx->bwd.g.release();
+    DevBuf *bb[] = {&ctx->bwd.sino, &ctx->bwd.dsino, &ctx->bwd.z, &ctx->bwd.medx,
+                    &ctx->bwd.meds, &ctx->bwd.rec};
+    for (DevBuf *b : bb)
+        b->release();
+    if (ctx->blas)
+        (void)rocblas_destroy_handle(ctx->blas);
     if (ctx->ev_in)
         (void)hipEventDestroy(ctx->ev_in);
     if (ctx->ev_out)
@@ -1123,6 +1145,284 @@ TT_API tt_status tt_backward(tt_context *ctx, const float *d_image, int n,
     TT_TRY(circus_vjp(ctx, sino, n, n_angles, nt, pf, np, d_dcircus,
                       ctx->bw_dsino.as<float>()));
     TT_TRY(image_vjp(ctx, n, n_angles, slot, ctx->bw_dsino.as<float>(), d_dimage));
+    return TT_OK;
+}
+
+#define TT_BLAS(expr)                                                          \
+    do {                                                                       \
+        rocblas_status b_ = (expr);                                            \
+        if (b_ != rocblas_status_success)                                      \
+            return fail(TT_ERR_DEVICE, "%s failed: rocblas status %d", #expr,  \
+                        (int)b_);                                              \
+    } while (0)
+
+TT_API tt_status tt_resize(tt_context *ctx, const float *d_in, int in_rows,
+                           int in_cols, float *d_out, int rows, int cols)
+{
+    TT_CHECK(ctx && d_in && d_out, "NULL argument");
+    TT_CHECK(in_rows >= 2 && in_cols >= 2 && rows >= 1 && cols >= 1,
+             "invalid sizes %d x %d -> %d x %d", in_rows, in_cols, rows, cols);
+    TT_TRY(ctx->set_device());
+    TT_HIP(ttk::launch_resize(d_in, in_rows, in_cols, d_out, rows, cols, ctx->stream));
+    return TT_OK;
+}
+
+TT_API tt_status tt_zscore(tt_context *ctx, const float *d_in, int n, int count,
+                           float *d_out)
+{
+    TT_CHECK(ctx && d_in && d_out, "NULL argument");
+    TT_CHECK(n >= 2 && count >= 1, "need n >= 2 and count >= 1");
+    TT_TRY(ctx->set_device());
+    TT_HIP(ttk::launch_zscore(d_in, n, count, d_out, ctx->stream));
+    return TT_OK;
+}
+
+TT_API tt_status tt_nearest_orthonormal_sinogram(tt_context *ctx,
+                                                 const float *d_sino, int n,
+                                                 int n_cols, float *d_out,
+                                                 int out_capacity_rows,
+                                                 int *out_rows, int *center)
+{
+    TT_CHECK(ctx && d_sino && d_out && out_rows && center, "NULL argument");
+    TT_CHECK(n >= 2 && n <= 2730 && n_cols >= 1, "invalid sinogram size %d x %d", n,
+             n_cols);
+    TT_TRY(ctx->set_device());
+    hipStream_t st = ctx->stream;
+    const int A = n_cols;
+    TT_HIP(ctx->nos_med.ensure(sizeof(int32_t) * A));
+    TT_HIP(ttk::launch_column_medians(d_sino, n, A, ctx->nos_med.as<int32_t>(), st));
+    std::vector<int32_t> med(A);
+    TT_HIP(hipMemcpyAsync(med.data(), ctx->nos_med.ptr, sizeof(int32_t) * A,
+                          hipMemcpyDeviceToHost, st));
+    TT_HIP(hipStreamSynchronize(st));
+    // c++/src/circus.cpp:72-84
+    const int sc = (int)floor((n - 1) / 2.0);
+    int mn = 1 << 30, mx = -(1 << 30);
+    for (int c = 0; c < A; c++) {
+        med[c] -= sc;
+        mn = med[c] < mn ? med[c] : mn;
+        mx = med[c] > mx ? med[c] : mx;
+    }
+    const int rows = n + abs(mx) + abs(mn);
+    *out_rows = rows;
+    *center = sc + mx;
+    TT_CHECK(rows >= A, "aligned sinogram %d x %d is wider than tall", rows, A);
+    TT_CHECK(out_capacity_rows >= rows, "output holds %d rows, %d needed",
+             out_capacity_rows, rows);
+    TT_HIP(hipMemcpyAsync(ctx->nos_med.ptr, med.data(), sizeof(int32_t) * A,
+                          hipMemcpyHostToDevice, st));
+    const size_t mA = (size_t)rows * A;
+    TT_HIP(ctx->nos_a.ensure(sizeof(double) * mA));
+    TT_HIP(ctx->nos_u.ensure(sizeof(double) * mA));
+    TT_HIP(ctx->nos_c.ensure(sizeof(double) * mA));
+    TT_HIP(ctx->nos_v.ensure(sizeof(double) * (size_t)A * A));
+    TT_HIP(ctx->nos_s.ensure(sizeof(double) * A));
+    TT_HIP(ctx->nos_e.ensure(sizeof(double) * A));
+    TT_HIP(ctx->nos_info.ensure(sizeof(rocblas_int)));
+    TT_HIP(ttk::launch_align(d_sino, n, A, ctx->nos_med.as<int32_t>(), mx, rows,
+                             ctx->nos_a.as<double>(), st));
+    if (!ctx->blas)
+        TT_BLAS(rocblas_create_handle(&ctx->blas));
+    TT_BLAS(rocblas_set_stream(ctx->blas, st));
+    // aligned = U S V^T (thin U, full V^T); NOS = U V^T
+    TT_BLAS(rocsolver_dgesvd(ctx->blas, rocblas_svect_singular, rocblas_svect_all, rows,
+                             A, ctx->nos_a.as<double>(), rows, ctx->nos_s.as<double>(),
+                             ctx->nos_u.as<double>(), rows, ctx->nos_v.as<double>(), A,
+                             ctx->nos_e.as<double>(), rocblas_outofplace,
+                             ctx->nos_info.as<rocblas_int>()));
+    const double one = 1.0, zero = 0.0;
+    TT_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
+    TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, rows,
+                          A, A, &one, ctx->nos_u.as<double>(), rows,
+                          ctx->nos_v.as<double>(), A, &zero, ctx->nos_c.as<double>(),
+                          rows));
+    TT_HIP(ttk::launch_to_float(ctx->nos_c.as<double>(), mA, d_out, st));
+    rocblas_int info = 0;
+    TT_HIP(hipMemcpyAsync(&info, ctx->nos_info.ptr, sizeof(info), hipMemcpyDeviceToHost,
+                          st));
+    TT_HIP(hipStreamSynchronize(st));
+    if (info != 0)
+        return fail(TT_ERR_DEVICE, "gesvd did not converge (info %d)", (int)info);
+    return TT_OK;
+}
+
+TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *images,
+                                     int nimg, int n_angles, float angle_step_deg,
+                                     const int32_t *tcodes, int nt,
+                                     const tt_pfunc *pf, int np)
+{
+    TT_CHECK(ctx, "ctx is NULL");
+    TT_CHECK(images && nimg >= 1, "images is NULL or nimg < 1");
+    TT_CHECK(n_angles >= 1, "n_angles must be >= 1 (got %d)", n_angles);
+    TT_CHECK(angle_step_deg > 0.0f && isfinite(angle_step_deg),
+             "angle step must be positive and finite");
+    int slot[6];
+    TT_TRY(adjoint_slots(tcodes, nt, slot));
+    TT_CHECK(np >= 1 && np <= ttk::kMaxP && pf, "np must be in 1..%d", ttk::kMaxP);
+    for (int q = 0; q < np; q++) {
+        if (pf[q].code == TT_P_HERMITE)
+            return fail(TT_ERR_UNSUPPORTED,
+                        "Hermite P-functionals are not available in the batched path");
+        TT_CHECK(pf[q].code >= 1 && pf[q].code <= 3,
+                 "invalid P-functional code %d at position %d", pf[q].code, q);
+    }
+    int nmax = 0;
+    for (int i = 0; i < nimg; i++) {
+        TT_CHECK(images[i].d_image && images[i].d_dcircus && images[i].d_dimage,
+                 "image %d: NULL pointer", i);
+        TT_CHECK(images[i].n >= 3 && images[i].n <= 1800,
+                 "image %d: padded side %d outside 3..1800", i, images[i].n);
+        nmax = images[i].n > nmax ? images[i].n : nmax;
+    }
+    TT_TRY(ctx->set_device());
+    std::vector<int64_t> key = {nimg, n_angles, (int64_t)(*(const uint32_t *)&angle_step_deg),
+                                nt, np};
+    for (int t = 0; t < nt; t++)
+        key.push_back(tcodes[t]);
+    for (int q = 0; q < np; q++)
+        key.push_back(pf[q].code);
+    for (int i = 0; i < nimg; i++) {
+        key.push_back((int64_t)(intptr_t)images[i].d_image);
+        key.push_back(images[i].n);
+        key.push_back((int64_t)(intptr_t)images[i].d_dcircus);
+        key.push_back((int64_t)(intptr_t)images[i].d_dimage);
+    }
+    BackwardPlan &P = ctx->bwd;
+    BatchPlan &B = P.g;
+    hipStream_t own = ctx->own;
+    const bool do345 = slot[3] >= 0 || slot[4] >= 0 || slot[5] >= 0;
+    if (!B.valid || B.key != key) {
+        B.drop_graph();
+        std::vector<int> order(nimg);
+        for (int i = 0; i < nimg; i++)
+            order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return images[a].n > images[b].n;
+        });
+        size_t sq = 0, ln = 0;
+        for (int i = 0; i < nimg; i++) {
+            sq += (size_t)images[i].n * images[i].n;
+            ln += (size_t)images[i].n * n_angles;
+        }
+        TT_HIP(B.imgT.ensure(sizeof(float) * sq));
+        TT_HIP(P.sino.ensure(sizeof(float) * nt * ln));
+        TT_HIP(P.dsino.ensure(sizeof(float) * nt * ln));
+        TT_HIP(P.z.ensure(sizeof(float2) * 3 * ln));
+        TT_HIP(P.medx.ensure(sizeof(int32_t) * ln));
+        TT_HIP(P.meds.ensure(sizeof(int32_t) * ln));
+        TT_HIP(P.rec.ensure(sizeof(ttk::LineRecord) * ln));
+        std::vector<ttk::BatchImage> d(nimg);
+        size_t po = 0, lo = 0;
+        int items = 0, cols = 0;
+        for (int k = 0; k < nimg; k++) {
+            const tt_image_grad &im = images[order[k]];
+            ttk::BatchImage &b = d[k];
+            const size_t l = (size_t)im.n * n_angles;
+            b.img = im.d_image;
+            b.imgT = B.imgT.as<float>() + po;
+            b.sino = P.sino.as<float>() + nt * lo;
+            b.med_x = P.medx.as<int32_t>() + lo;
+            b.med_s = P.meds.as<int32_t>() + lo;
+            b.n = im.n;
+            b.nb = (im.n + ttk::kWave - 1) / ttk::kWave;
+            b.item0 = items;
+            b.col0 = cols;
+            b.circus = nullptr;
+            b.z345 = P.z.as<float2>() + 3 * lo;
+            b.dcircus = im.d_dcircus;
+            b.dsino = P.dsino.as<float>() + nt * lo;
+            b.rec = P.rec.as<ttk::LineRecord>() + lo;
+            b.dimg = im.d_dimage;
+            b.line0 = (int)lo;
+            b.pix0 = (int)po;
+            items += b.nb * n_angles;
+            cols += nt * n_angles;
+            po += (size_t)im.n * im.n;
+            lo += l;
+        }
+        TT_CHECK(po < (1ull << 31) && lo < (1ull << 31), "batch too large");
+        TT_HIP(B.desc.ensure(sizeof(ttk::BatchImage) * nimg));
+        TT_HIP(hipMemcpy(B.desc.ptr, d.data(), sizeof(ttk::BatchImage) * nimg,
+                         hipMemcpyHostToDevice));
+        std::vector<float2> h;
+        make_trig(n_angles, angle_step_deg, h);
+        TT_HIP(B.trig.ensure(sizeof(float2) * n_angles));
+        TT_HIP(hipMemcpy(B.trig.ptr, h.data(), sizeof(float2) * n_angles,
+                         hipMemcpyHostToDevice));
+        make_tab345(nmax, h);
+        TT_HIP(B.tab.ensure(sizeof(float2) * h.size()));
+        TT_HIP(hipMemcpy(B.tab.ptr, h.data(), sizeof(float2) * h.size(),
+                         hipMemcpyHostToDevice));
+        TT_HIP(B.counter.ensure(sizeof(unsigned int)));
+
+        ttk::BatchSinoParams sp = {};
+        sp.images = B.desc.as<ttk::BatchImage>();
+        sp.nimg = nimg;
+        sp.items = items;
+        sp.trig = B.trig.as<float2>();
+        sp.tab345 = B.tab.as<float2>();
+        sp.tabn = nmax;
+        sp.A = n_angles;
+        sp.counter = B.counter.as<unsigned int>();
+        for (int k = 0; k < 6; k++)
+            sp.slot.s[k] = slot[k];
+        ttk::BatchAdjointParams ap = {};
+        ap.images = sp.images;
+        ap.nimg = nimg;
+        ap.A = n_angles;
+        ap.nt = nt;
+        ap.np = np;
+        ap.nmax = nmax;
+        ap.lines = (int)lo;
+        ap.pixels = (int)po;
+        ap.cols = cols;
+        ap.trig = sp.trig;
+        ap.tab345 = sp.tab345;
+        ap.tabn = nmax;
+        ap.do345 = do345;
+        ap.slot = sp.slot;
+        for (int q = 0; q < np; q++)
+            ap.code[q] = pf[q].code;
+        const int cus = device_cus(ctx->device);
+        B.waves_per_block = 4;
+        B.blocks = cus * 4;
+        TT_HIP(hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = ttk::launch_transpose_batched(sp.images, nimg, nmax, own);
+        if (ce == hipSuccess)
+            ce = ttk::launch_reset_counter(sp.counter, own);
+        if (ce == hipSuccess)
+            ce = ttk::launch_sino_t05_batched(sp, true, true, do345, B.waves_per_block,
+                                              B.blocks, own);
+        if (ce == hipSuccess)
+            ce = ttk::launch_circus_vjp_batched(ap, own);
+        if (ce == hipSuccess)
+            ce = ttk::launch_records_batched(ap, own);
+        if (ce == hipSuccess)
+            ce = ttk::launch_gather_batched(ap, own);
+        hipGraph_t g = nullptr;
+        hipError_t ee = hipStreamEndCapture(own, &g);
+        TT_HIP(ce);
+        TT_HIP(ee);
+        B.graph = g;
+        TT_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+        B.key = key;
+        B.valid = true;
+    }
+    if (!ctx->ev_in)
+        TT_HIP(hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming));
+    if (!ctx->ev_out)
+        TT_HIP(hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming));
+    if (ctx->stream != own) {
+        TT_HIP(hipEventRecord(ctx->ev_in, ctx->stream));
+        TT_HIP(hipStreamWaitEvent(own, ctx->ev_in, 0));
+    }
+    TT_HIP(ctx->timer.begin(TT_KERNEL_BATCH_BACKWARD, own));
+    TT_HIP(hipGraphLaunch(B.exec, own));
+    TT_HIP(ctx->timer.end(own));
+    if (ctx->stream != own) {
+        TT_HIP(hipEventRecord(ctx->ev_out, own));
+        TT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_out, 0));
+    }
     return TT_OK;
 }
 

@@ -35,6 +35,16 @@ struct SinoParams {
     Slots slot;
 };
 
+// Adjoint.  Per-line record of the rotated-domain gradient of one angle's
+// column: G(row) = g0 + [row >= m] (g1 (row-m) + g2 (row-m)^2)
+//                + [row > sm] sum_t (alpha_t W_t.re[row-sm] + beta_t W_t.im[row-sm]).
+struct LineRecord {
+    float g0, g1, g2;
+    int m, sm;
+    float ab[6];          // alpha3, beta3, alpha4, beta4, alpha5, beta5
+    float pad;
+};
+
 // One image of a batch (device array, ordered by first item).
 struct BatchImage {
     const float *img;     // n x n column-major
@@ -48,6 +58,14 @@ struct BatchImage {
     int item0;            // first wave item of this image
     int col0;             // first sinogram column (over all T) of this image
     float *circus;        // [nt][np][A]
+    // adjoint (tt_backward_batched); null in the forward-only batch
+    float2 *z345;         // [3][A][n] T3..T5 integrals
+    const float *dcircus; // [nt][np][A]
+    float *dsino;         // [nt][A][n]
+    LineRecord *rec;      // [A][n]
+    float *dimg;          // n x n
+    int line0;            // first line (angle*n + col) over the batch
+    int pix0;             // first pixel over the batch
 };
 
 struct BatchSinoParams {
@@ -108,16 +126,6 @@ struct CircusParams {
     int herm[kMaxP];      // row of herm_w for Hermite entries
 };
 
-// Adjoint.  Per-line record of the rotated-domain gradient of one angle's
-// column: G(row) = g0 + [row >= m] (g1 (row-m) + g2 (row-m)^2)
-//                + [row > sm] sum_t (alpha_t W_t.re[row-sm] + beta_t W_t.im[row-sm]).
-struct LineRecord {
-    float g0, g1, g2;
-    int m, sm;
-    float ab[6];          // alpha3, beta3, alpha4, beta4, alpha5, beta5
-    float pad;
-};
-
 struct RecordParams {
     const float *dsino;   // [nt][A][n] sinogram gradient
     const int32_t *med_x; // [A][n]
@@ -153,6 +161,18 @@ struct CircusVjpParams {
     int herm[kMaxP];
 };
 
+// Steps either side of the path and the orthonormal step.
+hipError_t launch_resize(const float *in, int in_rows, int in_cols, float *out,
+                         int rows, int cols, hipStream_t s);
+hipError_t launch_zscore(const float *in, int n, int count, float *out,
+                         hipStream_t s);
+hipError_t launch_column_medians(const float *in, int n, int cols, int32_t *med,
+                                 hipStream_t s);
+hipError_t launch_align(const float *in, int n, int cols, const int32_t *offset,
+                        int maxoff, int rows_out, double *out, hipStream_t s);
+hipError_t launch_to_float(const double *in, size_t count, float *out,
+                           hipStream_t s);
+
 hipError_t launch_records(const RecordParams &p, hipStream_t s);
 hipError_t launch_gather(const GatherParams &p, hipStream_t s);
 hipError_t launch_circus_vjp(const CircusVjpParams &p, hipStream_t s);
@@ -174,5 +194,27 @@ hipError_t launch_sino_t05_batched(const BatchSinoParams &p, bool need_x,
                                    bool need_s, bool do345, int waves_per_block,
                                    int blocks, hipStream_t s);
 hipError_t launch_circus_batched(const BatchCircusParams &p, hipStream_t s);
+
+// Batched adjoint stages (same descriptors as the forward batch).
+struct BatchAdjointParams {
+    const BatchImage *images;
+    int nimg;
+    int A;
+    int nt;
+    int np;
+    int nmax;
+    int lines;            // total lines (sum of A * n)
+    int pixels;           // total pixels (sum of n * n)
+    int cols;             // total sinogram columns (sum of nt * A)
+    const float2 *trig;
+    const float2 *tab345; // [3][tabn]
+    int tabn;
+    int do345;
+    Slots slot;
+    int code[kMaxP];
+};
+hipError_t launch_circus_vjp_batched(const BatchAdjointParams &p, hipStream_t s);
+hipError_t launch_records_batched(const BatchAdjointParams &p, hipStream_t s);
+hipError_t launch_gather_batched(const BatchAdjointParams &p, hipStream_t s);
 
 }  // namespace ttk

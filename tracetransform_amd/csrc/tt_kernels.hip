@@ -189,56 +189,68 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
         return;
 
     // ---- pass 2: median walk + integrals -----------------------------
-    // The walk is the reference's float prefix; the integrals only feed
-    // values, so they accumulate in double (full rate on CDNA4) and land
-    // within ~1e-7 of the FP64 oracle instead of inheriting the reference's
-    // sequential float error.
+    // The walk is the reference's float prefix.  The integrals only feed
+    // values: they accumulate with float FMAs over blocks of 64 rows, each
+    // block flushed into a double, so they land within ~1e-7 of the FP64
+    // oracle instead of inheriting the reference's sequential float error.
     float Px = 0.0f, Ps = 0.0f;
     double t1 = 0.0, t2 = 0.0;
     double r3 = 0.0, i3 = 0.0, r4 = 0.0, i4 = 0.0, r5 = 0.0, i5 = 0.0;
     int m = -1, sm = -1;
     const bool do3 = p.slot.s[3] >= 0, do4 = p.slot.s[4] >= 0, do5 = p.slot.s[5] >= 0;
-    for (int r = 0; r < n; r++) {
-        const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s,
-                                       (float)r - o);
-        const double vd = (double)v;
-        if (NEED_X) {
-            if (m < 0) {
-                Px += v;
-                if (2.0f * Px >= Sx)
-                    m = r;
+    for (int rb = 0; rb < n; rb += 64) {
+        const int re = rb + 64 < n ? rb + 64 : n;
+        float f1 = 0.0f, f2 = 0.0f;
+        float f3r = 0.0f, f3i = 0.0f, f4r = 0.0f, f4i = 0.0f, f5r = 0.0f, f5i = 0.0f;
+        for (int r = rb; r < re; r++) {
+            const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s,
+                                           (float)r - o);
+            if (NEED_X) {
+                if (m < 0) {
+                    Px += v;
+                    if (2.0f * Px >= Sx)
+                        m = r;
+                }
+                if (m >= 0) {
+                    const float rr = (float)(r - m);
+                    const float vr = v * rr;
+                    f1 += vr;
+                    f2 = fmaf(vr, rr, f2);
+                }
             }
-            if (m >= 0) {
-                const double rr = (double)(r - m);
-                const double vr = vd * rr;
-                t1 += vr;
-                t2 = fma(vr, rr, t2);
+            if (NEED_S) {
+                if (sm < 0) {
+                    Ps += sqrtf(v);
+                    if (2.0f * Ps >= Ss)
+                        sm = r;
+                } else if (DO345) {
+                    const int r1 = r - sm;
+                    if (do3) {
+                        const float2 w = s_tab[r1];
+                        f3r = fmaf(w.x, v, f3r);
+                        f3i = fmaf(w.y, v, f3i);
+                    }
+                    if (do4) {
+                        const float2 w = s_tab[p.tabn + r1];
+                        f4r = fmaf(w.x, v, f4r);
+                        f4i = fmaf(w.y, v, f4i);
+                    }
+                    if (do5) {
+                        const float2 w = s_tab[2 * p.tabn + r1];
+                        f5r = fmaf(w.x, v, f5r);
+                        f5i = fmaf(w.y, v, f5i);
+                    }
+                }
             }
         }
-        if (NEED_S) {
-            if (sm < 0) {
-                Ps += sqrtf(v);
-                if (2.0f * Ps >= Ss)
-                    sm = r;
-            } else if (DO345) {
-                const int r1 = r - sm;
-                if (do3) {
-                    const float2 w = s_tab[r1];
-                    r3 = fma((double)w.x, vd, r3);
-                    i3 = fma((double)w.y, vd, i3);
-                }
-                if (do4) {
-                    const float2 w = s_tab[p.tabn + r1];
-                    r4 = fma((double)w.x, vd, r4);
-                    i4 = fma((double)w.y, vd, i4);
-                }
-                if (do5) {
-                    const float2 w = s_tab[2 * p.tabn + r1];
-                    r5 = fma((double)w.x, vd, r5);
-                    i5 = fma((double)w.y, vd, i5);
-                }
-            }
-        }
+        t1 += f1;
+        t2 += f2;
+        r3 += f3r;
+        i3 += f3i;
+        r4 += f4r;
+        i4 += f4i;
+        r5 += f5r;
+        i5 += f5i;
     }
     if (m < 0)
         m = n - 1;
@@ -343,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void sino_t05_batched_kernel(BatchSinoParam
         if (!active)
             col = im.n - 1;
         const float2 cs = p.trig[a];
-        const LineDst dst = {im.sino, im.med_x, im.med_s, nullptr, im.n, p.A, p.tabn,
+        const LineDst dst = {im.sino, im.med_x, im.med_s, im.z345, im.n, p.A, p.tabn,
                              p.slot};
         if (fabsf(cs.x) > fabsf(cs.y))
             sino_line<true, NEED_X, NEED_S, DO345>(dst, im.imgT, s_tab, a, col,
@@ -981,34 +993,66 @@ __global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
 
 // One LineRecord per (angle, column): folds the sinogram gradient of every
 // requested T0..T5 into the coefficients of G(row).
+__device__ __forceinline__ LineRecord make_record(const float *dsino, const int32_t *med_x,
+                                                  const int32_t *med_s, const float2 *z345,
+                                                  size_t plane, size_t line, Slots slot)
+{
+    LineRecord r = {};
+    r.m = med_x ? med_x[line] : 0;
+    r.sm = med_s ? med_s[line] : 0;
+    if (slot.s[0] >= 0)
+        r.g0 = dsino[slot.s[0] * plane + line];
+    if (slot.s[1] >= 0)
+        r.g1 = dsino[slot.s[1] * plane + line];
+    if (slot.s[2] >= 0)
+        r.g2 = dsino[slot.s[2] * plane + line];
+    for (int k = 0; k < 3; k++) {
+        const int sl = slot.s[3 + k];
+        if (sl < 0)
+            continue;
+        const float2 z = z345[k * plane + line];
+        const float mag = hypotf(z.x, z.y);
+        if (mag > 0.0f) {
+            const float g = dsino[sl * plane + line] / mag;
+            r.ab[2 * k] = g * z.x;
+            r.ab[2 * k + 1] = g * z.y;
+        }
+    }
+    return r;
+}
+
 __global__ __launch_bounds__(kBlock) void records_kernel(RecordParams p)
 {
     const size_t lines = (size_t)p.A * p.n;
-    const size_t plane = lines;
     for (size_t line = (size_t)blockIdx.x * kBlock + threadIdx.x; line < lines;
-         line += (size_t)gridDim.x * kBlock) {
-        LineRecord r = {};
-        r.m = p.med_x ? p.med_x[line] : 0;
-        r.sm = p.med_s ? p.med_s[line] : 0;
-        if (p.slot.s[0] >= 0)
-            r.g0 = p.dsino[p.slot.s[0] * plane + line];
-        if (p.slot.s[1] >= 0)
-            r.g1 = p.dsino[p.slot.s[1] * plane + line];
-        if (p.slot.s[2] >= 0)
-            r.g2 = p.dsino[p.slot.s[2] * plane + line];
-        for (int k = 0; k < 3; k++) {
-            const int sl = p.slot.s[3 + k];
-            if (sl < 0)
-                continue;
-            const float2 z = p.z345[k * plane + line];
-            const float mag = hypotf(z.x, z.y);
-            if (mag > 0.0f) {
-                const float g = p.dsino[sl * plane + line] / mag;
-                r.ab[2 * k] = g * z.x;
-                r.ab[2 * k + 1] = g * z.y;
-            }
-        }
-        p.rec[line] = r;
+         line += (size_t)gridDim.x * kBlock)
+        p.rec[line] = make_record(p.dsino, p.med_x, p.med_s, p.z345, lines, line, p.slot);
+}
+
+template <class F>
+__device__ __forceinline__ int find_image(const BatchImage *images, int nimg, int v, F key)
+{
+    int lo = 0, hi = nimg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (key(images[mid]) <= v)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void records_batched_kernel(BatchAdjointParams p)
+{
+    for (int g = blockIdx.x * kBlock + threadIdx.x; g < p.lines; g += gridDim.x * kBlock) {
+        const int k = find_image(p.images, p.nimg, g,
+                                 [](const BatchImage &b) { return b.line0; });
+        const BatchImage &im = p.images[k];
+        const size_t plane = (size_t)p.A * im.n;
+        const size_t line = (size_t)(g - im.line0);
+        im.rec[line] = make_record(im.dsino, im.med_x, im.med_s, im.z345, plane, line,
+                                   p.slot);
     }
 }
 
@@ -1018,27 +1062,17 @@ __global__ __launch_bounds__(kBlock) void records_kernel(RecordParams p)
 // inverse image), recomputes each sample's forward float coordinates with
 // the exact operations of rot_sample, keeps those whose taps include (Y, X),
 // and accumulates G(row) times the tap weight.  Deterministic, no atomics.
-__global__ __launch_bounds__(kBlock) void gather_kernel(GatherParams p)
+__device__ __forceinline__ float gather_pixel(const LineRecord *recs, const float2 *trig,
+                                              const float2 *s_tab, int tabn, bool do345,
+                                              int n, int A, int X, int Y)
 {
-    extern __shared__ float2 s_tab[];
-    const int n = p.n;
-    if (p.do345) {
-        for (int i = threadIdx.x; i < 3 * n; i += kBlock)
-            s_tab[i] = p.tab345[i];
-    }
-    __syncthreads();
-    const int pix = blockIdx.x * kBlock + threadIdx.x;
-    if (pix >= n * n)
-        return;
-    const int X = pix / n;      // column
-    const int Y = pix - X * n;  // row
     const float o = (float)((n - 1) / 2.0);
     const float lim = (float)(n - 1);
     const float u = (float)X - o, v = (float)Y - o;
     const float R = 1.4163f;
     double acc = 0.0;
-    for (int a = 0; a < p.A; a++) {
-        const float2 cs = p.trig[a];
+    for (int a = 0; a < A; a++) {
+        const float2 cs = trig[a];
         const float c = cs.x, s = cs.y;
         const float pxc = u * c - v * s;   // inverse rotation of the pixel
         const float pyc = u * s + v * c;
@@ -1048,7 +1082,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherParams p)
         r0 = r0 < 0 ? 0 : r0;
         c1 = c1 > n - 1 ? n - 1 : c1;
         r1 = r1 > n - 1 ? n - 1 : r1;
-        const LineRecord *rec = p.rec + (size_t)a * n;
+        const LineRecord *rec = recs + (size_t)a * n;
         for (int col = c0; col <= c1; col++) {
             const float px = (float)col - o;
             const float pxcc = px * c;
@@ -1074,9 +1108,10 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherParams p)
                     const float d = (float)(row - r.m);
                     G = fmaf(d, fmaf(r.g2, d, r.g1), G);
                 }
-                if (p.do345 && row > r.sm) {
+                if (do345 && row > r.sm) {
                     const int q = row - r.sm;
-                    const float2 w3 = s_tab[q], w4 = s_tab[n + q], w5 = s_tab[2 * n + q];
+                    const float2 w3 = s_tab[q], w4 = s_tab[tabn + q],
+                                 w5 = s_tab[2 * tabn + q];
                     G = fmaf(r.ab[0], w3.x, G);
                     G = fmaf(r.ab[1], w3.y, G);
                     G = fmaf(r.ab[2], w4.x, G);
@@ -1088,99 +1123,156 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(GatherParams p)
             }
         }
     }
-    p.dimg[pix] = (float)acc;
+    return (float)acc;
+}
+
+// Transposed bilinear rotation as a gather: thread = one source pixel
+// (row Y, col X).  For every angle it enumerates the rotated samples whose
+// 2x2 footprint can cover the pixel (the rotated pixels within sqrt(2) of its
+// inverse image), recomputes each sample's forward float coordinates with
+// the exact operations of rot_sample, keeps those whose taps include (Y, X),
+// and accumulates G(row) times the tap weight.  Deterministic, no atomics.
+__global__ __launch_bounds__(kBlock) void gather_kernel(GatherParams p)
+{
+    extern __shared__ float2 s_tab[];
+    const int n = p.n;
+    if (p.do345) {
+        for (int i = threadIdx.x; i < 3 * n; i += kBlock)
+            s_tab[i] = p.tab345[i];
+    }
+    __syncthreads();
+    const int pix = blockIdx.x * kBlock + threadIdx.x;
+    if (pix >= n * n)
+        return;
+    const int X = pix / n, Y = pix - (pix / n) * n;
+    p.dimg[pix] = gather_pixel(p.rec, p.trig, s_tab, n, p.do345, n, p.A, X, Y);
+}
+
+__global__ __launch_bounds__(kBlock) void gather_batched_kernel(BatchAdjointParams p)
+{
+    extern __shared__ float2 s_tab[];
+    if (p.do345) {
+        for (int i = threadIdx.x; i < 3 * p.tabn; i += kBlock)
+            s_tab[i] = p.tab345[i];
+    }
+    __syncthreads();
+    for (int g = blockIdx.x * kBlock + threadIdx.x; g < p.pixels; g += gridDim.x * kBlock) {
+        const int k = find_image(p.images, p.nimg, g,
+                                 [](const BatchImage &b) { return b.pix0; });
+        const BatchImage &im = p.images[k];
+        const int pix = g - im.pix0;
+        const int X = pix / im.n, Y = pix - (pix / im.n) * im.n;
+        im.dimg[pix] = gather_pixel(im.rec, p.trig, s_tab, p.tabn, p.do345, im.n, p.A,
+                                    X, Y);
+    }
 }
 
 // dS = J_P^T dC for every sinogram column (P1: sign differences; P2: one-hot
 // at the source of the weighted-median rank of the stable sort; P3: the
 // derivative of sum_p w_p |X_q(p)|^4 / N^4, an inverse-DFT-shaped sum;
 // Hermite: its weights).
-__global__ __launch_bounds__(kBlock) void circus_vjp_kernel(CircusVjpParams p)
+// LDS layout of one circus-VJP workgroup for columns of height up to nmax.
+struct VjpLds {
+    uint64_t *key;  // pow2 >= nmax
+    float *x;       // nmax4
+    float *sq;      // nmax4
+    float *ds;      // nmax4
+    float2 *X;      // nmax/2+1
+    float *ck;      // (nmax/2+2) & ~1
+    float2 *tw;     // nmax
+};
+
+__device__ __forceinline__ VjpLds vjp_lds(unsigned char *smem, int nmax)
 {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int n = p.n;
-    const int n4 = (n + 3) & ~3;
+    const int n4 = (nmax + 3) & ~3;
     int m2 = 1;
-    while (m2 < n)
+    while (m2 < nmax)
         m2 <<= 1;
-    uint64_t *s_key = reinterpret_cast<uint64_t *>(smem_raw);     // m2
-    float *s_x = reinterpret_cast<float *>(s_key + m2);            // n4
-    float *s_sq = s_x + n4;                                        // n4
-    float *s_ds = s_sq + n4;                                       // n4
-    float2 *s_X = reinterpret_cast<float2 *>(s_ds + n4);           // n/2+1
-    float *s_ck = reinterpret_cast<float *>(s_X + (n / 2 + 1));    // n/2+1
-    float2 *s_tw = reinterpret_cast<float2 *>(s_ck + ((n / 2 + 2) & ~1));  // n
+    VjpLds l;
+    l.key = reinterpret_cast<uint64_t *>(smem);
+    l.x = reinterpret_cast<float *>(l.key + m2);
+    l.sq = l.x + n4;
+    l.ds = l.sq + n4;
+    l.X = reinterpret_cast<float2 *>(l.ds + n4);
+    l.ck = reinterpret_cast<float *>(l.X + (nmax / 2 + 1));
+    l.tw = reinterpret_cast<float2 *>(l.ck + ((nmax / 2 + 2) & ~1));
+    return l;
+}
+
+// J_P^T dC for the column x[0..n) already in L.x; dc[q * dc_stride] is the
+// upstream gradient of P-functional q at this column; result written to out.
+__device__ void circus_vjp_column(const VjpLds &L, int n, int np, const int *code,
+                                  const int *herm, const double *herm_w,
+                                  const float *dc, size_t dc_stride, float *out)
+{
     __shared__ float s_chain[8];
     __shared__ float s_sum;
     __shared__ int s_src;
-
-    const int colg = blockIdx.x;
-    const int sg = colg / p.ncols;
-    const int col = colg - sg * p.ncols;
-    const float *x = p.sino + (size_t)colg * n;
+    int m2 = 1;
+    while (m2 < n)
+        m2 <<= 1;
     bool p3 = false;
-    for (int q = 0; q < p.np; q++)
-        p3 = p3 || p.code[q] == 3;
+    for (int q = 0; q < np; q++)
+        p3 = p3 || code[q] == 3;
     for (int i = threadIdx.x; i < n; i += kBlock) {
-        s_x[i] = x[i];
-        s_ds[i] = 0.0f;
+        L.ds[i] = 0.0f;
         if (p3) {
             double sn, cn;
             sincospi(2.0 * (double)i / (double)n, &sn, &cn);
-            s_tw[i] = float2{(float)cn, (float)sn};
+            L.tw[i] = float2{(float)cn, (float)sn};
         }
     }
     __syncthreads();
-    for (int q = 0; q < p.np; q++) {
-        const float g = p.dcircus[((size_t)sg * p.np + q) * p.ncols + col];
-        const int code = p.code[q];
-        if (code == 1) {
+    for (int q = 0; q < np; q++) {
+        const float g = dc[q * dc_stride];
+        const int cd = code[q];
+        if (cd == 1) {
             for (int j = threadIdx.x; j < n; j += kBlock) {
                 float d = 0.0f;
                 if (j >= 1) {
-                    const float u = s_x[j - 1] - s_x[j];
+                    const float u = L.x[j - 1] - L.x[j];
                     d -= (float)((u > 0.0f) - (u < 0.0f));
                 }
                 if (j <= n - 2) {
-                    const float u = s_x[j] - s_x[j + 1];
+                    const float u = L.x[j] - L.x[j + 1];
                     d += (float)((u > 0.0f) - (u < 0.0f));
                 }
-                s_ds[j] += g * d;
+                L.ds[j] += g * d;
             }
-        } else if (code == 0) {
-            const double *w = p.herm_w + (size_t)p.herm[q] * n;
+        } else if (cd == 0) {
+            const double *w = herm_w + (size_t)herm[q] * n;
             for (int j = threadIdx.x; j < n; j += kBlock)
-                s_ds[j] += (float)((double)g * w[j]);
-        } else if (code == 2) {
+                L.ds[j] += (float)((double)g * w[j]);
+        } else if (cd == 2) {
             for (int i = threadIdx.x; i < m2; i += kBlock)
-                s_key[i] = i < n ? (((uint64_t)ord_key(s_x[i]) << 32) | (uint32_t)i)
+                L.key[i] = i < n ? (((uint64_t)ord_key(L.x[i]) << 32) | (uint32_t)i)
                                  : ~0ull;
             __syncthreads();
-            bitonic_lds(s_key, m2);
+            bitonic_lds(L.key, m2);
             for (int i = threadIdx.x; i < n; i += kBlock)
-                s_sq[i] = s_x[(int)(uint32_t)s_key[i]];
+                L.sq[i] = L.x[(int)(uint32_t)L.key[i]];
             __syncthreads();
-            eigen_sum_block(s_sq, n, s_chain, &s_sum);
+            eigen_sum_block(L.sq, n, s_chain, &s_sum);
             if (threadIdx.x == 0) {
-                const int m = weighted_median_walk(s_sq, n, s_sum);
-                s_src = (int)(uint32_t)s_key[m];
+                const int m = weighted_median_walk(L.sq, n, s_sum);
+                s_src = (int)(uint32_t)L.key[m];
             }
             __syncthreads();
             if (threadIdx.x == 0)
-                s_ds[s_src] += g;
+                L.ds[s_src] += g;
         } else {
             const int half = n / 2;
             for (int k = threadIdx.x; k <= half; k += kBlock) {
                 double re = 0.0, im = 0.0;
                 int idx = 0;
                 for (int j = 0; j < n; j++) {
-                    const float2 t = s_tw[idx];
+                    const float2 t = L.tw[idx];
                     idx += k;
                     idx -= idx >= n ? n : 0;
-                    re = fma((double)s_x[j], (double)t.x, re);
-                    im = fma(-(double)s_x[j], (double)t.y, im);
+                    re = fma((double)L.x[j], (double)t.x, re);
+                    im = fma(-(double)L.x[j], (double)t.y, im);
                 }
-                s_X[k] = float2{(float)re, (float)im};
+                L.X[k] = float2{(float)re, (float)im};
                 // trapz weights of the float abscissae, folded onto k
                 double w = 0.0;
                 for (int side = 0; side < 2; side++) {
@@ -1194,28 +1286,179 @@ __global__ __launch_bounds__(kBlock) void circus_vjp_kernel(CircusVjpParams p)
                         w += 0.5 * ((double)(float)(-1 + (qq + 1) * 2.0 / (n - 1)) - l);
                 }
                 const double n4d = (double)n * n * (double)n * n;
-                s_ck[k] = (float)(w * 4.0 * (re * re + im * im) / n4d);
+                L.ck[k] = (float)(w * 4.0 * (re * re + im * im) / n4d);
             }
             __syncthreads();
             for (int j = threadIdx.x; j < n; j += kBlock) {
                 double acc = 0.0;
                 int idx = 0;
                 for (int k = 0; k <= half; k++) {
-                    const float2 t = s_tw[idx];
+                    const float2 t = L.tw[idx];
                     idx += j;
                     idx -= idx >= n ? n : 0;
-                    const float2 X = s_X[k];
-                    acc = fma((double)s_ck[k],
+                    const float2 X = L.X[k];
+                    acc = fma((double)L.ck[k],
                               (double)X.x * (double)t.x - (double)X.y * (double)t.y, acc);
                 }
-                s_ds[j] += (float)((double)g * acc);
+                L.ds[j] += (float)((double)g * acc);
             }
         }
         __syncthreads();
     }
-    float *out = p.dsino + (size_t)colg * n;
     for (int i = threadIdx.x; i < n; i += kBlock)
-        out[i] = s_ds[i];
+        out[i] = L.ds[i];
+}
+
+// dS = J_P^T dC for every sinogram column (P1: sign differences; P2: one-hot
+// at the source of the weighted-median rank of the stable sort; P3: the
+// derivative of sum_p w_p |X_q(p)|^4 / N^4, an inverse-DFT-shaped sum;
+// Hermite: its weights).
+__global__ __launch_bounds__(kBlock) void circus_vjp_kernel(CircusVjpParams p)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n = p.n;
+    const VjpLds L = vjp_lds(smem_raw, n);
+    const int colg = blockIdx.x;
+    const int sg = colg / p.ncols;
+    const int col = colg - sg * p.ncols;
+    const float *x = p.sino + (size_t)colg * n;
+    for (int i = threadIdx.x; i < n; i += kBlock)
+        L.x[i] = x[i];
+    __syncthreads();
+    circus_vjp_column(L, n, p.np, p.code, p.herm, p.herm_w,
+                      p.dcircus + (size_t)sg * p.np * p.ncols + col, p.ncols,
+                      p.dsino + (size_t)colg * n);
+}
+
+__global__ __launch_bounds__(kBlock) void circus_vjp_batched_kernel(BatchAdjointParams p)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const VjpLds L = vjp_lds(smem_raw, p.nmax);
+    const int colg = blockIdx.x;
+    const int k = find_image(p.images, p.nimg, colg,
+                             [](const BatchImage &b) { return b.col0; });
+    const BatchImage &im = p.images[k];
+    const int n = im.n;
+    const int lc = colg - im.col0;       // t * A + a
+    const int t = lc / p.A;
+    const int a = lc - t * p.A;
+    const float *x = im.sino + (size_t)lc * n;
+    for (int i = threadIdx.x; i < n; i += kBlock)
+        L.x[i] = x[i];
+    __syncthreads();
+    const int herm_none[kMaxP] = {};
+    circus_vjp_column(L, n, p.np, p.code, herm_none, nullptr,
+                      im.dcircus + (size_t)t * p.np * p.A + a, p.A,
+                      im.dsino + (size_t)lc * n);
+}
+
+// ---------------------------------------------------------------------------
+// Steps either side of the path.
+// ---------------------------------------------------------------------------
+
+// c++/src/auxiliary.cpp:198-221 (resize): interior pixels sample the input at
+// ((col,row) + 0.5) * diag(in_rows/rows, in_cols/cols) - 0.5 with the
+// reference's float operations; the one-pixel border stays zero.  Negative
+// coordinates (upscaling) keep modf's semantics: integer part -0 -> index 0,
+// negative fraction.
+__global__ __launch_bounds__(kBlock) void resize_kernel(const float *in, int in_rows,
+                                                        int in_cols, float *out,
+                                                        int rows, int cols)
+{
+    const size_t total = (size_t)rows * cols;
+    const float t00 = (float)in_rows / (float)rows;
+    const float t11 = (float)in_cols / (float)cols;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * kBlock) {
+        const int row = (int)(i % rows), col = (int)(i / rows);
+        float v = 0.0f;
+        if (row >= 1 && row < rows - 1 && col >= 1 && col < cols - 1) {
+            const float px = (float)col + 0.5f, py = (float)row + 0.5f;
+            float x = px * t00 + py * 0.0f;
+            float y = px * 0.0f + py * t11;
+            x = x - 0.5f;
+            y = y - 0.5f;
+            const float xi = __builtin_truncf(x), yi = __builtin_truncf(y);
+            const float fx = x - xi, fy = y - yi;
+            const int X = (int)xi, Y = (int)yi;
+            const float s00 = in[(size_t)X * in_rows + Y];
+            const float s01 = in[(size_t)(X + 1) * in_rows + Y];
+            const float s10 = in[(size_t)X * in_rows + Y + 1];
+            const float s11 = in[(size_t)(X + 1) * in_rows + Y + 1];
+            v = s00 * (1 - fx) * (1 - fy) + s01 * fx * (1 - fy) +
+                s10 * (1 - fx) * fy + s11 * fx * fy;
+        }
+        out[i] = v;
+    }
+}
+
+// c++/src/auxiliary.cpp:277-318 (zscore): float mean from a sequential float
+// sum, variance in double over float differences, float result.  One
+// workgroup per vector.
+__global__ __launch_bounds__(kBlock) void zscore_kernel(const float *in, int n,
+                                                        float *out)
+{
+    __shared__ float s_mean;
+    __shared__ double s_red[kBlock / kWave];
+    const float *x = in + (size_t)blockIdx.x * n;
+    float *y = out + (size_t)blockIdx.x * n;
+    if (threadIdx.x == 0) {
+        float sum = 0.0f;
+        for (int i = 0; i < n; i++)
+            sum += x[i];
+        s_mean = sum / (float)n;
+    }
+    __syncthreads();
+    const float mean = s_mean;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+        const double d = (double)(x[i] - mean);
+        acc = fma(d, d, acc);
+    }
+    const double var = block_sum_d(acc, s_red);
+    const float stdev = (float)sqrt(var / (double)(n - 1));
+    for (int i = threadIdx.x; i < n; i += kBlock)
+        y[i] = (x[i] - mean) / stdev;
+}
+
+// findWeightedMedian of every column of an n x cols matrix (one workgroup per
+// column), as nearest_orthonormal_sinogram uses it (c++/src/circus.cpp:74-78).
+__global__ __launch_bounds__(kBlock) void column_median_kernel(const float *in, int n,
+                                                               int32_t *med)
+{
+    extern __shared__ __align__(16) float s_col[];
+    __shared__ float s_chain[8];
+    __shared__ float s_sum;
+    const float *x = in + (size_t)blockIdx.x * n;
+    for (int i = threadIdx.x; i < n; i += kBlock)
+        s_col[i] = x[i];
+    __syncthreads();
+    eigen_sum_block(s_col, n, s_chain, &s_sum);
+    if (threadIdx.x == 0)
+        med[blockIdx.x] = weighted_median_walk(s_col, n, s_sum);
+}
+
+// aligned(maxoff + row - offset[col], col) = in(row, col), zero elsewhere
+// (c++/src/circus.cpp:85-93), in double for the SVD.
+__global__ __launch_bounds__(kBlock) void align_kernel(const float *in, int n, int cols,
+                                                       const int32_t *offset, int maxoff,
+                                                       int rows_out, double *out)
+{
+    const size_t total = (size_t)rows_out * cols;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * kBlock) {
+        const int r = (int)(i % rows_out), c = (int)(i / rows_out);
+        const int src = r - (maxoff - offset[c]);
+        out[i] = (src >= 0 && src < n) ? (double)in[(size_t)c * n + src] : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void to_float_kernel(const double *in, size_t count,
+                                                          float *out)
+{
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * kBlock)
+        out[i] = (float)in[i];
 }
 
 }  // namespace
@@ -1380,17 +1623,89 @@ hipError_t launch_gather(const GatherParams &p, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_circus_vjp(const CircusVjpParams &p, hipStream_t s)
+static size_t vjp_shm(int nmax)
 {
     int m2 = 1;
-    while (m2 < p.n)
+    while (m2 < nmax)
         m2 <<= 1;
-    const size_t n4 = ((size_t)p.n + 3) & ~(size_t)3;
-    const size_t h = (size_t)p.n / 2 + 1;
-    const size_t shm = sizeof(uint64_t) * m2 + sizeof(float) * 3 * n4 +
-                       sizeof(float2) * h + sizeof(float) * ((h + 1) & ~(size_t)1) +
-                       sizeof(float2) * (size_t)p.n;
-    circus_vjp_kernel<<<p.ncols * p.nsino, kBlock, shm, s>>>(p);
+    const size_t n4 = ((size_t)nmax + 3) & ~(size_t)3;
+    const size_t h = (size_t)nmax / 2 + 1;
+    return sizeof(uint64_t) * m2 + sizeof(float) * 3 * n4 + sizeof(float2) * h +
+           sizeof(float) * ((h + 1) & ~(size_t)1) + sizeof(float2) * (size_t)nmax;
+}
+
+hipError_t launch_circus_vjp(const CircusVjpParams &p, hipStream_t s)
+{
+    circus_vjp_kernel<<<p.ncols * p.nsino, kBlock, vjp_shm(p.n), s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_circus_vjp_batched(const BatchAdjointParams &p, hipStream_t s)
+{
+    circus_vjp_batched_kernel<<<p.cols, kBlock, vjp_shm(p.nmax), s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_records_batched(const BatchAdjointParams &p, hipStream_t s)
+{
+    int blocks = (p.lines + kBlock - 1) / kBlock;
+    if (blocks > 8192)
+        blocks = 8192;
+    records_batched_kernel<<<blocks, kBlock, 0, s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_batched(const BatchAdjointParams &p, hipStream_t s)
+{
+    const int blocks = (p.pixels + kBlock - 1) / kBlock;
+    const size_t shm = p.do345 ? sizeof(float2) * 3 * (size_t)p.tabn : 0;
+    gather_batched_kernel<<<blocks, kBlock, shm, s>>>(p);
+    return hipGetLastError();
+}
+
+}  // namespace ttk
+
+namespace ttk {
+
+static int grid_for(size_t total)
+{
+    size_t b = (total + kBlock - 1) / kBlock;
+    return (int)(b > 4096 ? 4096 : (b < 1 ? 1 : b));
+}
+
+hipError_t launch_resize(const float *in, int in_rows, int in_cols, float *out,
+                         int rows, int cols, hipStream_t s)
+{
+    resize_kernel<<<grid_for((size_t)rows * cols), kBlock, 0, s>>>(in, in_rows, in_cols,
+                                                                   out, rows, cols);
+    return hipGetLastError();
+}
+
+hipError_t launch_zscore(const float *in, int n, int count, float *out, hipStream_t s)
+{
+    zscore_kernel<<<count, kBlock, 0, s>>>(in, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_column_medians(const float *in, int n, int cols, int32_t *med,
+                                 hipStream_t s)
+{
+    const size_t shm = sizeof(float) * (((size_t)n + 3) & ~(size_t)3);
+    column_median_kernel<<<cols, kBlock, shm, s>>>(in, n, med);
+    return hipGetLastError();
+}
+
+hipError_t launch_align(const float *in, int n, int cols, const int32_t *offset,
+                        int maxoff, int rows_out, double *out, hipStream_t s)
+{
+    align_kernel<<<grid_for((size_t)rows_out * cols), kBlock, 0, s>>>(
+        in, n, cols, offset, maxoff, rows_out, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_float(const double *in, size_t count, float *out, hipStream_t s)
+{
+    to_float_kernel<<<grid_for(count), kBlock, 0, s>>>(in, count, out);
     return hipGetLastError();
 }
 

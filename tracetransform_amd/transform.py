@@ -49,30 +49,27 @@ def getCircusFunctions(input, pfunctionals):
     return [out[p] for p in range(out.shape[0])]
 
 
-def zscore(x: torch.Tensor) -> torch.Tensor:
-    """c++/src/auxiliary.cpp:277-318: float mean, double variance over n-1."""
-    n = x.numel()
-    if n == 0:
-        return x.clone()
-    mean = x.sum(dtype=torch.float32) / n
-    var = ((x.double() - mean.double()) ** 2).sum() / (n - 1)
-    return (x - mean) / var.sqrt().float()
-
-
 class Transformer:
-    """c++/src/transform.cpp:28-112 without the CSV/PGM output."""
+    """c++/src/transform.cpp:28-112 without the CSV/PGM output.
+
+    ``orthonormal`` (all P-functionals Hermite, c++/src/demo.cpp:153-169)
+    stretches the image to ceil(ceil(360/step)/sqrt(2)) squared before
+    padding, replaces each sinogram by its nearest orthonormal sinogram and
+    sets every Hermite P-functional's centre from it."""
 
     def __init__(self, image, basename: str = "", angle_stepsize: float = 1,
                  orthonormal: bool = False):
         img = _as_device_matrix(image)
         if img.dim() != 2:
             raise ValueError("image must be a 2-D matrix")
-        if orthonormal:
-            raise NotImplementedError(
-                "orthonormal mode (resize + nearest orthonormal sinogram) is not "
-                "available on the device yet")
         rows, cols = img.shape
-        self._image = core.pad(img.t().contiguous(), rows, cols)  # [n, n] col-major
+        img_cm = img.t().contiguous()
+        if orthonormal:
+            ndiag = int(math.ceil(360.0 / angle_stepsize))
+            nsize = int(math.ceil(ndiag / math.sqrt(2)))
+            img_cm = core.resize(img_cm, rows, cols, nsize, nsize)
+            rows = cols = nsize
+        self._image = core.pad(img_cm, rows, cols)  # [n, n] column-major
         self._basename = basename
         self._orthonormal = orthonormal
         self._angle_stepsize = angle_stepsize
@@ -83,18 +80,29 @@ class Transformer:
         return self._image.t()
 
     def getTransform(self, tfunctionals, pfunctionals, write_data: bool = False):
+        """Signatures [A, nt*np] (column t*np + p = zscore of circus p of T t);
+        Hermite centres are written back into ``pfunctionals`` wrappers in
+        orthonormal mode, as c++/src/transform.cpp:80-84 does."""
         n_angles = core.angle_count(self._angle_stepsize)
         codes = tcodes_of(tfunctionals)
         pf = pfuncs_of(pfunctionals)
-        if any(p[0] == int(PFunctional.Hermite) for p in pf):
-            raise NotImplementedError("Hermite needs the orthonormal mode")
+        if any(p[0] == int(PFunctional.Hermite) for p in pf) and not self._orthonormal:
+            raise ValueError("Hermite P-functionals need the orthonormal mode")
         sino = core.sinograms(self._image, n_angles, float(self._angle_stepsize), codes)
         if not pf:
             return None
-        circ = core.circus(sino, pf)  # [nt, np, A]
-        sig = torch.empty((n_angles, len(codes) * len(pf)), dtype=torch.float32,
+        sig = torch.empty((len(codes), len(pf), n_angles), dtype=torch.float32,
                           device=sino.device)
         for t in range(len(codes)):
-            for p in range(len(pf)):
-                sig[:, t * len(pf) + p] = zscore(circ[t, p])
-        return sig
+            s_t = sino[t]
+            pf_t = pf
+            if self._orthonormal:
+                s_t, center = core.nearest_orthonormal_sinogram(s_t)
+                pf_t = [(c, o, center) if c == int(PFunctional.Hermite) else (c, o, ce)
+                        for (c, o, ce) in pf]
+                for w in pfunctionals:
+                    if hasattr(w, "arguments") and w.functional == PFunctional.Hermite:
+                        w.arguments.center = center
+            circ = core.circus(s_t.contiguous(), pf_t)  # [np, A]
+            sig[t] = core.zscore(circ)
+        return sig.reshape(len(codes) * len(pf), n_angles).t()
