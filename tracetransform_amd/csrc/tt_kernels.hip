@@ -24,6 +24,13 @@ __device__ __forceinline__ f2u ld2(const float *p)
     return *reinterpret_cast<const f2u *>(p);
 }
 
+// Same load at a 32-bit byte offset from a wave-uniform base: matches the
+// scalar-base + vector-offset form of global loads (no 64-bit VALU math).
+__device__ __forceinline__ f2u ld2b(const float *base, uint32_t byte_off)
+{
+    return *reinterpret_cast<const f2u *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
 __device__ __forceinline__ int wave_id()
 {
     return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -58,21 +65,27 @@ __device__ __forceinline__ float rot_sample(const float *__restrict__ src,
     float y = pxns + py * c;
     x = x + o;
     y = y + o;
-    const bool inb = (x >= 0.0f) && (x < lim) && (y >= 0.0f) && (y < lim);
+    // bitwise, not short-circuit: keeps the test branch-free
+    const bool inb = (x >= 0.0f) & (x < lim) & (y >= 0.0f) & (y < lim);
     const float xi = __builtin_truncf(x);
     const float yi = __builtin_truncf(y);
     const float fx = x - xi;
     const float fy = y - yi;
-    const int X = inb ? (int)xi : 0;
-    const int Y = inb ? (int)yi : 0;
+    // unsigned 32-bit offsets from a wave-uniform base: global loads in
+    // scalar-base + vector-offset form, no 64-bit address arithmetic
+    const uint32_t X = inb ? (uint32_t)xi : 0u;
+    const uint32_t Y = inb ? (uint32_t)yi : 0u;
+    const uint32_t un = (uint32_t)n;
     float s00, s01, s10, s11;
     if (!TR) {
-        const f2u a = ld2(src + (X * n + Y));
-        const f2u b = ld2(src + ((X + 1) * n + Y));
+        const uint32_t o0 = (X * un + Y) * 4u;
+        const f2u a = ld2b(src, o0);
+        const f2u b = ld2b(src, o0 + 4u * un);
         s00 = a.x; s10 = a.y; s01 = b.x; s11 = b.y;
     } else {
-        const f2u a = ld2(src + (Y * n + X));
-        const f2u b = ld2(src + ((Y + 1) * n + X));
+        const uint32_t o0 = (Y * un + X) * 4u;
+        const f2u a = ld2b(src, o0);
+        const f2u b = ld2b(src, o0 + 4u * un);
         s00 = a.x; s01 = a.y; s10 = b.x; s11 = b.y;
     }
     const float ofx = 1.0f - fx;
@@ -224,21 +237,22 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
                     if (2.0f * Ps >= Ss)
                         sm = r;
                 } else if (DO345) {
+                    // one 16-byte and one 8-byte LDS read per row for all three
+                    // tables (rows interleaved by the kernel prologue)
                     const int r1 = r - sm;
+                    const float4 w34 = reinterpret_cast<const float4 *>(s_tab)[r1];
+                    const float2 w5 = s_tab[2 * p.tabn + r1];
                     if (do3) {
-                        const float2 w = s_tab[r1];
-                        f3r = fmaf(w.x, v, f3r);
-                        f3i = fmaf(w.y, v, f3i);
+                        f3r = fmaf(w34.x, v, f3r);
+                        f3i = fmaf(w34.y, v, f3i);
                     }
                     if (do4) {
-                        const float2 w = s_tab[p.tabn + r1];
-                        f4r = fmaf(w.x, v, f4r);
-                        f4i = fmaf(w.y, v, f4i);
+                        f4r = fmaf(w34.z, v, f4r);
+                        f4i = fmaf(w34.w, v, f4i);
                     }
                     if (do5) {
-                        const float2 w = s_tab[2 * p.tabn + r1];
-                        f5r = fmaf(w.x, v, f5r);
-                        f5i = fmaf(w.y, v, f5i);
+                        f5r = fmaf(w5.x, v, f5r);
+                        f5i = fmaf(w5.y, v, f5i);
                     }
                 }
             }
@@ -282,13 +296,26 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
     }
 }
 
+// LDS copy of the T3..T5 tables ([3][tabn] float2 in global memory) with the
+// T3 and T4 entries of a row interleaved as one float4 (rows 0..tabn-1) and
+// T5 after them (float2 at 2*tabn + r).
+__device__ __forceinline__ void stage_tab345(float2 *s_tab, const float2 *g, int tabn,
+                                             int nthreads)
+{
+    float4 *s4 = reinterpret_cast<float4 *>(s_tab);
+    for (int r = threadIdx.x; r < tabn; r += nthreads) {
+        const float2 a = g[r], b = g[tabn + r];
+        s4[r] = float4{a.x, a.y, b.x, b.y};
+        s_tab[2 * tabn + r] = g[2 * tabn + r];
+    }
+}
+
 template <bool NEED_X, bool NEED_S, bool DO345>
 __global__ __launch_bounds__(kBlock) void sino_t05_kernel(SinoParams p)
 {
-    extern __shared__ float2 s_tab[];
+    extern __shared__ __align__(16) float2 s_tab[];
     if (DO345) {
-        for (int i = threadIdx.x; i < 3 * p.n; i += kBlock)
-            s_tab[i] = p.tab345[i];
+        stage_tab345(s_tab, p.tab345, p.n, kBlock);
         __syncthreads();
     }
     const int item = xcd_block() * (kBlock / kWave) + wave_id();
@@ -324,10 +351,9 @@ __global__ __launch_bounds__(kBlock) void sino_t05_kernel(SinoParams p)
 template <bool NEED_X, bool NEED_S, bool DO345>
 __global__ __launch_bounds__(kBlock) void sino_t05_batched_kernel(BatchSinoParams p)
 {
-    extern __shared__ float2 s_tab[];
+    extern __shared__ __align__(16) float2 s_tab[];
     if (DO345) {
-        for (int i = threadIdx.x; i < 3 * p.tabn; i += blockDim.x)
-            s_tab[i] = p.tab345[i];
+        stage_tab345(s_tab, p.tab345, p.tabn, blockDim.x);
         __syncthreads();
     }
     const int lane = threadIdx.x & (kWave - 1);
@@ -1093,16 +1119,18 @@ __device__ __forceinline__ float gather_pixel(const LineRecord *recs, const floa
                 float y = pxns + py * c;
                 x = x + o;
                 y = y + o;
-                if (!((x >= 0.0f) && (x < lim) && (y >= 0.0f) && (y < lim)))
-                    continue;
                 const float xi = __builtin_truncf(x), yi = __builtin_truncf(y);
                 const int Xi = (int)xi, Yi = (int)yi;
-                if ((Xi != X && Xi != X - 1) || (Yi != Y && Yi != Y - 1))
+                const bool hit = (x >= 0.0f) & (x < lim) & (y >= 0.0f) & (y < lim) &
+                                 ((Xi == X) | (Xi == X - 1)) & ((Yi == Y) | (Yi == Y - 1));
+                if (!hit)
                     continue;
                 const float fx = x - xi, fy = y - yi;
                 const float wx = Xi == X ? 1.0f - fx : fx;
                 const float wy = Yi == Y ? 1.0f - fy : fy;
-                const LineRecord r = rec[col];
+                const LineRecord r = *reinterpret_cast<const LineRecord *>(
+                    reinterpret_cast<const char *>(rec) +
+                    (uint32_t)col * (uint32_t)sizeof(LineRecord));
                 float G = r.g0;
                 if (row >= r.m) {
                     const float d = (float)(row - r.m);
