@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtt_amd.so")
+DEMO = os.path.join(HERE, "tt_demo")
 ARCH = os.environ.get("TT_OFFLOAD_ARCH", "gfx950")
 
 # Device code: no FMA contraction (median indices depend on exact float
@@ -71,7 +72,25 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_demo(verbose)
     return LIB
+
+
+def build_demo(verbose: bool = False) -> str:
+    """Host-only C++ front end ``tt_demo`` over the library (cpp/demo.cpp),
+    built with g++ against the HIP runtime API; it finds libtt_amd.so next to
+    itself through an $ORIGIN rpath."""
+    tmp = DEMO + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "cpp"),
+           "-I", "/opt/rocm/include", os.path.join(HERE, "cpp", "demo.cpp"),
+           "-L", HERE, "-ltt_amd", "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, DEMO)
+    return DEMO
 
 
 if __name__ == "__main__":
