@@ -3,7 +3,11 @@
 
 A step is one forward pass of the path over one synthetic image already in
 HBM: sinograms for every requested T-functional (tt_sinograms) followed by
-the circus functions of every sinogram (tt_circus).  ``value`` is sinogram
+the circus functions of every sinogram (tt_circus); for the large workload,
+whose P-functionals include Hermite, the Hermite functions run on the
+nearest orthonormal sinogram of each sinogram with its centre
+(tt_nearest_orthonormal_sinogram, c++/src/transform.cpp:75-92) and P1-P3 on
+the raw sinogram.  ``value`` is sinogram
 lines per second, a line being one (angle, column) trace with all requested
 T-functionals evaluated (lines per image = A * n).
 
@@ -31,12 +35,13 @@ HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
     # One 1024x1024 image, sum of 32 Gaussian blobs (seed 2), padded to 1449,
-    # 360 angles at 1 deg, T0-T7, P1-P3 + Hermite orders 1-5 (centre
-    # floor((n-1)/2) of the raw sinogram; see DESIGN.md on the orthonormal
-    # step).
+    # 360 angles at 1 deg, T0-T7; P1-P3 on each raw sinogram and Hermite
+    # orders 1-5 on its nearest orthonormal sinogram with that sinogram's
+    # centre (no 255x255 stretch).
     "large-1024-blob-T0-T7-P1-P3-H1-H5": dict(
         side=1024, kind="blob", seed=2, step=1.0, tcodes=[0, 1, 2, 3, 4, 5, 6, 7],
-        pfuncs=[(1,), (2,), (3,)] + [(0, k, -1) for k in range(1, 6)]),
+        pfuncs=[(1,), (2,), (3,)] + [(0, k, -1) for k in range(1, 6)],
+        hermite_on="nos"),
     # One 256x256 U[0,1) image (seed 1) padded to 363, 360 angles at 1 deg,
     # T0-T5, P1-P3.
     "plain-256-uniform-T0-T5-P1-P3": dict(
@@ -61,10 +66,10 @@ def make_image(w, side=None, seed=None):
     return inputs.blob_image(side, seed)
 
 
-def resolve_pfuncs(w, n):
-    """Hermite entries carry centre -1 in the table: use floor((n-1)/2)."""
-    return [(p[0], p[1], (n - 1) // 2) if p[0] == 0 and p[2] < 0 else p
-            for p in w["pfuncs"]]
+def split_pfuncs(w):
+    """(regular P-functionals, Hermite orders) of a workload."""
+    return ([p for p in w["pfuncs"] if p[0] != 0],
+            [p[1] for p in w["pfuncs"] if p[0] == 0])
 
 
 def sino_bytes(n, A, nt):
@@ -87,25 +92,52 @@ def cpu_threads():
     return max(1, min(threads, os.cpu_count() or 1))
 
 
+def cpu_transform_nos(tt_oracle, img, n, A, w, threads):
+    """Reference getTransform with P1-P3 on the raw sinograms and Hermite on
+    each nearest orthonormal sinogram (numpy FP64 SVD), then z-scores."""
+    reg, orders = split_pfuncs(w)
+    sino, _ = tt_oracle.sinograms(img, n, A, w["step"], w["tcodes"], threads)
+    sig = []
+    for t in range(len(w["tcodes"])):
+        if reg:
+            c, _ = tt_oracle.circus(sino[t], n, A, reg, threads)
+            sig.extend(tt_oracle.zscore(row) for row in c)
+        if orders:
+            nos, center = tt_oracle.nearest_orthonormal_sinogram(sino[t])
+            c, _ = tt_oracle.circus(nos, nos.shape[1], A,
+                                    [(0, k, center) for k in orders], threads)
+            sig.extend(tt_oracle.zscore(row) for row in c)
+    return sig
+
+
 def cpu_baseline(img_cm, w, n, A, iters, side):
     """OpenMP port of the reference pipeline on this host's cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import tt_oracle
     threads = cpu_threads()
     img_cm = tt_oracle.pad(img_cm.ravel(), side, side)
-    pf = resolve_pfuncs(w, n)
-    tt_oracle.transform(img_cm, n, A, w["step"], w["tcodes"], pf, threads)
+    nos = w.get("hermite_on") == "nos"
+
+    def once():
+        if nos:
+            cpu_transform_nos(tt_oracle, img_cm, n, A, w, threads)
+        else:
+            tt_oracle.transform(img_cm, n, A, w["step"], w["tcodes"], w["pfuncs"], threads)
+
+    once()
     times = []
     for _ in range(iters):
         t0 = time.perf_counter()
-        tt_oracle.transform(img_cm, n, A, w["step"], w["tcodes"], pf, threads)
+        once()
         times.append(time.perf_counter() - t0)
     times.sort()
     med = times[len(times) // 2]
+    what = ("sinograms+circus P1-P3+NOS (numpy FP64 SVD)+Hermite+zscore" if nos
+            else "sinograms+circus+zscore")
     return {"value": A * n / med, "unit": "lines/s", "cores": threads,
             "kind": "port",
             "sample": f"full workload ({A} angles x {n} lines), 1 warm-up + "
-                      f"{iters} timed runs of sinograms+circus+zscore, median"}
+                      f"{iters} timed runs of {what}, median"}
 
 
 def cpu_baseline_batch(imgs_cm, sides, w, A, budget_s=20.0):
@@ -170,7 +202,7 @@ def read_kernels(ctx, steps):
     from tracetransform_amd import _lib
     kinds = (("sino_t05", _lib.TT_KERNEL_SINO_T05), ("sino_sort", _lib.TT_KERNEL_SINO_SORT),
              ("circus", _lib.TT_KERNEL_CIRCUS), ("aux", _lib.TT_KERNEL_AUX),
-             ("batch_graph", _lib.TT_KERNEL_BATCH))
+             ("batch_graph", _lib.TT_KERNEL_BATCH), ("nos", _lib.TT_KERNEL_NOS))
     return {name: ctx.profile_read(k) for name, k in kinds}
 
 
@@ -179,8 +211,7 @@ def timed(ctx, step, steps, warmup):
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
-    for k in range(5):
-        ctx.profile_read(k)
+    read_kernels(ctx, 0)
     ctx.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -212,18 +243,28 @@ def run_single(args, w):
     side = w["side"]
     n, _, _ = core.pad_geometry(side, side)
     A = core.angle_count(w["step"])
-    pf = resolve_pfuncs(w, n)
-    nt, npf = len(w["tcodes"]), len(pf)
+    reg, orders = split_pfuncs(w)
+    if orders and w.get("hermite_on") != "nos":
+        raise ValueError("Hermite P-functionals need the orthonormal sinogram")
+    nt = len(w["tcodes"])
     dev = torch.device("cuda", 0)
     d_raw = torch.from_numpy(np.ascontiguousarray(img_cm)).to(dev)
     d_img = core.pad(d_raw, side, side)
     sino = torch.empty((nt, A, n), dtype=torch.float32, device=dev)
-    circ = torch.empty((nt, npf, A), dtype=torch.float32, device=dev)
+    circ = torch.empty((nt, len(reg), A), dtype=torch.float32, device=dev)
+    circ_h = torch.empty((nt, len(orders), A), dtype=torch.float32, device=dev)
+    centers = [None] * nt
     ctx = core.get_context(0)
 
     def step():
         core.sinograms(d_img, A, w["step"], w["tcodes"], out=sino)
-        core.circus(sino, pf, out=circ)
+        if reg:
+            core.circus(sino, reg, out=circ)
+        if orders:
+            for t in range(nt):
+                nos, centers[t] = core.nearest_orthonormal_sinogram(sino[t])
+                core.circus(nos, [(0, k, centers[t]) for k in orders],
+                            out=circ_h[t:t + 1])
 
     ms_step, kern = timed(ctx, step, args.steps, args.warmup)
     lines = A * n
@@ -237,7 +278,10 @@ def run_single(args, w):
         "config": {"workload": args.workload, "image": f"{side}x{side} {w['kind']} "
                    f"seed {w['seed']}", "padded_n": n, "angles": A,
                    "angle_step_deg": w["step"], "tfunctionals": w["tcodes"],
-                   "pfunctionals": [list(p) for p in pf], "lines_per_step": lines},
+                   "pfunctionals": [p[0] for p in reg] + [f"H{k}" for k in orders],
+                   "hermite_on": ("nearest orthonormal sinogram, centres "
+                                  f"{sorted(set(centers))}") if orders else None,
+                   "lines_per_step": lines},
         "roofline": roof,
         "kernel_ms_per_step": {k: round(v[0] / max(args.steps, 1), 4)
                                for k, v in kern.items() if v[1]},

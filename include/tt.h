@@ -244,7 +244,9 @@ enum {
     TT_KERNEL_GATHER = 5,     /* adjoint: transposed rotation (+ records)   */
     TT_KERNEL_CIRCUS_VJP = 6, /* adjoint of the P-functionals               */
     TT_KERNEL_BATCH_BACKWARD = 7, /* one replay of the batched backward graph */
-    TT_KERNEL_KINDS = 8
+    TT_KERNEL_NOS = 8,        /* nearest orthonormal sinogram after the
+                                 medians: align, gesvd, gemm, to float      */
+    TT_KERNEL_KINDS = 9
 };
 TT_API tt_status tt_profile_enable(tt_context *ctx, int enable);
 TT_API tt_status tt_profile_read(tt_context *ctx, int kind, double *total_ms,

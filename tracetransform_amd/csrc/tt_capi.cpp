@@ -1219,6 +1219,7 @@ TT_API tt_status tt_nearest_orthonormal_sinogram(tt_context *ctx,
     TT_HIP(ctx->nos_s.ensure(sizeof(double) * A));
     TT_HIP(ctx->nos_e.ensure(sizeof(double) * A));
     TT_HIP(ctx->nos_info.ensure(sizeof(rocblas_int)));
+    TT_HIP(ctx->timer.begin(TT_KERNEL_NOS, st));
     TT_HIP(ttk::launch_align(d_sino, n, A, ctx->nos_med.as<int32_t>(), mx, rows,
                              ctx->nos_a.as<double>(), st));
     if (!ctx->blas)
@@ -1237,6 +1238,7 @@ TT_API tt_status tt_nearest_orthonormal_sinogram(tt_context *ctx,
                           ctx->nos_v.as<double>(), A, &zero, ctx->nos_c.as<double>(),
                           rows));
     TT_HIP(ttk::launch_to_float(ctx->nos_c.as<double>(), mA, d_out, st));
+    TT_HIP(ctx->timer.end(st));
     rocblas_int info = 0;
     TT_HIP(hipMemcpyAsync(&info, ctx->nos_info.ptr, sizeof(info), hipMemcpyDeviceToHost,
                           st));
