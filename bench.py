@@ -53,6 +53,20 @@ WORKLOADS = {
     "batched-256-blob-T0-T5-P1-P3": dict(
         batch=256, lo=64, hi=512, kind="blob", seed=3, step=2.0,
         tcodes=[0, 1, 2, 3, 4, 5], pfuncs=[(1,), (2,), (3,)]),
+    # Forward + adjoint of one 512x512 blob image (seed 4) padded to 725, 360
+    # angles at 1 deg, T0/T3/T4/T5, P1-P3, loss = sum of squared circus
+    # differences against a second image (seed 5): a step is sinograms,
+    # circus, dC = 2 (C - C2) and the backward pass to the image.
+    "adjoint-512-blob-T0-T3-T5-P1-P3": dict(
+        side=512, kind="blob", seed=4, target_seed=5, step=1.0, tcodes=[0, 3, 4, 5],
+        pfuncs=[(1,), (2,), (3,)], adjoint=True),
+    # Batched forward + adjoint: 64 blob images with sides uniform in
+    # 256..1024 (seed 6), 720 angles at 0.5 deg, T0/T3/T4/T5, P1-P3, targets
+    # from a second set of images; a step is one batched forward replay, the
+    # loss gradient and one batched backward replay.
+    "batched-adjoint-64-blob-T0-T3-T5-P1-P3": dict(
+        batch=64, lo=256, hi=1024, kind="blob", seed=6, step=0.5, tcodes=[0, 3, 4, 5],
+        pfuncs=[(1,), (2,), (3,)], adjoint=True),
 }
 DEFAULT_WORKLOAD = "large-1024-blob-T0-T7-P1-P3-H1-H5"
 
@@ -202,7 +216,10 @@ def read_kernels(ctx, steps):
     from tracetransform_amd import _lib
     kinds = (("sino_t05", _lib.TT_KERNEL_SINO_T05), ("sino_sort", _lib.TT_KERNEL_SINO_SORT),
              ("circus", _lib.TT_KERNEL_CIRCUS), ("aux", _lib.TT_KERNEL_AUX),
-             ("batch_graph", _lib.TT_KERNEL_BATCH), ("nos", _lib.TT_KERNEL_NOS))
+             ("batch_graph", _lib.TT_KERNEL_BATCH), ("gather", _lib.TT_KERNEL_GATHER),
+             ("circus_vjp", _lib.TT_KERNEL_CIRCUS_VJP),
+             ("batch_backward_graph", _lib.TT_KERNEL_BATCH_BACKWARD),
+             ("nos", _lib.TT_KERNEL_NOS))
     return {name: ctx.profile_read(k) for name, k in kinds}
 
 
@@ -253,6 +270,7 @@ def run_single(args, w):
     sino = torch.empty((nt, A, n), dtype=torch.float32, device=dev)
     circ = torch.empty((nt, len(reg), A), dtype=torch.float32, device=dev)
     circ_h = torch.empty((nt, len(orders), A), dtype=torch.float32, device=dev)
+    nos_ws = torch.empty((nt, A * 3 * n), dtype=torch.float32, device=dev)
     centers = [None] * nt
     ctx = core.get_context(0)
 
@@ -261,9 +279,9 @@ def run_single(args, w):
         if reg:
             core.circus(sino, reg, out=circ)
         if orders:
+            nos, centers[:] = core.nearest_orthonormal_sinograms(sino, out=nos_ws)
             for t in range(nt):
-                nos, centers[t] = core.nearest_orthonormal_sinogram(sino[t])
-                core.circus(nos, [(0, k, centers[t]) for k in orders],
+                core.circus(nos[t], [(0, k, centers[t]) for k in orders],
                             out=circ_h[t:t + 1])
 
     ms_step, kern = timed(ctx, step, args.steps, args.warmup)
@@ -356,6 +374,216 @@ def run_batched(args, w):
     return out
 
 
+def adjoint_bytes(n, A, nt, npf):
+    """Algorithmic bytes of forward + adjoint for one image: three 4*n^2
+    passes per angle (forward, recompute, transposed rotation), the image
+    and its gradient, sinograms written and read, circus and its gradient."""
+    return (12.0 * n * n * A + 8.0 * n * n + 8.0 * n * A * nt
+            + 8.0 * A * nt * npf)
+
+
+def gather_bytes(n, A):
+    """Algorithmic bytes of one transposed-rotation launch: one 4*n^2 pass
+    over the rotated-domain gradient per angle, the gradient written once."""
+    return 4.0 * n * n * A + 4.0 * n * n
+
+
+def cpu_baseline_adjoint(imgs, targets, ns, w, A, budget_s=20.0):
+    """The oracle's FP32 forward (OpenMP) and FP64 analytic backward (OpenMP)
+    per image on this host, over a prefix of the images until ~budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import tt_oracle
+    threads = cpu_threads()
+    done_lines, spent, used = 0, 0.0, 0
+    for img, tgt, n in zip(imgs, targets, ns):
+        t0 = time.perf_counter()
+        sino, aux = tt_oracle.sinograms(img, n, A, w["step"], w["tcodes"], threads)
+        circ, src = [], []
+        for t in range(len(w["tcodes"])):
+            c, a = tt_oracle.circus(sino[t], n, A, w["pfuncs"], threads)
+            circ.append(c)
+            src.append(a["p2_src"])
+        dc = 2.0 * (np.stack(circ).astype(np.float64) - tgt)
+        tt_oracle.backward_f64(img, n, A, w["step"], w["tcodes"], w["pfuncs"], aux,
+                               np.stack(src), dc, threads)
+        spent += time.perf_counter() - t0
+        done_lines += A * n
+        used += 1
+        if spent >= budget_s:
+            break
+    return {"value": done_lines / spent, "unit": "lines/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {used} of {len(imgs)} images one after another: FP32 "
+                      f"forward + FP64 analytic backward of the oracle, {spent:.1f} s"}
+
+
+def run_adjoint(args, w):
+    """Forward + adjoint of one image (plain path: it is one image)."""
+    import numpy as np
+    import torch
+    from tracetransform_amd import core
+
+    side = w["side"]
+    n, _, _ = core.pad_geometry(side, side)
+    A = core.angle_count(w["step"])
+    nt, npf = len(w["tcodes"]), len(w["pfuncs"])
+    dev = torch.device("cuda", 0)
+    img_cm = make_image(w)
+    tgt_cm = make_image(w, seed=w["target_seed"])
+    d_img = core.pad(torch.from_numpy(np.ascontiguousarray(img_cm)).to(dev), side, side)
+    d_tgt = core.pad(torch.from_numpy(np.ascontiguousarray(tgt_cm)).to(dev), side, side)
+    target = core.circus(core.sinograms(d_tgt, A, w["step"], w["tcodes"]), w["pfuncs"])
+    sino = torch.empty((nt, A, n), dtype=torch.float32, device=dev)
+    circ = torch.empty((nt, npf, A), dtype=torch.float32, device=dev)
+    dcirc = torch.empty_like(circ)
+    grad = [None]
+
+    def step():
+        core.sinograms(d_img, A, w["step"], w["tcodes"], out=sino)
+        core.circus(sino, w["pfuncs"], out=circ)
+        torch.sub(circ, target, out=dcirc)
+        dcirc.mul_(2.0)
+        grad[0] = core.backward(d_img, A, w["step"], w["tcodes"], w["pfuncs"], dcirc)
+
+    ctx = core.get_context(0)
+    ms_step, kern = timed(ctx, step, args.steps, args.warmup)
+    lines = A * n
+    ms_g, cnt_g = kern["gather"]
+    avg_g = ms_g / max(cnt_g, 1)
+    gb = gather_bytes(n, A)
+    total = adjoint_bytes(n, A, nt, npf)
+    out = {
+        "metric": "sinogram lines/s", "value": round(lines / (ms_step * 1e-3), 1),
+        "unit": "lines/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "image": f"{side}x{side} {w['kind']} "
+                   f"seed {w['seed']}, target seed {w['target_seed']}", "padded_n": n,
+                   "angles": A, "angle_step_deg": w["step"], "tfunctionals": w["tcodes"],
+                   "pfunctionals": [p[0] for p in w["pfuncs"]], "lines_per_step": lines,
+                   "step": "forward + loss gradient + backward"},
+        "roofline": {"bound": "hbm", "achieved": round(gb / (avg_g * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gb / (avg_g * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic(args.workload),
+                     "kernel": "gather_kernel (transposed rotation, with line records)",
+                     "avg_launch_ms": round(avg_g, 4),
+                     "algorithmic_bytes_per_launch": gb},
+        "step_roofline": {"algorithmic_bytes": total,
+                          "achieved_gbs": round(total / (ms_step * 1e-3) / 1e9, 1),
+                          "frac": round(total / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "kernel_ms_per_step": {k: round(v[0] / max(args.steps, 1), 4)
+                               for k, v in kern.items() if v[1]},
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import tt_oracle
+        p_img = tt_oracle.pad(img_cm.ravel(), side, side)
+        p_tgt = tt_oracle.pad(tgt_cm.ravel(), side, side)
+        s2, _ = tt_oracle.sinograms(p_tgt, n, A, w["step"], w["tcodes"], cpu_threads())
+        tgt = np.stack([tt_oracle.circus(s2[t], n, A, w["pfuncs"], cpu_threads())[0]
+                        for t in range(nt)]).astype(np.float64)
+        out["cpu_baseline"] = cpu_baseline_adjoint([p_img], [tgt], [n], w, A)
+    return out
+
+
+def run_batched_adjoint(args, w):
+    """Batched forward + adjoint replays beside the plain per-image path."""
+    import numpy as np
+    import torch
+    from tracetransform_amd import core, inputs
+
+    sides = inputs.batch_sides(w["batch"], w["lo"], w["hi"], w["seed"])
+    dev = torch.device("cuda", 0)
+    imgs_cm, imgs, tgts = [], [], []
+    for k, sd in enumerate(sides):
+        sd = int(sd)
+        a = make_image(w, side=sd, seed=w["seed"] * 1000 + k)
+        b = make_image(w, side=sd, seed=w["seed"] * 1000 + 500 + k)
+        imgs_cm.append(a)
+        imgs.append(core.pad(torch.from_numpy(np.ascontiguousarray(a)).to(dev), sd, sd))
+        tgts.append(core.pad(torch.from_numpy(np.ascontiguousarray(b)).to(dev), sd, sd))
+    A = core.angle_count(w["step"])
+    ns = [im.shape[0] for im in imgs]
+    nt, npf = len(w["tcodes"]), len(w["pfuncs"])
+    lines = A * sum(ns)
+    ctx = core.get_context(0)
+    target = torch.stack([core.circus(core.sinograms(t, A, w["step"], w["tcodes"]),
+                                      w["pfuncs"]) for t in tgts])
+    del tgts
+    fwd = core.Batch(imgs, A, w["step"], w["tcodes"], w["pfuncs"])
+    bwd = core.BatchBackward(imgs, A, w["step"], w["tcodes"], w["pfuncs"])
+
+    def step():
+        fwd.run()
+        torch.sub(fwd.circus_all, target, out=bwd.dcircus_all)
+        bwd.dcircus_all.mul_(2.0)
+        bwd.run()
+
+    fwd.run(autotune=bool(args.autotune))
+    step()
+    torch.cuda.synchronize()
+    ms_step, kern = timed(ctx, step, args.steps, args.warmup)
+
+    sinos = [torch.empty((nt, A, n), dtype=torch.float32, device=dev) for n in ns]
+    circs = torch.empty((len(ns), nt, npf, A), dtype=torch.float32, device=dev)
+    dcircs = torch.empty_like(circs)
+
+    def plain():
+        for k, im in enumerate(imgs):
+            core.sinograms(im, A, w["step"], w["tcodes"], out=sinos[k])
+            core.circus(sinos[k], w["pfuncs"], out=circs[k])
+        torch.sub(circs, target, out=dcircs)
+        dcircs.mul_(2.0)
+        for k, im in enumerate(imgs):
+            core.backward(im, A, w["step"], w["tcodes"], w["pfuncs"], dcircs[k])
+
+    ms_plain, _ = timed(ctx, plain, max(2, args.steps // 2), 1)
+    total = sum(adjoint_bytes(n, A, nt, npf) for n in ns)
+    ms_f, cnt_f = kern["batch_graph"]
+    ms_b, cnt_b = kern["batch_backward_graph"]
+    avg_b = ms_b / max(cnt_b, 1)
+    bb = sum(8.0 * n * n * A + 8.0 * n * n for n in ns)  # recompute + gather passes
+    out = {
+        "metric": "sinogram lines/s", "value": round(lines / (ms_step * 1e-3), 1),
+        "unit": "lines/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "images": len(ns),
+                   "sides": f"{int(min(sides))}..{int(max(sides))}",
+                   "padded_n": f"{min(ns)}..{max(ns)}", "angles": A,
+                   "angle_step_deg": w["step"], "tfunctionals": w["tcodes"],
+                   "pfunctionals": [p[0] for p in w["pfuncs"]], "lines_per_step": lines,
+                   "step": "batched forward + loss gradient + batched backward",
+                   "launch_shape": list(fwd.launch_shape())},
+        "roofline": {"bound": "hbm", "achieved": round(bb / (avg_b * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bb / (avg_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic(args.workload),
+                     "kernel": "batched backward graph (recompute, circus VJP, records, "
+                               "gather)",
+                     "avg_launch_ms": round(avg_b, 4),
+                     "algorithmic_bytes_per_launch": bb},
+        "step_roofline": {"algorithmic_bytes": total,
+                          "achieved_gbs": round(total / (ms_step * 1e-3) / 1e9, 1),
+                          "frac": round(total / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "graph_ms_per_step": {"forward": round(ms_f / max(cnt_f, 1), 4),
+                              "backward": round(avg_b, 4)},
+        "plain_per_image": {"ms_per_step": round(ms_plain, 4),
+                            "value": round(lines / (ms_plain * 1e-3), 1),
+                            "batched_speedup": round(ms_plain / ms_step, 3)},
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import tt_oracle
+        tgt_np = target.cpu().numpy().astype(np.float64)
+        p_imgs = [tt_oracle.pad(a.ravel(), int(sd), int(sd)) for a, sd in
+                  zip(imgs_cm[:8], sides[:8])]
+        out["cpu_baseline"] = cpu_baseline_adjoint(p_imgs, list(tgt_np[:8]), ns[:8], w, A)
+    return out
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -367,7 +595,10 @@ def main():
         print(json.dumps(calibrate_copy(args.calibrate_copy_mib)), flush=True)
         return
     w = WORKLOADS[args.workload]
-    out = run_batched(args, w) if "batch" in w else run_single(args, w)
+    if w.get("adjoint"):
+        out = run_batched_adjoint(args, w) if "batch" in w else run_adjoint(args, w)
+    else:
+        out = run_batched(args, w) if "batch" in w else run_single(args, w)
     print(json.dumps(out), flush=True)
 
 

@@ -147,7 +147,7 @@ TT_API tt_status tt_zscore(tt_context *ctx, const float *d_in, int n, int count,
 /* Replaces nearest_orthonormal_sinogram (c++/src/circus.cpp:68-102): column
  * medians, alignment to the sinogram centre, then the polar factor
  * U[:, :A] V^T of the aligned (n + |max| + |min|) x A matrix, computed in
- * double (rocSOLVER gesvd, rocBLAS gemm) and stored as float.  d_out holds
+ * double (rocSOLVER gesvdj, rocBLAS gemm) and stored as float.  d_out holds
  * out_capacity_rows x A floats; *out_rows receives the aligned height and
  * *center the Hermite centre (c++/src/transform.cpp:78-84).  The aligned
  * height must be at least A (the reference's U block needs it). */
@@ -156,6 +156,18 @@ TT_API tt_status tt_nearest_orthonormal_sinogram(tt_context *ctx,
                                                  int n_cols, float *d_out,
                                                  int out_capacity_rows,
                                                  int *out_rows, int *center);
+
+/* nearest_orthonormal_sinogram (c++/src/circus.cpp:68-102) of n_sino
+ * sinograms (d_sino: n_sino x n_cols x n, as tt_sinograms writes them) in
+ * one batched solve, as getTransform applies it to each T-functional's
+ * sinogram (c++/src/transform.cpp:75-84).  Sinogram t's factor is an
+ * out_rows[t] x n_cols column-major matrix at d_out + t * out_capacity_rows
+ * * n_cols; centers[t] is its Hermite centre (host arrays of n_sino). */
+TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
+                                                  const float *d_sino, int n,
+                                                  int n_cols, int n_sino, float *d_out,
+                                                  int out_capacity_rows,
+                                                  int *out_rows, int *centers);
 
 /* ---- batched forward (no reference counterpart) ------------------------- */
 

@@ -113,8 +113,28 @@ float tto_interpolate(const float *src, int n, float x, float y)
     return interp_ld(src, n, x, y);
 }
 
+/* Bilinear sample whose taps outside the in_rows x in_cols matrix read 0. */
+static float interp_zero_ext(const float *src, int in_rows, int in_cols, float x,
+                             float y)
+{
+    float x_int, y_int;
+    float x_fract = modff(x, &x_int);
+    float y_fract = modff(y, &y_int);
+    long xi = (long)x_int, yi = (long)y_int;
+#define TAP(r, c) (((r) >= 0 && (r) < in_rows && (c) >= 0 && (c) < in_cols) \
+                       ? src[(r) + (c) * (long)in_rows] : 0.0f)
+    float s00 = TAP(yi, xi), s01 = TAP(yi, xi + 1);
+    float s10 = TAP(yi + 1, xi), s11 = TAP(yi + 1, xi + 1);
+#undef TAP
+    return s00 * (1 - x_fract) * (1 - y_fract) + s01 * x_fract * (1 - y_fract) +
+           s10 * (1 - x_fract) * y_fract + s11 * x_fract * y_fract;
+}
+
 /* c++/src/auxiliary.cpp:198-221 (resize): borders stay zero; interior
- * p = ((col,row) + 0.5) * diag(in_rows/rows, in_cols/cols) - 0.5. */
+ * p = ((col,row) + 0.5) * diag(in_rows/rows, in_cols/cols) - 0.5.  The
+ * reference scales the column coordinate by the row ratio and vice versa,
+ * so a non-square input reads outside the matrix (undefined there); here
+ * such taps read 0.  Square inputs never reach them. */
 void tto_resize(const float *in, int in_rows, int in_cols, float *out,
                 int rows, int cols)
 {
@@ -128,7 +148,8 @@ void tto_resize(const float *in, int in_rows, int in_cols, float *out,
             float y = px * 0.0f + py * t11;
             x = x - 0.5f;
             y = y - 0.5f;
-            out[(size_t)row + (size_t)col * rows] = interp_ld(in, in_rows, x, y);
+            out[(size_t)row + (size_t)col * rows] =
+                interp_zero_ext(in, in_rows, in_cols, x, y);
         }
 }
 

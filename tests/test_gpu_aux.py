@@ -54,6 +54,31 @@ def test_nos_matches_fp64_polar_factor(oracle, gpu):
         assert np.abs(gram - np.eye(A)).max() <= 1e-5
 
 
+def test_nos_batched_matches_single_and_fp64(oracle, gpu):
+    # sinograms whose columns peak at different positions, so each has its
+    # own offsets, aligned height and centre
+    rng = np.random.default_rng(6)
+    n, A = 181, 72
+    rows_idx = np.arange(n)
+    sino = np.empty((4, A, n), np.float32)
+    for t in range(4):
+        for a in range(A):
+            peak = n // 2 + int(round((t + 1) * 9 * np.sin(2 * np.pi * a / A + t)))
+            sino[t, a] = rng.random(n) * 0.3 + np.exp(-0.5 * ((rows_idx - peak) / 12.0) ** 2)
+    views, centers = core.nearest_orthonormal_sinograms(dev(sino))
+    assert len(set(v.shape[1] for v in views)) > 1   # uneven heights in one batch
+    for t in range(4):
+        single, c1 = core.nearest_orthonormal_sinogram(dev(sino[t]))
+        ref, rc = oracle.nearest_orthonormal_sinogram(sino[t])
+        print(f"[nos batch] t={t} rows={ref.shape[1]} center={rc}", flush=True)
+        assert centers[t] == c1 == rc and tuple(views[t].shape) == ref.shape
+        got = views[t].cpu().numpy()
+        assert normwise_err(got, single.cpu().numpy()) <= 1e-6
+        assert normwise_err(got, ref) <= 1e-6
+        q = views[t].double()
+        assert np.abs((q @ q.t()).cpu().numpy() - np.eye(A)).max() <= 1e-5
+
+
 def test_transformer_orthonormal_pipeline(oracle, gpu):
     import tracetransform_amd as tt
     img = torch.from_numpy(inputs.uniform_image(64, 9).T.copy())

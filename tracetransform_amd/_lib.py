@@ -27,7 +27,7 @@ EXPORTED = (
     "tt_forward_batched", "tt_context_set_autotune", "tt_batched_launch_shape",
     "tt_circus_backward", "tt_sinograms_backward", "tt_backward",
     "tt_resize", "tt_zscore", "tt_nearest_orthonormal_sinogram",
-    "tt_backward_batched",
+    "tt_nearest_orthonormal_sinograms", "tt_backward_batched",
 )
 
 

@@ -217,8 +217,11 @@ class Batch:
         nt, npf, A = len(self.codes), len(self.pf), self.n_angles
         self.sino = [torch.empty((nt, A, im.shape[0]), dtype=torch.float32, device=dev)
                      for im in self.images]
-        self.circus = [torch.empty((nt, npf, A), dtype=torch.float32, device=dev)
-                       if npf else None for _ in self.images]
+        # one [nimg, nt, np, A] tensor so callers can form losses in one op
+        self.circus_all = torch.empty((len(self.images), nt, npf, A), dtype=torch.float32,
+                                      device=dev) if npf else None
+        self.circus = [self.circus_all[k] if npf else None
+                       for k in range(len(self.images))]
         arr = (_lib.tt_image * len(self.images))()
         for k, im in enumerate(self.images):
             arr[k].d_image = im.data_ptr()
@@ -380,6 +383,35 @@ def nearest_orthonormal_sinogram(sino: torch.Tensor):
     return nos, center.value
 
 
+def nearest_orthonormal_sinograms(sino: torch.Tensor, out: torch.Tensor | None = None):
+    """nearest_orthonormal_sinogram of every sinogram of ``sino``
+    ([n_sino, A, n], as ``sinograms`` returns them) in one batched solve.
+    Returns (list of [A, n'_t] views, list of centres); ``out`` is an
+    optional [n_sino, A * 3n] workspace the views point into."""
+    _require_cuda_f32("sino", sino)
+    if sino.dim() != 3:
+        raise ValueError("sino must be [n_sino, A, n]")
+    n_sino, A, n = sino.shape
+    cap = 3 * n
+    if out is None:
+        out = torch.empty((n_sino, A * cap), dtype=torch.float32, device=sino.device)
+    else:
+        _require_cuda_f32("out", out)
+        if tuple(out.shape) != (n_sino, A * cap):
+            raise ValueError(f"out must have shape {(n_sino, A * cap)}")
+    rows = (C.c_int * n_sino)()
+    centers = (C.c_int * n_sino)()
+    ctx = get_context(sino.device.index)
+    ctx.bind_current_stream()
+    _lib.check("tt_nearest_orthonormal_sinograms",
+               _lib.load().tt_nearest_orthonormal_sinograms(
+                   ctx.handle, _dptr(sino), n, A, n_sino, _dptr(out), cap, rows, centers))
+    # sinogram t's factor is a rows[t] x A column-major matrix at the start
+    # of out[t]
+    views = [out[t, : rows[t] * A].view(A, rows[t]) for t in range(n_sino)]
+    return views, [centers[t] for t in range(n_sino)]
+
+
 class BatchBackward:
     """A batch for ``tt_backward_batched``: per image an upstream gradient
     [nt, np, A] in, the image gradient [n, n] out; reusing the object replays
@@ -394,8 +426,10 @@ class BatchBackward:
         self.pf = pfuncs_of(pfunctionals)
         dev = self.images[0].device
         nt, npf = len(self.codes), len(self.pf)
-        self.dcircus = [torch.zeros((nt, npf, self.n_angles), dtype=torch.float32,
-                                    device=dev) for _ in self.images]
+        # one [nimg, nt, np, A] tensor; dcircus[k] is image k's view
+        self.dcircus_all = torch.zeros((len(self.images), nt, npf, self.n_angles),
+                                       dtype=torch.float32, device=dev)
+        self.dcircus = [self.dcircus_all[k] for k in range(len(self.images))]
         self.dimage = [torch.empty_like(im) for im in self.images]
         arr = (_lib.tt_image_grad * len(self.images))()
         for k, im in enumerate(self.images):

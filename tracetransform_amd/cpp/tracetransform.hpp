@@ -418,19 +418,20 @@ class Transformer {
                             nullptr),
                   "getCircusFunctions");
         } else {
+            // every sinogram's nearest orthonormal sinogram in one batched solve
             const int cap = 3 * n;
-            _d_nos.reserve((size_t)cap * A);
+            _d_nos.reserve((size_t)nt * cap * A);
+            std::vector<int> rows(nt), centers(nt);
+            check(tt_nearest_orthonormal_sinograms(ctx, _d_sino.get(), n, A, nt, _d_nos.get(),
+                                                   cap, rows.data(), centers.data()),
+                  "nearest_orthonormal_sinogram");
             for (int t = 0; t < nt; t++) {
-                int rows = 0, center = 0;
-                check(tt_nearest_orthonormal_sinogram(ctx, _d_sino.get() + (size_t)t * A * n, n,
-                                                      A, _d_nos.get(), cap, &rows, &center),
-                      "nearest_orthonormal_sinogram");
                 for (auto &p : pfunctionals)
                     if (p.functional == PFunctional::Hermite)
-                        p.arguments.center = (size_t)center;
+                        p.arguments.center = (size_t)centers[t];
                 const std::vector<tt_pfunc> pf = pfuncs(pfunctionals);
-                check(tt_circus(ctx, _d_nos.get(), rows, A, 1, pf.data(), np,
-                                _d_circus.get() + (size_t)t * np * A, nullptr),
+                check(tt_circus(ctx, _d_nos.get() + (size_t)t * cap * A, rows[t], A, 1,
+                                pf.data(), np, _d_circus.get() + (size_t)t * np * A, nullptr),
                       "getCircusFunctions");
             }
         }

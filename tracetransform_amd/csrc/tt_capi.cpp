@@ -271,7 +271,7 @@ struct tt_context {
     hipStream_t stream = nullptr;
     DevBuf imgT, trig, tab345, dft_tw, herm_w, io_a, io_b, med_x, med_s;
     DevBuf bw_sino, bw_dsino, bw_z, bw_rec;
-    DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info;
+    DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
     rocblas_handle blas = nullptr;
     int trig_A = -1;
     float trig_step = 0.0f;
@@ -427,7 +427,7 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
                       &ctx->med_s, &ctx->bw_sino, &ctx->bw_dsino, &ctx->bw_z,
                       &ctx->bw_rec, &ctx->nos_med, &ctx->nos_a, &ctx->nos_u,
                       &ctx->nos_v, &ctx->nos_s, &ctx->nos_e, &ctx->nos_c,
-                      &ctx->nos_info};
+                      &ctx->nos_info, &ctx->nos_sweeps};
     for (DevBuf *b : bufs)
         b->release();
     ctx->timer.release();
@@ -1177,75 +1177,113 @@ TT_API tt_status tt_zscore(tt_context *ctx, const float *d_in, int n, int count,
     return TT_OK;
 }
 
+TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
+                                                  const float *d_sino, int n,
+                                                  int n_cols, int n_sino, float *d_out,
+                                                  int out_capacity_rows,
+                                                  int *out_rows, int *centers)
+{
+    TT_CHECK(ctx && d_sino && d_out && out_rows && centers, "NULL argument");
+    TT_CHECK(n >= 2 && n <= 2730 && n_cols >= 1, "invalid sinogram size %d x %d", n,
+             n_cols);
+    TT_CHECK(n_sino >= 1, "n_sino must be >= 1 (got %d)", n_sino);
+    TT_TRY(ctx->set_device());
+    hipStream_t st = ctx->stream;
+    const int A = n_cols;
+    const size_t cols_all = (size_t)n_sino * A;
+    // weighted medians of every column of every sinogram, one launch
+    TT_HIP(ctx->nos_med.ensure(sizeof(int32_t) * cols_all));
+    TT_HIP(ttk::launch_column_medians(d_sino, n, (int)cols_all, ctx->nos_med.as<int32_t>(),
+                                      st));
+    std::vector<int32_t> med(cols_all);
+    TT_HIP(hipMemcpyAsync(med.data(), ctx->nos_med.ptr, sizeof(int32_t) * cols_all,
+                          hipMemcpyDeviceToHost, st));
+    TT_HIP(hipStreamSynchronize(st));
+    // c++/src/circus.cpp:72-84: offsets to floor((n-1)/2), aligned height
+    // n + |max| + |min| and the Hermite centre floor((n-1)/2) + max
+    const int sc = (int)floor((n - 1) / 2.0);
+    std::vector<int> maxoff(n_sino);
+    int rows_max = 0;
+    for (int t = 0; t < n_sino; t++) {
+        int mn = 1 << 30, mx = -(1 << 30);
+        for (int c = 0; c < A; c++) {
+            int32_t &o = med[(size_t)t * A + c];
+            o -= sc;
+            mn = o < mn ? o : mn;
+            mx = o > mx ? o : mx;
+        }
+        const int rows = n + abs(mx) + abs(mn);
+        TT_CHECK(rows >= A, "sinogram %d: aligned sinogram %d x %d is wider than tall", t,
+                 rows, A);
+        TT_CHECK(out_capacity_rows >= rows, "sinogram %d: output holds %d rows, %d needed",
+                 t, out_capacity_rows, rows);
+        out_rows[t] = rows;
+        centers[t] = sc + mx;
+        maxoff[t] = mx;
+        rows_max = rows > rows_max ? rows : rows_max;
+    }
+    TT_HIP(hipMemcpyAsync(ctx->nos_med.ptr, med.data(), sizeof(int32_t) * cols_all,
+                          hipMemcpyHostToDevice, st));
+    // Every aligned matrix is zero-padded to rows_max rows so one batched
+    // solver call covers them all.  Zero rows leave the polar factor's other
+    // rows unchanged and stay exactly zero through the Householder QR that
+    // forms U, so row t of the batch is sinogram t's factor plus zero rows.
+    const size_t mA = (size_t)rows_max * A, AA = (size_t)A * A;
+    TT_HIP(ctx->nos_a.ensure(sizeof(double) * mA * n_sino));
+    TT_HIP(ctx->nos_u.ensure(sizeof(double) * mA * n_sino));
+    TT_HIP(ctx->nos_c.ensure(sizeof(double) * mA * n_sino));
+    TT_HIP(ctx->nos_v.ensure(sizeof(double) * AA * n_sino));
+    TT_HIP(ctx->nos_s.ensure(sizeof(double) * (size_t)A * n_sino));
+    TT_HIP(ctx->nos_e.ensure(sizeof(double) * (size_t)n_sino));
+    TT_HIP(ctx->nos_info.ensure(sizeof(rocblas_int) * n_sino));
+    TT_HIP(ctx->nos_sweeps.ensure(sizeof(rocblas_int) * n_sino));
+    TT_HIP(ctx->timer.begin(TT_KERNEL_NOS, st));
+    for (int t = 0; t < n_sino; t++)
+        TT_HIP(ttk::launch_align(d_sino + (size_t)t * A * n, n, A,
+                                 ctx->nos_med.as<int32_t>() + (size_t)t * A, maxoff[t],
+                                 rows_max, ctx->nos_a.as<double>() + mA * t, st));
+    if (!ctx->blas)
+        TT_BLAS(rocblas_create_handle(&ctx->blas));
+    TT_BLAS(rocblas_set_stream(ctx->blas, st));
+    TT_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
+    // aligned = U S V^T (thin U, full V^T): V from Jacobi eigenvectors of
+    // A^T A to machine precision, U from a QR of A V; NOS = U V^T
+    TT_BLAS(rocsolver_dgesvdj_strided_batched(
+        ctx->blas, rocblas_svect_singular, rocblas_svect_all, rows_max, A,
+        ctx->nos_a.as<double>(), rows_max, (rocblas_stride)mA, 0.0,
+        ctx->nos_e.as<double>(), 100, ctx->nos_sweeps.as<rocblas_int>(),
+        ctx->nos_s.as<double>(), (rocblas_stride)A, ctx->nos_u.as<double>(), rows_max,
+        (rocblas_stride)mA, ctx->nos_v.as<double>(), A, (rocblas_stride)AA,
+        ctx->nos_info.as<rocblas_int>(), n_sino));
+    const double one = 1.0, zero = 0.0;
+    TT_BLAS(rocblas_dgemm_strided_batched(
+        ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A, &one,
+        ctx->nos_u.as<double>(), rows_max, (rocblas_stride)mA, ctx->nos_v.as<double>(), A,
+        (rocblas_stride)AA, &zero, ctx->nos_c.as<double>(), rows_max, (rocblas_stride)mA,
+        n_sino));
+    for (int t = 0; t < n_sino; t++)
+        TT_HIP(ttk::launch_to_float(ctx->nos_c.as<double>() + mA * t, rows_max, out_rows[t],
+                                    A, d_out + (size_t)t * out_capacity_rows * A, st));
+    TT_HIP(ctx->timer.end(st));
+    std::vector<rocblas_int> info(n_sino);
+    TT_HIP(hipMemcpyAsync(info.data(), ctx->nos_info.ptr, sizeof(rocblas_int) * n_sino,
+                          hipMemcpyDeviceToHost, st));
+    TT_HIP(hipStreamSynchronize(st));
+    for (int t = 0; t < n_sino; t++)
+        if (info[t] != 0)
+            return fail(TT_ERR_DEVICE, "sinogram %d: Jacobi SVD did not converge (info %d)",
+                        t, (int)info[t]);
+    return TT_OK;
+}
+
 TT_API tt_status tt_nearest_orthonormal_sinogram(tt_context *ctx,
                                                  const float *d_sino, int n,
                                                  int n_cols, float *d_out,
                                                  int out_capacity_rows,
                                                  int *out_rows, int *center)
 {
-    TT_CHECK(ctx && d_sino && d_out && out_rows && center, "NULL argument");
-    TT_CHECK(n >= 2 && n <= 2730 && n_cols >= 1, "invalid sinogram size %d x %d", n,
-             n_cols);
-    TT_TRY(ctx->set_device());
-    hipStream_t st = ctx->stream;
-    const int A = n_cols;
-    TT_HIP(ctx->nos_med.ensure(sizeof(int32_t) * A));
-    TT_HIP(ttk::launch_column_medians(d_sino, n, A, ctx->nos_med.as<int32_t>(), st));
-    std::vector<int32_t> med(A);
-    TT_HIP(hipMemcpyAsync(med.data(), ctx->nos_med.ptr, sizeof(int32_t) * A,
-                          hipMemcpyDeviceToHost, st));
-    TT_HIP(hipStreamSynchronize(st));
-    // c++/src/circus.cpp:72-84
-    const int sc = (int)floor((n - 1) / 2.0);
-    int mn = 1 << 30, mx = -(1 << 30);
-    for (int c = 0; c < A; c++) {
-        med[c] -= sc;
-        mn = med[c] < mn ? med[c] : mn;
-        mx = med[c] > mx ? med[c] : mx;
-    }
-    const int rows = n + abs(mx) + abs(mn);
-    *out_rows = rows;
-    *center = sc + mx;
-    TT_CHECK(rows >= A, "aligned sinogram %d x %d is wider than tall", rows, A);
-    TT_CHECK(out_capacity_rows >= rows, "output holds %d rows, %d needed",
-             out_capacity_rows, rows);
-    TT_HIP(hipMemcpyAsync(ctx->nos_med.ptr, med.data(), sizeof(int32_t) * A,
-                          hipMemcpyHostToDevice, st));
-    const size_t mA = (size_t)rows * A;
-    TT_HIP(ctx->nos_a.ensure(sizeof(double) * mA));
-    TT_HIP(ctx->nos_u.ensure(sizeof(double) * mA));
-    TT_HIP(ctx->nos_c.ensure(sizeof(double) * mA));
-    TT_HIP(ctx->nos_v.ensure(sizeof(double) * (size_t)A * A));
-    TT_HIP(ctx->nos_s.ensure(sizeof(double) * A));
-    TT_HIP(ctx->nos_e.ensure(sizeof(double) * A));
-    TT_HIP(ctx->nos_info.ensure(sizeof(rocblas_int)));
-    TT_HIP(ctx->timer.begin(TT_KERNEL_NOS, st));
-    TT_HIP(ttk::launch_align(d_sino, n, A, ctx->nos_med.as<int32_t>(), mx, rows,
-                             ctx->nos_a.as<double>(), st));
-    if (!ctx->blas)
-        TT_BLAS(rocblas_create_handle(&ctx->blas));
-    TT_BLAS(rocblas_set_stream(ctx->blas, st));
-    // aligned = U S V^T (thin U, full V^T); NOS = U V^T
-    TT_BLAS(rocsolver_dgesvd(ctx->blas, rocblas_svect_singular, rocblas_svect_all, rows,
-                             A, ctx->nos_a.as<double>(), rows, ctx->nos_s.as<double>(),
-                             ctx->nos_u.as<double>(), rows, ctx->nos_v.as<double>(), A,
-                             ctx->nos_e.as<double>(), rocblas_outofplace,
-                             ctx->nos_info.as<rocblas_int>()));
-    const double one = 1.0, zero = 0.0;
-    TT_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
-    TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, rows,
-                          A, A, &one, ctx->nos_u.as<double>(), rows,
-                          ctx->nos_v.as<double>(), A, &zero, ctx->nos_c.as<double>(),
-                          rows));
-    TT_HIP(ttk::launch_to_float(ctx->nos_c.as<double>(), mA, d_out, st));
-    TT_HIP(ctx->timer.end(st));
-    rocblas_int info = 0;
-    TT_HIP(hipMemcpyAsync(&info, ctx->nos_info.ptr, sizeof(info), hipMemcpyDeviceToHost,
-                          st));
-    TT_HIP(hipStreamSynchronize(st));
-    if (info != 0)
-        return fail(TT_ERR_DEVICE, "gesvd did not converge (info %d)", (int)info);
-    return TT_OK;
+    return tt_nearest_orthonormal_sinograms(ctx, d_sino, n, n_cols, 1, d_out,
+                                            out_capacity_rows, out_rows, center);
 }
 
 TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *images,

@@ -170,7 +170,7 @@ hipError_t launch_column_medians(const float *in, int n, int cols, int32_t *med,
                                  hipStream_t s);
 hipError_t launch_align(const float *in, int n, int cols, const int32_t *offset,
                         int maxoff, int rows_out, double *out, hipStream_t s);
-hipError_t launch_to_float(const double *in, size_t count, float *out,
+hipError_t launch_to_float(const double *in, int ld, int rows, int cols, float *out,
                            hipStream_t s);
 
 hipError_t launch_records(const RecordParams &p, hipStream_t s);

@@ -1409,10 +1409,16 @@ __global__ __launch_bounds__(kBlock) void resize_kernel(const float *in, int in_
             const float xi = __builtin_truncf(x), yi = __builtin_truncf(y);
             const float fx = x - xi, fy = y - yi;
             const int X = (int)xi, Y = (int)yi;
-            const float s00 = in[(size_t)X * in_rows + Y];
-            const float s01 = in[(size_t)(X + 1) * in_rows + Y];
-            const float s10 = in[(size_t)X * in_rows + Y + 1];
-            const float s11 = in[(size_t)(X + 1) * in_rows + Y + 1];
+            // The reference scales the column coordinate by the row ratio
+            // (and vice versa), so a non-square input reaches past the
+            // matrix; those taps read 0 here.
+            auto tap = [&](int r, int c) {
+                return (r >= 0 && r < in_rows && c >= 0 && c < in_cols)
+                           ? in[(size_t)c * in_rows + r]
+                           : 0.0f;
+            };
+            const float s00 = tap(Y, X), s01 = tap(Y, X + 1);
+            const float s10 = tap(Y + 1, X), s11 = tap(Y + 1, X + 1);
             v = s00 * (1 - fx) * (1 - fy) + s01 * fx * (1 - fy) +
                 s10 * (1 - fx) * fy + s11 * fx * fy;
         }
@@ -1481,12 +1487,17 @@ __global__ __launch_bounds__(kBlock) void align_kernel(const float *in, int n, i
     }
 }
 
-__global__ __launch_bounds__(kBlock) void to_float_kernel(const double *in, size_t count,
-                                                          float *out)
+// out (rows x cols, leading dimension rows) = float of the first rows rows
+// of in (leading dimension ld).
+__global__ __launch_bounds__(kBlock) void to_float_kernel(const double *in, int ld,
+                                                          int rows, int cols, float *out)
 {
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < count;
-         i += (size_t)gridDim.x * kBlock)
-        out[i] = (float)in[i];
+    const size_t total = (size_t)rows * cols;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * kBlock) {
+        const int r = (int)(i % rows), c = (int)(i / rows);
+        out[i] = (float)in[(size_t)c * ld + r];
+    }
 }
 
 }  // namespace
@@ -1731,9 +1742,11 @@ hipError_t launch_align(const float *in, int n, int cols, const int32_t *offset,
     return hipGetLastError();
 }
 
-hipError_t launch_to_float(const double *in, size_t count, float *out, hipStream_t s)
+hipError_t launch_to_float(const double *in, int ld, int rows, int cols, float *out,
+                           hipStream_t s)
 {
-    to_float_kernel<<<grid_for(count), kBlock, 0, s>>>(in, count, out);
+    to_float_kernel<<<grid_for((size_t)rows * cols), kBlock, 0, s>>>(in, ld, rows, cols,
+                                                                    out);
     return hipGetLastError();
 }
 

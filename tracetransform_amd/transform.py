@@ -91,18 +91,18 @@ class Transformer:
         sino = core.sinograms(self._image, n_angles, float(self._angle_stepsize), codes)
         if not pf:
             return None
+        if not self._orthonormal:
+            circ = core.circus(sino, pf)  # [nt, np, A]
+            return core.zscore(circ.reshape(-1, n_angles)).t()
         sig = torch.empty((len(codes), len(pf), n_angles), dtype=torch.float32,
                           device=sino.device)
+        nos, centers = core.nearest_orthonormal_sinograms(sino)
         for t in range(len(codes)):
-            s_t = sino[t]
-            pf_t = pf
-            if self._orthonormal:
-                s_t, center = core.nearest_orthonormal_sinogram(s_t)
-                pf_t = [(c, o, center) if c == int(PFunctional.Hermite) else (c, o, ce)
-                        for (c, o, ce) in pf]
-                for w in pfunctionals:
-                    if hasattr(w, "arguments") and w.functional == PFunctional.Hermite:
-                        w.arguments.center = center
-            circ = core.circus(s_t.contiguous(), pf_t)  # [np, A]
-            sig[t] = core.zscore(circ)
-        return sig.reshape(len(codes) * len(pf), n_angles).t()
+            center = centers[t]
+            pf_t = [(c, o, center) if c == int(PFunctional.Hermite) else (c, o, ce)
+                    for (c, o, ce) in pf]
+            for w in pfunctionals:
+                if hasattr(w, "arguments") and w.functional == PFunctional.Hermite:
+                    w.arguments.center = center
+            sig[t] = core.circus(nos[t], pf_t)  # [np, A]
+        return core.zscore(sig.reshape(-1, n_angles)).t()
