@@ -42,6 +42,11 @@ WORKLOADS = {
         side=1024, kind="blob", seed=2, step=1.0, tcodes=[0, 1, 2, 3, 4, 5, 6, 7],
         pfuncs=[(1,), (2,), (3,)] + [(0, k, -1) for k in range(1, 6)],
         hermite_on="nos"),
+    # The same 1024x1024 blob image, T0-T5 and P1-P3 only: the line kernel
+    # at the large size (no sort, no orthonormal step).
+    "large-1024-blob-T0-T5-P1-P3": dict(
+        side=1024, kind="blob", seed=2, step=1.0, tcodes=[0, 1, 2, 3, 4, 5],
+        pfuncs=[(1,), (2,), (3,)]),
     # One 256x256 U[0,1) image (seed 1) padded to 363, 360 angles at 1 deg,
     # T0-T5, P1-P3.
     "plain-256-uniform-T0-T5-P1-P3": dict(

@@ -99,6 +99,22 @@ def test_all_T_blob_1024_few_angles(oracle, gpu):
     check_forward(oracle, img, n, 8, 45.0, T07)
 
 
+@pytest.mark.parametrize("side", [300, 700, 1024])
+def test_T6_T7_long_tails(oracle, gpu, side):
+    # the mass sits in one corner, so along the diagonal lines the median is
+    # near the line's start and the non-zero part of the tail is long: up to
+    # 403 / 968 / 1426 samples for sides 300 / 700 / 1024 (n = 425 / 991 /
+    # 1449).  Parts longer than 512 take the queued wide pass, longer than
+    # 1024 its widest network.
+    rng = np.random.default_rng(side)
+    img = 0.01 + 1e-3 * rng.random((side, side))
+    img[:30, :30] += 5.0
+    img_cm = np.ascontiguousarray(img.T, np.float32).ravel()
+    n, _, _ = oracle.pad_geometry(side, side)
+    pimg = oracle.pad(img_cm, side, side)
+    check_forward(oracle, pimg, n, 8, 45.0, [6, 7, 0], f64=False)
+
+
 @pytest.mark.parametrize("rows,cols", [(1, 1), (3, 8), (30, 30)])
 def test_T6_T7_small_and_ties(oracle, gpu, rows, cols):
     rng = np.random.default_rng(rows + 7 * cols)
@@ -144,6 +160,8 @@ def test_negative_and_zero_images(oracle, gpu):
     img_cm = (rng.random(side * side, dtype=np.float32) - 0.3).astype(np.float32)
     img, n = padded(oracle, img_cm, side, side)
     check_forward(oracle, img, n, 90, 4.0, T05, f64=False)
+    # negative samples make the sqrt sums NaN: T6/T7 pick the last position
+    check_forward(oracle, img, n, 90, 4.0, [6, 7], f64=False)
     zero = np.zeros(n * n, np.float32)
     g, _ = check_forward(oracle, zero, n, 8, 45.0, T05, f64=False)
     assert not g.any()

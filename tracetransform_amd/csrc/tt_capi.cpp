@@ -272,6 +272,7 @@ struct tt_context {
     DevBuf imgT, trig, tab345, dft_tw, herm_w, io_a, io_b, med_x, med_s;
     DevBuf bw_sino, bw_dsino, bw_z, bw_rec;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
+    DevBuf sort_qa, sort_qb, sort_count;
     rocblas_handle blas = nullptr;
     int trig_A = -1;
     float trig_step = 0.0f;
@@ -427,7 +428,8 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
                       &ctx->med_s, &ctx->bw_sino, &ctx->bw_dsino, &ctx->bw_z,
                       &ctx->bw_rec, &ctx->nos_med, &ctx->nos_a, &ctx->nos_u,
                       &ctx->nos_v, &ctx->nos_s, &ctx->nos_e, &ctx->nos_c,
-                      &ctx->nos_info, &ctx->nos_sweeps};
+                      &ctx->nos_info, &ctx->nos_sweeps, &ctx->sort_qa,
+                      &ctx->sort_qb, &ctx->sort_count};
     for (DevBuf *b : bufs)
         b->release();
     ctx->timer.release();
@@ -602,8 +604,19 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
         q.A = n_angles;
         q.slot6 = slot[6];
         q.slot7 = slot[7];
+        int32_t *qa = nullptr, *qb = nullptr;
+        unsigned int *counts = nullptr;
+        if (n > 512) {
+            const size_t qbytes = sizeof(int32_t) * 2 * (size_t)n * n_angles;
+            TT_HIP(ctx->sort_qa.ensure(qbytes));
+            TT_HIP(ctx->sort_qb.ensure(qbytes));
+            TT_HIP(ctx->sort_count.ensure(2 * sizeof(unsigned int)));
+            qa = ctx->sort_qa.as<int32_t>();
+            qb = ctx->sort_qb.as<int32_t>();
+            counts = ctx->sort_count.as<unsigned int>();
+        }
         TT_HIP(ctx->timer.begin(TT_KERNEL_SINO_SORT, st));
-        TT_HIP(ttk::launch_sino_sort(q, st));
+        TT_HIP(ttk::launch_sino_sort(q, qa, qb, counts, st));
         TT_HIP(ctx->timer.end(st));
     }
     const size_t plane = sizeof(float) * (size_t)n * n_angles;

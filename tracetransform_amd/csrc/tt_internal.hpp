@@ -105,6 +105,12 @@ struct SortTParams {
     int32_t *t6_rank;
     int32_t *t6_src;
     int32_t *t7_rank;
+    // (line, which) queues: a pass reads defer_in (all lines when NULL) and
+    // queues the entries it cannot hold in defer_out (2 * A * n entries each)
+    const int32_t *defer_in;
+    const unsigned int *defer_in_count;
+    int32_t *defer_out;
+    unsigned int *defer_out_count;
     int n;
     int A;
     int slot6;
@@ -182,7 +188,9 @@ hipError_t launch_pad(const float *in, int rows, int cols, float *out, int n,
                       int dfx, int dfy, hipStream_t s);
 hipError_t launch_sino_t05(const SinoParams &p, bool need_x, bool need_s,
                            bool do345, hipStream_t s);
-hipError_t launch_sino_sort(const SortTParams &p, hipStream_t s);
+// n > 512 needs two queues of 2 * A * n entries and two counters.
+hipError_t launch_sino_sort(const SortTParams &p, int32_t *queue_a, int32_t *queue_b,
+                            unsigned int *counts, hipStream_t s);
 hipError_t launch_circus(const CircusParams &p, hipStream_t s);
 
 // Batched path; `waves_per_block` in {1, 2, 4} and `blocks` (persistent grid)

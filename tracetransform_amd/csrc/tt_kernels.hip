@@ -19,12 +19,7 @@ namespace {
 // global memory on CDNA).
 typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
 
-__device__ __forceinline__ f2u ld2(const float *p)
-{
-    return *reinterpret_cast<const f2u *>(p);
-}
-
-// Same load at a 32-bit byte offset from a wave-uniform base: matches the
+// Load at a 32-bit byte offset from a wave-uniform base: matches the
 // scalar-base + vector-offset form of global loads (no 64-bit VALU math).
 __device__ __forceinline__ f2u ld2b(const float *base, uint32_t byte_off)
 {
@@ -873,23 +868,73 @@ __device__ __forceinline__ void wave_bitonic_sort(uint64_t (&k)[KPL], int lane)
     }
 }
 
-// Eigen packet-order sum of x[0..n) in a wave's LDS slice: lanes 0..7 run the
-// stride-8 chains, lane 0 combines; the result is returned to every lane.
-__device__ __forceinline__ float eigen_sum_wave(const float *x, int n, int lane)
+// Wave-wide reductions.
+__device__ __forceinline__ int wave_max_i(int v)
 {
+#pragma unroll
+    for (int m = kWave / 2; m > 0; m >>= 1) {
+        const int o = __shfl_xor(v, m, kWave);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_i(int v)
+{
+#pragma unroll
+    for (int m = kWave / 2; m > 0; m >>= 1)
+        v += __shfl_xor(v, m, kWave);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v)
+{
+#pragma unroll
+    for (int m = kWave / 2; m > 0; m >>= 1) {
+        const uint64_t o = shfl_xor64(v, m);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// The sorted square roots of a tail of length L as a compact LDS array c of
+// its Le non-trailing entries: position j of the full sequence holds c[j]
+// for j < ins, 0 for ins <= j < ins + zt, and c[j - zt] beyond.
+struct TailView {
+    const float *c;
+    int ins, zt, L;
+    __device__ __forceinline__ float at(int j) const
+    {
+        return j < ins ? c[j] : (j < ins + zt ? 0.0f : c[j - zt]);
+    }
+};
+
+// Eigen packet-order sum of a TailView's full sequence (as eigen_sum_block):
+// lanes 0..7 run the stride-8 chains, lane 0 combines; the result is
+// returned to every lane.
+__device__ __forceinline__ float eigen_sum_view(const TailView &v, int lane)
+{
+    const int n = v.L;
     const int a4 = n & ~3, a8 = n & ~7;
     float r = 0.0f;
     if (lane < 8 && n >= 8) {
         int i = lane;
         for (; i + 24 < a8; i += 32) {
-            const float v0 = x[i], v1 = x[i + 8], v2 = x[i + 16], v3 = x[i + 24];
+            const float v0 = v.at(i), v1 = v.at(i + 8), v2 = v.at(i + 16), v3 = v.at(i + 24);
             r += v0;
             r += v1;
             r += v2;
             r += v3;
         }
         for (; i < a8; i += 8)
-            r += x[i];
+            r += v.at(i);
     }
     float ch[8];
 #pragma unroll
@@ -898,115 +943,249 @@ __device__ __forceinline__ float eigen_sum_wave(const float *x, int n, int lane)
     float res = 0.0f;
     if (lane == 0) {
         if (a4 == 0) {
-            res = n > 0 ? x[0] : 0.0f;
+            res = n > 0 ? v.at(0) : 0.0f;
             for (int i = 1; i < n; i++)
-                res += x[i];
+                res += v.at(i);
         } else {
             float ax[4];
             for (int j = 0; j < 4; j++)
-                ax[j] = n >= 8 ? ch[j] + ch[j + 4] : x[j];
+                ax[j] = n >= 8 ? ch[j] + ch[j + 4] : v.at(j);
             if (n >= 8 && a4 > a8)
                 for (int j = 0; j < 4; j++)
-                    ax[j] += x[a8 + j];
+                    ax[j] += v.at(a8 + j);
             res = (ax[0] + ax[2]) + (ax[1] + ax[3]);
             for (int i = a4; i < n; i++)
-                res += x[i];
+                res += v.at(i);
         }
     }
     return __shfl(res, 0, kWave);
 }
 
-// One wave per line (persistent over lines).  T6: keys (orderable(w) << 32 |
-// r) for w = float(r) * x[sm + r]; after the sort each lane re-samples the
-// rows its keys point at to form sqrt(x[sm + idx[k]]).  T7: keys
-// (orderable(x[m + r]) << 32 | r).  The sorted square roots go to the wave's
-// LDS slice for the Eigen-order sum and findWeightedMedian's walk.
-template <int KPL>
-__global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
+// findWeightedMedian's sequential walk over a TailView (one lane).  Adding
+// the zero run leaves the prefix unchanged, so the test can first pass there
+// only at its first position (when nothing before it passed, that is only
+// possible with ins == 0 and S <= 0).
+// Walks c[j0..j1) (8 loads issued ahead of the dependent adds); returns the
+// first passing j + off, or -1 with the prefix carried in *integral.
+__device__ __forceinline__ int walk_segment(const float *c, int j0, int j1, int off, float S,
+                                            float *integral)
 {
-    constexpr int M = kWave * KPL;
-    __shared__ __align__(16) float s_sq_all[(kBlock / kWave) * M];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = wave_id();
-    float *s_sq = s_sq_all + wv * M;
+    float acc = *integral;
+    int j = j0;
+    for (; j + 8 <= j1; j += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            v[u] = c[j + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            acc += v[u];
+            if (2.0f * acc >= S)
+                return j + u + off;
+        }
+    }
+    for (; j < j1; j++) {
+        acc += c[j];
+        if (2.0f * acc >= S)
+            return j + off;
+    }
+    *integral = acc;
+    return -1;
+}
+
+__device__ int weighted_median_walk_view(const TailView &v, int Le, float S)
+{
+    float integral = 0.0f;
+    int k = walk_segment(v.c, 0, v.ins, 0, S, &integral);
+    if (k >= 0)
+        return k;
+    if (v.zt > 0 && 2.0f * integral >= S)
+        return v.ins;
+    k = walk_segment(v.c, v.ins, Le, v.zt, S, &integral);
+    return k >= 0 ? k : v.L - 1;
+}
+
+// Sorted tail of one line for T6 (T6 = true) or T7, given the tail's first
+// Le samples in xs[0..Le), no sample negative or NaN, and every sample at
+// i >= Le zero (the line has left the rotated image).  Only the first Le keys
+// are sorted (K keys per lane, 64*K >= Le); the L - Le trailing zeros are
+// placed without sorting: for T7 (value keys) they are the smallest values
+// and lead the order; for T6 (w = float(i) * x keys) they have w = 0 and the
+// largest indices, so they follow the zero-w keys of the sorted part (stable
+// order: ties by ascending index).  Replaces xs[0..Le) by the square roots
+// of the sorted part, takes the Eigen-order sum and the sequential walk over
+// the full sequence, and returns the key at the weighted-median position k.
+template <int K, bool T6>
+__device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le, int lane,
+                                                       int *k_out)
+{
+    uint64_t key[K];
+    int zin = 0;
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const int i = r * kWave + lane;  // any placement: the sort orders them
+        key[r] = ~0ull;
+        if (i < Le) {
+            const float x = xs[i];
+            const float kv = T6 ? (float)i * x : x;
+            key[r] = ((uint64_t)ord_key(kv) << 32) | (uint32_t)i;
+            zin += (T6 && kv == 0.0f) ? 1 : 0;
+        }
+    }
+    wave_bitonic_sort<K>(key, lane);
+    const int ins = T6 ? wave_sum_i(zin) : 0;  // where the trailing zeros go
+    const int zt = L - Le;
+    // gather the sorted samples before overwriting the slice with roots
+    float sq[K];
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const int pos = lane * K + r;
+        sq[r] = 0.0f;
+        if (pos < Le)
+            sq[r] = sqrtf(T6 ? xs[(uint32_t)key[r]] : ord_val((uint32_t)(key[r] >> 32)));
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const int pos = lane * K + r;
+        if (pos < Le)
+            xs[pos] = sq[r];
+    }
+    wave_lds_sync();
+    const TailView view = {xs, ins, zt, L};
+    const float S = eigen_sum_view(view, lane);
+    int k = 0;
+    if (lane == 0)
+        k = weighted_median_walk_view(view, Le, S);
+    k = __shfl(k, 0, kWave);
+    *k_out = k;
+    if (k >= ins && k < ins + zt)  // a trailing zero: w (or x) = +0, index Le + ...
+        return ((uint64_t)ord_key(0.0f) << 32) | (uint32_t)(Le + (k - ins));
+    const int pos = k < ins ? k : k - zt;
+    uint64_t kk = 0;
+#pragma unroll
+    for (int r = 0; r < K; r++)
+        kk = (r == pos % K) ? key[r] : kk;
+    return shfl64(kk, pos / K);
+}
+
+// T6 (which = 0, tail from the sqrt-median sm) or T7 (which = 1, tail from
+// the median m) of one line by one wave.  The wave samples the tail into its
+// LDS slice, then
+//   * if any sample is negative, sqrt makes the Eigen sum NaN, no prefix
+//     passes the test and findWeightedMedian returns the last position: the
+//     result is the largest key (no sort needed);
+//   * otherwise it sorts the part before the trailing zeros with the
+//     smallest register network that holds it (K = 1..KMAX keys per lane).
+// With DEFER, a line whose sorted part needs more than KMAX keys per lane is
+// queued in p.defer for the wide pass instead.
+template <int KMAX, bool DEFER>
+__device__ __forceinline__ void sort_line(const SortTParams &p, int line, int which,
+                                          float *xs, int lane)
+{
+    constexpr int M = kWave * KMAX;  // capacity of the wave's LDS slice
     const int n = p.n;
+    const int slotw = which == 0 ? p.slot6 : p.slot7;
+    if (slotw < 0)
+        return;
     const float o = (float)((n - 1) / 2.0);
     const float lim = (float)(n - 1);
-    const size_t tstride = (size_t)p.A * n;
-    const int lines = p.A * n;
-    const int waves = gridDim.x * (kBlock / kWave);
-    for (int line = blockIdx.x * (kBlock / kWave) + wv; line < lines; line += waves) {
-        const int a = line / n;
-        const int col = line - a * n;
-        const float2 cs = p.trig[a];
-        const float px = (float)col - o;
-        const float pxc = px * cs.x, pxns = px * (-cs.y);
-        const bool tr = fabsf(cs.y) > fabsf(cs.x);
-        const float *src = tr ? p.imgT : p.img;
-        auto sample = [&](int row) -> float {
-            return tr ? rot_sample<true>(src, n, lim, o, pxc, pxns, cs.x, cs.y,
-                                         (float)row - o)
-                      : rot_sample<false>(src, n, lim, o, pxc, pxns, cs.x, cs.y,
-                                          (float)row - o);
-        };
-        for (int which = 0; which < 2; which++) {
-            const int slotw = which == 0 ? p.slot6 : p.slot7;
-            if (slotw < 0)
-                continue;
-            const int base = which == 0 ? p.med_s[line] : p.med_x[line];
-            const int L = n - base;
-            uint64_t key[KPL];
-#pragma unroll
-            for (int r = 0; r < KPL; r++) {
-                const int i = lane * KPL + r;
-                key[r] = ~0ull;
-                if (i < L) {
-                    const float x = sample(base + i);
-                    const float kv = which == 0 ? (float)i * x : x;
-                    key[r] = ((uint64_t)ord_key(kv) << 32) | (uint32_t)i;
-                }
-            }
-            wave_bitonic_sort<KPL>(key, lane);
-#pragma unroll
-            for (int r = 0; r < KPL; r++) {
-                const int i = lane * KPL + r;
-                if (i < L) {
-                    const float x = which == 0 ? sample(base + (int)(uint32_t)key[r])
-                                               : ord_val((uint32_t)(key[r] >> 32));
-                    s_sq[i] = sqrtf(x);
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            const float S = eigen_sum_wave(s_sq, L, lane);
-            int k = 0;
+    const int a = line / n;
+    const int col = line - a * n;
+    const float2 cs = p.trig[a];
+    const float px = (float)col - o;
+    const float pxc = px * cs.x, pxns = px * (-cs.y);
+    const bool tr = fabsf(cs.y) > fabsf(cs.x);
+    const float *src = tr ? p.imgT : p.img;
+    const bool t6 = which == 0;
+    const int base = t6 ? p.med_s[line] : p.med_x[line];
+    const int L = n - base;
+    // consecutive lanes take consecutive rows (neighbouring taps)
+    int last_nz = -1;
+    bool bad = false;
+    uint64_t kmax = 0;
+    for (int i = lane; i < L; i += kWave) {
+        const float pyo = (float)(base + i) - o;
+        const float x = tr ? rot_sample<true>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo)
+                           : rot_sample<false>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo);
+        if (i < M)  // a longer sorted part is queued (DEFER) or cannot occur
+            xs[i] = x;
+        last_nz = x != 0.0f ? i : last_nz;
+        bad |= !(x >= 0.0f);  // negative or NaN
+        const float kv = t6 ? (float)i * x : x;
+        const uint64_t kk = ((uint64_t)ord_key(kv) << 32) | (uint32_t)i;
+        kmax = kk > kmax ? kk : kmax;
+    }
+    int k = L - 1;
+    uint64_t res;
+    if (__any(bad)) {
+        res = wave_max_u64(kmax);
+    } else {
+        const int Le = wave_max_i(last_nz) + 1;
+        const int kneed = (Le + kWave - 1) / kWave;
+        if (DEFER && kneed > KMAX) {
             if (lane == 0)
-                k = weighted_median_walk(s_sq, L, S);
-            k = __shfl(k, 0, kWave);
-            // the key at sorted position k sits in lane k / KPL, register k % KPL
-            uint64_t kk = 0;
-#pragma unroll
-            for (int r = 0; r < KPL; r++)
-                kk = (r == k % KPL) ? key[r] : kk;
-            kk = shfl64(kk, k / KPL);
-            if (lane == 0) {
-                if (which == 0) {
-                    const int srcr = (int)(uint32_t)kk;
-                    p.sino[slotw * tstride + line] = ord_val((uint32_t)(kk >> 32));
-                    if (p.t6_rank)
-                        p.t6_rank[line] = k;
-                    if (p.t6_src)
-                        p.t6_src[line] = srcr;
-                } else {
-                    p.sino[slotw * tstride + line] = ord_val((uint32_t)(kk >> 32));
-                    if (p.t7_rank)
-                        p.t7_rank[line] = k;
-                }
+                p.defer_out[atomicAdd(p.defer_out_count, 1u)] = line * 2 + which;
+            return;
+        }
+        wave_lds_sync();
+#define TT_TAIL(K) (t6 ? sorted_tail_median<(KMAX < K ? KMAX : K), true>(xs, L, Le, lane, &k) \
+                       : sorted_tail_median<(KMAX < K ? KMAX : K), false>(xs, L, Le, lane, &k))
+        if (kneed <= 1 || KMAX == 1)
+            res = TT_TAIL(1);
+        else if (kneed <= 2 || KMAX == 2)
+            res = TT_TAIL(2);
+        else if (kneed <= 4 || KMAX == 4)
+            res = TT_TAIL(4);
+        else if (kneed <= 8 || KMAX == 8)
+            res = TT_TAIL(8);
+        else if (kneed <= 16 || KMAX == 16)
+            res = TT_TAIL(16);
+        else
+            res = TT_TAIL(32);
+#undef TT_TAIL
+    }
+    if (lane == 0) {
+        p.sino[(size_t)slotw * p.A * n + line] = ord_val((uint32_t)(res >> 32));
+        if (t6) {
+            if (p.t6_rank)
+                p.t6_rank[line] = k;
+            if (p.t6_src)
+                p.t6_src[line] = (int)(uint32_t)res;
+        } else if (p.t7_rank) {
+            p.t7_rank[line] = k;
+        }
+    }
+}
+
+// One wave per line, persistent over lines, T6 then T7 of each line; or, with
+// p.defer_in set, one wave per queued (line, which) entry.  With DEFER,
+// entries whose sorted part needs more than KMAX keys per lane are queued in
+// p.defer_out for a wider pass (a wide network's VGPR budget would otherwise
+// set the occupancy of every wave).
+template <int KMAX, bool DEFER>
+__global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
+{
+    constexpr int M = kWave * KMAX;
+    __shared__ __align__(16) float s_all[(kBlock / kWave) * M];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = wave_id();
+    float *xs = s_all + wv * M;
+    const int waves = gridDim.x * (kBlock / kWave);
+    const int first = blockIdx.x * (kBlock / kWave) + wv;
+    if (!p.defer_in) {
+        const int lines = p.A * p.n;
+        for (int line = first; line < lines; line += waves)
+            for (int which = 0; which < 2; which++) {
+                sort_line<KMAX, DEFER>(p, line, which, xs, lane);
+                wave_lds_sync();
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        const int count = (int)*p.defer_in_count;
+        for (int j = first; j < count; j += waves) {
+            const int e = p.defer_in[j];
+            sort_line<KMAX, DEFER>(p, e >> 1, e & 1, xs, lane);
+            wave_lds_sync();
         }
     }
 }
@@ -1611,8 +1790,12 @@ hipError_t launch_sino_t05_batched(const BatchSinoParams &p, bool need_x,
     return hipGetLastError();
 }
 
-hipError_t launch_sino_sort(const SortTParams &p, hipStream_t s)
+hipError_t launch_sino_sort(const SortTParams &p_in, int32_t *queue_a,
+                            int32_t *queue_b, unsigned int *counts, hipStream_t s)
 {
+    SortTParams p = p_in;
+    p.defer_in = nullptr;
+    p.defer_in_count = nullptr;
     const int lines = p.A * p.n;
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
@@ -1621,22 +1804,43 @@ hipError_t launch_sino_sort(const SortTParams &p, hipStream_t s)
     const int need = (lines + kBlock / kWave - 1) / (kBlock / kWave);
     if (blocks > need)
         blocks = need;
-#define TT_SORT(K) sino_sort_kernel<K><<<blocks, kBlock, 0, s>>>(p)
-    if (p.n <= 64)
-        TT_SORT(1);
-    else if (p.n <= 128)
-        TT_SORT(2);
-    else if (p.n <= 256)
-        TT_SORT(4);
-    else if (p.n <= 512)
-        TT_SORT(8);
-    else if (p.n <= 1024)
-        TT_SORT(16);
-    else if (p.n <= 2048)
-        TT_SORT(32);
-    else
+    if (p.n > 2048)
         return hipErrorInvalidValue;
+    if (p.n <= 512) {
+#define TT_SORT(K) sino_sort_kernel<K, false><<<blocks, kBlock, 0, s>>>(p)
+        if (p.n <= 64)
+            TT_SORT(1);
+        else if (p.n <= 128)
+            TT_SORT(2);
+        else if (p.n <= 256)
+            TT_SORT(4);
+        else
+            TT_SORT(8);
 #undef TT_SORT
+        return hipGetLastError();
+    }
+    // sorted parts up to 512 keys in the first pass, up to 1024 in the
+    // second, the rest in the third; each pass queues what it cannot hold
+    if (!queue_a || !queue_b || !counts)
+        return hipErrorInvalidValue;
+    const int queued_blocks = cus * 4;
+    reset_counter_kernel<<<1, kWave, 0, s>>>(counts);
+    reset_counter_kernel<<<1, kWave, 0, s>>>(counts + 1);
+    p.defer_out = queue_a;
+    p.defer_out_count = counts;
+    sino_sort_kernel<8, true><<<blocks, kBlock, 0, s>>>(p);
+    p.defer_in = queue_a;
+    p.defer_in_count = counts;
+    if (p.n <= 1024) {
+        sino_sort_kernel<16, false><<<queued_blocks, kBlock, 0, s>>>(p);
+        return hipGetLastError();
+    }
+    p.defer_out = queue_b;
+    p.defer_out_count = counts + 1;
+    sino_sort_kernel<16, true><<<queued_blocks, kBlock, 0, s>>>(p);
+    p.defer_in = queue_b;
+    p.defer_in_count = counts + 1;
+    sino_sort_kernel<32, false><<<queued_blocks, kBlock, 0, s>>>(p);
     return hipGetLastError();
 }
 
