@@ -79,6 +79,31 @@ def test_nos_batched_matches_single_and_fp64(oracle, gpu):
         assert np.abs((q @ q.t()).cpu().numpy() - np.eye(A)).max() <= 1e-5
 
 
+def test_nos_rank_deficient(oracle, gpu):
+    # repeated columns: the aligned matrix has rank 20 of 40, its Gram matrix
+    # is singular and the polar factor is not unique; any valid one has
+    # orthonormal columns and makes H = P^T A symmetric positive semidefinite
+    # (A = P H)
+    rng = np.random.default_rng(8)
+    n, A = 121, 40
+    base = (rng.random((20, n)) + 0.05).astype(np.float32)
+    sino = np.concatenate([base, base])
+    nos, center = core.nearest_orthonormal_sinogram(dev(sino))
+    ref, rc = oracle.nearest_orthonormal_sinogram(sino)
+    assert center == rc and tuple(nos.shape) == ref.shape
+    p = nos.double().cpu().numpy().T          # rows x A
+    rows = p.shape[0]
+    offs = np.array([oracle.weighted_median(sino[a]) for a in range(A)]) - (n - 1) // 2
+    aligned = np.zeros((rows, A))
+    for a in range(A):
+        aligned[offs.max() - offs[a]: offs.max() - offs[a] + n, a] = sino[a]
+    assert np.abs(p.T @ p - np.eye(A)).max() <= 1e-5
+    h = p.T @ aligned
+    scale = np.abs(h).max()
+    assert np.abs(h - h.T).max() <= 1e-5 * scale
+    assert np.linalg.eigvalsh(0.5 * (h + h.T)).min() >= -1e-5 * scale
+
+
 def test_transformer_orthonormal_pipeline(oracle, gpu):
     import tracetransform_amd as tt
     img = torch.from_numpy(inputs.uniform_image(64, 9).T.copy())

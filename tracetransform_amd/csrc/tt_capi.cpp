@@ -272,6 +272,7 @@ struct tt_context {
     DevBuf imgT, trig, tab345, dft_tw, herm_w, io_a, io_b, med_x, med_s;
     DevBuf bw_sino, bw_dsino, bw_z, bw_rec;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
+    DevBuf nos_w, nos_m;
     DevBuf sort_qa, sort_qb, sort_count;
     rocblas_handle blas = nullptr;
     int trig_A = -1;
@@ -428,7 +429,8 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
                       &ctx->med_s, &ctx->bw_sino, &ctx->bw_dsino, &ctx->bw_z,
                       &ctx->bw_rec, &ctx->nos_med, &ctx->nos_a, &ctx->nos_u,
                       &ctx->nos_v, &ctx->nos_s, &ctx->nos_e, &ctx->nos_c,
-                      &ctx->nos_info, &ctx->nos_sweeps, &ctx->sort_qa,
+                      &ctx->nos_info, &ctx->nos_sweeps, &ctx->nos_w, &ctx->nos_m,
+                      &ctx->sort_qa,
                       &ctx->sort_qb, &ctx->sort_count};
     for (DevBuf *b : bufs)
         b->release();
@@ -1238,16 +1240,17 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
     TT_HIP(hipMemcpyAsync(ctx->nos_med.ptr, med.data(), sizeof(int32_t) * cols_all,
                           hipMemcpyHostToDevice, st));
     // Every aligned matrix is zero-padded to rows_max rows so one batched
-    // solver call covers them all.  Zero rows leave the polar factor's other
-    // rows unchanged and stay exactly zero through the Householder QR that
-    // forms U, so row t of the batch is sinogram t's factor plus zero rows.
+    // solve covers them all; zero rows leave the polar factor's other rows
+    // unchanged and stay exactly zero in it.
     const size_t mA = (size_t)rows_max * A, AA = (size_t)A * A;
     TT_HIP(ctx->nos_a.ensure(sizeof(double) * mA * n_sino));
     TT_HIP(ctx->nos_u.ensure(sizeof(double) * mA * n_sino));
     TT_HIP(ctx->nos_c.ensure(sizeof(double) * mA * n_sino));
     TT_HIP(ctx->nos_v.ensure(sizeof(double) * AA * n_sino));
+    TT_HIP(ctx->nos_w.ensure(sizeof(double) * AA * n_sino));
+    TT_HIP(ctx->nos_m.ensure(sizeof(double) * AA * n_sino));
     TT_HIP(ctx->nos_s.ensure(sizeof(double) * (size_t)A * n_sino));
-    TT_HIP(ctx->nos_e.ensure(sizeof(double) * (size_t)n_sino));
+    TT_HIP(ctx->nos_e.ensure(sizeof(double) * (size_t)A * n_sino));
     TT_HIP(ctx->nos_info.ensure(sizeof(rocblas_int) * n_sino));
     TT_HIP(ctx->nos_sweeps.ensure(sizeof(rocblas_int) * n_sino));
     TT_HIP(ctx->timer.begin(TT_KERNEL_NOS, st));
@@ -1259,33 +1262,93 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
         TT_BLAS(rocblas_create_handle(&ctx->blas));
     TT_BLAS(rocblas_set_stream(ctx->blas, st));
     TT_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
-    // aligned = U S V^T (thin U, full V^T): V from Jacobi eigenvectors of
-    // A^T A to machine precision, U from a QR of A V; NOS = U V^T
-    TT_BLAS(rocsolver_dgesvdj_strided_batched(
-        ctx->blas, rocblas_svect_singular, rocblas_svect_all, rows_max, A,
-        ctx->nos_a.as<double>(), rows_max, (rocblas_stride)mA, 0.0,
-        ctx->nos_e.as<double>(), 100, ctx->nos_sweeps.as<rocblas_int>(),
-        ctx->nos_s.as<double>(), (rocblas_stride)A, ctx->nos_u.as<double>(), rows_max,
-        (rocblas_stride)mA, ctx->nos_v.as<double>(), A, (rocblas_stride)AA,
-        ctx->nos_info.as<rocblas_int>(), n_sino));
     const double one = 1.0, zero = 0.0;
+    double *a_d = ctx->nos_a.as<double>(), *g = ctx->nos_v.as<double>();
+    double *lam = ctx->nos_s.as<double>();
+    rocblas_int *info_d = ctx->nos_info.as<rocblas_int>();
+    // Polar factor P = A (A^T A)^(-1/2): Gram matrix, its eigenvectors by
+    // divide and conquer, M = V diag(lambda^-1/2) V^T and P = A M, then two
+    // Newton-Schulz steps P <- P (1.5 I - 0.5 P^T P) that restore
+    // orthonormality to rounding.  Forming A^T A squares the condition
+    // number: the factor is accurate to about eps * kappa(A)^2 relative
+    // (the reference's FP32 Jacobi SVD carries about 6e-8 * kappa(A)).
     TT_BLAS(rocblas_dgemm_strided_batched(
-        ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A, &one,
-        ctx->nos_u.as<double>(), rows_max, (rocblas_stride)mA, ctx->nos_v.as<double>(), A,
-        (rocblas_stride)AA, &zero, ctx->nos_c.as<double>(), rows_max, (rocblas_stride)mA,
-        n_sino));
-    for (int t = 0; t < n_sino; t++)
-        TT_HIP(ttk::launch_to_float(ctx->nos_c.as<double>() + mA * t, rows_max, out_rows[t],
-                                    A, d_out + (size_t)t * out_capacity_rows * A, st));
-    TT_HIP(ctx->timer.end(st));
-    std::vector<rocblas_int> info(n_sino);
-    TT_HIP(hipMemcpyAsync(info.data(), ctx->nos_info.ptr, sizeof(rocblas_int) * n_sino,
+        ctx->blas, rocblas_operation_transpose, rocblas_operation_none, A, A, rows_max, &one,
+        a_d, rows_max, (rocblas_stride)mA, a_d, rows_max, (rocblas_stride)mA, &zero, g, A,
+        (rocblas_stride)AA, n_sino));
+    TT_BLAS(rocsolver_dsyevd_strided_batched(ctx->blas, rocblas_evect_original,
+                                             rocblas_fill_upper, A, g, A, (rocblas_stride)AA,
+                                             lam, (rocblas_stride)A, ctx->nos_e.as<double>(),
+                                             (rocblas_stride)A, info_d, n_sino));
+    std::vector<double> lam_h((size_t)A * n_sino);
+    std::vector<rocblas_int> info((size_t)n_sino);
+    TT_HIP(hipMemcpyAsync(lam_h.data(), lam, sizeof(double) * lam_h.size(),
+                          hipMemcpyDeviceToHost, st));
+    TT_HIP(hipMemcpyAsync(info.data(), info_d, sizeof(rocblas_int) * n_sino,
                           hipMemcpyDeviceToHost, st));
     TT_HIP(hipStreamSynchronize(st));
+    bool gram_ok = true;
+    for (int t = 0; t < n_sino; t++) {
+        // ascending eigenvalues; a (numerically) singular Gram matrix has no
+        // inverse square root
+        const double lo = lam_h[(size_t)t * A], hi = lam_h[(size_t)t * A + A - 1];
+        if (info[t] != 0 || !(hi > 0.0) || !(lo > hi * 1e-15))
+            gram_ok = false;
+    }
+    double *result = nullptr;
+    if (gram_ok) {
+        double *vs = ctx->nos_w.as<double>(), *m = ctx->nos_m.as<double>();
+        double *p0 = ctx->nos_c.as<double>(), *p1 = ctx->nos_u.as<double>();
+        TT_HIP(ttk::launch_scale_eigvec(g, lam, A, n_sino, vs, st));
+        TT_BLAS(rocblas_dgemm_strided_batched(
+            ctx->blas, rocblas_operation_none, rocblas_operation_transpose, A, A, A, &one, vs,
+            A, (rocblas_stride)AA, g, A, (rocblas_stride)AA, &zero, m, A, (rocblas_stride)AA,
+            n_sino));
+        TT_BLAS(rocblas_dgemm_strided_batched(
+            ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A, &one,
+            a_d, rows_max, (rocblas_stride)mA, m, A, (rocblas_stride)AA, &zero, p0, rows_max,
+            (rocblas_stride)mA, n_sino));
+        for (int it = 0; it < 2; it++) {
+            TT_BLAS(rocblas_dgemm_strided_batched(
+                ctx->blas, rocblas_operation_transpose, rocblas_operation_none, A, A, rows_max,
+                &one, p0, rows_max, (rocblas_stride)mA, p0, rows_max, (rocblas_stride)mA,
+                &zero, m, A, (rocblas_stride)AA, n_sino));
+            TT_HIP(ttk::launch_newton_schulz_factor(m, A, n_sino, st));
+            TT_BLAS(rocblas_dgemm_strided_batched(
+                ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A,
+                &one, p0, rows_max, (rocblas_stride)mA, m, A, (rocblas_stride)AA, &zero, p1,
+                rows_max, (rocblas_stride)mA, n_sino));
+            std::swap(p0, p1);
+        }
+        result = p0;
+    } else {
+        // Rank-deficient input: thin U from a Jacobi SVD (V from A^T A, U
+        // from a QR of A V, orthonormal even where sigma = 0); NOS = U V^T
+        TT_BLAS(rocsolver_dgesvdj_strided_batched(
+            ctx->blas, rocblas_svect_singular, rocblas_svect_all, rows_max, A, a_d, rows_max,
+            (rocblas_stride)mA, 0.0, ctx->nos_e.as<double>(), 100,
+            ctx->nos_sweeps.as<rocblas_int>(), lam, (rocblas_stride)A,
+            ctx->nos_u.as<double>(), rows_max, (rocblas_stride)mA, g, A, (rocblas_stride)AA,
+            info_d, n_sino));
+        TT_BLAS(rocblas_dgemm_strided_batched(
+            ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A, &one,
+            ctx->nos_u.as<double>(), rows_max, (rocblas_stride)mA, g, A, (rocblas_stride)AA,
+            &zero, ctx->nos_c.as<double>(), rows_max, (rocblas_stride)mA, n_sino));
+        result = ctx->nos_c.as<double>();
+        TT_HIP(hipMemcpyAsync(info.data(), info_d, sizeof(rocblas_int) * n_sino,
+                              hipMemcpyDeviceToHost, st));
+    }
     for (int t = 0; t < n_sino; t++)
-        if (info[t] != 0)
-            return fail(TT_ERR_DEVICE, "sinogram %d: Jacobi SVD did not converge (info %d)",
-                        t, (int)info[t]);
+        TT_HIP(ttk::launch_to_float(result + mA * t, rows_max, out_rows[t], A,
+                                    d_out + (size_t)t * out_capacity_rows * A, st));
+    TT_HIP(ctx->timer.end(st));
+    TT_HIP(hipStreamSynchronize(st));
+    if (!gram_ok)
+        for (int t = 0; t < n_sino; t++)
+            if (info[t] != 0)
+                return fail(TT_ERR_DEVICE,
+                            "sinogram %d: Jacobi SVD did not converge (info %d)", t,
+                            (int)info[t]);
     return TT_OK;
 }
 

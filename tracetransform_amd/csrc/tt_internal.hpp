@@ -178,6 +178,9 @@ hipError_t launch_align(const float *in, int n, int cols, const int32_t *offset,
                         int maxoff, int rows_out, double *out, hipStream_t s);
 hipError_t launch_to_float(const double *in, int ld, int rows, int cols, float *out,
                            hipStream_t s);
+hipError_t launch_scale_eigvec(const double *v, const double *lambda, int k, int count,
+                               double *out, hipStream_t s);
+hipError_t launch_newton_schulz_factor(double *x, int k, int count, hipStream_t s);
 
 hipError_t launch_records(const RecordParams &p, hipStream_t s);
 hipError_t launch_gather(const GatherParams &p, hipStream_t s);

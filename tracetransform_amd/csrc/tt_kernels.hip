@@ -1679,6 +1679,34 @@ __global__ __launch_bounds__(kBlock) void to_float_kernel(const double *in, int 
     }
 }
 
+// out(i, j) = v(i, j) / sqrt(lambda_j) for count column-major k x k matrices
+// (eigenvectors v, eigenvalues lambda, k per matrix).
+__global__ __launch_bounds__(kBlock) void scale_eigvec_kernel(const double *v,
+                                                             const double *lambda, int k,
+                                                             int count, double *out)
+{
+    const size_t kk = (size_t)k * k, total = kk * count;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * kBlock) {
+        const size_t m = i / kk;
+        const int j = (int)((i - m * kk) / k);
+        out[i] = v[i] / sqrt(lambda[m * k + j]);
+    }
+}
+
+// x <- 1.5 I - 0.5 x for count k x k matrices (one Newton-Schulz factor).
+__global__ __launch_bounds__(kBlock) void newton_schulz_factor_kernel(double *x, int k,
+                                                                     int count)
+{
+    const size_t kk = (size_t)k * k, total = kk * count;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * kBlock) {
+        const size_t e = i % kk;
+        const int r = (int)(e % k), c = (int)(e / k);
+        x[i] = (r == c ? 1.5 : 0.0) - 0.5 * x[i];
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1943,6 +1971,21 @@ hipError_t launch_align(const float *in, int n, int cols, const int32_t *offset,
 {
     align_kernel<<<grid_for((size_t)rows_out * cols), kBlock, 0, s>>>(
         in, n, cols, offset, maxoff, rows_out, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_eigvec(const double *v, const double *lambda, int k, int count,
+                               double *out, hipStream_t s)
+{
+    scale_eigvec_kernel<<<grid_for((size_t)k * k * count), kBlock, 0, s>>>(v, lambda, k,
+                                                                          count, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_newton_schulz_factor(double *x, int k, int count, hipStream_t s)
+{
+    newton_schulz_factor_kernel<<<grid_for((size_t)k * k * count), kBlock, 0, s>>>(x, k,
+                                                                                  count);
     return hipGetLastError();
 }
 
