@@ -225,7 +225,7 @@ struct KernelTimer {
 // rewrite the context's tables) cannot change what the graph reads.
 struct BatchPlan {
     std::vector<int64_t> key;
-    DevBuf desc, imgT, trig, tab, counter;
+    DevBuf desc, imgT, trig, tab, counter, bbox;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int waves_per_block = 4, blocks = 0;
@@ -249,6 +249,7 @@ struct BatchPlan {
         trig.release();
         tab.release();
         counter.release();
+        bbox.release();
         key.clear();
     }
 };
@@ -273,7 +274,7 @@ struct tt_context {
     DevBuf bw_sino, bw_dsino, bw_z, bw_rec;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
     DevBuf nos_w, nos_m;
-    DevBuf sort_qa, sort_qb, sort_count;
+    DevBuf sort_qa, sort_qb, sort_count, bbox;
     rocblas_handle blas = nullptr;
     int trig_A = -1;
     float trig_step = 0.0f;
@@ -431,7 +432,7 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
                       &ctx->nos_v, &ctx->nos_s, &ctx->nos_e, &ctx->nos_c,
                       &ctx->nos_info, &ctx->nos_sweeps, &ctx->nos_w, &ctx->nos_m,
                       &ctx->sort_qa,
-                      &ctx->sort_qb, &ctx->sort_count};
+                      &ctx->sort_qb, &ctx->sort_count, &ctx->bbox};
     for (DevBuf *b : bufs)
         b->release();
     ctx->timer.release();
@@ -570,8 +571,10 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
     if (do345)
         TT_TRY(ctx->ensure_tab345(n));
     TT_HIP(ctx->imgT.ensure(sizeof(float) * (size_t)n * n));
+    TT_HIP(ctx->bbox.ensure(4 * sizeof(int)));
     TT_HIP(ctx->timer.begin(TT_KERNEL_AUX, st));
     TT_HIP(ttk::launch_transpose(d_image, ctx->imgT.as<float>(), n, st));
+    TT_HIP(ttk::launch_image_bbox(d_image, n, ctx->bbox.as<int>(), st));
     TT_HIP(ctx->timer.end(st));
 
     ttk::SinoParams p = {};
@@ -582,6 +585,7 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
     p.sino = d_sino;
     p.med_x = med_x;
     p.med_s = med_s;
+    p.bbox = ctx->bbox.as<int>();
     p.n = n;
     p.A = n_angles;
     p.nb = (n + ttk::kWave - 1) / ttk::kWave;
@@ -602,6 +606,7 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
         q.t6_rank = ax.t6_rank;
         q.t6_src = ax.t6_src;
         q.t7_rank = ax.t7_rank;
+        q.bbox = ctx->bbox.as<int>();
         q.n = n;
         q.A = n_angles;
         q.slot6 = slot[6];
@@ -742,6 +747,8 @@ hipError_t enqueue_batch(tt_context *ctx, const ttk::BatchSinoParams &sp,
     hipError_t e = hipSuccess;
     if (with_transpose)
         e = ttk::launch_transpose_batched(sp.images, nimg, nmax, st);
+    if (with_transpose && e == hipSuccess)
+        e = ttk::launch_image_bbox_batched(sp.images, nimg, nmax, st);
     if (e == hipSuccess)
         e = ttk::launch_reset_counter(sp.counter, st);
     if (e == hipSuccess)
@@ -827,6 +834,15 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
     }
     BatchPlan &B = ctx->batch;
     hipStream_t own = ctx->own;
+    // everything below (autotune trials included) reads the caller's images
+    if (!ctx->ev_in)
+        TT_HIP(hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming));
+    if (!ctx->ev_out)
+        TT_HIP(hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming));
+    if (ctx->stream != own) {
+        TT_HIP(hipEventRecord(ctx->ev_in, ctx->stream));
+        TT_HIP(hipStreamWaitEvent(own, ctx->ev_in, 0));
+    }
     const bool do345 = slot[3] >= 0 || slot[4] >= 0 || slot[5] >= 0;
     const bool need_x = slot[1] >= 0 || slot[2] >= 0;
     const bool need_s = do345;
@@ -845,12 +861,14 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         for (int i = 0; i < nimg; i++)
             tot += (size_t)images[i].n * images[i].n;
         TT_HIP(B.imgT.ensure(sizeof(float) * tot));
+        TT_HIP(B.bbox.ensure(4 * sizeof(int) * (size_t)nimg));
         size_t off = 0;
         int items = 0, cols = 0;
         for (int k = 0; k < nimg; k++) {
             const tt_image &im = images[order[k]];
             ttk::BatchImage &b = d[k];
             b.img = im.d_image;
+            b.bbox = B.bbox.as<int>() + 4 * k;
             b.imgT = B.imgT.as<float>() + off;
             off += (size_t)im.n * im.n;
             b.sino = im.d_sino;
@@ -905,6 +923,7 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         int best_w = 4, best_b = cus * 4;
         if (ctx->autotune) {
             TT_HIP(ttk::launch_transpose_batched(sp.images, nimg, nmax, own));
+            TT_HIP(ttk::launch_image_bbox_batched(sp.images, nimg, nmax, own));
             hipEvent_t e0, e1;
             TT_HIP(hipEventCreate(&e0));
             TT_HIP(hipEventCreate(&e1));
@@ -949,14 +968,6 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         TT_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
         B.key = key;
         B.valid = true;
-    }
-    if (!ctx->ev_in)
-        TT_HIP(hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming));
-    if (!ctx->ev_out)
-        TT_HIP(hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming));
-    if (ctx->stream != own) {
-        TT_HIP(hipEventRecord(ctx->ev_in, ctx->stream));
-        TT_HIP(hipStreamWaitEvent(own, ctx->ev_in, 0));
     }
     TT_HIP(ctx->timer.begin(TT_KERNEL_BATCH, own));
     TT_HIP(hipGraphLaunch(B.exec, own));

@@ -28,6 +28,7 @@ struct SinoParams {
     int32_t *med_x;       // [A][n] or null
     int32_t *med_s;       // [A][n] or null
     float2 *z345;         // [3][A][n] complex T3..T5 integrals, or null
+    const int *bbox;      // non-zero pixels' col/row bounds {x0, x1, y0, y1}, or null
     int n;
     int A;
     int nb;               // column blocks per angle: ceil(n / 64)
@@ -53,6 +54,7 @@ struct BatchImage {
     float *sino;          // [nt][A][n]
     int32_t *med_x;
     int32_t *med_s;
+    const int *bbox;      // non-zero pixels' col/row bounds {x0, x1, y0, y1}, or null
     int n;
     int nb;               // column blocks per angle
     int item0;            // first wave item of this image
@@ -105,6 +107,7 @@ struct SortTParams {
     int32_t *t6_rank;
     int32_t *t6_src;
     int32_t *t7_rank;
+    const int *bbox;      // non-zero pixels' col/row bounds {x0, x1, y0, y1}, or null
     // (line, which) queues: a pass reads defer_in (all lines when NULL) and
     // queues the entries it cannot hold in defer_out (2 * A * n entries each)
     const int32_t *defer_in;
@@ -189,6 +192,13 @@ hipError_t launch_circus_vjp(const CircusVjpParams &p, hipStream_t s);
 hipError_t launch_transpose(const float *in, float *out, int n, hipStream_t s);
 hipError_t launch_pad(const float *in, int rows, int cols, float *out, int n,
                       int dfx, int dfy, hipStream_t s);
+// Bounding box {x0, x1, y0, y1} (columns, rows) of the non-zero pixels of an
+// n x n column-major image, for the line kernels' row ranges; an all-zero
+// image gives x0 > x1.
+hipError_t launch_image_bbox(const float *img, int n, int *bbox, hipStream_t s);
+// The same for every image of a batch (bbox of image k at images[k].bbox).
+hipError_t launch_image_bbox_batched(const BatchImage *images, int nimg, int nmax,
+                                     hipStream_t s);
 hipError_t launch_sino_t05(const SinoParams &p, bool need_x, bool need_s,
                            bool do345, hipStream_t s);
 // n > 512 needs two queues of 2 * A * n entries and two counters.

@@ -103,6 +103,69 @@ __device__ __forceinline__ float rot_sample(const float *__restrict__ src,
 // ---------------------------------------------------------------------------
 // Where one line's results go.  slot[t] is the output slot of T-code t
 // (-1 if not requested); tabn is the row stride of the T3..T5 tables in LDS.
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+    for (int m = kWave / 2; m > 0; m >>= 1) {
+        const int o = __shfl_xor(v, m, kWave);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_min_i(int v)
+{
+#pragma unroll
+    for (int m = kWave / 2; m > 0; m >>= 1) {
+        const int o = __shfl_xor(v, m, kWave);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// Rows r with base + (r - o) * slope in [lo_v, hi_v] (exact arithmetic),
+// intersected into [*lo, *hi].
+__device__ __forceinline__ void slab_rows(double base, double slope, double lo_v,
+                                          double hi_v, double o, double *lo, double *hi)
+{
+    if (fabs(slope) < 1e-30) {
+        if (base < lo_v || base > hi_v)
+            *lo = 1.0, *hi = 0.0;
+        return;
+    }
+    double r1 = (lo_v - base) / slope + o, r2 = (hi_v - base) / slope + o;
+    if (r1 > r2) {
+        const double t = r1;
+        r1 = r2;
+        r2 = t;
+    }
+    *lo = r1 > *lo ? r1 : *lo;
+    *hi = r2 < *hi ? r2 : *hi;
+}
+
+// Rows of column col (angle with trig c, s) whose sample can be non-zero:
+// the sample at row r reads taps (trunc(x), trunc(y)) + {0,1}^2 with
+// x = px c + (r - o) s + o, y = -px s + (r - o) c + o.  Outside the rows
+// where the exact x and y lie within 2 pixels of the bounding box of the
+// non-zero pixels (float rounding of x, y is below 1e-3 here), every tap
+// lies outside that box, so the sample is exactly 0.  Returns [lo, hi]
+// (lo > hi when empty); no box means every row.
+__device__ __forceinline__ int2 line_rows(const int *bbox, int n, int col, float c, float s)
+{
+    if (!bbox)
+        return int2{0, n - 1};
+    const int bx0 = bbox[0], bx1 = bbox[1], by0 = bbox[2], by1 = bbox[3];
+    if (bx0 > bx1 || by0 > by1)
+        return int2{n, -1};
+    const double o = (n - 1) / 2.0, px = (double)col - o;
+    double lo = 0.0, hi = (double)(n - 1);
+    slab_rows(px * (double)c + o, (double)s, bx0 - 2.0, bx1 + 2.0, o, &lo, &hi);
+    slab_rows(-px * (double)s + o, (double)c, by0 - 2.0, by1 + 2.0, o, &lo, &hi);
+    if (!(lo <= hi))
+        return int2{n, -1};
+    return int2{(int)ceil(lo), (int)floor(hi)};
+}
+
 struct LineDst {
     float *sino;
     int32_t *med_x;
@@ -117,8 +180,10 @@ struct LineDst {
 template <bool TR, bool NEED_X, bool NEED_S, bool DO345>
 __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
                                           const float2 *s_tab, int a, int col,
-                                          bool active, float c, float s)
+                                          bool active, float c, float s, int R0, int R1)
 {
+    // Rows outside [R0, R1) sample exactly 0 for every lane of the wave: they
+    // add nothing to any sum, prefix or integral, so the loops skip them.
     const int n = p.n;
     const float o = (float)((n - 1) / 2.0);
     const float lim = (float)(n - 1);
@@ -135,7 +200,8 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
         rx[j] = 0.0f;
         rs[j] = 0.0f;
     }
-    for (int r0 = 0; r0 < a8; r0 += 8) {
+    const int a8_end = a8 < R1 ? a8 : R1;
+    for (int r0 = R0 & ~7; r0 < a8_end; r0 += 8) {
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s,
@@ -180,7 +246,8 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
         }
         Sx = (ax[0] + ax[2]) + (ax[1] + ax[3]);
         Ss = (as[0] + as[2]) + (as[1] + as[3]);
-        for (int r = a4; r < n; r++) {
+        const int tail_end = n < R1 ? n : R1;
+        for (int r = a4; r < tail_end; r++) {
             const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s,
                                            (float)r - o);
             Sx += v;
@@ -206,8 +273,15 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
     double r3 = 0.0, i3 = 0.0, r4 = 0.0, i4 = 0.0, r5 = 0.0, i5 = 0.0;
     int m = -1, sm = -1;
     const bool do3 = p.slot.s[3] >= 0, do4 = p.slot.s[4] >= 0, do5 = p.slot.s[5] >= 0;
-    for (int rb = 0; rb < n; rb += 64) {
-        const int re = rb + 64 < n ? rb + 64 : n;
+    if (R0 > 0 || R0 >= R1) {
+        // the walk's first row is a skipped zero: it passes there iff S <= 0
+        if (NEED_X && 2.0f * 0.0f >= Sx)
+            m = 0;
+        if (NEED_S && 2.0f * 0.0f >= Ss)
+            sm = 0;
+    }
+    for (int rb = R0; rb < R1; rb += 64) {
+        const int re = rb + 64 < R1 ? rb + 64 : R1;
         float f1 = 0.0f, f2 = 0.0f;
         float f3r = 0.0f, f3i = 0.0f, f4r = 0.0f, f4i = 0.0f, f5r = 0.0f, f5i = 0.0f;
         for (int r = rb; r < re; r++) {
@@ -325,14 +399,16 @@ __global__ __launch_bounds__(kBlock) void sino_t05_kernel(SinoParams p)
         col = p.n - 1;
     const float2 cs = p.trig[a];
     const LineDst dst = {p.sino, p.med_x, p.med_s, p.z345, p.n, p.A, p.n, p.slot};
+    const int2 rows = line_rows(p.bbox, p.n, col, cs.x, cs.y);
+    const int R0 = wave_min_i(rows.x), R1 = wave_max_i(rows.y) + 1;
     // Lanes step (dx, dy) = (c, -s) per column: read the copy whose fast axis
     // is the one the lanes mostly move along.
     if (fabsf(cs.x) > fabsf(cs.y))
         sino_line<true, NEED_X, NEED_S, DO345>(dst, p.imgT, s_tab, a, col, active,
-                                              cs.x, cs.y);
+                                              cs.x, cs.y, R0, R1);
     else
         sino_line<false, NEED_X, NEED_S, DO345>(dst, p.img, s_tab, a, col, active,
-                                               cs.x, cs.y);
+                                               cs.x, cs.y, R0, R1);
 }
 
 // ---------------------------------------------------------------------------
@@ -378,12 +454,14 @@ __global__ __launch_bounds__(kBlock) void sino_t05_batched_kernel(BatchSinoParam
         const float2 cs = p.trig[a];
         const LineDst dst = {im.sino, im.med_x, im.med_s, im.z345, im.n, p.A, p.tabn,
                              p.slot};
+        const int2 rows = line_rows(im.bbox, im.n, col, cs.x, cs.y);
+        const int R0 = wave_min_i(rows.x), R1 = wave_max_i(rows.y) + 1;
         if (fabsf(cs.x) > fabsf(cs.y))
             sino_line<true, NEED_X, NEED_S, DO345>(dst, im.imgT, s_tab, a, col,
-                                                  active, cs.x, cs.y);
+                                                  active, cs.x, cs.y, R0, R1);
         else
             sino_line<false, NEED_X, NEED_S, DO345>(dst, im.img, s_tab, a, col,
-                                                   active, cs.x, cs.y);
+                                                   active, cs.x, cs.y, R0, R1);
     }
 }
 
@@ -869,16 +947,6 @@ __device__ __forceinline__ void wave_bitonic_sort(uint64_t (&k)[KPL], int lane)
 }
 
 // Wave-wide reductions.
-__device__ __forceinline__ int wave_max_i(int v)
-{
-#pragma unroll
-    for (int m = kWave / 2; m > 0; m >>= 1) {
-        const int o = __shfl_xor(v, m, kWave);
-        v = o > v ? o : v;
-    }
-    return v;
-}
-
 __device__ __forceinline__ int wave_sum_i(int v)
 {
 #pragma unroll
@@ -1100,11 +1168,15 @@ __device__ __forceinline__ void sort_line(const SortTParams &p, int line, int wh
     const bool t6 = which == 0;
     const int base = t6 ? p.med_s[line] : p.med_x[line];
     const int L = n - base;
+    // rows past the line's last possibly non-zero row sample exactly 0
+    const int2 rows = line_rows(p.bbox, n, col, cs.x, cs.y);
+    const int Ls = rows.y + 1 - base < L ? rows.y + 1 - base : L;
     // consecutive lanes take consecutive rows (neighbouring taps)
     int last_nz = -1;
     bool bad = false;
-    uint64_t kmax = 0;
-    for (int i = lane; i < L; i += kWave) {
+    // the largest key among the unsampled zeros: (0, L - 1)
+    uint64_t kmax = Ls < L ? (((uint64_t)ord_key(0.0f) << 32) | (uint32_t)(L - 1)) : 0ull;
+    for (int i = lane; i < Ls; i += kWave) {
         const float pyo = (float)(base + i) - o;
         const float x = tr ? rot_sample<true>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo)
                            : rot_sample<false>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo);
@@ -1679,6 +1751,89 @@ __global__ __launch_bounds__(kBlock) void to_float_kernel(const double *in, int 
     }
 }
 
+// Bounding box of the non-zero pixels of an n x n column-major image
+// (pixel (row Y, col X) at X * n + Y): bb = {x0, x1, y0, y1}, set to
+// {n, -1, n, -1} beforehand.  Block-local bounds, one atomic each per block.
+// Block b takes columns b, b + nblocks, ...; its threads stride down rows.
+__device__ __forceinline__ void bbox_accumulate(const float *img, int n, int block,
+                                                int nblocks, int *bb)
+{
+    __shared__ int s_b[4][kBlock / kWave];
+    int x0 = n, x1 = -1, y0 = n, y1 = -1;
+    for (int X = block; X < n; X += nblocks) {
+        const float *col = img + (size_t)X * n;
+        for (int Y = threadIdx.x; Y < n; Y += kBlock) {
+            if (col[Y] != 0.0f) {
+                x0 = X < x0 ? X : x0;
+                x1 = X > x1 ? X : x1;
+                y0 = Y < y0 ? Y : y0;
+                y1 = Y > y1 ? Y : y1;
+            }
+        }
+    }
+    x0 = wave_min_i(x0);
+    x1 = wave_max_i(x1);
+    y0 = wave_min_i(y0);
+    y1 = wave_max_i(y1);
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    if (lane == 0) {
+        s_b[0][w] = x0;
+        s_b[1][w] = x1;
+        s_b[2][w] = y0;
+        s_b[3][w] = y1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / kWave; k++) {
+            x0 = s_b[0][k] < x0 ? s_b[0][k] : x0;
+            x1 = s_b[1][k] > x1 ? s_b[1][k] : x1;
+            y0 = s_b[2][k] < y0 ? s_b[2][k] : y0;
+            y1 = s_b[3][k] > y1 ? s_b[3][k] : y1;
+        }
+        if (x1 >= 0) {
+            atomicMin(bb + 0, x0);
+            atomicMax(bb + 1, x1);
+            atomicMin(bb + 2, y0);
+            atomicMax(bb + 3, y1);
+        }
+    }
+}
+
+__global__ void bbox_init_kernel(int *bb, int n)
+{
+    if (threadIdx.x == 0) {
+        bb[0] = n;
+        bb[1] = -1;
+        bb[2] = n;
+        bb[3] = -1;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void image_bbox_kernel(const float *img, int n, int *bb)
+{
+    bbox_accumulate(img, n, blockIdx.x, gridDim.x, bb);
+}
+
+__global__ void bbox_init_batched_kernel(const BatchImage *images, int nimg)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nimg) {
+        int *bb = const_cast<int *>(images[k].bbox);
+        const int n = images[k].n;
+        bb[0] = n;
+        bb[1] = -1;
+        bb[2] = n;
+        bb[3] = -1;
+    }
+}
+
+// grid.y walks the images
+__global__ __launch_bounds__(kBlock) void image_bbox_batched_kernel(const BatchImage *images)
+{
+    const BatchImage &im = images[blockIdx.y];
+    bbox_accumulate(im.img, im.n, blockIdx.x, gridDim.x, const_cast<int *>(im.bbox));
+}
+
 // out(i, j) = v(i, j) / sqrt(lambda_j) for count column-major k x k matrices
 // (eigenvectors v, eigenvalues lambda, k per matrix).
 __global__ __launch_bounds__(kBlock) void scale_eigvec_kernel(const double *v,
@@ -1971,6 +2126,21 @@ hipError_t launch_align(const float *in, int n, int cols, const int32_t *offset,
 {
     align_kernel<<<grid_for((size_t)rows_out * cols), kBlock, 0, s>>>(
         in, n, cols, offset, maxoff, rows_out, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_image_bbox(const float *img, int n, int *bbox, hipStream_t s)
+{
+    bbox_init_kernel<<<1, kWave, 0, s>>>(bbox, n);
+    image_bbox_kernel<<<n < 1024 ? n : 1024, kBlock, 0, s>>>(img, n, bbox);
+    return hipGetLastError();
+}
+
+hipError_t launch_image_bbox_batched(const BatchImage *images, int nimg, int nmax,
+                                     hipStream_t s)
+{
+    bbox_init_batched_kernel<<<(nimg + kBlock - 1) / kBlock, kBlock, 0, s>>>(images, nimg);
+    image_bbox_batched_kernel<<<dim3(nmax < 64 ? nmax : 64, nimg), kBlock, 0, s>>>(images);
     return hipGetLastError();
 }
 
