@@ -10,6 +10,7 @@
 #include "tt_internal.hpp"
 
 #include <math.h>
+#include <type_traits>
 
 namespace ttk {
 
@@ -901,45 +902,68 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m)
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
-// Ascending bitonic sort of 64*KPL keys held by one wave in registers,
-// blocked layout: lane l holds positions l*KPL .. l*KPL+KPL-1.  Strides below
-// KPL exchange registers inside a lane; larger strides exchange with lane
-// l ^ (stride / KPL) through shuffles.  Wave-synchronous: no barriers.
-template <int KPL>
-__device__ __forceinline__ void wave_bitonic_sort(uint64_t (&k)[KPL], int lane)
+__device__ __forceinline__ uint32_t shfl_xor_key(uint32_t v, int m)
+{
+    return (uint32_t)__shfl_xor((int)v, m, kWave);
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_key(uint64_t v, int m)
+{
+    return shfl_xor64(v, m);
+}
+
+// Ascending bitonic sort of 64*KPL keys (uint32_t or uint64_t) held by one
+// wave in registers, blocked layout: lane l holds positions l*KPL ..
+// l*KPL+KPL-1.  Merge sizes up to KPL stay inside a lane and are unrolled
+// with their directions known at compile time; for larger sizes the
+// direction depends on the lane only, strides of KPL and more exchange with
+// lane l ^ (stride / KPL) through shuffles.  Wave-synchronous: no barriers.
+// (DPP and ds_swizzle forms of the exchanges were measured slower here:
+// they raise the VGPR count and cost occupancy.)
+template <int KPL, class Key>
+__device__ __forceinline__ void wave_bitonic_sort(Key (&k)[KPL], int lane)
 {
     constexpr int M = kWave * KPL;
-    for (int size = 2; size <= M; size <<= 1) {
-        // strides that cross lanes (runtime loop; the register index is static)
-        for (int stride = size >> 1; stride >= KPL; stride >>= 1) {
-            const int lx = stride / KPL;
-            const bool lower = (lane & lx) == 0;
+    const bool lane_up = (lane & 1) == 0;  // direction of the size-KPL merge
+#pragma unroll
+    for (int size = 2; size <= KPL; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
 #pragma unroll
             for (int r = 0; r < KPL; r++) {
-                const int pos = lane * KPL + r;
-                const bool up = (pos & size) == 0;
-                const uint64_t o = shfl_xor64(k[r], lx);
-                const bool keep_min = lower == up;
-                const uint64_t mn = k[r] < o ? k[r] : o;
-                const uint64_t mx = k[r] < o ? o : k[r];
-                k[r] = keep_min ? mn : mx;
+                const int r2 = r ^ stride;
+                if (r2 > r) {
+                    const bool up = size == KPL ? lane_up : (r & size) == 0;
+                    const Key a = k[r], b = k[r2];
+                    const bool sw = (a > b) == up;
+                    k[r] = sw ? b : a;
+                    k[r2] = sw ? a : b;
+                }
             }
         }
-        // strides inside a lane
+    }
+    for (int size = 2 * KPL; size <= M; size <<= 1) {
+        const bool up = ((lane * KPL) & size) == 0;
+        for (int stride = size >> 1; stride >= KPL; stride >>= 1) {
+            const int lx = stride / KPL;
+            // the lower lane of each pair keeps the minimum when ascending
+            const bool keep_min = ((lane & lx) == 0) == up;
+#pragma unroll
+            for (int r = 0; r < KPL; r++) {
+                const Key o = shfl_xor_key(k[r], lx);
+                k[r] = ((k[r] < o) == keep_min) ? k[r] : o;
+            }
+        }
 #pragma unroll
         for (int stride = KPL / 2; stride > 0; stride >>= 1) {
-            if (stride < size) {
 #pragma unroll
-                for (int r = 0; r < KPL; r++) {
-                    const int r2 = r ^ stride;
-                    if (r2 > r) {
-                        const int pos = lane * KPL + r;
-                        const bool up = (pos & size) == 0;
-                        const uint64_t a = k[r], b = k[r2];
-                        const bool sw = (a > b) == up;
-                        k[r] = sw ? b : a;
-                        k[r2] = sw ? a : b;
-                    }
+            for (int r = 0; r < KPL; r++) {
+                const int r2 = r ^ stride;
+                if (r2 > r) {
+                    const Key a = k[r], b = k[r2];
+                    const bool sw = (a > b) == up;
+                    k[r] = sw ? b : a;
+                    k[r2] = sw ? a : b;
                 }
             }
         }
@@ -1087,17 +1111,25 @@ template <int K, bool T6>
 __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le, int lane,
                                                        int *k_out)
 {
-    uint64_t key[K];
+    // T6 keys carry the index (stable order, and the source row is an
+    // output); T7 needs only the order of the values: equal values are
+    // interchangeable, so its keys are the 32-bit orderable values
+    using Key = typename std::conditional<T6, uint64_t, uint32_t>::type;
+    Key key[K];
     int zin = 0;
 #pragma unroll
     for (int r = 0; r < K; r++) {
         const int i = r * kWave + lane;  // any placement: the sort orders them
-        key[r] = ~0ull;
+        key[r] = (Key)~(Key)0;
         if (i < Le) {
             const float x = xs[i];
-            const float kv = T6 ? (float)i * x : x;
-            key[r] = ((uint64_t)ord_key(kv) << 32) | (uint32_t)i;
-            zin += (T6 && kv == 0.0f) ? 1 : 0;
+            if constexpr (T6) {
+                const float kv = (float)i * x;
+                key[r] = ((uint64_t)ord_key(kv) << 32) | (uint32_t)i;
+                zin += kv == 0.0f ? 1 : 0;
+            } else {
+                key[r] = ord_key(x);
+            }
         }
     }
     wave_bitonic_sort<K>(key, lane);
@@ -1109,8 +1141,12 @@ __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le,
     for (int r = 0; r < K; r++) {
         const int pos = lane * K + r;
         sq[r] = 0.0f;
-        if (pos < Le)
-            sq[r] = sqrtf(T6 ? xs[(uint32_t)key[r]] : ord_val((uint32_t)(key[r] >> 32)));
+        if (pos < Le) {
+            if constexpr (T6)
+                sq[r] = sqrtf(xs[(uint32_t)key[r]]);
+            else
+                sq[r] = sqrtf(ord_val(key[r]));
+        }
     }
     wave_lds_sync();
 #pragma unroll
@@ -1130,11 +1166,14 @@ __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le,
     if (k >= ins && k < ins + zt)  // a trailing zero: w (or x) = +0, index Le + ...
         return ((uint64_t)ord_key(0.0f) << 32) | (uint32_t)(Le + (k - ins));
     const int pos = k < ins ? k : k - zt;
-    uint64_t kk = 0;
+    Key kk = 0;
 #pragma unroll
     for (int r = 0; r < K; r++)
         kk = (r == pos % K) ? key[r] : kk;
-    return shfl64(kk, pos / K);
+    if constexpr (T6)
+        return shfl64(kk, pos / K);
+    else  // value key in the high word (no index: T7 exports none)
+        return (uint64_t)(uint32_t)__shfl((int)kk, pos / K, kWave) << 32;
 }
 
 // T6 (which = 0, tail from the sqrt-median sm) or T7 (which = 1, tail from
@@ -2132,7 +2171,8 @@ hipError_t launch_align(const float *in, int n, int cols, const int32_t *offset,
 hipError_t launch_image_bbox(const float *img, int n, int *bbox, hipStream_t s)
 {
     bbox_init_kernel<<<1, kWave, 0, s>>>(bbox, n);
-    image_bbox_kernel<<<n < 1024 ? n : 1024, kBlock, 0, s>>>(img, n, bbox);
+    // few blocks: each ends with four atomics on the same four words
+    image_bbox_kernel<<<n < 256 ? n : 256, kBlock, 0, s>>>(img, n, bbox);
     return hipGetLastError();
 }
 
