@@ -64,6 +64,8 @@ typedef struct tt_sino_aux {
     int32_t *t6_rank;     /* T6: position k* in the sorted weighted tail    */
     int32_t *t6_src;      /* T6: tail offset r of that element (idx[k*])    */
     int32_t *t7_rank;     /* T7: position k* in the sorted tail             */
+    int32_t *t7_src;      /* T7: row m + idx[k*] of that element, idx the
+                             stable order of the tail (the adjoint's source) */
 } tt_sino_aux;
 
 typedef struct tt_context tt_context;
@@ -203,7 +205,10 @@ TT_API tt_status tt_batched_launch_shape(tt_context *ctx, int *waves_per_block,
  * and P2's rank held at their forward values (they are piecewise constant in
  * the input) and the sample positions of the forward rotation.  The forward
  * quantities are recomputed from d_image; rotated images are never stored.
- * Supported: T0..T5, P1, P2, P3, Hermite; padded sides up to 1800. */
+ * T6 and T7 hold their sort (hence their source row) fixed as well: with it,
+ * T6 = q x[sm + q] and T7 = x[src] are linear in one sample each.
+ * Supported: T0..T7 (the batched form T0..T5), P1, P2, P3, Hermite; padded
+ * sides up to 1800. */
 
 /* d_dsino (n_sino x n_cols x n) = J_circus^T d_dcircus (n_sino x np x n_cols)
  * for the sinograms d_sino. */

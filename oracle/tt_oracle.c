@@ -1151,30 +1151,27 @@ static void circus_vjp_f64(const double *x, int n, const int *pcodes,
     }
 }
 
-int tto_sinograms_vjp_f64(const float *img, int n, int n_angles, float step_deg,
-                          const int *tcodes, int nt, const int32_t *med_x,
-                          const int32_t *med_s, const double *dsino, double *dimg,
-                          int nthreads);
-
-/* J^T dC of the whole pipeline (sinograms of T codes 0..5, then circus),
- * in FP64 with reference geometry, medians (med_x, med_s) and P2 sources
- * (p2_src [nt][np][A]) fixed at the FP32 run's values.  dcircus is
- * [nt][np][A]; dimg is n*n (column-major).  T3..T5 use their double tables. */
+/* J^T dC of the whole pipeline (sinograms of T codes 0..7, then circus),
+ * in FP64 with reference geometry, medians (med_x, med_s), T6/T7 sources
+ * (t6_src relative to sm, t7_src absolute, [A][n]) and P2 sources (p2_src
+ * [nt][np][A]) fixed at the FP32 run's values.  dcircus is [nt][np][A];
+ * dimg is n*n (column-major).  T3..T5 use their double tables. */
 int tto_backward_f64(const float *img, int n, int n_angles, float step_deg,
                      const int *tcodes, int nt, const int *pcodes,
                      const int *orders, const int *centers, int np,
                      const int32_t *med_x, const int32_t *med_s,
+                     const int32_t *t6_src, const int32_t *t7_src,
                      const int32_t *p2_src, const double *dcircus, double *dimg,
                      int nthreads)
 {
-    if (n <= 0 || n_angles <= 0 || nt <= 0)
+    if (n <= 0 || n_angles <= 0 || nt <= 0 || !valid_tcodes(tcodes, nt))
         return -1;
     for (int t = 0; t < nt; t++)
-        if (tcodes[t] < 0 || tcodes[t] > 5)
+        if ((tcodes[t] == 6 && !t6_src) || (tcodes[t] == 7 && !t7_src))
             return -1;
     const int A = n_angles;
     double *sino = (double *)malloc(sizeof(double) * (size_t)nt * n * A);
-    tto_sinograms_f64(img, n, A, step_deg, tcodes, nt, med_x, med_s, NULL, NULL, 1,
+    tto_sinograms_f64(img, n, A, step_deg, tcodes, nt, med_x, med_s, t6_src, t7_src, 1,
                       sino, nthreads);
     double *dsino = (double *)calloc((size_t)nt * n * A, sizeof(double));
     if (nthreads <= 0)
@@ -1201,23 +1198,26 @@ int tto_backward_f64(const float *img, int n, int n_angles, float step_deg,
         free(src);
     }
     int rc = tto_sinograms_vjp_f64(img, n, A, step_deg, tcodes, nt, med_x, med_s,
-                                   dsino, dimg, nthreads);
+                                   t6_src, t7_src, dsino, dimg, nthreads);
     free(sino);
     free(dsino);
     return rc;
 }
 
 /* J_sinograms^T dsino ([nt][A][n]) in FP64, reference geometry, medians
- * fixed; dimg is n*n column-major. */
+ * and T6/T7 sources fixed; dimg is n*n column-major.  With the sort fixed,
+ * T6 = q * x[sm + q] (q = t6_src) and T7 = x[t7_src] are linear in one
+ * sample each (c++/src/functionals.cpp:226-276). */
 int tto_sinograms_vjp_f64(const float *img, int n, int n_angles, float step_deg,
                           const int *tcodes, int nt, const int32_t *med_x,
-                          const int32_t *med_s, const double *dsino, double *dimg,
+                          const int32_t *med_s, const int32_t *t6_src,
+                          const int32_t *t7_src, const double *dsino, double *dimg,
                           int nthreads)
 {
-    if (n <= 0 || n_angles <= 0 || nt <= 0)
+    if (n <= 0 || n_angles <= 0 || nt <= 0 || !valid_tcodes(tcodes, nt))
         return -1;
     for (int t = 0; t < nt; t++)
-        if (tcodes[t] < 0 || tcodes[t] > 5)
+        if ((tcodes[t] == 6 && !t6_src) || (tcodes[t] == 7 && !t7_src))
             return -1;
     if (nthreads <= 0)
         nthreads = 1;
@@ -1225,7 +1225,7 @@ int tto_sinograms_vjp_f64(const float *img, int n, int n_angles, float step_deg,
     double *tre[6] = {0}, *tim[6] = {0};
     for (int t = 0; t < nt; t++) {
         int code = tcodes[t];
-        if (code >= 3 && !tre[code]) {
+        if (code >= 3 && code <= 5 && !tre[code]) {
             tre[code] = (double *)malloc(sizeof(double) * (size_t)n);
             tim[code] = (double *)malloc(sizeof(double) * (size_t)n);
             t345_table_f64(code, n, tre[code], tim[code]);
@@ -1261,6 +1261,11 @@ int tto_sinograms_vjp_f64(const float *img, int n, int n_angles, float step_deg,
                         for (int r = m; r < n; r++)
                             G[r] += g * (code == 1 ? (double)(r - m)
                                                    : (double)(r - m) * (r - m));
+                    } else if (code == 6) {
+                        const int q = t6_src[line];
+                        G[med_s[line] + q] += g * (double)q;
+                    } else if (code == 7) {
+                        G[t7_src[line]] += g;
                     } else {
                         const int sm = med_s[line];
                         double zr = 0.0, zi = 0.0;

@@ -88,17 +88,20 @@ void tto_dft_f64(const double *x, int n, double *re, double *im);
 
 /* ---- FP64 adjoint: J^T dC with medians and P2 sources fixed ------------ */
 /* dcircus and p2_src are [nt][np][A]; dimg is n*n column-major.  T codes
- * 0..5, P codes 0..3, reference geometry. */
+ * 0..7 (T6 and T7 with their sources t6_src, relative to sm, and t7_src,
+ * absolute, fixed as well), P codes 0..3, reference geometry. */
 int tto_backward_f64(const float *img, int n, int n_angles, float step_deg,
                      const int *tcodes, int nt, const int *pcodes,
                      const int *orders, const int *centers, int np,
                      const int32_t *med_x, const int32_t *med_s,
+                     const int32_t *t6_src, const int32_t *t7_src,
                      const int32_t *p2_src, const double *dcircus, double *dimg,
                      int nthreads);
 
 int tto_sinograms_vjp_f64(const float *img, int n, int n_angles, float step_deg,
                           const int *tcodes, int nt, const int32_t *med_x,
-                          const int32_t *med_s, const double *dsino, double *dimg,
+                          const int32_t *med_s, const int32_t *t6_src,
+                          const int32_t *t7_src, const double *dsino, double *dimg,
                           int nthreads);
 
 /* ---- full reference step: sinograms -> circus -> zscore ---------------- */

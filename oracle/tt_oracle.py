@@ -89,10 +89,11 @@ def lib():
         L.tto_dft_f64.argtypes = [_f64p, C.c_int, _f64p, _f64p]
         L.tto_backward_f64.argtypes = [_f32p, C.c_int, C.c_int, C.c_float, _i32p,
                                        C.c_int, _i32p, _i32p, _i32p, C.c_int, vp, vp,
-                                       vp, _f64p, _f64p, C.c_int]
+                                       vp, vp, vp, _f64p, _f64p, C.c_int]
         L.tto_backward_f64.restype = C.c_int
         L.tto_sinograms_vjp_f64.argtypes = [_f32p, C.c_int, C.c_int, C.c_float, _i32p,
-                                            C.c_int, vp, vp, _f64p, _f64p, C.c_int]
+                                            C.c_int, vp, vp, vp, vp, _f64p, _f64p,
+                                            C.c_int]
         L.tto_sinograms_vjp_f64.restype = C.c_int
         L.tto_transform.argtypes = [_f32p, C.c_int, C.c_int, C.c_float, _i32p,
                                     C.c_int, _i32p, _i32p, _i32p, C.c_int, _f32p,
@@ -368,24 +369,37 @@ def backward_f64(img, n, n_angles, step_deg, tcodes, pfuncs, aux, p2_src, dcircu
     dimg = np.empty(n * n, np.float64)
     mx = np.ascontiguousarray(aux["med_x"], np.int32).ravel()
     ms = np.ascontiguousarray(aux["med_s"], np.int32).ravel()
+    s6, s7 = _t67_src(aux)
     src = np.ascontiguousarray(p2_src, np.int32).ravel()
     dc = np.ascontiguousarray(dcircus, np.float64).ravel()
     rc = lib().tto_backward_f64(_f32(img).ravel(), n, n_angles, step_deg, tc, tc.size,
                                 codes, orders, centers, codes.size, _ptr(mx), _ptr(ms),
-                                _ptr(src), dc, dimg, nthreads)
+                                _ptr(s6), _ptr(s7), _ptr(src), dc, dimg, nthreads)
     if rc:
         raise ValueError("tto_backward_f64 rejected its arguments")
     return dimg
 
 
+def _t67_src(aux):
+    """T6 (relative to sm) and T7 (absolute row) sources of an aux dict, or
+    None where absent."""
+    out = []
+    for k in ("t6_src", "t7_src"):
+        v = aux.get(k)
+        out.append(None if v is None else np.ascontiguousarray(v, np.int32).ravel())
+    return out
+
+
 def sinograms_vjp_f64(img, n, n_angles, step_deg, tcodes, aux, dsino, nthreads=1):
-    """J_sinograms^T dsino ([nt, A, n]) in FP64 with fixed medians."""
+    """J_sinograms^T dsino ([nt, A, n]) in FP64 with fixed medians and T6/T7
+    sources."""
     tc = np.ascontiguousarray(tcodes, np.int32)
     dimg = np.empty(n * n, np.float64)
     mx = np.ascontiguousarray(aux["med_x"], np.int32).ravel()
     ms = np.ascontiguousarray(aux["med_s"], np.int32).ravel()
+    s6, s7 = _t67_src(aux)
     rc = lib().tto_sinograms_vjp_f64(_f32(img).ravel(), n, n_angles, step_deg, tc,
-                                     tc.size, _ptr(mx), _ptr(ms),
+                                     tc.size, _ptr(mx), _ptr(ms), _ptr(s6), _ptr(s7),
                                      np.ascontiguousarray(dsino, np.float64).ravel(),
                                      dimg, nthreads)
     if rc:

@@ -2,6 +2,9 @@
 tt_backward) with the weighted medians and P2 ranks held at their forward
 values.
 
+T6 and T7 hold their sort fixed too (their source rows), which makes them
+linear in one sample each.
+
 Bar: normwise relative error <= 1e-5 against the oracle's FP64 analytic
 vector-Jacobian products (reference geometry, same fixed indices), whose own
 correctness is pinned on the CPU against FP64 central differences
@@ -37,7 +40,7 @@ def test_sinogram_vjp_matches_fp64(oracle, gpu):
     side, A, step = 40, 90, 4.0
     img, n = padded(oracle, inputs.blob_image(side, 31, nblobs=6, sigma=(3, 12)),
                     side, side)
-    codes = [0, 1, 2, 3, 4, 5]
+    codes = [0, 1, 2, 3, 4, 5, 6, 7]
     _, aux = oracle.sinograms(img, n, A, step, codes, nthreads=NTHREADS)
     rng = np.random.default_rng(1)
     dS = rng.standard_normal((len(codes), A, n)).astype(np.float32)
@@ -77,7 +80,7 @@ def test_full_backward_matches_fp64(oracle, gpu, side, A, step):
     img, n = padded(oracle, inputs.blob_image(side, 41, nblobs=6, sigma=(2, 10)),
                     side, side)
     pf = P123 + [(0, 2, (n - 1) // 2)]
-    codes = [0, 1, 2, 3, 4, 5]
+    codes = [0, 1, 2, 3, 4, 5, 6, 7]
     aux, src = fixed_indices(oracle, img, n, A, step, codes, pf)
     rng = np.random.default_rng(3)
     dC = rng.standard_normal((len(codes), len(pf), A)).astype(np.float32)
@@ -147,7 +150,10 @@ def test_autograd_wrapper_loss_gradient(oracle, gpu):
 
 def test_backward_rejects_unsupported(gpu):
     img = torch.zeros((7, 7), device="cuda")
+    with pytest.raises(ValueError):
+        core.backward(img, 4, 90.0, [8], P123, torch.zeros((1, 3, 4), device="cuda"))
+    # the batched adjoint covers T0..T5
     with pytest.raises(Exception):
-        core.backward(img, 4, 90.0, [6], P123, torch.zeros((1, 3, 4), device="cuda"))
+        core.BatchBackward([img], 4, 90.0, [6], P123).run()
     with pytest.raises(ValueError):
         core.backward(img, 4, 90.0, [0, 0], P123, torch.zeros((2, 3, 4), device="cuda"))

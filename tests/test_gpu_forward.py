@@ -50,6 +50,7 @@ def check_forward(oracle, img, n, A, step, codes, f64=True, margin=True):
         assert_same_indices("t6_src", gaux["t6_src"], oaux["t6_src"])
     if 7 in codes:
         assert_same_indices("t7_rank", gaux["t7_rank"], oaux["t7_rank"])
+        assert_same_indices("t7_src", gaux["t7_src"], oaux["t7_src"])
     for t, c in enumerate(codes):
         if c in (0, 6, 7):  # float sum in Eigen order / selected elements
             assert np.array_equal(g[t], o[t]), (c, ulp_diff(g[t], o[t]))

@@ -424,25 +424,30 @@ def test_backward_f64_matches_central_differences(oracle):
     n, dx, dy = oracle.pad_geometry(side, side)
     rng = np.random.default_rng(21)
     img = oracle.pad(rng.random(side * side, dtype=np.float32) + 0.1, side, side)
-    codes = [0, 1, 2, 3, 4, 5]
-    pf = [(1,), (2,), (3,), (0, 2, (n - 1) // 2)]
-    aux, src = _fixed_indices(oracle, img, n, A, step, codes, pf)
-    dC = rng.standard_normal((len(codes), len(pf), A))
-    grad = oracle.backward_f64(img, n, A, step, codes, pf, aux, src, dC, nthreads=4)
-    pix = [(dy + r) + (dx + c) * n for r, c in
-           ((0, 0), (3, 4), (9, 9), (5, 1), (7, 8), (2, 6))]
-    for i in pix:
-        h = 1e-3
-        ip, im = img.copy(), img.copy()
-        ip[i] += h
-        im[i] -= h
-        Lp = (dC * oracle.forward_f64_fixed(ip, n, A, step, codes, pf, aux, src)).sum()
-        Lm = (dC * oracle.forward_f64_fixed(im, n, A, step, codes, pf, aux, src)).sum()
-        fd = (Lp - Lm) / (float(ip[i]) - float(im[i]))
-        assert abs(fd - grad[i]) <= 1e-6 * max(1.0, abs(grad[i])), (i, fd, grad[i])
-    # pixels of the zero margin receive gradient only through interpolation
-    # weights, and the whole gradient is finite
-    assert np.isfinite(grad).all()
+    herm = (0, 2, (n - 1) // 2)
+    # T0-T5 with every P-functional; T6/T7 (sources fixed too) with the
+    # smooth ones: adjacent T6/T7 values can lie within a step of each
+    # other, where P1's |difference| has a kink the central difference
+    # would straddle
+    for codes, pf in (([0, 1, 2, 3, 4, 5], [(1,), (2,), (3,), herm]),
+                      ([6, 7], [(2,), (3,), herm])):
+        aux, src = _fixed_indices(oracle, img, n, A, step, codes, pf)
+        dC = rng.standard_normal((len(codes), len(pf), A))
+        grad = oracle.backward_f64(img, n, A, step, codes, pf, aux, src, dC, nthreads=4)
+        pix = [(dy + r) + (dx + c) * n for r, c in
+               ((0, 0), (3, 4), (9, 9), (5, 1), (7, 8), (2, 6))]
+        for i in pix:
+            h = 1e-3
+            ip, im = img.copy(), img.copy()
+            ip[i] += h
+            im[i] -= h
+            Lp = (dC * oracle.forward_f64_fixed(ip, n, A, step, codes, pf, aux, src)).sum()
+            Lm = (dC * oracle.forward_f64_fixed(im, n, A, step, codes, pf, aux, src)).sum()
+            fd = (Lp - Lm) / (float(ip[i]) - float(im[i]))
+            assert abs(fd - grad[i]) <= 1e-6 * max(1.0, abs(grad[i])), (codes, i, fd, grad[i])
+        # pixels of the zero margin receive gradient only through
+        # interpolation weights, and the whole gradient is finite
+        assert np.isfinite(grad).all()
 
 
 def test_backward_f64_linear_parts(oracle):

@@ -56,7 +56,7 @@ class tt_image_grad(C.Structure):
 class tt_sino_aux(C.Structure):
     _fields_ = [("median_x", C.c_void_p), ("median_sqrt", C.c_void_p),
                 ("t6_rank", C.c_void_p), ("t6_src", C.c_void_p),
-                ("t7_rank", C.c_void_p)]
+                ("t7_rank", C.c_void_p), ("t7_src", C.c_void_p)]
 
 
 _lib = None

@@ -122,7 +122,7 @@ def sinograms(image_cm: torch.Tensor, n_angles: int, step_deg: float, tfunctiona
 
     Returns ``sino`` [nt, A, n], or ``(sino, aux)`` with int32 [A, n] index
     tensors ``median_x`` and ``median_sqrt`` (plus ``t6_rank``, ``t6_src``
-    and ``t7_rank`` when T6/T7 are requested) when ``with_aux``."""
+    and ``t7_rank``, ``t7_src`` when T6/T7 are requested) when ``with_aux``."""
     _require_cuda_f32("image", image_cm)
     if image_cm.dim() != 2 or image_cm.shape[0] != image_cm.shape[1]:
         raise ValueError("image must be a square padded [n, n] tensor")
@@ -143,13 +143,14 @@ def sinograms(image_cm: torch.Tensor, n_angles: int, step_deg: float, tfunctiona
         if 6 in codes:
             names += ["t6_rank", "t6_src"]
         if 7 in codes:
-            names += ["t7_rank"]
+            names += ["t7_rank", "t7_src"]
         aux = {k: torch.full((n_angles, n), -1, dtype=torch.int32, device=dev)
                for k in names}
         ptr = {k: (aux[k].data_ptr() if k in aux else None)
-               for k in ("median_x", "median_sqrt", "t6_rank", "t6_src", "t7_rank")}
+               for k in ("median_x", "median_sqrt", "t6_rank", "t6_src", "t7_rank",
+                         "t7_src")}
         aux_s = _lib.tt_sino_aux(ptr["median_x"], ptr["median_sqrt"], ptr["t6_rank"],
-                                 ptr["t6_src"], ptr["t7_rank"])
+                                 ptr["t6_src"], ptr["t7_rank"], ptr["t7_src"])
     ctx = get_context(dev.index)
     ctx.bind_current_stream()
     carr = (C.c_int32 * max(nt, 1))(*codes)

@@ -271,7 +271,7 @@ struct tt_context {
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     DevBuf imgT, trig, tab345, dft_tw, herm_w, io_a, io_b, med_x, med_s;
-    DevBuf bw_sino, bw_dsino, bw_z, bw_rec;
+    DevBuf bw_sino, bw_dsino, bw_z, bw_rec, bw_t6src, bw_t7src;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
     DevBuf nos_w, nos_m;
     DevBuf sort_qa, sort_qb, sort_count, bbox;
@@ -428,7 +428,7 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
     DevBuf *bufs[] = {&ctx->imgT, &ctx->trig, &ctx->tab345, &ctx->dft_tw,
                       &ctx->herm_w, &ctx->io_a, &ctx->io_b, &ctx->med_x,
                       &ctx->med_s, &ctx->bw_sino, &ctx->bw_dsino, &ctx->bw_z,
-                      &ctx->bw_rec, &ctx->nos_med, &ctx->nos_a, &ctx->nos_u,
+                      &ctx->bw_rec, &ctx->bw_t6src, &ctx->bw_t7src, &ctx->nos_med, &ctx->nos_a, &ctx->nos_u,
                       &ctx->nos_v, &ctx->nos_s, &ctx->nos_e, &ctx->nos_c,
                       &ctx->nos_info, &ctx->nos_sweeps, &ctx->nos_w, &ctx->nos_m,
                       &ctx->sort_qa,
@@ -590,7 +590,7 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
     p.A = n_angles;
     p.nb = (n + ttk::kWave - 1) / ttk::kWave;
     p.items = p.nb * n_angles;
-    for (int k = 0; k < 6; k++)
+    for (int k = 0; k < 8; k++)
         p.slot.s[k] = slot[k];
     TT_HIP(ctx->timer.begin(TT_KERNEL_SINO_T05, st));
     TT_HIP(ttk::launch_sino_t05(p, need_x, need_s, do345, st));
@@ -606,6 +606,7 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
         q.t6_rank = ax.t6_rank;
         q.t6_src = ax.t6_src;
         q.t7_rank = ax.t7_rank;
+        q.t7_src = ax.t7_src;
         q.bbox = ctx->bbox.as<int>();
         q.n = n;
         q.A = n_angles;
@@ -791,7 +792,7 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
              "angle step must be positive and finite");
     TT_TRY(check_tcodes(tcodes, nt));
     TT_CHECK(nt >= 1, "at least one T-functional is required");
-    int slot[6] = {-1, -1, -1, -1, -1, -1};
+    int slot[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
     for (int t = 0; t < nt; t++) {
         if (tcodes[t] > 5)
             return fail(TT_ERR_UNSUPPORTED,
@@ -905,7 +906,7 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         sp.tabn = nmax;
         sp.A = n_angles;
         sp.counter = B.counter.as<unsigned int>();
-        for (int k = 0; k < 6; k++)
+        for (int k = 0; k < 8; k++)
             sp.slot.s[k] = slot[k];
         ttk::BatchCircusParams cp = {};
         cp.images = sp.images;
@@ -1051,12 +1052,43 @@ tt_status forward_for_adjoint(tt_context *ctx, const float *d_image, int n, int 
     p.A = A;
     p.nb = (n + ttk::kWave - 1) / ttk::kWave;
     p.items = p.nb * A;
-    for (int k = 0; k < 6; k++)
+    for (int k = 0; k < 8; k++)
         p.slot.s[k] = slot[k];
     const bool do345 = slot[3] >= 0 || slot[4] >= 0 || slot[5] >= 0;
     TT_HIP(ctx->timer.begin(TT_KERNEL_SINO_T05, st));
     TT_HIP(ttk::launch_sino_t05(p, true, true, do345, st));
     TT_HIP(ctx->timer.end(st));
+    if (slot[6] >= 0 || slot[7] >= 0) {
+        // T6/T7 values and the sources their adjoints need
+        TT_HIP(ctx->bw_t6src.ensure(sizeof(int32_t) * lines));
+        TT_HIP(ctx->bw_t7src.ensure(sizeof(int32_t) * lines));
+        ttk::SortTParams q = {};
+        q.img = d_image;
+        q.imgT = ctx->imgT.as<float>();
+        q.trig = ctx->trig.as<float2>();
+        q.sino = ctx->bw_sino.as<float>();
+        q.med_x = ctx->med_x.as<int32_t>();
+        q.med_s = ctx->med_s.as<int32_t>();
+        q.t6_src = slot[6] >= 0 ? ctx->bw_t6src.as<int32_t>() : nullptr;
+        q.t7_src = slot[7] >= 0 ? ctx->bw_t7src.as<int32_t>() : nullptr;
+        q.n = n;
+        q.A = A;
+        q.slot6 = slot[6];
+        q.slot7 = slot[7];
+        int32_t *qa = nullptr, *qb = nullptr;
+        unsigned int *counts = nullptr;
+        if (n > 512) {
+            TT_HIP(ctx->sort_qa.ensure(sizeof(int32_t) * 2 * lines));
+            TT_HIP(ctx->sort_qb.ensure(sizeof(int32_t) * 2 * lines));
+            TT_HIP(ctx->sort_count.ensure(2 * sizeof(unsigned int)));
+            qa = ctx->sort_qa.as<int32_t>();
+            qb = ctx->sort_qb.as<int32_t>();
+            counts = ctx->sort_count.as<unsigned int>();
+        }
+        TT_HIP(ctx->timer.begin(TT_KERNEL_SINO_SORT, st));
+        TT_HIP(ttk::launch_sino_sort(q, qa, qb, counts, st));
+        TT_HIP(ctx->timer.end(st));
+    }
     *sino_out = ctx->bw_sino.as<float>();
     return TT_OK;
 }
@@ -1071,11 +1103,13 @@ tt_status image_vjp(tt_context *ctx, int n, int A, const int *slot,
     r.dsino = d_dsino;
     r.med_x = ctx->med_x.as<int32_t>();
     r.med_s = ctx->med_s.as<int32_t>();
+    r.t6_src = slot[6] >= 0 ? ctx->bw_t6src.as<int32_t>() : nullptr;
+    r.t7_src = slot[7] >= 0 ? ctx->bw_t7src.as<int32_t>() : nullptr;
     r.z345 = ctx->bw_z.as<float2>();
     r.rec = ctx->bw_rec.as<ttk::LineRecord>();
     r.n = n;
     r.A = A;
-    for (int k = 0; k < 6; k++)
+    for (int k = 0; k < 8; k++)
         r.slot.s[k] = slot[k];
     ttk::GatherParams g = {};
     g.rec = r.rec;
@@ -1092,15 +1126,16 @@ tt_status image_vjp(tt_context *ctx, int n, int A, const int *slot,
     return TT_OK;
 }
 
-tt_status adjoint_slots(const int32_t *tcodes, int nt, int *slot)
+// Output slot of each T-functional (slot[8]); T6/T7 only where with_sort.
+tt_status adjoint_slots(const int32_t *tcodes, int nt, int *slot, bool with_sort)
 {
     TT_TRY(check_tcodes(tcodes, nt));
     TT_CHECK(nt >= 1, "at least one T-functional is required");
-    for (int k = 0; k < 6; k++)
+    for (int k = 0; k < 8; k++)
         slot[k] = -1;
     for (int t = 0; t < nt; t++) {
-        if (tcodes[t] > 5)
-            return fail(TT_ERR_UNSUPPORTED, "the adjoint covers T0..T5 (got T%d)",
+        if (tcodes[t] > (with_sort ? 7 : 5))
+            return fail(TT_ERR_UNSUPPORTED, "the batched adjoint covers T0..T5 (got T%d)",
                         tcodes[t]);
         TT_CHECK(slot[tcodes[t]] < 0, "duplicate T-functional %d", tcodes[t]);
         slot[tcodes[t]] = t;
@@ -1138,8 +1173,8 @@ TT_API tt_status tt_sinograms_backward(tt_context *ctx, const float *d_image,
     TT_CHECK(n_angles >= 1, "n_angles must be >= 1");
     TT_CHECK(angle_step_deg > 0.0f && isfinite(angle_step_deg),
              "angle step must be positive and finite");
-    int slot[6];
-    TT_TRY(adjoint_slots(tcodes, nt, slot));
+    int slot[8];
+    TT_TRY(adjoint_slots(tcodes, nt, slot, true));
     TT_TRY(ctx->set_device());
     float *sino = nullptr;
     TT_TRY(forward_for_adjoint(ctx, d_image, n, n_angles, angle_step_deg, slot, nt,
@@ -1158,8 +1193,8 @@ TT_API tt_status tt_backward(tt_context *ctx, const float *d_image, int n,
     TT_CHECK(n_angles >= 1, "n_angles must be >= 1");
     TT_CHECK(angle_step_deg > 0.0f && isfinite(angle_step_deg),
              "angle step must be positive and finite");
-    int slot[6];
-    TT_TRY(adjoint_slots(tcodes, nt, slot));
+    int slot[8];
+    TT_TRY(adjoint_slots(tcodes, nt, slot, true));
     TT_TRY(check_pfuncs(pf, np, n));
     TT_CHECK(np >= 1 && d_dcircus, "tt_backward needs P-functionals and d_dcircus");
     TT_TRY(ctx->set_device());
@@ -1383,8 +1418,8 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
     TT_CHECK(n_angles >= 1, "n_angles must be >= 1 (got %d)", n_angles);
     TT_CHECK(angle_step_deg > 0.0f && isfinite(angle_step_deg),
              "angle step must be positive and finite");
-    int slot[6];
-    TT_TRY(adjoint_slots(tcodes, nt, slot));
+    int slot[8];
+    TT_TRY(adjoint_slots(tcodes, nt, slot, false));
     TT_CHECK(np >= 1 && np <= ttk::kMaxP && pf, "np must be in 1..%d", ttk::kMaxP);
     for (int q = 0; q < np; q++) {
         if (pf[q].code == TT_P_HERMITE)
@@ -1491,7 +1526,7 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         sp.tabn = nmax;
         sp.A = n_angles;
         sp.counter = B.counter.as<unsigned int>();
-        for (int k = 0; k < 6; k++)
+        for (int k = 0; k < 8; k++)
             sp.slot.s[k] = slot[k];
         ttk::BatchAdjointParams ap = {};
         ap.images = sp.images;

@@ -14,7 +14,7 @@ constexpr int kNumXcd = 8;       // MI355X: 8 XCDs, workgroups round-robin
 
 // Output slot of each T code (T0..T5), -1 when not requested.
 struct Slots {
-    int s[6];
+    int s[8];  // output slot of T-functional k (0..7), or -1
 };
 
 // Forward T-functional sweep for one padded image (T0..T5).  One wave owns
@@ -39,11 +39,16 @@ struct SinoParams {
 // Adjoint.  Per-line record of the rotated-domain gradient of one angle's
 // column: G(row) = g0 + [row >= m] (g1 (row-m) + g2 (row-m)^2)
 //                + [row > sm] sum_t (alpha_t W_t.re[row-sm] + beta_t W_t.im[row-sm]).
+//          + [row == r6] c6 + [row == r7] c7
+// (T6 = q x[sm + q] and T7 = x[src7] with the sort held fixed: c6 = dT6 * q
+// at row r6 = sm + q, c7 = dT7 at row r7 = src7; rows -1 when absent).
 struct LineRecord {
     float g0, g1, g2;
     int m, sm;
     float ab[6];          // alpha3, beta3, alpha4, beta4, alpha5, beta5
-    float pad;
+    int r6, r7;
+    float c6, c7;
+    float pad[3];
 };
 
 // One image of a batch (device array, ordered by first item).
@@ -107,6 +112,7 @@ struct SortTParams {
     int32_t *t6_rank;
     int32_t *t6_src;
     int32_t *t7_rank;
+    int32_t *t7_src;      // T7 source row (stable order of the tail), or null
     const int *bbox;      // non-zero pixels' col/row bounds {x0, x1, y0, y1}, or null
     // (line, which) queues: a pass reads defer_in (all lines when NULL) and
     // queues the entries it cannot hold in defer_out (2 * A * n entries each)
@@ -139,6 +145,8 @@ struct RecordParams {
     const float *dsino;   // [nt][A][n] sinogram gradient
     const int32_t *med_x; // [A][n]
     const int32_t *med_s; // [A][n]
+    const int32_t *t6_src; // [A][n] T6 source offset from sm, or null
+    const int32_t *t7_src; // [A][n] T7 source row, or null
     const float2 *z345;   // [3][A][n]
     LineRecord *rec;      // [A][n]
     int n;

@@ -1107,14 +1107,16 @@ __device__ int weighted_median_walk_view(const TailView &v, int Le, float S)
 // order: ties by ascending index).  Replaces xs[0..Le) by the square roots
 // of the sorted part, takes the Eigen-order sum and the sequential walk over
 // the full sequence, and returns the key at the weighted-median position k.
-template <int K, bool T6>
+template <int K, bool T6, bool IDX7>
 __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le, int lane,
                                                        int *k_out)
 {
     // T6 keys carry the index (stable order, and the source row is an
-    // output); T7 needs only the order of the values: equal values are
-    // interchangeable, so its keys are the 32-bit orderable values
-    using Key = typename std::conditional<T6, uint64_t, uint32_t>::type;
+    // output); T7 needs only the order of the values (equal values are
+    // interchangeable), so its keys are the 32-bit orderable values unless
+    // its source row is asked for (IDX7: stable order, index in the key)
+    constexpr bool IDX = T6 || IDX7;
+    using Key = typename std::conditional<IDX, uint64_t, uint32_t>::type;
     Key key[K];
     int zin = 0;
 #pragma unroll
@@ -1123,17 +1125,19 @@ __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le,
         key[r] = (Key)~(Key)0;
         if (i < Le) {
             const float x = xs[i];
-            if constexpr (T6) {
-                const float kv = (float)i * x;
+            const float kv = T6 ? (float)i * x : x;
+            if constexpr (IDX) {
                 key[r] = ((uint64_t)ord_key(kv) << 32) | (uint32_t)i;
                 zin += kv == 0.0f ? 1 : 0;
             } else {
-                key[r] = ord_key(x);
+                key[r] = ord_key(kv);
             }
         }
     }
     wave_bitonic_sort<K>(key, lane);
-    const int ins = T6 ? wave_sum_i(zin) : 0;  // where the trailing zeros go
+    // where the trailing zeros go: after the sorted part's zero keys in the
+    // stable (index) order, or first when the order of equal values is free
+    const int ins = IDX ? wave_sum_i(zin) : 0;
     const int zt = L - Le;
     // gather the sorted samples before overwriting the slice with roots
     float sq[K];
@@ -1144,6 +1148,8 @@ __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le,
         if (pos < Le) {
             if constexpr (T6)
                 sq[r] = sqrtf(xs[(uint32_t)key[r]]);
+            else if constexpr (IDX7)
+                sq[r] = sqrtf(ord_val((uint32_t)(key[r] >> 32)));
             else
                 sq[r] = sqrtf(ord_val(key[r]));
         }
@@ -1170,9 +1176,9 @@ __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le,
 #pragma unroll
     for (int r = 0; r < K; r++)
         kk = (r == pos % K) ? key[r] : kk;
-    if constexpr (T6)
+    if constexpr (IDX)
         return shfl64(kk, pos / K);
-    else  // value key in the high word (no index: T7 exports none)
+    else  // value key in the high word (no index asked for)
         return (uint64_t)(uint32_t)__shfl((int)kk, pos / K, kWave) << 32;
 }
 
@@ -1186,7 +1192,7 @@ __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le,
 //     smallest register network that holds it (K = 1..KMAX keys per lane).
 // With DEFER, a line whose sorted part needs more than KMAX keys per lane is
 // queued in p.defer for the wide pass instead.
-template <int KMAX, bool DEFER>
+template <int KMAX, bool DEFER, bool IDX7>
 __device__ __forceinline__ void sort_line(const SortTParams &p, int line, int which,
                                           float *xs, int lane)
 {
@@ -1240,8 +1246,9 @@ __device__ __forceinline__ void sort_line(const SortTParams &p, int line, int wh
             return;
         }
         wave_lds_sync();
-#define TT_TAIL(K) (t6 ? sorted_tail_median<(KMAX < K ? KMAX : K), true>(xs, L, Le, lane, &k) \
-                       : sorted_tail_median<(KMAX < K ? KMAX : K), false>(xs, L, Le, lane, &k))
+#define TT_TAIL(K)                                                                           \
+    (t6 ? sorted_tail_median<(KMAX < K ? KMAX : K), true, false>(xs, L, Le, lane, &k)          \
+        : sorted_tail_median<(KMAX < K ? KMAX : K), false, IDX7>(xs, L, Le, lane, &k))
         if (kneed <= 1 || KMAX == 1)
             res = TT_TAIL(1);
         else if (kneed <= 2 || KMAX == 2)
@@ -1263,8 +1270,11 @@ __device__ __forceinline__ void sort_line(const SortTParams &p, int line, int wh
                 p.t6_rank[line] = k;
             if (p.t6_src)
                 p.t6_src[line] = (int)(uint32_t)res;
-        } else if (p.t7_rank) {
-            p.t7_rank[line] = k;
+        } else {
+            if (p.t7_rank)
+                p.t7_rank[line] = k;
+            if (IDX7)
+                p.t7_src[line] = base + (int)(uint32_t)res;
         }
     }
 }
@@ -1274,7 +1284,7 @@ __device__ __forceinline__ void sort_line(const SortTParams &p, int line, int wh
 // entries whose sorted part needs more than KMAX keys per lane are queued in
 // p.defer_out for a wider pass (a wide network's VGPR budget would otherwise
 // set the occupancy of every wave).
-template <int KMAX, bool DEFER>
+template <int KMAX, bool DEFER, bool IDX7>
 __global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
 {
     constexpr int M = kWave * KMAX;
@@ -1288,14 +1298,14 @@ __global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
         const int lines = p.A * p.n;
         for (int line = first; line < lines; line += waves)
             for (int which = 0; which < 2; which++) {
-                sort_line<KMAX, DEFER>(p, line, which, xs, lane);
+                sort_line<KMAX, DEFER, IDX7>(p, line, which, xs, lane);
                 wave_lds_sync();
             }
     } else {
         const int count = (int)*p.defer_in_count;
         for (int j = first; j < count; j += waves) {
             const int e = p.defer_in[j];
-            sort_line<KMAX, DEFER>(p, e >> 1, e & 1, xs, lane);
+            sort_line<KMAX, DEFER, IDX7>(p, e >> 1, e & 1, xs, lane);
             wave_lds_sync();
         }
     }
@@ -1310,12 +1320,24 @@ __global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
 // One LineRecord per (angle, column): folds the sinogram gradient of every
 // requested T0..T5 into the coefficients of G(row).
 __device__ __forceinline__ LineRecord make_record(const float *dsino, const int32_t *med_x,
-                                                  const int32_t *med_s, const float2 *z345,
+                                                  const int32_t *med_s, const int32_t *t6_src,
+                                                  const int32_t *t7_src, const float2 *z345,
                                                   size_t plane, size_t line, Slots slot)
 {
     LineRecord r = {};
     r.m = med_x ? med_x[line] : 0;
     r.sm = med_s ? med_s[line] : 0;
+    r.r6 = -1;
+    r.r7 = -1;
+    if (slot.s[6] >= 0 && t6_src) {
+        const int q = t6_src[line];
+        r.r6 = r.sm + q;
+        r.c6 = dsino[slot.s[6] * plane + line] * (float)q;
+    }
+    if (slot.s[7] >= 0 && t7_src) {
+        r.r7 = t7_src[line];
+        r.c7 = dsino[slot.s[7] * plane + line];
+    }
     if (slot.s[0] >= 0)
         r.g0 = dsino[slot.s[0] * plane + line];
     if (slot.s[1] >= 0)
@@ -1342,7 +1364,8 @@ __global__ __launch_bounds__(kBlock) void records_kernel(RecordParams p)
     const size_t lines = (size_t)p.A * p.n;
     for (size_t line = (size_t)blockIdx.x * kBlock + threadIdx.x; line < lines;
          line += (size_t)gridDim.x * kBlock)
-        p.rec[line] = make_record(p.dsino, p.med_x, p.med_s, p.z345, lines, line, p.slot);
+        p.rec[line] = make_record(p.dsino, p.med_x, p.med_s, p.t6_src, p.t7_src, p.z345,
+                                  lines, line, p.slot);
 }
 
 template <class F>
@@ -1367,8 +1390,8 @@ __global__ __launch_bounds__(kBlock) void records_batched_kernel(BatchAdjointPar
         const BatchImage &im = p.images[k];
         const size_t plane = (size_t)p.A * im.n;
         const size_t line = (size_t)(g - im.line0);
-        im.rec[line] = make_record(im.dsino, im.med_x, im.med_s, im.z345, plane, line,
-                                   p.slot);
+        im.rec[line] = make_record(im.dsino, im.med_x, im.med_s, nullptr, nullptr, im.z345,
+                                   plane, line, p.slot);
     }
 }
 
@@ -1422,6 +1445,8 @@ __device__ __forceinline__ float gather_pixel(const LineRecord *recs, const floa
                     reinterpret_cast<const char *>(rec) +
                     (uint32_t)col * (uint32_t)sizeof(LineRecord));
                 float G = r.g0;
+                G += row == r.r6 ? r.c6 : 0.0f;
+                G += row == r.r7 ? r.c7 : 0.0f;
                 if (row >= r.m) {
                     const float d = (float)(row - r.m);
                     G = fmaf(d, fmaf(r.g2, d, r.g1), G);
@@ -2012,7 +2037,8 @@ hipError_t launch_sino_t05_batched(const BatchSinoParams &p, bool need_x,
     return hipGetLastError();
 }
 
-hipError_t launch_sino_sort(const SortTParams &p_in, int32_t *queue_a,
+template <bool IDX7>
+static hipError_t sino_sort_passes(const SortTParams &p_in, int32_t *queue_a,
                             int32_t *queue_b, unsigned int *counts, hipStream_t s)
 {
     SortTParams p = p_in;
@@ -2029,7 +2055,7 @@ hipError_t launch_sino_sort(const SortTParams &p_in, int32_t *queue_a,
     if (p.n > 2048)
         return hipErrorInvalidValue;
     if (p.n <= 512) {
-#define TT_SORT(K) sino_sort_kernel<K, false><<<blocks, kBlock, 0, s>>>(p)
+#define TT_SORT(K) sino_sort_kernel<K, false, IDX7><<<blocks, kBlock, 0, s>>>(p)
         if (p.n <= 64)
             TT_SORT(1);
         else if (p.n <= 128)
@@ -2050,20 +2076,27 @@ hipError_t launch_sino_sort(const SortTParams &p_in, int32_t *queue_a,
     reset_counter_kernel<<<1, kWave, 0, s>>>(counts + 1);
     p.defer_out = queue_a;
     p.defer_out_count = counts;
-    sino_sort_kernel<8, true><<<blocks, kBlock, 0, s>>>(p);
+    sino_sort_kernel<8, true, IDX7><<<blocks, kBlock, 0, s>>>(p);
     p.defer_in = queue_a;
     p.defer_in_count = counts;
     if (p.n <= 1024) {
-        sino_sort_kernel<16, false><<<queued_blocks, kBlock, 0, s>>>(p);
+        sino_sort_kernel<16, false, IDX7><<<queued_blocks, kBlock, 0, s>>>(p);
         return hipGetLastError();
     }
     p.defer_out = queue_b;
     p.defer_out_count = counts + 1;
-    sino_sort_kernel<16, true><<<queued_blocks, kBlock, 0, s>>>(p);
+    sino_sort_kernel<16, true, IDX7><<<queued_blocks, kBlock, 0, s>>>(p);
     p.defer_in = queue_b;
     p.defer_in_count = counts + 1;
-    sino_sort_kernel<32, false><<<queued_blocks, kBlock, 0, s>>>(p);
+    sino_sort_kernel<32, false, IDX7><<<queued_blocks, kBlock, 0, s>>>(p);
     return hipGetLastError();
+}
+
+hipError_t launch_sino_sort(const SortTParams &p, int32_t *queue_a, int32_t *queue_b,
+                            unsigned int *counts, hipStream_t s)
+{
+    return p.t7_src ? sino_sort_passes<true>(p, queue_a, queue_b, counts, s)
+                    : sino_sort_passes<false>(p, queue_a, queue_b, counts, s);
 }
 
 }  // namespace ttk
