@@ -1474,7 +1474,7 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         TT_HIP(P.meds.ensure(sizeof(int32_t) * ln));
         TT_HIP(P.rec.ensure(sizeof(ttk::LineRecord) * ln));
         std::vector<ttk::BatchImage> d(nimg);
-        size_t po = 0, lo = 0;
+        size_t po = 0, lo = 0, tiles = 0;
         int items = 0, cols = 0;
         for (int k = 0; k < nimg; k++) {
             const tt_image_grad &im = images[order[k]];
@@ -1497,6 +1497,9 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
             b.dimg = im.d_dimage;
             b.line0 = (int)lo;
             b.pix0 = (int)po;
+            b.tile0 = (int)tiles;
+            const size_t tn = (size_t)(im.n + 15) / 16;  // 16x16 gather tiles
+            tiles += tn * tn;
             items += b.nb * n_angles;
             cols += nt * n_angles;
             po += (size_t)im.n * im.n;
@@ -1537,6 +1540,7 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         ap.nmax = nmax;
         ap.lines = (int)lo;
         ap.pixels = (int)po;
+        ap.tiles = (int)tiles;
         ap.cols = cols;
         ap.trig = sp.trig;
         ap.tab345 = sp.tab345;

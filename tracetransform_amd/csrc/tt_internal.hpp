@@ -73,6 +73,7 @@ struct BatchImage {
     float *dimg;          // n x n
     int line0;            // first line (angle*n + col) over the batch
     int pix0;             // first pixel over the batch
+    int tile0;            // first 16x16 gather tile over the batch
 };
 
 struct BatchSinoParams {
@@ -234,6 +235,7 @@ struct BatchAdjointParams {
     int nmax;
     int lines;            // total lines (sum of A * n)
     int pixels;           // total pixels (sum of n * n)
+    int tiles;            // total 16x16 gather tiles
     int cols;             // total sinogram columns (sum of nt * A)
     const float2 *trig;
     const float2 *tab345; // [3][tabn]

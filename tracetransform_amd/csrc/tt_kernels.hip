@@ -1395,25 +1395,67 @@ __global__ __launch_bounds__(kBlock) void records_batched_kernel(BatchAdjointPar
     }
 }
 
-// Transposed bilinear rotation as a gather: thread = one source pixel
-// (row Y, col X).  For every angle it enumerates the rotated samples whose
-// 2x2 footprint can cover the pixel (the rotated pixels within sqrt(2) of its
-// inverse image), recomputes each sample's forward float coordinates with
-// the exact operations of rot_sample, keeps those whose taps include (Y, X),
-// and accumulates G(row) times the tap weight.  Deterministic, no atomics.
-__device__ __forceinline__ float gather_pixel(const LineRecord *recs, const float2 *trig,
-                                              const float2 *s_tab, int tabn, bool do345,
-                                              int n, int A, int X, int Y)
+// G(row) of one line record (the rotated-domain gradient at that row).
+__device__ __forceinline__ float record_g(const LineRecord &r, int row, const float2 *s_tab,
+                                          int tabn, bool do345)
 {
+    float G = r.g0;
+    G += row == r.r6 ? r.c6 : 0.0f;
+    G += row == r.r7 ? r.c7 : 0.0f;
+    if (row >= r.m) {
+        const float d = (float)(row - r.m);
+        G = fmaf(d, fmaf(r.g2, d, r.g1), G);
+    }
+    if (do345 && row > r.sm) {
+        const int q = row - r.sm;
+        const float2 w3 = s_tab[q], w4 = s_tab[tabn + q], w5 = s_tab[2 * tabn + q];
+        G = fmaf(r.ab[0], w3.x, G);
+        G = fmaf(r.ab[1], w3.y, G);
+        G = fmaf(r.ab[2], w4.x, G);
+        G = fmaf(r.ab[3], w4.y, G);
+        G = fmaf(r.ab[4], w5.x, G);
+        G = fmaf(r.ab[5], w5.y, G);
+    }
+    return G;
+}
+
+// Transposed bilinear rotation as a gather over 16x16 pixel tiles.  Per
+// angle the workgroup evaluates every rotated sample that any of its pixels
+// can receive from once: the sample's forward float coordinates (the exact
+// operations of rot_sample), its taps (X, Y) and fractions, and G(row) from
+// the line record, into LDS.  Each pixel then visits the rotated samples
+// within sqrt(2) of its inverse image (those whose 2x2 footprint can cover
+// it) column by column, keeps those whose taps include it and accumulates
+// G * weight in double.  The visiting order is fixed, so the result is
+// deterministic and needs no atomics.
+constexpr int kTile = 16;
+constexpr int kBoxCap = 32;  // box side bound: a tile's candidates span <= 25
+
+struct GatherLds {
+    LineRecord rec[kBoxCap];
+    float4 smp[kBoxCap * kBoxCap];  // (taps X | Y << 16, or -1), G, fx, fy
+    int box[4][kBlock / kWave];
+};
+
+__device__ __forceinline__ float gather_tile(const LineRecord *recs, const float2 *trig,
+                                             const float2 *s_tab, int tabn, bool do345,
+                                             int n, int A, int X0, int Y0, GatherLds &L)
+{
+    const int tid = threadIdx.x;
+    const int X = X0 + tid / kTile, Y = Y0 + tid % kTile;
+    const bool valid = X < n && Y < n;
     const float o = (float)((n - 1) / 2.0);
     const float lim = (float)(n - 1);
     const float u = (float)X - o, v = (float)Y - o;
     const float R = 1.4163f;
+    const int wv = tid / kWave, lane = tid & (kWave - 1);
     double acc = 0.0;
     for (int a = 0; a < A; a++) {
         const float2 cs = trig[a];
         const float c = cs.x, s = cs.y;
-        const float pxc = u * c - v * s;   // inverse rotation of the pixel
+        // this pixel's candidate box (the rotated samples within R of its
+        // inverse image), and the tile's union of them
+        const float pxc = u * c - v * s;
         const float pyc = u * s + v * c;
         int c0 = (int)ceilf(pxc - R + o), c1 = (int)floorf(pxc + R + o);
         int r0 = (int)ceilf(pyc - R + o), r1 = (int)floorf(pyc + R + o);
@@ -1421,93 +1463,120 @@ __device__ __forceinline__ float gather_pixel(const LineRecord *recs, const floa
         r0 = r0 < 0 ? 0 : r0;
         c1 = c1 > n - 1 ? n - 1 : c1;
         r1 = r1 > n - 1 ? n - 1 : r1;
+        int bc0 = wave_min_i(valid ? c0 : n), bc1 = wave_max_i(valid ? c1 : -1);
+        int br0 = wave_min_i(valid ? r0 : n), br1 = wave_max_i(valid ? r1 : -1);
+        if (lane == 0) {
+            L.box[0][wv] = bc0;
+            L.box[1][wv] = bc1;
+            L.box[2][wv] = br0;
+            L.box[3][wv] = br1;
+        }
+        __syncthreads();
+        for (int w = 0; w < kBlock / kWave; w++) {
+            bc0 = L.box[0][w] < bc0 ? L.box[0][w] : bc0;
+            bc1 = L.box[1][w] > bc1 ? L.box[1][w] : bc1;
+            br0 = L.box[2][w] < br0 ? L.box[2][w] : br0;
+            br1 = L.box[3][w] > br1 ? L.box[3][w] : br1;
+        }
+        int bw = bc1 - bc0 + 1, bh = br1 - br0 + 1;
+        bw = bw < 0 ? 0 : (bw > kBoxCap ? kBoxCap : bw);  // bounded by geometry
+        bh = bh < 0 ? 0 : (bh > kBoxCap ? kBoxCap : bh);
+        // the box's line records
         const LineRecord *rec = recs + (size_t)a * n;
-        for (int col = c0; col <= c1; col++) {
-            const float px = (float)col - o;
-            const float pxcc = px * c;
-            const float pxns = px * (-s);
-            for (int row = r0; row <= r1; row++) {
-                const float py = (float)row - o;
-                float x = pxcc + py * s;
-                float y = pxns + py * c;
-                x = x + o;
-                y = y + o;
-                const float xi = __builtin_truncf(x), yi = __builtin_truncf(y);
-                const int Xi = (int)xi, Yi = (int)yi;
-                const bool hit = (x >= 0.0f) & (x < lim) & (y >= 0.0f) & (y < lim) &
-                                 ((Xi == X) | (Xi == X - 1)) & ((Yi == Y) | (Yi == Y - 1));
-                if (!hit)
-                    continue;
-                const float fx = x - xi, fy = y - yi;
-                const float wx = Xi == X ? 1.0f - fx : fx;
-                const float wy = Yi == Y ? 1.0f - fy : fy;
-                const LineRecord r = *reinterpret_cast<const LineRecord *>(
-                    reinterpret_cast<const char *>(rec) +
-                    (uint32_t)col * (uint32_t)sizeof(LineRecord));
-                float G = r.g0;
-                G += row == r.r6 ? r.c6 : 0.0f;
-                G += row == r.r7 ? r.c7 : 0.0f;
-                if (row >= r.m) {
-                    const float d = (float)(row - r.m);
-                    G = fmaf(d, fmaf(r.g2, d, r.g1), G);
+        constexpr int kRecWords = (int)(sizeof(LineRecord) / 4);
+        for (int j = tid; j < bw * kRecWords; j += kBlock)
+            reinterpret_cast<float *>(L.rec)[j] =
+                reinterpret_cast<const float *>(rec + bc0)[j];
+        __syncthreads();
+        // every rotated sample of the box, once
+        for (int j = tid; j < bw * bh; j += kBlock) {
+            const int col = bc0 + j % bw, row = br0 + j / bw;
+            const float px = (float)col - o, py = (float)row - o;
+            float x = px * c + py * s;
+            float y = px * (-s) + py * c;
+            x = x + o;
+            y = y + o;
+            const bool inb = (x >= 0.0f) & (x < lim) & (y >= 0.0f) & (y < lim);
+            const float xi = __builtin_truncf(x), yi = __builtin_truncf(y);
+            float4 e;
+            e.x = __int_as_float(inb ? ((int)xi | ((int)yi << 16)) : -1);
+            e.y = record_g(L.rec[col - bc0], row, s_tab, tabn, do345);
+            e.z = x - xi;
+            e.w = y - yi;
+            L.smp[j] = e;
+        }
+        __syncthreads();
+        if (valid) {
+            // the pixel's candidates, column by column (as the samples were
+            // laid out); those outside [c0, c1] x [r0, r1] are skipped
+#pragma unroll
+            for (int dc = 0; dc < 3; dc++) {
+                const int col = c0 + dc;
+#pragma unroll
+                for (int dr = 0; dr < 3; dr++) {
+                    const int row = r0 + dr;
+                    // inside the box by construction; the test keeps every
+                    // LDS index in range regardless
+                    const bool cand = (col <= c1) & (row <= r1) &
+                                      ((unsigned)(col - bc0) < (unsigned)bw) &
+                                      ((unsigned)(row - br0) < (unsigned)bh);
+                    const int j = (row - br0) * bw + (col - bc0);
+                    const float4 e = L.smp[cand ? j : 0];
+                    const int taps = __float_as_int(e.x);
+                    const int Xi = taps & 0xFFFF, Yi = taps >> 16;
+                    const bool hit = cand & (taps >= 0) & ((Xi == X) | (Xi == X - 1)) &
+                                     ((Yi == Y) | (Yi == Y - 1));
+                    const float wx = Xi == X ? 1.0f - e.z : e.z;
+                    const float wy = Yi == Y ? 1.0f - e.w : e.w;
+                    const double t = fma((double)e.y, (double)wx * (double)wy, acc);
+                    acc = hit ? t : acc;
                 }
-                if (do345 && row > r.sm) {
-                    const int q = row - r.sm;
-                    const float2 w3 = s_tab[q], w4 = s_tab[tabn + q],
-                                 w5 = s_tab[2 * tabn + q];
-                    G = fmaf(r.ab[0], w3.x, G);
-                    G = fmaf(r.ab[1], w3.y, G);
-                    G = fmaf(r.ab[2], w4.x, G);
-                    G = fmaf(r.ab[3], w4.y, G);
-                    G = fmaf(r.ab[4], w5.x, G);
-                    G = fmaf(r.ab[5], w5.y, G);
-                }
-                acc = fma((double)G, (double)wx * (double)wy, acc);
             }
         }
+        __syncthreads();
     }
     return (float)acc;
 }
 
-// Transposed bilinear rotation as a gather: thread = one source pixel
-// (row Y, col X).  For every angle it enumerates the rotated samples whose
-// 2x2 footprint can cover the pixel (the rotated pixels within sqrt(2) of its
-// inverse image), recomputes each sample's forward float coordinates with
-// the exact operations of rot_sample, keeps those whose taps include (Y, X),
-// and accumulates G(row) times the tap weight.  Deterministic, no atomics.
 __global__ __launch_bounds__(kBlock) void gather_kernel(GatherParams p)
 {
     extern __shared__ float2 s_tab[];
+    __shared__ GatherLds L;
     const int n = p.n;
     if (p.do345) {
         for (int i = threadIdx.x; i < 3 * n; i += kBlock)
             s_tab[i] = p.tab345[i];
     }
     __syncthreads();
-    const int pix = blockIdx.x * kBlock + threadIdx.x;
-    if (pix >= n * n)
-        return;
-    const int X = pix / n, Y = pix - (pix / n) * n;
-    p.dimg[pix] = gather_pixel(p.rec, p.trig, s_tab, n, p.do345, n, p.A, X, Y);
+    const int tn = (n + kTile - 1) / kTile;
+    const int X0 = blockIdx.x / tn * kTile, Y0 = blockIdx.x % tn * kTile;
+    const float g = gather_tile(p.rec, p.trig, s_tab, n, p.do345, n, p.A, X0, Y0, L);
+    const int X = X0 + threadIdx.x / kTile, Y = Y0 + threadIdx.x % kTile;
+    if (X < n && Y < n)
+        p.dimg[(size_t)X * n + Y] = g;
 }
 
 __global__ __launch_bounds__(kBlock) void gather_batched_kernel(BatchAdjointParams p)
 {
     extern __shared__ float2 s_tab[];
+    __shared__ GatherLds L;
     if (p.do345) {
         for (int i = threadIdx.x; i < 3 * p.tabn; i += kBlock)
             s_tab[i] = p.tab345[i];
     }
     __syncthreads();
-    for (int g = blockIdx.x * kBlock + threadIdx.x; g < p.pixels; g += gridDim.x * kBlock) {
-        const int k = find_image(p.images, p.nimg, g,
-                                 [](const BatchImage &b) { return b.pix0; });
-        const BatchImage &im = p.images[k];
-        const int pix = g - im.pix0;
-        const int X = pix / im.n, Y = pix - (pix / im.n) * im.n;
-        im.dimg[pix] = gather_pixel(im.rec, p.trig, s_tab, p.tabn, p.do345, im.n, p.A,
-                                    X, Y);
-    }
+    const int t = blockIdx.x;
+    const int k = find_image(p.images, p.nimg, t,
+                             [](const BatchImage &b) { return b.tile0; });
+    const BatchImage &im = p.images[k];
+    const int n = im.n;
+    const int tn = (n + kTile - 1) / kTile;
+    const int local = t - im.tile0;
+    const int X0 = local / tn * kTile, Y0 = local % tn * kTile;
+    const float g = gather_tile(im.rec, p.trig, s_tab, p.tabn, p.do345, n, p.A, X0, Y0, L);
+    const int X = X0 + threadIdx.x / kTile, Y = Y0 + threadIdx.x % kTile;
+    if (X < n && Y < n)
+        im.dimg[(size_t)X * n + Y] = g;
 }
 
 // dS = J_P^T dC for every sinogram column (P1: sign differences; P2: one-hot
@@ -2115,9 +2184,9 @@ hipError_t launch_records(const RecordParams &p, hipStream_t s)
 
 hipError_t launch_gather(const GatherParams &p, hipStream_t s)
 {
-    const int blocks = (p.n * p.n + kBlock - 1) / kBlock;
+    const int tn = (p.n + kTile - 1) / kTile;
     const size_t shm = p.do345 ? sizeof(float2) * 3 * (size_t)p.n : 0;
-    gather_kernel<<<blocks, kBlock, shm, s>>>(p);
+    gather_kernel<<<tn * tn, kBlock, shm, s>>>(p);
     return hipGetLastError();
 }
 
@@ -2155,9 +2224,8 @@ hipError_t launch_records_batched(const BatchAdjointParams &p, hipStream_t s)
 
 hipError_t launch_gather_batched(const BatchAdjointParams &p, hipStream_t s)
 {
-    const int blocks = (p.pixels + kBlock - 1) / kBlock;
     const size_t shm = p.do345 ? sizeof(float2) * 3 * (size_t)p.tabn : 0;
-    gather_batched_kernel<<<blocks, kBlock, shm, s>>>(p);
+    gather_batched_kernel<<<p.tiles, kBlock, shm, s>>>(p);
     return hipGetLastError();
 }
 
