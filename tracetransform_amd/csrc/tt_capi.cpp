@@ -1035,11 +1035,14 @@ tt_status forward_for_adjoint(tt_context *ctx, const float *d_image, int n, int 
     TT_TRY(ctx->ensure_trig(A, step));
     TT_TRY(ctx->ensure_tab345(n));
     TT_HIP(ctx->imgT.ensure(sizeof(float) * (size_t)n * n));
+    TT_HIP(ctx->bbox.ensure(4 * sizeof(int)));
     hipStream_t st = ctx->stream;
     TT_HIP(ctx->timer.begin(TT_KERNEL_AUX, st));
     TT_HIP(ttk::launch_transpose(d_image, ctx->imgT.as<float>(), n, st));
+    TT_HIP(ttk::launch_image_bbox(d_image, n, ctx->bbox.as<int>(), st));
     TT_HIP(ctx->timer.end(st));
     ttk::SinoParams p = {};
+    p.bbox = ctx->bbox.as<int>();
     p.img = d_image;
     p.imgT = ctx->imgT.as<float>();
     p.trig = ctx->trig.as<float2>();
@@ -1071,6 +1074,7 @@ tt_status forward_for_adjoint(tt_context *ctx, const float *d_image, int n, int 
         q.med_s = ctx->med_s.as<int32_t>();
         q.t6_src = slot[6] >= 0 ? ctx->bw_t6src.as<int32_t>() : nullptr;
         q.t7_src = slot[7] >= 0 ? ctx->bw_t7src.as<int32_t>() : nullptr;
+        q.bbox = ctx->bbox.as<int>();
         q.n = n;
         q.A = A;
         q.slot6 = slot[6];
@@ -1473,12 +1477,14 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         TT_HIP(P.medx.ensure(sizeof(int32_t) * ln));
         TT_HIP(P.meds.ensure(sizeof(int32_t) * ln));
         TT_HIP(P.rec.ensure(sizeof(ttk::LineRecord) * ln));
+        TT_HIP(B.bbox.ensure(4 * sizeof(int) * (size_t)nimg));
         std::vector<ttk::BatchImage> d(nimg);
         size_t po = 0, lo = 0, tiles = 0;
         int items = 0, cols = 0;
         for (int k = 0; k < nimg; k++) {
             const tt_image_grad &im = images[order[k]];
             ttk::BatchImage &b = d[k];
+            b.bbox = B.bbox.as<int>() + 4 * k;
             const size_t l = (size_t)im.n * n_angles;
             b.img = im.d_image;
             b.imgT = B.imgT.as<float>() + po;
@@ -1554,6 +1560,8 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         B.blocks = cus * 4;
         TT_HIP(hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal));
         hipError_t ce = ttk::launch_transpose_batched(sp.images, nimg, nmax, own);
+        if (ce == hipSuccess)
+            ce = ttk::launch_image_bbox_batched(sp.images, nimg, nmax, own);
         if (ce == hipSuccess)
             ce = ttk::launch_reset_counter(sp.counter, own);
         if (ce == hipSuccess)
