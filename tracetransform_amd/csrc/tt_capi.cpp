@@ -613,15 +613,14 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
         q.slot6 = slot[6];
         q.slot7 = slot[7];
         int32_t *qa = nullptr, *qb = nullptr;
-        unsigned int *counts = nullptr;
+        TT_HIP(ctx->sort_count.ensure(ttk::kSortCounters * sizeof(unsigned int)));
+        unsigned int *counts = ctx->sort_count.as<unsigned int>();
         if (n > 512) {
             const size_t qbytes = sizeof(int32_t) * 2 * (size_t)n * n_angles;
             TT_HIP(ctx->sort_qa.ensure(qbytes));
             TT_HIP(ctx->sort_qb.ensure(qbytes));
-            TT_HIP(ctx->sort_count.ensure(2 * sizeof(unsigned int)));
             qa = ctx->sort_qa.as<int32_t>();
             qb = ctx->sort_qb.as<int32_t>();
-            counts = ctx->sort_count.as<unsigned int>();
         }
         TT_HIP(ctx->timer.begin(TT_KERNEL_SINO_SORT, st));
         TT_HIP(ttk::launch_sino_sort(q, qa, qb, counts, st));
@@ -1080,14 +1079,13 @@ tt_status forward_for_adjoint(tt_context *ctx, const float *d_image, int n, int 
         q.slot6 = slot[6];
         q.slot7 = slot[7];
         int32_t *qa = nullptr, *qb = nullptr;
-        unsigned int *counts = nullptr;
+        TT_HIP(ctx->sort_count.ensure(ttk::kSortCounters * sizeof(unsigned int)));
+        unsigned int *counts = ctx->sort_count.as<unsigned int>();
         if (n > 512) {
             TT_HIP(ctx->sort_qa.ensure(sizeof(int32_t) * 2 * lines));
             TT_HIP(ctx->sort_qb.ensure(sizeof(int32_t) * 2 * lines));
-            TT_HIP(ctx->sort_count.ensure(2 * sizeof(unsigned int)));
             qa = ctx->sort_qa.as<int32_t>();
             qb = ctx->sort_qb.as<int32_t>();
-            counts = ctx->sort_count.as<unsigned int>();
         }
         TT_HIP(ctx->timer.begin(TT_KERNEL_SINO_SORT, st));
         TT_HIP(ttk::launch_sino_sort(q, qa, qb, counts, st));

@@ -121,6 +121,8 @@ struct SortTParams {
     const unsigned int *defer_in_count;
     int32_t *defer_out;
     unsigned int *defer_out_count;
+    unsigned int *work;   // work-item counter of the pass (set by launch_sino_sort)
+    int chunk;            // items taken per atomic on work
     int n;
     int A;
     int slot6;
@@ -210,7 +212,9 @@ hipError_t launch_image_bbox_batched(const BatchImage *images, int nimg, int nma
                                      hipStream_t s);
 hipError_t launch_sino_t05(const SinoParams &p, bool need_x, bool need_s,
                            bool do345, hipStream_t s);
-// n > 512 needs two queues of 2 * A * n entries and two counters.
+// counts: kSortCounters counters (always); n > 512 also needs two queues of
+// 2 * A * n entries.
+constexpr int kSortCounters = 5;
 hipError_t launch_sino_sort(const SortTParams &p, int32_t *queue_a, int32_t *queue_b,
                             unsigned int *counts, hipStream_t s);
 hipError_t launch_circus(const CircusParams &p, hipStream_t s);

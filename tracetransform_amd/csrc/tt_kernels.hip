@@ -472,6 +472,12 @@ __global__ void reset_counter_kernel(unsigned int *c)
         c[0] = 0u;
 }
 
+__global__ void reset_counters_kernel(unsigned int *c, int count)
+{
+    if ((int)threadIdx.x < count)
+        c[threadIdx.x] = 0u;
+}
+
 // Row-major copies of every image of a batch: grid.y walks the images.
 __global__ __launch_bounds__(kBlock) void transpose_batched_kernel(
     const BatchImage *images)
@@ -1008,25 +1014,40 @@ struct TailView {
     }
 };
 
+// Sequential float sum of c[i0], c[i0 + 8], ... (i < i1), added to acc.
+__device__ __forceinline__ float chain_sum(const float *c, int i0, int i1, float acc)
+{
+    int i = i0;
+    for (; i + 24 < i1; i += 32) {
+        const float v0 = c[i], v1 = c[i + 8], v2 = c[i + 16], v3 = c[i + 24];
+        acc += v0;
+        acc += v1;
+        acc += v2;
+        acc += v3;
+    }
+    for (; i < i1; i += 8)
+        acc += c[i];
+    return acc;
+}
+
 // Eigen packet-order sum of a TailView's full sequence (as eigen_sum_block):
 // lanes 0..7 run the stride-8 chains, lane 0 combines; the result is
-// returned to every lane.
+// returned to every lane.  Adding the zero run's +0 leaves a chain of
+// non-negative values unchanged, so chain r sums the compact entries before
+// the run (positions r, r + 8, ... below ins) and after it (positions
+// congruent to r from ins + zt on, entry = position - zt) and skips the run.
 __device__ __forceinline__ float eigen_sum_view(const TailView &v, int lane)
 {
     const int n = v.L;
     const int a4 = n & ~3, a8 = n & ~7;
     float r = 0.0f;
     if (lane < 8 && n >= 8) {
-        int i = lane;
-        for (; i + 24 < a8; i += 32) {
-            const float v0 = v.at(i), v1 = v.at(i + 8), v2 = v.at(i + 16), v3 = v.at(i + 24);
-            r += v0;
-            r += v1;
-            r += v2;
-            r += v3;
-        }
-        for (; i < a8; i += 8)
-            r += v.at(i);
+        const int e1 = v.ins < a8 ? v.ins : a8;
+        r = chain_sum(v.c, lane, e1, r);
+        const int q0 = v.ins + v.zt;
+        const int p0 = q0 + ((lane - q0) & 7);  // first position >= q0 in chain lane
+        if (p0 < a8)
+            r = chain_sum(v.c, p0 - v.zt, a8 - v.zt, r);
     }
     float ch[8];
 #pragma unroll
@@ -1057,8 +1078,11 @@ __device__ __forceinline__ float eigen_sum_view(const TailView &v, int lane)
 // the zero run leaves the prefix unchanged, so the test can first pass there
 // only at its first position (when nothing before it passed, that is only
 // possible with ins == 0 and S <= 0).
-// Walks c[j0..j1) (8 loads issued ahead of the dependent adds); returns the
-// first passing j + off, or -1 with the prefix carried in *integral.
+// Walks c[j0..j1); returns the first passing j + off, or -1 with the prefix
+// carried in *integral.  Every c is >= 0, so the prefix never decreases and
+// the test, once passed, stays passed: a block of 8 is summed first and
+// walked element by element only when its last prefix passes (the same
+// additions in the same order, so the same prefixes).
 __device__ __forceinline__ int walk_segment(const float *c, int j0, int j1, int off, float S,
                                             float *integral)
 {
@@ -1069,12 +1093,20 @@ __device__ __forceinline__ int walk_segment(const float *c, int j0, int j1, int 
 #pragma unroll
         for (int u = 0; u < 8; u++)
             v[u] = c[j + u];
+        float t = acc;
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            acc += v[u];
-            if (2.0f * acc >= S)
-                return j + u + off;
+        for (int u = 0; u < 8; u++)
+            t += v[u];
+        if (2.0f * t >= S) {
+#pragma unroll
+            for (int u = 0; u < 7; u++) {
+                acc += v[u];
+                if (2.0f * acc >= S)
+                    return j + u + off;
+            }
+            return j + 7 + off;
         }
+        acc = t;
     }
     for (; j < j1; j++) {
         acc += c[j];
@@ -1216,6 +1248,14 @@ __device__ __forceinline__ void sort_line(const SortTParams &p, int line, int wh
     // rows past the line's last possibly non-zero row sample exactly 0
     const int2 rows = line_rows(p.bbox, n, col, cs.x, cs.y);
     const int Ls = rows.y + 1 - base < L ? rows.y + 1 - base : L;
+    // a sampled part longer than the slice cannot sort here: queue it for a
+    // wider pass before sampling (its sorted part may turn out shorter;
+    // the wider pass then sorts it with the smaller network)
+    if (DEFER && Ls > M) {
+        if (lane == 0)
+            p.defer_out[atomicAdd(p.defer_out_count, 1u)] = line * 2 + which;
+        return;
+    }
     // consecutive lanes take consecutive rows (neighbouring taps)
     int last_nz = -1;
     bool bad = false;
@@ -1279,34 +1319,42 @@ __device__ __forceinline__ void sort_line(const SortTParams &p, int line, int wh
     }
 }
 
-// One wave per line, persistent over lines, T6 then T7 of each line; or, with
-// p.defer_in set, one wave per queued (line, which) entry.  With DEFER,
-// entries whose sorted part needs more than KMAX keys per lane are queued in
-// p.defer_out for a wider pass (a wide network's VGPR budget would otherwise
-// set the occupancy of every wave).
+// Persistent waves take work items from the counter p.work (zeroed ahead of
+// the launch), CHUNK at a time: a line (T6 then T7) or, with p.defer_in set,
+// a queued (line, which) entry.  One counter serialises its atomics across
+// the device (tens of nanoseconds each), so the cheap items of the first pass
+// go in chunks of several lines.  With DEFER, entries whose sorted part needs
+// more than KMAX keys per lane are queued in p.defer_out for a wider pass (a
+// wide network's VGPR budget would otherwise set the occupancy of every
+// wave).
 template <int KMAX, bool DEFER, bool IDX7>
 __global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
 {
     constexpr int M = kWave * KMAX;
     __shared__ __align__(16) float s_all[(kBlock / kWave) * M];
     const int lane = threadIdx.x & (kWave - 1);
-    const int wv = wave_id();
-    float *xs = s_all + wv * M;
-    const int waves = gridDim.x * (kBlock / kWave);
-    const int first = blockIdx.x * (kBlock / kWave) + wv;
-    if (!p.defer_in) {
-        const int lines = p.A * p.n;
-        for (int line = first; line < lines; line += waves)
-            for (int which = 0; which < 2; which++) {
-                sort_line<KMAX, DEFER, IDX7>(p, line, which, xs, lane);
+    float *xs = s_all + wave_id() * M;
+    const int total = p.defer_in ? (int)*p.defer_in_count : p.A * p.n;
+    const int chunk = p.chunk;
+    for (;;) {
+        int j0 = 0;
+        if (lane == 0)
+            j0 = (int)atomicAdd(p.work, (unsigned int)chunk);
+        j0 = __builtin_amdgcn_readfirstlane(j0);
+        if (j0 >= total)
+            break;
+        const int j1 = j0 + chunk < total ? j0 + chunk : total;
+        for (int j = j0; j < j1; j++) {
+            if (!p.defer_in) {
+                for (int which = 0; which < 2; which++) {
+                    sort_line<KMAX, DEFER, IDX7>(p, j, which, xs, lane);
+                    wave_lds_sync();
+                }
+            } else {
+                const int e = p.defer_in[j];
+                sort_line<KMAX, DEFER, IDX7>(p, e >> 1, e & 1, xs, lane);
                 wave_lds_sync();
             }
-    } else {
-        const int count = (int)*p.defer_in_count;
-        for (int j = first; j < count; j += waves) {
-            const int e = p.defer_in[j];
-            sort_line<KMAX, DEFER, IDX7>(p, e >> 1, e & 1, xs, lane);
-            wave_lds_sync();
         }
     }
 }
@@ -2106,6 +2154,23 @@ hipError_t launch_sino_t05_batched(const BatchSinoParams &p, bool need_x,
     return hipGetLastError();
 }
 
+// Workgroups of one sort pass that fit on the device at once (the waves are
+// persistent, so more would only wait).
+template <int KMAX, bool DEFER, bool IDX7>
+static int sort_blocks(int cus, int items = 0x7fffffff)
+{
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &b, sino_sort_kernel<KMAX, DEFER, IDX7>, kBlock, 0) != hipSuccess || b < 1)
+            b = 2;
+        per_cu = b;
+    }
+    const int need = (int)(((long long)items + kBlock / kWave - 1) / (kBlock / kWave));
+    return cus * per_cu < need ? cus * per_cu : need;
+}
+
 template <bool IDX7>
 static hipError_t sino_sort_passes(const SortTParams &p_in, int32_t *queue_a,
                             int32_t *queue_b, unsigned int *counts, hipStream_t s)
@@ -2113,18 +2178,18 @@ static hipError_t sino_sort_passes(const SortTParams &p_in, int32_t *queue_a,
     SortTParams p = p_in;
     p.defer_in = nullptr;
     p.defer_in_count = nullptr;
+    if (p.n > 2048 || !counts)
+        return hipErrorInvalidValue;
     const int lines = p.A * p.n;
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    int blocks = cus * 8;
-    const int need = (lines + kBlock / kWave - 1) / (kBlock / kWave);
-    if (blocks > need)
-        blocks = need;
-    if (p.n > 2048)
-        return hipErrorInvalidValue;
+    // counts: [0], [1] queue lengths, [2..4] the passes' work counters
+    reset_counters_kernel<<<1, kWave, 0, s>>>(counts, kSortCounters);
+    p.work = counts + 2;
+    p.chunk = 8;
     if (p.n <= 512) {
-#define TT_SORT(K) sino_sort_kernel<K, false, IDX7><<<blocks, kBlock, 0, s>>>(p)
+#define TT_SORT(K) sino_sort_kernel<K, false, IDX7><<<sort_blocks<K, false, IDX7>(cus, lines), kBlock, 0, s>>>(p)
         if (p.n <= 64)
             TT_SORT(1);
         else if (p.n <= 128)
@@ -2138,26 +2203,27 @@ static hipError_t sino_sort_passes(const SortTParams &p_in, int32_t *queue_a,
     }
     // sorted parts up to 512 keys in the first pass, up to 1024 in the
     // second, the rest in the third; each pass queues what it cannot hold
-    if (!queue_a || !queue_b || !counts)
+    if (!queue_a || !queue_b)
         return hipErrorInvalidValue;
-    const int queued_blocks = cus * 4;
-    reset_counter_kernel<<<1, kWave, 0, s>>>(counts);
-    reset_counter_kernel<<<1, kWave, 0, s>>>(counts + 1);
     p.defer_out = queue_a;
     p.defer_out_count = counts;
-    sino_sort_kernel<8, true, IDX7><<<blocks, kBlock, 0, s>>>(p);
+    sino_sort_kernel<8, true, IDX7><<<sort_blocks<8, true, IDX7>(cus, lines), kBlock, 0, s>>>(p);
     p.defer_in = queue_a;
     p.defer_in_count = counts;
+    p.work = counts + 3;
+    p.chunk = 2;
     if (p.n <= 1024) {
-        sino_sort_kernel<16, false, IDX7><<<queued_blocks, kBlock, 0, s>>>(p);
+        sino_sort_kernel<16, false, IDX7><<<sort_blocks<16, false, IDX7>(cus), kBlock, 0, s>>>(p);
         return hipGetLastError();
     }
     p.defer_out = queue_b;
     p.defer_out_count = counts + 1;
-    sino_sort_kernel<16, true, IDX7><<<queued_blocks, kBlock, 0, s>>>(p);
+    sino_sort_kernel<16, true, IDX7><<<sort_blocks<16, true, IDX7>(cus), kBlock, 0, s>>>(p);
     p.defer_in = queue_b;
     p.defer_in_count = counts + 1;
-    sino_sort_kernel<32, false, IDX7><<<queued_blocks, kBlock, 0, s>>>(p);
+    p.work = counts + 4;
+    p.chunk = 1;
+    sino_sort_kernel<32, false, IDX7><<<sort_blocks<32, false, IDX7>(cus), kBlock, 0, s>>>(p);
     return hipGetLastError();
 }
 
