@@ -273,7 +273,7 @@ struct tt_context {
     DevBuf imgT, trig, tab345, dft_tw, herm_w, io_a, io_b, med_x, med_s;
     DevBuf bw_sino, bw_dsino, bw_z, bw_rec, bw_t6src, bw_t7src;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
-    DevBuf nos_w, nos_m;
+    DevBuf nos_w, nos_m, nos_q, nos_r, nos_dev, nos_sp, nos_ptrs;
     DevBuf sort_qa, sort_qb, sort_count, bbox;
     rocblas_handle blas = nullptr;
     int trig_A = -1;
@@ -431,7 +431,8 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
                       &ctx->bw_rec, &ctx->bw_t6src, &ctx->bw_t7src, &ctx->nos_med, &ctx->nos_a, &ctx->nos_u,
                       &ctx->nos_v, &ctx->nos_s, &ctx->nos_e, &ctx->nos_c,
                       &ctx->nos_info, &ctx->nos_sweeps, &ctx->nos_w, &ctx->nos_m,
-                      &ctx->sort_qa,
+                      &ctx->nos_q, &ctx->nos_r, &ctx->nos_dev, &ctx->nos_sp,
+                      &ctx->nos_ptrs, &ctx->sort_qa,
                       &ctx->sort_qb, &ctx->sort_count, &ctx->bbox};
     for (DevBuf *b : bufs)
         b->release();
@@ -1240,6 +1241,31 @@ TT_API tt_status tt_zscore(tt_context *ctx, const float *d_in, int n, int count,
     return TT_OK;
 }
 
+// Polynomial schedule of the polar iteration X <- X q(X^T X), q(y) = c0 +
+// c1 y + c2 y^2 + c3 y^3 + c4 y^4, so that each step maps every singular
+// value x of X to x q(x^2) and leaves the singular vectors alone (generated
+// by csrc/polar_schedule.py): eight growth steps that multiply singular
+// values below ~0.01 by 7.28 without letting any fall below 0.1 of the
+// upper end, two minimax steps and two steps of fifth-order convergence.
+// From X_0 = A / ||A^T A||_F^(1/2) every singular value x in [1e-8, 1.02]
+// ends within 2e-15 of 1; smaller ones (condition numbers beyond ~1e8) do
+// not converge and the caller falls back to the eigendecomposition route.
+static const ttk::PolarStep kPolarSteps[] = {
+    {{7.2816756381049732, -66.063394187123038, 209.00402730908877, -256.85470452555404, 107.07762732431864}},
+    {{7.2816756381049732, -66.063394187123038, 209.00402730908877, -256.85470452555404, 107.07762732431864}},
+    {{7.2816756381049732, -66.063394187123038, 209.00402730908877, -256.85470452555404, 107.07762732431864}},
+    {{7.2816756381049732, -66.063394187123038, 209.00402730908877, -256.85470452555404, 107.07762732431864}},
+    {{7.2816756381049732, -66.063394187123038, 209.00402730908877, -256.85470452555404, 107.07762732431864}},
+    {{7.2816756381049732, -66.063394187123038, 209.00402730908877, -256.85470452555404, 107.07762732431864}},
+    {{7.2816756381049732, -66.063394187123038, 209.00402730908877, -256.85470452555404, 107.07762732431864}},
+    {{7.2816756381049732, -66.063394187123038, 209.00402730908877, -256.85470452555404, 107.07762732431864}},
+    {{8.6756202485986371, -63.534465944226177, 187.09388164456479, -221.83925019608748, 90.517402439679913}},
+    {{2.6119822462388274, -3.7135489280995135, 3.3244544471543302, -1.4797355908332768, 0.25480859276722512}},
+    {{1.9493886477464955, -1.2740525232146713, 0, 0.52460041621275599, -0.19993651672502183}},
+    {{2.4609375, -3.28125, 2.953125, -1.40625, 0.2734375}},
+};
+constexpr int kPolarStepCount = (int)(sizeof(kPolarSteps) / sizeof(kPolarSteps[0]));
+
 TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
                                                   const float *d_sino, int n,
                                                   int n_cols, int n_sino, float *d_out,
@@ -1289,7 +1315,12 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
                           hipMemcpyHostToDevice, st));
     // Every aligned matrix is zero-padded to rows_max rows so one batched
     // solve covers them all; zero rows leave the polar factor's other rows
-    // unchanged and stay exactly zero in it.
+    // unchanged and stay exactly zero in it.  rows_max is rounded up to
+    // kGramChunks row chunks of a multiple of 8 rows, over which the Gram
+    // GEMMs are split (more workgroups than the 360 x 360 outputs give).
+    constexpr int kGramChunks = 4;
+    const int chunk_rows = ((rows_max + kGramChunks - 1) / kGramChunks + 7) / 8 * 8;
+    rows_max = chunk_rows * kGramChunks;
     const size_t mA = (size_t)rows_max * A, AA = (size_t)A * A;
     TT_HIP(ctx->nos_a.ensure(sizeof(double) * mA * n_sino));
     TT_HIP(ctx->nos_u.ensure(sizeof(double) * mA * n_sino));
@@ -1301,6 +1332,10 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
     TT_HIP(ctx->nos_e.ensure(sizeof(double) * (size_t)A * n_sino));
     TT_HIP(ctx->nos_info.ensure(sizeof(rocblas_int) * n_sino));
     TT_HIP(ctx->nos_sweeps.ensure(sizeof(rocblas_int) * n_sino));
+    TT_HIP(ctx->nos_q.ensure(sizeof(double) * AA * n_sino));
+    TT_HIP(ctx->nos_sp.ensure(sizeof(double) * AA * n_sino * kGramChunks));
+    TT_HIP(ctx->nos_r.ensure(sizeof(double) * 2 * n_sino));
+    TT_HIP(ctx->nos_dev.ensure(sizeof(double) * ttk::kNosSlices * n_sino));
     TT_HIP(ctx->timer.begin(TT_KERNEL_NOS, st));
     for (int t = 0; t < n_sino; t++)
         TT_HIP(ttk::launch_align(d_sino + (size_t)t * A * n, n, A,
@@ -1312,79 +1347,150 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
     TT_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
     const double one = 1.0, zero = 0.0;
     double *a_d = ctx->nos_a.as<double>(), *g = ctx->nos_v.as<double>();
-    double *lam = ctx->nos_s.as<double>();
-    rocblas_int *info_d = ctx->nos_info.as<rocblas_int>();
-    // Polar factor P = A (A^T A)^(-1/2): Gram matrix, its eigenvectors by
-    // divide and conquer, M = V diag(lambda^-1/2) V^T and P = A M, then two
-    // Newton-Schulz steps P <- P (1.5 I - 0.5 P^T P) that restore
-    // orthonormality to rounding.  Forming A^T A squares the condition
-    // number: the factor is accurate to about eps * kappa(A)^2 relative
-    // (the reference's FP32 Jacobi SVD carries about 6e-8 * kappa(A)).
-    TT_BLAS(rocblas_dgemm_strided_batched(
-        ctx->blas, rocblas_operation_transpose, rocblas_operation_none, A, A, rows_max, &one,
-        a_d, rows_max, (rocblas_stride)mA, a_d, rows_max, (rocblas_stride)mA, &zero, g, A,
-        (rocblas_stride)AA, n_sino));
-    TT_BLAS(rocsolver_dsyevd_strided_batched(ctx->blas, rocblas_evect_original,
-                                             rocblas_fill_upper, A, g, A, (rocblas_stride)AA,
-                                             lam, (rocblas_stride)A, ctx->nos_e.as<double>(),
-                                             (rocblas_stride)A, info_d, n_sino));
-    std::vector<double> lam_h((size_t)A * n_sino);
-    std::vector<rocblas_int> info((size_t)n_sino);
-    TT_HIP(hipMemcpyAsync(lam_h.data(), lam, sizeof(double) * lam_h.size(),
-                          hipMemcpyDeviceToHost, st));
-    TT_HIP(hipMemcpyAsync(info.data(), info_d, sizeof(rocblas_int) * n_sino,
-                          hipMemcpyDeviceToHost, st));
-    TT_HIP(hipStreamSynchronize(st));
-    bool gram_ok = true;
-    for (int t = 0; t < n_sino; t++) {
-        // ascending eigenvalues; a (numerically) singular Gram matrix has no
-        // inverse square root
-        const double lo = lam_h[(size_t)t * A], hi = lam_h[(size_t)t * A + A - 1];
-        if (info[t] != 0 || !(hi > 0.0) || !(lo > hi * 1e-15))
-            gram_ok = false;
+    const rocblas_stride sA = (rocblas_stride)mA, sG = (rocblas_stride)AA;
+    // S = X^T X for the batch: one batched GEMM over the n_sino * kGramChunks
+    // row chunks (pointer tables: chunk c of matrix m of each X buffer, and
+    // the partial products), then the chunk sums in order
+    double *sp = ctx->nos_sp.as<double>();
+    double *xbuf[3] = {a_d, ctx->nos_c.as<double>(), ctx->nos_u.as<double>()};
+    const int nb = n_sino * kGramChunks;
+    std::vector<double *> tab((size_t)4 * nb);
+    for (int b = 0; b < nb; b++) {
+        const size_t off = mA * (size_t)(b / kGramChunks) +
+                           (size_t)chunk_rows * (b % kGramChunks);
+        for (int w = 0; w < 3; w++)
+            tab[(size_t)w * nb + b] = xbuf[w] + off;
+        tab[(size_t)3 * nb + b] = sp + AA * b;
     }
+    TT_HIP(ctx->nos_ptrs.ensure(sizeof(double *) * tab.size()));
+    TT_HIP(hipMemcpyAsync(ctx->nos_ptrs.ptr, tab.data(), sizeof(double *) * tab.size(),
+                          hipMemcpyHostToDevice, st));
+    double **tab_d = ctx->nos_ptrs.as<double *>();
+    auto gram = [&](const double *x, double *out) -> tt_status {
+        int w = 0;
+        while (w < 3 && xbuf[w] != x)
+            w++;
+        if (w == 3)
+            return fail(TT_ERR_INVALID, "internal: unknown NOS buffer");
+        const double *const *xs = tab_d + (size_t)w * nb;
+        TT_BLAS(rocblas_dgemm_batched(ctx->blas, rocblas_operation_transpose,
+                                      rocblas_operation_none, A, A, chunk_rows, &one, xs,
+                                      rows_max, xs, rows_max, &zero, tab_d + (size_t)3 * nb,
+                                      A, nb));
+        TT_HIP(ttk::launch_nos_reduce(sp, kGramChunks, A, n_sino, out, st));
+        return TT_OK;
+    };
+    // Polar factor by the polynomial iteration X <- X q(X^T X) (every step
+    // batched GEMMs in FP64 on the matrix cores): from X_0 = A r,
+    // r = ||A^T A||_F^(-1/2), the schedule drives every singular value to
+    // 1 while the singular vectors stay those of A, so X converges to
+    // U V^T.  Rounding in each step perturbs the factor by about
+    // eps * kappa(A) relative (the reference's FP32 Jacobi SVD carries
+    // about 6e-8 * kappa(A)).  Convergence is checked on X^T X; a matrix
+    // beyond the schedule's reach (kappa above ~1e8, rank deficiency) falls
+    // back to the eigendecomposition route below.
     double *result = nullptr;
-    if (gram_ok) {
-        double *vs = ctx->nos_w.as<double>(), *m = ctx->nos_m.as<double>();
-        double *p0 = ctx->nos_c.as<double>(), *p1 = ctx->nos_u.as<double>();
-        TT_HIP(ttk::launch_scale_eigvec(g, lam, A, n_sino, vs, st));
-        TT_BLAS(rocblas_dgemm_strided_batched(
-            ctx->blas, rocblas_operation_none, rocblas_operation_transpose, A, A, A, &one, vs,
-            A, (rocblas_stride)AA, g, A, (rocblas_stride)AA, &zero, m, A, (rocblas_stride)AA,
-            n_sino));
-        TT_BLAS(rocblas_dgemm_strided_batched(
-            ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A, &one,
-            a_d, rows_max, (rocblas_stride)mA, m, A, (rocblas_stride)AA, &zero, p0, rows_max,
-            (rocblas_stride)mA, n_sino));
-        for (int it = 0; it < 2; it++) {
+    {
+        double *S = g, *S2 = ctx->nos_w.as<double>(), *T = ctx->nos_m.as<double>();
+        double *Q = ctx->nos_q.as<double>(), *scale = ctx->nos_r.as<double>();
+        double *r = scale + n_sino, *part = ctx->nos_dev.as<double>();
+        double *xa = ctx->nos_c.as<double>(), *xb = ctx->nos_u.as<double>();
+        const double *x = a_d;
+        TT_TRY(gram(a_d, S));
+        TT_HIP(ttk::launch_nos_scales(S, A, n_sino, part, scale, r, st));
+        for (int step = 0; step < kPolarStepCount; step++) {
+            if (step > 0)
+                TT_TRY(gram(x, S));
             TT_BLAS(rocblas_dgemm_strided_batched(
-                ctx->blas, rocblas_operation_transpose, rocblas_operation_none, A, A, rows_max,
-                &one, p0, rows_max, (rocblas_stride)mA, p0, rows_max, (rocblas_stride)mA,
-                &zero, m, A, (rocblas_stride)AA, n_sino));
-            TT_HIP(ttk::launch_newton_schulz_factor(m, A, n_sino, st));
+                ctx->blas, rocblas_operation_none, rocblas_operation_none, A, A, A, &one, S, A,
+                sG, S, A, sG, &zero, S2, A, sG, n_sino));
+            TT_HIP(ttk::launch_nos_poly(S, S2, A, n_sino, kPolarSteps[step],
+                                        step == 0 ? scale : nullptr, step == 0 ? r : nullptr,
+                                        T, Q, st));
             TT_BLAS(rocblas_dgemm_strided_batched(
-                ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A,
-                &one, p0, rows_max, (rocblas_stride)mA, m, A, (rocblas_stride)AA, &zero, p1,
-                rows_max, (rocblas_stride)mA, n_sino));
-            std::swap(p0, p1);
+                ctx->blas, rocblas_operation_none, rocblas_operation_none, A, A, A, &one, S2, A,
+                sG, T, A, sG, &one, Q, A, sG, n_sino));
+            TT_BLAS(rocblas_dgemm_strided_batched(
+                ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A, &one,
+                x, rows_max, sA, Q, A, sG, &zero, xa, rows_max, sA, n_sino));
+            x = xa;
+            std::swap(xa, xb);
         }
-        result = p0;
-    } else {
-        // Rank-deficient input: thin U from a Jacobi SVD (V from A^T A, U
-        // from a QR of A V, orthonormal even where sigma = 0); NOS = U V^T
-        TT_BLAS(rocsolver_dgesvdj_strided_batched(
-            ctx->blas, rocblas_svect_singular, rocblas_svect_all, rows_max, A, a_d, rows_max,
-            (rocblas_stride)mA, 0.0, ctx->nos_e.as<double>(), 100,
-            ctx->nos_sweeps.as<rocblas_int>(), lam, (rocblas_stride)A,
-            ctx->nos_u.as<double>(), rows_max, (rocblas_stride)mA, g, A, (rocblas_stride)AA,
-            info_d, n_sino));
-        TT_BLAS(rocblas_dgemm_strided_batched(
-            ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A, &one,
-            ctx->nos_u.as<double>(), rows_max, (rocblas_stride)mA, g, A, (rocblas_stride)AA,
-            &zero, ctx->nos_c.as<double>(), rows_max, (rocblas_stride)mA, n_sino));
-        result = ctx->nos_c.as<double>();
+        TT_TRY(gram(x, S));
+        TT_HIP(ttk::launch_nos_deviation(S, A, n_sino, part, st));
+        std::vector<double> dev((size_t)ttk::kNosSlices * n_sino);
+        TT_HIP(hipMemcpyAsync(dev.data(), part, sizeof(double) * dev.size(),
+                              hipMemcpyDeviceToHost, st));
+        TT_HIP(hipStreamSynchronize(st));
+        bool converged = true;
+        for (double d : dev)
+            converged = converged && d <= 1e-10;  // false for NaN
+        if (converged)
+            result = const_cast<double *>(x);
+    }
+    std::vector<rocblas_int> info((size_t)n_sino);
+    bool gram_ok = true;
+    if (!result) {
+        // Eigendecomposition route: P = A (A^T A)^(-1/2) with the Gram
+        // matrix's eigenvectors by divide and conquer, M = V diag(lambda^-1/2)
+        // V^T and P = A M, then two Newton-Schulz steps
+        // P <- P (1.5 I - 0.5 P^T P) that restore orthonormality to rounding
+        // (accurate to about eps * kappa(A)^2 relative).
+        double *lam = ctx->nos_s.as<double>();
+        rocblas_int *info_d = ctx->nos_info.as<rocblas_int>();
+        TT_TRY(gram(a_d, g));
+        TT_BLAS(rocsolver_dsyevd_strided_batched(ctx->blas, rocblas_evect_original,
+                                                 rocblas_fill_upper, A, g, A, sG, lam,
+                                                 (rocblas_stride)A, ctx->nos_e.as<double>(),
+                                                 (rocblas_stride)A, info_d, n_sino));
+        std::vector<double> lam_h((size_t)A * n_sino);
+        TT_HIP(hipMemcpyAsync(lam_h.data(), lam, sizeof(double) * lam_h.size(),
+                              hipMemcpyDeviceToHost, st));
         TT_HIP(hipMemcpyAsync(info.data(), info_d, sizeof(rocblas_int) * n_sino,
                               hipMemcpyDeviceToHost, st));
+        TT_HIP(hipStreamSynchronize(st));
+        for (int t = 0; t < n_sino; t++) {
+            // ascending eigenvalues; a (numerically) singular Gram matrix has
+            // no inverse square root
+            const double lo = lam_h[(size_t)t * A], hi = lam_h[(size_t)t * A + A - 1];
+            if (info[t] != 0 || !(hi > 0.0) || !(lo > hi * 1e-15))
+                gram_ok = false;
+        }
+        if (gram_ok) {
+            double *vs = ctx->nos_w.as<double>(), *m = ctx->nos_m.as<double>();
+            double *p0 = ctx->nos_c.as<double>(), *p1 = ctx->nos_u.as<double>();
+            TT_HIP(ttk::launch_scale_eigvec(g, lam, A, n_sino, vs, st));
+            TT_BLAS(rocblas_dgemm_strided_batched(
+                ctx->blas, rocblas_operation_none, rocblas_operation_transpose, A, A, A, &one,
+                vs, A, sG, g, A, sG, &zero, m, A, sG, n_sino));
+            TT_BLAS(rocblas_dgemm_strided_batched(
+                ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A, &one,
+                a_d, rows_max, sA, m, A, sG, &zero, p0, rows_max, sA, n_sino));
+            for (int it = 0; it < 2; it++) {
+                TT_TRY(gram(p0, m));
+                TT_HIP(ttk::launch_newton_schulz_factor(m, A, n_sino, st));
+                TT_BLAS(rocblas_dgemm_strided_batched(
+                    ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A,
+                    &one, p0, rows_max, sA, m, A, sG, &zero, p1, rows_max, sA, n_sino));
+                std::swap(p0, p1);
+            }
+            result = p0;
+        } else {
+            // Rank-deficient input: thin U from a Jacobi SVD (V from A^T A, U
+            // from a QR of A V, orthonormal even where sigma = 0); NOS = U V^T
+            TT_BLAS(rocsolver_dgesvdj_strided_batched(
+                ctx->blas, rocblas_svect_singular, rocblas_svect_all, rows_max, A, a_d,
+                rows_max, sA, 0.0, ctx->nos_e.as<double>(), 100,
+                ctx->nos_sweeps.as<rocblas_int>(), lam, (rocblas_stride)A,
+                ctx->nos_u.as<double>(), rows_max, sA, g, A, sG, info_d, n_sino));
+            TT_BLAS(rocblas_dgemm_strided_batched(
+                ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A, &one,
+                ctx->nos_u.as<double>(), rows_max, sA, g, A, sG, &zero,
+                ctx->nos_c.as<double>(), rows_max, sA, n_sino));
+            result = ctx->nos_c.as<double>();
+            TT_HIP(hipMemcpyAsync(info.data(), info_d, sizeof(rocblas_int) * n_sino,
+                                  hipMemcpyDeviceToHost, st));
+        }
     }
     for (int t = 0; t < n_sino; t++)
         TT_HIP(ttk::launch_to_float(result + mA * t, rows_max, out_rows[t], A,

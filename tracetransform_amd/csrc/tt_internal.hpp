@@ -195,6 +195,26 @@ hipError_t launch_to_float(const double *in, int ld, int rows, int cols, float *
 hipError_t launch_scale_eigvec(const double *v, const double *lambda, int k, int count,
                                double *out, hipStream_t s);
 hipError_t launch_newton_schulz_factor(double *x, int k, int count, hipStream_t s);
+// Polar iteration pieces (nearest orthonormal sinogram): coefficients of
+// q(y) = c0 + c1 y + c2 y^2 + c3 y^3 + c4 y^4 for one step.
+struct PolarStep {
+    double c[5];
+};
+// Gram matrices from row-chunk partials (sum over chunks, in order).
+hipError_t launch_nos_reduce(const double *part, int chunks, int k, int count, double *S,
+                             hipStream_t s);
+// First-step scales s = ||G||_F^-1, r = ||G||_F^(-1/2) per matrix (part:
+// kNosSlices * count doubles of scratch).
+constexpr int kNosSlices = 64;
+hipError_t launch_nos_scales(const double *G, int k, int count, double *part, double *scale,
+                             double *r, hipStream_t s);
+// part[m * kNosSlices + j] = max |S_m - I| over slice j (the host takes the
+// maximum).
+hipError_t launch_nos_deviation(const double *S, int k, int count, double *part,
+                                hipStream_t s);
+hipError_t launch_nos_poly(const double *S, const double *S2, int k, int count,
+                           const PolarStep &c, const double *scale, const double *r, double *T,
+                           double *Q, hipStream_t s);
 
 hipError_t launch_records(const RecordParams &p, hipStream_t s);
 hipError_t launch_gather(const GatherParams &p, hipStream_t s);

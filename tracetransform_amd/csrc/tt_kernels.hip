@@ -1328,7 +1328,8 @@ __device__ __forceinline__ void sort_line(const SortTParams &p, int line, int wh
 // wide network's VGPR budget would otherwise set the occupancy of every
 // wave).
 template <int KMAX, bool DEFER, bool IDX7>
-__global__ __launch_bounds__(kBlock) void sino_sort_kernel(SortTParams p)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KMAX <= 8 ? 5 : KMAX <= 16 ? 4 : 1)))
+void sino_sort_kernel(SortTParams p)
 {
     constexpr int M = kWave * KMAX;
     __shared__ __align__(16) float s_all[(kBlock / kWave) * M];
@@ -2043,6 +2044,106 @@ __global__ __launch_bounds__(kBlock) void newton_schulz_factor_kernel(double *x,
     }
 }
 
+// Polar iteration of the nearest orthonormal sinogram (X <- X q(X^T X)).
+// The Gram matrices come from a GEMM split over row chunks: S_m = sum over
+// c < chunks of part[m * chunks + c], summed in chunk order (deterministic).
+__global__ __launch_bounds__(kBlock) void nos_reduce_kernel(const double *part, int chunks,
+                                                           size_t kk, int count, double *S)
+{
+    const size_t total = kk * count;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * kBlock) {
+        const size_t m = i / kk, e = i - m * kk;
+        const double *p = part + m * chunks * kk + e;
+        double acc = p[0];
+        for (int c = 1; c < chunks; c++)
+            acc += p[c * kk];
+        S[i] = acc;
+    }
+}
+
+// Per-slice partial reductions of each k x k matrix (grid: kNosSlices x
+// count): sum of squares (WHAT = 0) or max |S - I| (WHAT = 1, NaN
+// propagates), into part[m * kNosSlices + slice].
+template <int WHAT>
+__global__ __launch_bounds__(kBlock) void nos_slice_kernel(const double *S, int k, double *part)
+{
+    __shared__ double s_red[kBlock / kWave];
+    const size_t kk = (size_t)k * k;
+    const double *M = S + kk * blockIdx.y;
+    const size_t per = (kk + gridDim.x - 1) / gridDim.x;
+    const size_t i0 = per * blockIdx.x, i1 = i0 + per < kk ? i0 + per : kk;
+    double acc = 0.0;
+    for (size_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
+        if (WHAT == 0) {
+            acc += M[i] * M[i];
+        } else {
+            const double v = fabs(M[i] - ((i % k) == (i / k) ? 1.0 : 0.0));
+            acc = (v > acc || v != v) ? v : acc;
+        }
+    }
+    if (WHAT == 0) {
+        acc = block_sum_d(acc, s_red);
+    } else {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double v = __shfl_xor(acc, o, kWave);
+            acc = (v > acc || v != v) ? v : acc;
+        }
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0)
+            s_red[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        acc = s_red[0];
+        for (int w = 1; w < kBlock / kWave; w++)
+            acc = (s_red[w] > acc || s_red[w] != s_red[w]) ? s_red[w] : acc;
+    }
+    if (threadIdx.x == 0)
+        part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = acc;
+}
+
+// Scales of the first step from the Gram matrices' squared Frobenius norms
+// f2 (sums of the slices, in order): s = ||G||_F^-1 normalises S_0 = s G
+// (every eigenvalue in [0, 1]) and r = ||G||_F^(-1/2) scales X_0 = A r; both
+// 0 for a zero matrix, which then never converges.
+__global__ void nos_scales_kernel(const double *part, int slices, int count, double *s,
+                                  double *r)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < count) {
+        double f2 = 0.0;
+        for (int j = 0; j < slices; j++)
+            f2 += part[(size_t)m * slices + j];
+        const double f = sqrt(f2);
+        s[m] = f > 0.0 ? 1.0 / f : 0.0;
+        r[m] = f > 0.0 ? 1.0 / sqrt(f) : 0.0;
+    }
+}
+
+// The elementwise part of Q = q(S) = c0 I + c1 S + c2 S^2 + c3 S^3 + c4 S^4
+// (Paterson-Stockmeyer: Q = (c0 I + c1 S + c2 S^2) + S^2 (c3 S + c4 S^2),
+// the product added by a GEMM).  In the first step S and S2 hold G and G G
+// unnormalised, S_0 = s G, and Q is scaled by r (X_1 = A r q(S_0)):
+// a = s S, b = s^2 S2, Q = r (c0 I + c1 a + c2 b) and T = s^2 r (c3 a +
+// c4 b), so that S2 T = S_0^2 r (c3 S_0 + c4 S_0^2).  Later steps: s = r = 1
+// (null pointers).
+__global__ __launch_bounds__(kBlock) void nos_poly_kernel(const double *S, const double *S2,
+                                                         int k, int count, PolarStep c,
+                                                         const double *scale, const double *r,
+                                                         double *T, double *Q)
+{
+    const size_t kk = (size_t)k * k, total = kk * count;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * kBlock) {
+        const size_t m = i / kk, e = i - m * kk;
+        const double sc = scale ? scale[m] : 1.0, rr = r ? r[m] : 1.0;
+        const double a = sc * S[i], b = sc * sc * S2[i];
+        const double d = (e % k) == (e / k) ? c.c[0] : 0.0;
+        T[i] = sc * sc * rr * (c.c[3] * a + c.c[4] * b);
+        Q[i] = rr * (d + c.c[1] * a + c.c[2] * b);
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -2363,6 +2464,39 @@ hipError_t launch_newton_schulz_factor(double *x, int k, int count, hipStream_t 
 {
     newton_schulz_factor_kernel<<<grid_for((size_t)k * k * count), kBlock, 0, s>>>(x, k,
                                                                                   count);
+    return hipGetLastError();
+}
+
+hipError_t launch_nos_reduce(const double *part, int chunks, int k, int count, double *S,
+                             hipStream_t s)
+{
+    const size_t kk = (size_t)k * k;
+    nos_reduce_kernel<<<grid_for(kk * count), kBlock, 0, s>>>(part, chunks, kk, count, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_nos_scales(const double *G, int k, int count, double *part, double *scale,
+                             double *r, hipStream_t s)
+{
+    nos_slice_kernel<0><<<dim3(kNosSlices, count), kBlock, 0, s>>>(G, k, part);
+    nos_scales_kernel<<<(count + kWave - 1) / kWave, kWave, 0, s>>>(part, kNosSlices, count,
+                                                                    scale, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_nos_deviation(const double *S, int k, int count, double *part,
+                                hipStream_t s)
+{
+    nos_slice_kernel<1><<<dim3(kNosSlices, count), kBlock, 0, s>>>(S, k, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_nos_poly(const double *S, const double *S2, int k, int count,
+                           const PolarStep &c, const double *scale, const double *r, double *T,
+                           double *Q, hipStream_t s)
+{
+    nos_poly_kernel<<<grid_for((size_t)k * k * count), kBlock, 0, s>>>(S, S2, k, count, c,
+                                                                       scale, r, T, Q);
     return hipGetLastError();
 }
 
