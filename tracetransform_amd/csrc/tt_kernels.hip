@@ -1222,17 +1222,17 @@ __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le,
 //     result is the largest key (no sort needed);
 //   * otherwise it sorts the part before the trailing zeros with the
 //     smallest register network that holds it (K = 1..KMAX keys per lane).
-// With DEFER, a line whose sorted part needs more than KMAX keys per lane is
-// queued in p.defer for the wide pass instead.
+// With DEFER, an entry whose sorted part needs more than KMAX keys per lane
+// is left for a wider pass: sort_line then returns true (nothing written).
 template <int KMAX, bool DEFER, bool IDX7>
-__device__ __forceinline__ void sort_line(const SortTParams &p, int line, int which,
+__device__ __forceinline__ bool sort_line(const SortTParams &p, int line, int which,
                                           float *xs, int lane)
 {
     constexpr int M = kWave * KMAX;  // capacity of the wave's LDS slice
     const int n = p.n;
     const int slotw = which == 0 ? p.slot6 : p.slot7;
     if (slotw < 0)
-        return;
+        return false;
     const float o = (float)((n - 1) / 2.0);
     const float lim = (float)(n - 1);
     const int a = line / n;
@@ -1251,11 +1251,8 @@ __device__ __forceinline__ void sort_line(const SortTParams &p, int line, int wh
     // a sampled part longer than the slice cannot sort here: queue it for a
     // wider pass before sampling (its sorted part may turn out shorter;
     // the wider pass then sorts it with the smaller network)
-    if (DEFER && Ls > M) {
-        if (lane == 0)
-            p.defer_out[atomicAdd(p.defer_out_count, 1u)] = line * 2 + which;
-        return;
-    }
+    if (DEFER && Ls > M)
+        return true;
     // consecutive lanes take consecutive rows (neighbouring taps)
     int last_nz = -1;
     bool bad = false;
@@ -1280,11 +1277,8 @@ __device__ __forceinline__ void sort_line(const SortTParams &p, int line, int wh
     } else {
         const int Le = wave_max_i(last_nz) + 1;
         const int kneed = (Le + kWave - 1) / kWave;
-        if (DEFER && kneed > KMAX) {
-            if (lane == 0)
-                p.defer_out[atomicAdd(p.defer_out_count, 1u)] = line * 2 + which;
-            return;
-        }
+        if (DEFER && kneed > KMAX)
+            return true;
         wave_lds_sync();
 #define TT_TAIL(K)                                                                           \
     (t6 ? sorted_tail_median<(KMAX < K ? KMAX : K), true, false>(xs, L, Le, lane, &k)          \
@@ -1317,16 +1311,18 @@ __device__ __forceinline__ void sort_line(const SortTParams &p, int line, int wh
                 p.t7_src[line] = base + (int)(uint32_t)res;
         }
     }
+    return false;
 }
 
 // Persistent waves take work items from the counter p.work (zeroed ahead of
-// the launch), CHUNK at a time: a line (T6 then T7) or, with p.defer_in set,
-// a queued (line, which) entry.  One counter serialises its atomics across
-// the device (tens of nanoseconds each), so the cheap items of the first pass
-// go in chunks of several lines.  With DEFER, entries whose sorted part needs
-// more than KMAX keys per lane are queued in p.defer_out for a wider pass (a
-// wide network's VGPR budget would otherwise set the occupancy of every
-// wave).
+// the launch), p.chunk at a time: a line (T6 then T7) or, with p.defer_in
+// set, a queued (line, which) entry.  One counter serialises its atomics
+// across the device (tens of nanoseconds each), so the cheap items of the
+// first pass go in chunks of several lines.  With DEFER, entries whose
+// sorted part needs more than KMAX keys per lane are queued in p.defer_out
+// for a wider pass (a wide network's VGPR budget would otherwise set the
+// occupancy of every wave); a wave collects a chunk's deferrals in one
+// register (lane i holds the i-th) and queues them with one atomic.
 template <int KMAX, bool DEFER, bool IDX7>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KMAX <= 8 ? 5 : KMAX <= 16 ? 4 : 1)))
 void sino_sort_kernel(SortTParams p)
@@ -1336,7 +1332,7 @@ void sino_sort_kernel(SortTParams p)
     const int lane = threadIdx.x & (kWave - 1);
     float *xs = s_all + wave_id() * M;
     const int total = p.defer_in ? (int)*p.defer_in_count : p.A * p.n;
-    const int chunk = p.chunk;
+    const int chunk = p.chunk;  // at most 32: two entries per line fit a wave
     for (;;) {
         int j0 = 0;
         if (lane == 0)
@@ -1345,17 +1341,32 @@ void sino_sort_kernel(SortTParams p)
         if (j0 >= total)
             break;
         const int j1 = j0 + chunk < total ? j0 + chunk : total;
+        int queued = 0, nq = 0;
         for (int j = j0; j < j1; j++) {
             if (!p.defer_in) {
                 for (int which = 0; which < 2; which++) {
-                    sort_line<KMAX, DEFER, IDX7>(p, j, which, xs, lane);
+                    if (sort_line<KMAX, DEFER, IDX7>(p, j, which, xs, lane)) {
+                        queued = lane == nq ? j * 2 + which : queued;
+                        nq++;
+                    }
                     wave_lds_sync();
                 }
             } else {
                 const int e = p.defer_in[j];
-                sort_line<KMAX, DEFER, IDX7>(p, e >> 1, e & 1, xs, lane);
+                if (sort_line<KMAX, DEFER, IDX7>(p, e >> 1, e & 1, xs, lane)) {
+                    queued = lane == nq ? e : queued;
+                    nq++;
+                }
                 wave_lds_sync();
             }
+        }
+        if (DEFER && nq > 0) {
+            unsigned int at = 0;
+            if (lane == 0)
+                at = atomicAdd(p.defer_out_count, (unsigned int)nq);
+            at = __builtin_amdgcn_readfirstlane(at);
+            if (lane < nq)
+                p.defer_out[at + lane] = queued;
         }
     }
 }
