@@ -1030,31 +1030,34 @@ __device__ __forceinline__ float chain_sum(const float *c, int i0, int i1, float
     return acc;
 }
 
-// Eigen packet-order sum of a TailView's full sequence (as eigen_sum_block):
-// lanes 0..7 run the stride-8 chains, lane 0 combines; the result is
-// returned to every lane.  Adding the zero run's +0 leaves a chain of
-// non-negative values unchanged, so chain r sums the compact entries before
-// the run (positions r, r + 8, ... below ins) and after it (positions
-// congruent to r from ins + zt on, entry = position - zt) and skips the run.
-__device__ __forceinline__ float eigen_sum_view(const TailView &v, int lane)
+// Eigen packet-order sum of a TailView's full sequence (as eigen_sum_block),
+// for up to 8 views at once: lanes 8g .. 8g + 7 run the stride-8 chains of
+// the view they pass (group g), lane 8g combines them; the sum of group g is
+// returned on lane 8g (other lanes: unspecified).  Lanes of no group pass
+// active = false.  Adding the zero run's +0 leaves a chain of non-negative
+// values unchanged, so chain r sums the compact entries before the run
+// (positions r, r + 8, ... below ins) and after it (positions congruent to r
+// from ins + zt on, entry = position - zt) and skips the run.
+__device__ __forceinline__ float eigen_sum_group(const TailView &v, int lane, bool active)
 {
     const int n = v.L;
     const int a4 = n & ~3, a8 = n & ~7;
+    const int ch_i = lane & 7;
     float r = 0.0f;
-    if (lane < 8 && n >= 8) {
+    if (active && n >= 8) {
         const int e1 = v.ins < a8 ? v.ins : a8;
-        r = chain_sum(v.c, lane, e1, r);
+        r = chain_sum(v.c, ch_i, e1, r);
         const int q0 = v.ins + v.zt;
-        const int p0 = q0 + ((lane - q0) & 7);  // first position >= q0 in chain lane
+        const int p0 = q0 + ((ch_i - q0) & 7);  // first position >= q0 in chain ch_i
         if (p0 < a8)
             r = chain_sum(v.c, p0 - v.zt, a8 - v.zt, r);
     }
     float ch[8];
 #pragma unroll
     for (int j = 0; j < 8; j++)
-        ch[j] = __shfl(r, j, kWave);
+        ch[j] = __shfl(r, (lane & ~7) + j, kWave);
     float res = 0.0f;
-    if (lane == 0) {
+    if (active && ch_i == 0) {
         if (a4 == 0) {
             res = n > 0 ? v.at(0) : 0.0f;
             for (int i = 1; i < n; i++)
@@ -1071,7 +1074,13 @@ __device__ __forceinline__ float eigen_sum_view(const TailView &v, int lane)
                 res += v.at(i);
         }
     }
-    return __shfl(res, 0, kWave);
+    return res;
+}
+
+// The same for one view (lanes 0..7), the sum returned to every lane.
+__device__ __forceinline__ float eigen_sum_view(const TailView &v, int lane)
+{
+    return __shfl(eigen_sum_group(v, lane, lane < 8), 0, kWave);
 }
 
 // findWeightedMedian's sequential walk over a TailView (one lane).  Adding
@@ -1117,7 +1126,7 @@ __device__ __forceinline__ int walk_segment(const float *c, int j0, int j1, int 
     return -1;
 }
 
-__device__ int weighted_median_walk_view(const TailView &v, int Le, float S)
+__device__ __forceinline__ int weighted_median_walk_view(const TailView &v, int Le, float S)
 {
     float integral = 0.0f;
     int k = walk_segment(v.c, 0, v.ins, 0, S, &integral);
@@ -1139,45 +1148,68 @@ __device__ int weighted_median_walk_view(const TailView &v, int Le, float S)
 // order: ties by ascending index).  Replaces xs[0..Le) by the square roots
 // of the sorted part, takes the Eigen-order sum and the sequential walk over
 // the full sequence, and returns the key at the weighted-median position k.
-template <int K, bool T6, bool IDX7>
-__device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le, int lane,
-                                                       int *k_out)
+template <bool IDX>
+using TailKey = typename std::conditional<IDX, uint64_t, uint32_t>::type;
+
+// Sort key of tail sample i: T6 w = float(i) * x with the index (stable
+// order, and the source row is an output); T7 the value x, with the index
+// when its source row is asked for (IDX7), else alone (equal values are
+// interchangeable).  Counts the zero keys of the index-carrying kinds.
+template <bool T6, bool IDX7>
+__device__ __forceinline__ TailKey<T6 || IDX7> tail_key(const float *xs, int i, int *zin)
 {
-    // T6 keys carry the index (stable order, and the source row is an
-    // output); T7 needs only the order of the values (equal values are
-    // interchangeable), so its keys are the 32-bit orderable values unless
-    // its source row is asked for (IDX7: stable order, index in the key)
-    constexpr bool IDX = T6 || IDX7;
-    using Key = typename std::conditional<IDX, uint64_t, uint32_t>::type;
-    Key key[K];
-    int zin = 0;
-#pragma unroll
-    for (int r = 0; r < K; r++) {
-        const int i = r * kWave + lane;  // any placement: the sort orders them
-        key[r] = (Key)~(Key)0;
-        if (i < Le) {
-            const float x = xs[i];
-            const float kv = T6 ? (float)i * x : x;
-            if constexpr (IDX) {
-                key[r] = ((uint64_t)ord_key(kv) << 32) | (uint32_t)i;
-                zin += kv == 0.0f ? 1 : 0;
-            } else {
-                key[r] = ord_key(kv);
-            }
-        }
+    const float x = xs[i];
+    const float kv = T6 ? (float)i * x : x;
+    if constexpr (T6 || IDX7) {
+        *zin += kv == 0.0f ? 1 : 0;
+        return ((uint64_t)ord_key(kv) << 32) | (uint32_t)i;
+    } else {
+        return ord_key(kv);
     }
-    wave_bitonic_sort<K>(key, lane);
-    // where the trailing zeros go: after the sorted part's zero keys in the
-    // stable (index) order, or first when the order of equal values is free
-    const int ins = IDX ? wave_sum_i(zin) : 0;
+}
+
+// The key at position k of a tail's full sequence (L = Le + zt entries,
+// the zt trailing zeros at ins .. ins + zt - 1), the sorted part's keys in
+// blocked layout with KO per lane; returned to every lane as
+// (orderable value << 32) | index (index 0 for value-only keys).
+template <int KR, bool IDX>
+__device__ __forceinline__ uint64_t tail_key_at(const TailKey<IDX> (&key)[KR], int KO, int k,
+                                                int ins, int zt, int Le)
+{
+    if (k >= ins && k < ins + zt)  // a trailing zero: w (or x) = +0, index Le + ...
+        return ((uint64_t)ord_key(0.0f) << 32) | (uint32_t)(Le + (k - ins));
+    const int pos = k < ins ? k : k - zt;
+    const int pr = pos % KO;
+    TailKey<IDX> kk = 0;
+#pragma unroll
+    for (int r = 0; r < KR; r++)
+        kk = (r == pr) ? key[r] : kk;
+    if constexpr (IDX)
+        return shfl64(kk, pos / KO);
+    else  // value key in the high word (no index asked for)
+        return (uint64_t)(uint32_t)__shfl((int)kk, pos / KO, kWave) << 32;
+}
+
+// Second half of a sorted tail's median: the sorted part's keys are in
+// blocked layout (lane l holds positions l*KO .. l*KO + KO - 1, KO <= KR),
+// ins is where the trailing zeros go.  Replaces xs[0..Le) by the square
+// roots of the sorted samples, takes the Eigen-order sum and the sequential
+// walk over the full sequence, and returns the key at the weighted-median
+// position k.
+template <int KR, bool T6, bool IDX7>
+__device__ __forceinline__ uint64_t tail_finish(const TailKey<T6 || IDX7> (&key)[KR], int KO,
+                                                int ins, float *xs, int L, int Le, int lane,
+                                                int *k_out)
+{
+    constexpr bool IDX = T6 || IDX7;
     const int zt = L - Le;
     // gather the sorted samples before overwriting the slice with roots
-    float sq[K];
+    float sq[KR];
 #pragma unroll
-    for (int r = 0; r < K; r++) {
-        const int pos = lane * K + r;
+    for (int r = 0; r < KR; r++) {
+        const int pos = lane * KO + r;
         sq[r] = 0.0f;
-        if (pos < Le) {
+        if (r < KO && pos < Le) {
             if constexpr (T6)
                 sq[r] = sqrtf(xs[(uint32_t)key[r]]);
             else if constexpr (IDX7)
@@ -1188,9 +1220,9 @@ __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le,
     }
     wave_lds_sync();
 #pragma unroll
-    for (int r = 0; r < K; r++) {
-        const int pos = lane * K + r;
-        if (pos < Le)
+    for (int r = 0; r < KR; r++) {
+        const int pos = lane * KO + r;
+        if (r < KO && pos < Le)
             xs[pos] = sq[r];
     }
     wave_lds_sync();
@@ -1201,17 +1233,136 @@ __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le,
         k = weighted_median_walk_view(view, Le, S);
     k = __shfl(k, 0, kWave);
     *k_out = k;
-    if (k >= ins && k < ins + zt)  // a trailing zero: w (or x) = +0, index Le + ...
-        return ((uint64_t)ord_key(0.0f) << 32) | (uint32_t)(Le + (k - ins));
-    const int pos = k < ins ? k : k - zt;
-    Key kk = 0;
+    return tail_key_at<KR, IDX>(key, KO, k, ins, zt, Le);
+}
+
+// Sorted tail of one line for T6 (T6 = true) or T7, given the tail's first
+// Le samples in xs[0..Le), no sample negative or NaN, and every sample at
+// i >= Le zero (the line has left the rotated image).  Only the first Le keys
+// are sorted (K keys per lane, 64*K >= Le); the L - Le trailing zeros are
+// placed without sorting: for T7 (value keys) they are the smallest values
+// and lead the order; for T6 (w = float(i) * x keys) they have w = 0 and the
+// largest indices, so they follow the zero-w keys of the sorted part (stable
+// order: ties by ascending index).
+template <int K, bool T6, bool IDX7>
+__device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le, int lane,
+                                                       int *k_out)
+{
+    constexpr bool IDX = T6 || IDX7;
+    using Key = TailKey<IDX>;
+    Key key[K];
+    int zin = 0;
 #pragma unroll
-    for (int r = 0; r < K; r++)
-        kk = (r == pos % K) ? key[r] : kk;
-    if constexpr (IDX)
-        return shfl64(kk, pos / K);
-    else  // value key in the high word (no index asked for)
-        return (uint64_t)(uint32_t)__shfl((int)kk, pos / K, kWave) << 32;
+    for (int r = 0; r < K; r++) {
+        const int i = r * kWave + lane;  // any placement: the sort orders them
+        key[r] = (Key)~(Key)0;
+        if (i < Le)
+            key[r] = tail_key<T6, IDX7>(xs, i, &zin);
+    }
+    wave_bitonic_sort<K>(key, lane);
+    // where the trailing zeros go: after the sorted part's zero keys in the
+    // stable (index) order, or first when the order of equal values is free
+    const int ins = IDX ? wave_sum_i(zin) : 0;
+    return tail_finish<K, T6, IDX7>(key, K, ins, xs, L, Le, lane, k_out);
+}
+
+// The sorted tail with the smallest register network that holds its Le
+// sorted keys (at most KMAX per lane).
+template <int KMAX, bool T6, bool IDX7>
+__device__ __forceinline__ uint64_t sorted_tail_dispatch(float *xs, int L, int Le, int lane,
+                                                         int *k)
+{
+    const int kneed = (Le + kWave - 1) / kWave;
+    if (kneed <= 1 || KMAX == 1)
+        return sorted_tail_median<1, T6, IDX7>(xs, L, Le, lane, k);
+    if constexpr (KMAX >= 2)
+        if (kneed <= 2 || KMAX == 2)
+            return sorted_tail_median<2, T6, IDX7>(xs, L, Le, lane, k);
+    if constexpr (KMAX >= 4)
+        if (kneed <= 4 || KMAX == 4)
+            return sorted_tail_median<4, T6, IDX7>(xs, L, Le, lane, k);
+    if constexpr (KMAX >= 8)
+        if (kneed <= 8 || KMAX == 8)
+            return sorted_tail_median<8, T6, IDX7>(xs, L, Le, lane, k);
+    if constexpr (KMAX >= 16)
+        if (kneed <= 16 || KMAX == 16)
+            return sorted_tail_median<16, T6, IDX7>(xs, L, Le, lane, k);
+    if constexpr (KMAX >= 32)
+        return sorted_tail_median<32, T6, IDX7>(xs, L, Le, lane, k);
+    return 0;  // not reached: kneed <= KMAX
+}
+
+// Keys per lane of the network for Le keys: 1, 2, 4, ... up to KMAX.
+template <int KMAX>
+__device__ __forceinline__ int keys_per_lane(int Le)
+{
+    int K = 1;
+    while (K < KMAX && K * kWave < Le)
+        K <<= 1;
+    return K;
+}
+
+// Sorts the first K of KMAX keys per lane (the rest hold the largest key)
+// with the K-per-lane network (K a power of two up to KMAX).
+template <int KMAX, class Key>
+__device__ __forceinline__ void sort_prefix(Key (&k)[KMAX], int K, int lane)
+{
+    if (K <= 1 || KMAX == 1) {
+        wave_bitonic_sort<1>(reinterpret_cast<Key(&)[1]>(k), lane);
+        return;
+    }
+    if constexpr (KMAX >= 2)
+        if (K <= 2 || KMAX == 2) {
+            wave_bitonic_sort<2>(reinterpret_cast<Key(&)[2]>(k), lane);
+            return;
+        }
+    if constexpr (KMAX >= 4)
+        if (K <= 4 || KMAX == 4) {
+            wave_bitonic_sort<4>(reinterpret_cast<Key(&)[4]>(k), lane);
+            return;
+        }
+    if constexpr (KMAX >= 8)
+        wave_bitonic_sort<8>(reinterpret_cast<Key(&)[8]>(k), lane);
+}
+
+// The largest key of a tail whose first Ls samples are in xs (the rest of
+// its L are zero): with a negative or NaN sample, sqrt makes the Eigen sum
+// NaN, no prefix passes the test and findWeightedMedian returns the last
+// position, i.e. the largest key.
+__device__ __forceinline__ uint64_t tail_max_key(const float *xs, int Ls, int L, bool t6,
+                                                 int lane)
+{
+    // the unsampled zeros' largest key: (0, L - 1)
+    uint64_t kmax = Ls < L ? (((uint64_t)ord_key(0.0f) << 32) | (uint32_t)(L - 1)) : 0ull;
+    for (int i = lane; i < Ls; i += kWave) {
+        const float x = xs[i];
+        const float kv = t6 ? (float)i * x : x;
+        const uint64_t kk = ((uint64_t)ord_key(kv) << 32) | (uint32_t)i;
+        kmax = kk > kmax ? kk : kmax;
+    }
+    return wave_max_u64(kmax);
+}
+
+// Lane 0 stores one tail's result: the value, its rank k and source row.
+template <bool IDX7>
+__device__ __forceinline__ void store_tail_result(const SortTParams &p, int line, int which,
+                                                  uint64_t res, int k, int base, int lane)
+{
+    if (lane != 0)
+        return;
+    const int slotw = which == 0 ? p.slot6 : p.slot7;
+    p.sino[(size_t)slotw * p.A * p.n + line] = ord_val((uint32_t)(res >> 32));
+    if (which == 0) {
+        if (p.t6_rank)
+            p.t6_rank[line] = k;
+        if (p.t6_src)
+            p.t6_src[line] = (int)(uint32_t)res;
+    } else {
+        if (p.t7_rank)
+            p.t7_rank[line] = k;
+        if (IDX7)
+            p.t7_src[line] = base + (int)(uint32_t)res;
+    }
 }
 
 // T6 (which = 0, tail from the sqrt-median sm) or T7 (which = 1, tail from
@@ -1256,8 +1407,6 @@ __device__ __forceinline__ bool sort_line(const SortTParams &p, int line, int wh
     // consecutive lanes take consecutive rows (neighbouring taps)
     int last_nz = -1;
     bool bad = false;
-    // the largest key among the unsampled zeros: (0, L - 1)
-    uint64_t kmax = Ls < L ? (((uint64_t)ord_key(0.0f) << 32) | (uint32_t)(L - 1)) : 0ull;
     for (int i = lane; i < Ls; i += kWave) {
         const float pyo = (float)(base + i) - o;
         const float x = tr ? rot_sample<true>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo)
@@ -1266,52 +1415,157 @@ __device__ __forceinline__ bool sort_line(const SortTParams &p, int line, int wh
             xs[i] = x;
         last_nz = x != 0.0f ? i : last_nz;
         bad |= !(x >= 0.0f);  // negative or NaN
-        const float kv = t6 ? (float)i * x : x;
-        const uint64_t kk = ((uint64_t)ord_key(kv) << 32) | (uint32_t)i;
-        kmax = kk > kmax ? kk : kmax;
     }
     int k = L - 1;
     uint64_t res;
     if (__any(bad)) {
-        res = wave_max_u64(kmax);
+        wave_lds_sync();
+        res = tail_max_key(xs, Ls < M ? Ls : M, L, t6, lane);
     } else {
         const int Le = wave_max_i(last_nz) + 1;
         const int kneed = (Le + kWave - 1) / kWave;
         if (DEFER && kneed > KMAX)
             return true;
         wave_lds_sync();
-#define TT_TAIL(K)                                                                           \
-    (t6 ? sorted_tail_median<(KMAX < K ? KMAX : K), true, false>(xs, L, Le, lane, &k)          \
-        : sorted_tail_median<(KMAX < K ? KMAX : K), false, IDX7>(xs, L, Le, lane, &k))
-        if (kneed <= 1 || KMAX == 1)
-            res = TT_TAIL(1);
-        else if (kneed <= 2 || KMAX == 2)
-            res = TT_TAIL(2);
-        else if (kneed <= 4 || KMAX == 4)
-            res = TT_TAIL(4);
-        else if (kneed <= 8 || KMAX == 8)
-            res = TT_TAIL(8);
-        else if (kneed <= 16 || KMAX == 16)
-            res = TT_TAIL(16);
-        else
-            res = TT_TAIL(32);
-#undef TT_TAIL
+        res = t6 ? sorted_tail_dispatch<KMAX, true, false>(xs, L, Le, lane, &k)
+                 : sorted_tail_dispatch<KMAX, false, IDX7>(xs, L, Le, lane, &k);
     }
-    if (lane == 0) {
-        p.sino[(size_t)slotw * p.A * n + line] = ord_val((uint32_t)(res >> 32));
-        if (t6) {
-            if (p.t6_rank)
-                p.t6_rank[line] = k;
-            if (p.t6_src)
-                p.t6_src[line] = (int)(uint32_t)res;
-        } else {
-            if (p.t7_rank)
-                p.t7_rank[line] = k;
-            if (IDX7)
-                p.t7_src[line] = base + (int)(uint32_t)res;
+    store_tail_result<IDX7>(p, line, which, res, k, base, lane);
+    return false;
+}
+
+// T6 and T7 of one line by one wave (a first pass, both requested).  The
+// tails [sm, n) and [m, n) share their rows from max(sm, m) on, so the wave
+// samples the union [min(sm, m), E) once into xs (E: the row after the
+// line's last possibly non-zero row), sorts both tails' keys, writes the T6
+// roots over xs and the T7 roots into xs2, and runs the two Eigen sums on
+// lanes 0..7 and 8..15 and the two walks on lanes 0 and 1 together.
+// Returns the entries left for a wider pass (bit 0: T6, bit 1: T7; nothing
+// written for them).  A pair whose union exceeds the slice goes entry by
+// entry; so do the tails of a pair with a negative or NaN sample (largest
+// key), from the shared samples.
+template <int KMAX, bool DEFER, bool IDX7>
+__device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *xs, float *xs2,
+                                         int lane)
+{
+    constexpr int M = kWave * KMAX;
+    const int n = p.n;
+    const float o = (float)((n - 1) / 2.0);
+    const float lim = (float)(n - 1);
+    const int a = line / n;
+    const int col = line - a * n;
+    const float2 cs = p.trig[a];
+    const float px = (float)col - o;
+    const float pxc = px * cs.x, pxns = px * (-cs.y);
+    const bool tr = fabsf(cs.y) > fabsf(cs.x);
+    const float *src = tr ? p.imgT : p.img;
+    const int sm = p.med_s[line], m = p.med_x[line];
+    const int L6 = n - sm, L7 = n - m;
+    const int2 rows = line_rows(p.bbox, n, col, cs.x, cs.y);
+    const int E = rows.y + 1 < n ? rows.y + 1 : n;
+    const int Ls6 = E - sm, Ls7 = E - m;  // sampled lengths (<= 0: no sample)
+    int deferred = 0;
+    if (DEFER)
+        deferred = (Ls6 > M ? 1 : 0) | (Ls7 > M ? 2 : 0);
+    const int b0 = sm < m ? sm : m;
+    const int Lu = E - b0;
+    if (deferred || Lu > M) {
+        for (int which = 0; which < 2; which++) {
+            if (!((deferred >> which) & 1) &&
+                sort_line<KMAX, DEFER, IDX7>(p, line, which, xs, lane))
+                deferred |= 1 << which;
+            wave_lds_sync();
+        }
+        return deferred;
+    }
+    int last_nz = -1, last_bad = -1;
+    for (int i = lane; i < Lu; i += kWave) {
+        const float pyo = (float)(b0 + i) - o;
+        const float x = tr ? rot_sample<true>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo)
+                           : rot_sample<false>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo);
+        xs[i] = x;
+        last_nz = x != 0.0f ? i : last_nz;
+        last_bad = !(x >= 0.0f) ? i : last_bad;  // negative or NaN
+    }
+    const int o6 = sm - b0, o7 = m - b0;
+    const int lnz = wave_max_i(last_nz), lbad = wave_max_i(last_bad);
+    const int Le6 = lnz + 1 - o6 > 0 ? lnz + 1 - o6 : 0;
+    const int Le7 = lnz + 1 - o7 > 0 ? lnz + 1 - o7 : 0;
+    wave_lds_sync();
+    if (lbad >= o6 || lbad >= o7) {
+        // the bad tails read the shared samples first; then at most one
+        // good tail sorts (and overwrites them)
+        const bool bad6 = lbad >= o6, bad7 = lbad >= o7;
+        uint64_t r6 = 0, r7 = 0;
+        int k6 = L6 - 1, k7 = L7 - 1;
+        if (bad6)
+            r6 = tail_max_key(xs + o6, Ls6, L6, true, lane);
+        if (bad7)
+            r7 = tail_max_key(xs + o7, Ls7, L7, false, lane);
+        wave_lds_sync();
+        if (!bad6)
+            r6 = sorted_tail_dispatch<KMAX, true, false>(xs + o6, L6, Le6, lane, &k6);
+        if (!bad7)
+            r7 = sorted_tail_dispatch<KMAX, false, IDX7>(xs + o7, L7, Le7, lane, &k7);
+        store_tail_result<IDX7>(p, line, 0, r6, k6, sm, lane);
+        store_tail_result<IDX7>(p, line, 1, r7, k7, m, lane);
+        return 0;
+    }
+    using Key7 = TailKey<IDX7>;
+    const int K6 = keys_per_lane<KMAX>(Le6), K7 = keys_per_lane<KMAX>(Le7);
+    uint64_t key6[KMAX];
+    Key7 key7[KMAX];
+    int zin6 = 0, zin7 = 0;
+#pragma unroll
+    for (int r = 0; r < KMAX; r++) {
+        const int i = r * kWave + lane;  // < 64 * K: within the first K slots
+        key6[r] = i < Le6 ? tail_key<true, false>(xs + o6, i, &zin6) : ~0ull;
+        key7[r] = i < Le7 ? tail_key<false, IDX7>(xs + o7, i, &zin7) : (Key7)~(Key7)0;
+    }
+    const int ins6 = wave_sum_i(zin6), ins7 = IDX7 ? wave_sum_i(zin7) : 0;
+    // T6: sort, gather its samples by index, then write the roots over xs
+    // (both key sets are built, so the samples are no longer needed)
+    sort_prefix<KMAX>(key6, K6, lane);
+    {
+        float sq6[KMAX];
+#pragma unroll
+        for (int r = 0; r < KMAX; r++) {
+            const int p6 = lane * K6 + r;
+            sq6[r] = (r < K6 && p6 < Le6) ? sqrtf(xs[o6 + (uint32_t)key6[r]]) : 0.0f;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < KMAX; r++) {
+            const int p6 = lane * K6 + r;
+            if (r < K6 && p6 < Le6)
+                xs[p6] = sq6[r];
         }
     }
-    return false;
+    // T7: sort, roots from the key values into xs2
+    sort_prefix<KMAX>(key7, K7, lane);
+#pragma unroll
+    for (int r = 0; r < KMAX; r++) {
+        const int p7 = lane * K7 + r;
+        if (r < K7 && p7 < Le7) {
+            if constexpr (IDX7)
+                xs2[p7] = sqrtf(ord_val((uint32_t)(key7[r] >> 32)));
+            else
+                xs2[p7] = sqrtf(ord_val(key7[r]));
+        }
+    }
+    wave_lds_sync();
+    const TailView v6 = {xs, ins6, L6 - Le6, L6}, v7 = {xs2, ins7, L7 - Le7, L7};
+    const float sg = eigen_sum_group(lane >= 8 ? v7 : v6, lane, lane < 16);
+    const float S6 = __shfl(sg, 0, kWave), S7 = __shfl(sg, 8, kWave);
+    int kw = 0;
+    if (lane < 2)  // lane 0 walks T6, lane 1 T7
+        kw = weighted_median_walk_view(lane ? v7 : v6, lane ? Le7 : Le6, lane ? S7 : S6);
+    const int k6 = __shfl(kw, 0, kWave), k7 = __shfl(kw, 1, kWave);
+    const uint64_t r6 = tail_key_at<KMAX, true>(key6, K6, k6, ins6, L6 - Le6, Le6);
+    const uint64_t r7 = tail_key_at<KMAX, IDX7>(key7, K7, k7, ins7, L7 - Le7, Le7);
+    store_tail_result<IDX7>(p, line, 0, r6, k6, sm, lane);
+    store_tail_result<IDX7>(p, line, 1, r7, k7, m, lane);
+    return 0;
 }
 
 // Persistent waves take work items from the counter p.work (zeroed ahead of
@@ -1328,9 +1582,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KMAX <= 
 void sino_sort_kernel(SortTParams p)
 {
     constexpr int M = kWave * KMAX;
-    __shared__ __align__(16) float s_all[(kBlock / kWave) * M];
+    // first-pass kernels (at most 8 keys per lane) take T6 and T7 of a line
+    // together and hold two slices per wave
+    constexpr bool PAIR = KMAX <= 8;
+    constexpr int W = PAIR ? 2 * M : M;
+    __shared__ __align__(16) float s_all[(kBlock / kWave) * W];
     const int lane = threadIdx.x & (kWave - 1);
-    float *xs = s_all + wave_id() * M;
+    float *xs = s_all + wave_id() * W;
+    const bool pair = PAIR && !p.defer_in && p.slot6 >= 0 && p.slot7 >= 0;
     const int total = p.defer_in ? (int)*p.defer_in_count : p.A * p.n;
     const int chunk = p.chunk;  // at most 32: two entries per line fit a wave
     for (;;) {
@@ -1343,7 +1602,15 @@ void sino_sort_kernel(SortTParams p)
         const int j1 = j0 + chunk < total ? j0 + chunk : total;
         int queued = 0, nq = 0;
         for (int j = j0; j < j1; j++) {
-            if (!p.defer_in) {
+            if (pair) {
+                const int d = sort_pair<KMAX, DEFER, IDX7>(p, j, xs, xs + M, lane);
+                for (int which = 0; which < 2; which++)
+                    if ((d >> which) & 1) {
+                        queued = lane == nq ? j * 2 + which : queued;
+                        nq++;
+                    }
+                wave_lds_sync();
+            } else if (!p.defer_in) {
                 for (int which = 0; which < 2; which++) {
                     if (sort_line<KMAX, DEFER, IDX7>(p, j, which, xs, lane)) {
                         queued = lane == nq ? j * 2 + which : queued;
