@@ -271,6 +271,7 @@ struct tt_context {
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     DevBuf imgT, trig, tab345, dft_tw, herm_w, io_a, io_b, med_x, med_s;
+    DevBuf dft_w, p3_x, p3_spec;
     DevBuf bw_sino, bw_dsino, bw_z, bw_rec, bw_t6src, bw_t7src;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
     DevBuf nos_w, nos_m, nos_q, nos_r, nos_dev, nos_sp, nos_ptrs;
@@ -278,7 +279,7 @@ struct tt_context {
     rocblas_handle blas = nullptr;
     int trig_A = -1;
     float trig_step = 0.0f;
-    int tab_n = -1, dft_n = -1;
+    int tab_n = -1, dft_n = -1, dftw_n = -1;
     std::vector<int32_t> herm_key;
     std::vector<float2> h_trig, h_tab, h_tw;
     std::vector<double> h_herm;
@@ -391,6 +392,14 @@ tt_status check_pfuncs(const tt_pfunc *pf, int np, int n)
             return s_;                                                         \
     } while (0)
 
+#define TT_BLAS(expr)                                                          \
+    do {                                                                       \
+        rocblas_status b_ = (expr);                                            \
+        if (b_ != rocblas_status_success)                                      \
+            return fail(TT_ERR_DEVICE, "%s failed: rocblas status %d", #expr,  \
+                        (int)b_);                                              \
+    } while (0)
+
 extern "C" {
 
 TT_API const char *tt_last_error(void) { return g_err.c_str(); }
@@ -425,7 +434,8 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
         return TT_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->imgT, &ctx->trig, &ctx->tab345, &ctx->dft_tw,
+    DevBuf *bufs[] = {&ctx->dft_w, &ctx->p3_x, &ctx->p3_spec,
+                      &ctx->imgT, &ctx->trig, &ctx->tab345, &ctx->dft_tw,
                       &ctx->herm_w, &ctx->io_a, &ctx->io_b, &ctx->med_x,
                       &ctx->med_s, &ctx->bw_sino, &ctx->bw_dsino, &ctx->bw_z,
                       &ctx->bw_rec, &ctx->bw_t6src, &ctx->bw_t7src, &ctx->nos_med, &ctx->nos_a, &ctx->nos_u,
@@ -692,17 +702,41 @@ TT_API tt_status tt_circus(tt_context *ctx, const float *d_sino, int n,
         }
         p3 = p3 || pf[q].code == TT_P3;
     }
-    if (p3) {
-        TT_TRY(ctx->ensure_dft(n));
-        p.dft_tw = ctx->dft_tw.as<float2>();
-    }
     if (!hkey.empty()) {
         TT_TRY(ctx->ensure_hermite(n, hkey));
         p.herm_w = ctx->herm_w.as<double>();
     }
-    TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS, ctx->stream));
-    TT_HIP(ttk::launch_circus(p, ctx->stream));
-    TT_HIP(ctx->timer.end(ctx->stream));
+    hipStream_t st = ctx->stream;
+    const int h = n / 2 + 1;
+    const size_t cols = (size_t)n_sino * n_cols;
+    if (p3) {
+        // P3's half spectra of every column at once: one FP64 GEMM with the
+        // DFT matrix, spec (2h x cols) = W^T X, on the matrix cores
+        if (ctx->dftw_n != n) {
+            TT_HIP(ctx->dft_w.ensure(sizeof(double) * (size_t)n * 2 * h));
+            TT_HIP(ttk::launch_dft_matrix(n, h, ctx->dft_w.as<double>(), st));
+            ctx->dftw_n = n;
+        }
+        TT_HIP(ctx->p3_x.ensure(sizeof(double) * cols * n));
+        TT_HIP(ctx->p3_spec.ensure(sizeof(double) * cols * 2 * h));
+        if (!ctx->blas)
+            TT_BLAS(rocblas_create_handle(&ctx->blas));
+        TT_BLAS(rocblas_set_stream(ctx->blas, st));
+        TT_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
+        p.p3_spec = ctx->p3_spec.as<double>();
+        p.p3_h = h;
+    }
+    TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS, st));
+    if (p3) {
+        const double one = 1.0, zero = 0.0;
+        TT_HIP(ttk::launch_to_double(d_sino, cols * n, ctx->p3_x.as<double>(), st));
+        TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
+                              2 * h, (int)cols, n, &one, ctx->dft_w.as<double>(), n,
+                              ctx->p3_x.as<double>(), n, &zero, ctx->p3_spec.as<double>(),
+                              2 * h));
+    }
+    TT_HIP(ttk::launch_circus(p, st));
+    TT_HIP(ctx->timer.end(st));
     return TT_OK;
 }
 
@@ -1211,14 +1245,6 @@ TT_API tt_status tt_backward(tt_context *ctx, const float *d_image, int n,
     TT_TRY(image_vjp(ctx, n, n_angles, slot, ctx->bw_dsino.as<float>(), d_dimage));
     return TT_OK;
 }
-
-#define TT_BLAS(expr)                                                          \
-    do {                                                                       \
-        rocblas_status b_ = (expr);                                            \
-        if (b_ != rocblas_status_success)                                      \
-            return fail(TT_ERR_DEVICE, "%s failed: rocblas status %d", #expr,  \
-                        (int)b_);                                              \
-    } while (0)
 
 TT_API tt_status tt_resize(tt_context *ctx, const float *d_in, int in_rows,
                            int in_cols, float *d_out, int rows, int cols)

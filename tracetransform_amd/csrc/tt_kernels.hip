@@ -703,7 +703,8 @@ __device__ __forceinline__ CircusLds circus_lds(float *smem, int nmax)
 // in L.tw when P3 is requested).  Result q goes to out[q * out_stride].
 __device__ void circus_column(const CircusLds &L, int n, int np, const int *code,
                               const int *herm, const double *herm_w, float *out,
-                              size_t out_stride, int32_t *p2rank)
+                              size_t out_stride, int32_t *p2rank,
+                              const double *spec = nullptr, int spec_h = 0)
 {
     __shared__ float s_red[kBlock / kWave];
     __shared__ double s_red_d[kBlock / kWave];
@@ -739,6 +740,26 @@ __device__ void circus_column(const CircusLds &L, int n, int np, const int *code
             res = L.sort[s_rank];
             if (p2rank && threadIdx.x == 0)
                 p2rank[q * out_stride] = s_rank;
+        } else if (spec) {
+            // P3 from the column's FP64 half spectrum (a GEMM with the DFT
+            // matrix): pow(hypot(Re/N, Im/N), 4) and the trapz sum below
+            const int half = n / 2;
+            for (int k = threadIdx.x; k <= half; k += kBlock) {
+                const double h = hypot(spec[k] / n, spec[spec_h + k] / n);
+                const double h2 = h * h;
+                L.mod[k] = (float)(h2 * h2);
+            }
+            __syncthreads();
+            for (int k = half + 1 + threadIdx.x; k < n; k += kBlock)
+                L.mod[k] = L.mod[n - k];
+            __syncthreads();
+            float acc = 0.0f;
+            for (int i = threadIdx.x; i < n - 1; i += kBlock) {
+                const float l0 = (float)(-1 + i * 2.0 / (n - 1));
+                const float l1 = (float)(-1 + (i + 1) * 2.0 / (n - 1));
+                acc += (l1 - l0) * (L.mod[i + 1] + L.mod[i]);
+            }
+            res = (float)(block_sum(acc, s_red) * 0.5);
         } else {
             // P3: half spectrum by direct DFT (twiddles from LDS), then
             // pow(hypot(Re/N, Im/N), 4) in double and the trapz sum.
@@ -805,7 +826,30 @@ __global__ __launch_bounds__(kBlock) void circus_kernel(CircusParams p)
     __syncthreads();
     const size_t o = (size_t)sg * p.np * p.ncols + col;
     circus_column(L, n, p.np, p.code, p.herm, p.herm_w, p.out + o, p.ncols,
-                  p.p2rank ? p.p2rank + o : nullptr);
+                  p.p2rank ? p.p2rank + o : nullptr,
+                  p.p3_spec ? p.p3_spec + (size_t)colg * 2 * p.p3_h : nullptr, p.p3_h);
+}
+
+__global__ __launch_bounds__(kBlock) void dft_matrix_kernel(int n, int h, double *w)
+{
+    const size_t total = (size_t)n * 2 * h;
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * kBlock) {
+        const int c = (int)(e / n), j = (int)(e - (size_t)c * n);
+        const int k = c < h ? c : c - h;
+        const long long t = ((long long)j * k) % n;
+        double sn, cn;
+        sincospi(2.0 * (double)t / (double)n, &sn, &cn);
+        w[e] = c < h ? cn : -sn;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void to_double_kernel(const float *in, size_t count,
+                                                          double *out)
+{
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * kBlock)
+        out[i] = (double)in[i];
 }
 
 // Every sinogram column of a batch; the image is found by binary search over
@@ -2485,6 +2529,7 @@ hipError_t launch_circus(const CircusParams &p, hipStream_t s)
     return hipGetLastError();
 }
 
+
 hipError_t launch_circus_batched(const BatchCircusParams &p, hipStream_t s)
 {
     circus_batched_kernel<<<p.cols, kBlock, circus_shm(p.nmax), s>>>(p);
@@ -2775,6 +2820,18 @@ hipError_t launch_nos_poly(const double *S, const double *S2, int k, int count,
 {
     nos_poly_kernel<<<grid_for((size_t)k * k * count), kBlock, 0, s>>>(S, S2, k, count, c,
                                                                        scale, r, T, Q);
+    return hipGetLastError();
+}
+
+hipError_t launch_dft_matrix(int n, int h, double *w, hipStream_t s)
+{
+    dft_matrix_kernel<<<grid_for((size_t)n * 2 * h), kBlock, 0, s>>>(n, h, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_double(const float *in, size_t count, double *out, hipStream_t s)
+{
+    to_double_kernel<<<grid_for(count), kBlock, 0, s>>>(in, count, out);
     return hipGetLastError();
 }
 
