@@ -274,7 +274,7 @@ struct tt_context {
     DevBuf dft_w, p3_x, p3_spec;
     DevBuf bw_sino, bw_dsino, bw_z, bw_rec, bw_t6src, bw_t7src;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
-    DevBuf nos_w, nos_m, nos_q, nos_r, nos_dev, nos_sp, nos_ptrs;
+    DevBuf nos_w, nos_m, nos_q, nos_r, nos_dev, nos_sp, nos_ptrs, nos_h;
     DevBuf sort_qa, sort_qb, sort_count, bbox;
     rocblas_handle blas = nullptr;
     int trig_A = -1;
@@ -442,7 +442,7 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
                       &ctx->nos_v, &ctx->nos_s, &ctx->nos_e, &ctx->nos_c,
                       &ctx->nos_info, &ctx->nos_sweeps, &ctx->nos_w, &ctx->nos_m,
                       &ctx->nos_q, &ctx->nos_r, &ctx->nos_dev, &ctx->nos_sp,
-                      &ctx->nos_ptrs, &ctx->sort_qa,
+                      &ctx->nos_ptrs, &ctx->nos_h, &ctx->sort_qa,
                       &ctx->sort_qb, &ctx->sort_count, &ctx->bbox};
     for (DevBuf *b : bufs)
         b->release();
@@ -1291,6 +1291,9 @@ static const ttk::PolarStep kPolarSteps[] = {
     {{2.4609375, -3.28125, 2.953125, -1.40625, 0.2734375}},
 };
 constexpr int kPolarStepCount = (int)(sizeof(kPolarSteps) / sizeof(kPolarSteps[0]));
+// steps that update X itself; the others run on X^T X (every singular value
+// of X is above ~1e-3 of the largest by then)
+constexpr int kPolarTallSteps = 6;
 
 TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
                                                   const float *d_sino, int n,
@@ -1360,6 +1363,7 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
     TT_HIP(ctx->nos_sweeps.ensure(sizeof(rocblas_int) * n_sino));
     TT_HIP(ctx->nos_q.ensure(sizeof(double) * AA * n_sino));
     TT_HIP(ctx->nos_sp.ensure(sizeof(double) * AA * n_sino * kGramChunks));
+    TT_HIP(ctx->nos_h.ensure(sizeof(double) * AA * n_sino * 4));
     TT_HIP(ctx->nos_r.ensure(sizeof(double) * 2 * n_sino));
     TT_HIP(ctx->nos_dev.ensure(sizeof(double) * ttk::kNosSlices * n_sino));
     TT_HIP(ctx->timer.begin(TT_KERNEL_NOS, st));
@@ -1412,7 +1416,8 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
     // 1 while the singular vectors stay those of A, so X converges to
     // U V^T.  Rounding in each step perturbs the factor by about
     // eps * kappa(A) relative (the reference's FP32 Jacobi SVD carries
-    // about 6e-8 * kappa(A)).  Convergence is checked on X^T X; a matrix
+    // about 6e-8 * kappa(A)).  Convergence is checked on X^T X (the last
+    // Gram-space S, equal to it up to rounding); a matrix
     // beyond the schedule's reach (kappa above ~1e8, rank deficiency) falls
     // back to the eigendecomposition route below.
     double *result = nullptr;
@@ -1424,25 +1429,57 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
         const double *x = a_d;
         TT_TRY(gram(a_d, S));
         TT_HIP(ttk::launch_nos_scales(S, A, n_sino, part, scale, r, st));
-        for (int step = 0; step < kPolarStepCount; step++) {
-            if (step > 0)
-                TT_TRY(gram(x, S));
-            TT_BLAS(rocblas_dgemm_strided_batched(
-                ctx->blas, rocblas_operation_none, rocblas_operation_none, A, A, A, &one, S, A,
-                sG, S, A, sG, &zero, S2, A, sG, n_sino));
+        // C = P Q for batches of A x A matrices
+        auto mul = [&](const double *P, const double *Qm, double beta, double *C) {
+            return rocblas_dgemm_strided_batched(ctx->blas, rocblas_operation_none,
+                                                 rocblas_operation_none, A, A, A, &one, P, A,
+                                                 sG, Qm, A, sG, &beta, C, A, sG, n_sino);
+        };
+        // Q = q(S) of the step (the first step scales by s and r)
+        auto poly = [&](int step) -> tt_status {
+            TT_BLAS(mul(S, S, 0.0, S2));
             TT_HIP(ttk::launch_nos_poly(S, S2, A, n_sino, kPolarSteps[step],
                                         step == 0 ? scale : nullptr, step == 0 ? r : nullptr,
                                         T, Q, st));
-            TT_BLAS(rocblas_dgemm_strided_batched(
-                ctx->blas, rocblas_operation_none, rocblas_operation_none, A, A, A, &one, S2, A,
-                sG, T, A, sG, &one, Q, A, sG, n_sino));
-            TT_BLAS(rocblas_dgemm_strided_batched(
-                ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A, &one,
-                x, rows_max, sA, Q, A, sG, &zero, xa, rows_max, sA, n_sino));
-            x = xa;
-            std::swap(xa, xb);
+            TT_BLAS(mul(S2, T, 1.0, Q));
+            return TT_OK;
+        };
+        // The first kPolarTallSteps steps update X itself; once X is well
+        // conditioned (every singular value above ~1e-3 of the largest), the
+        // rest run in the Gram space, exact in exact arithmetic as q(S)
+        // commutes with S: S <- S q(S)^2 and M <- M q(S), and X <- X M once
+        // at the end.  Forming S squares only X's condition number at the
+        // switch, so the factor stays at the eps * kappa(A) level.
+        double *M = ctx->nos_h.as<double>(), *Mn = M + AA * n_sino;
+        double *Q2 = M + 2 * AA * n_sino, *Sn = M + 3 * AA * n_sino;
+        for (int step = 0; step < kPolarStepCount; step++) {
+            if (step > 0 && step <= kPolarTallSteps)
+                TT_TRY(gram(x, S));
+            TT_TRY(poly(step));
+            if (step < kPolarTallSteps) {
+                TT_BLAS(rocblas_dgemm_strided_batched(
+                    ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A,
+                    &one, x, rows_max, sA, Q, A, sG, &zero, xa, rows_max, sA, n_sino));
+                x = xa;
+                std::swap(xa, xb);
+                continue;
+            }
+            if (step == kPolarTallSteps) {
+                std::swap(M, Q);  // M = Q (Q gets M's buffer)
+            } else {
+                TT_BLAS(mul(M, Q, 0.0, Mn));
+                std::swap(M, Mn);
+            }
+            const double *Qs = step == kPolarTallSteps ? M : Q;
+            TT_BLAS(mul(Qs, Qs, 0.0, Q2));
+            TT_BLAS(mul(S, Q2, 0.0, Sn));
+            std::swap(S, Sn);
         }
-        TT_TRY(gram(x, S));
+        TT_BLAS(rocblas_dgemm_strided_batched(
+            ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A, &one, x,
+            rows_max, sA, M, A, sG, &zero, xa, rows_max, sA, n_sino));
+        x = xa;
+        // convergence: S (= X^T X up to rounding) against I
         TT_HIP(ttk::launch_nos_deviation(S, A, n_sino, part, st));
         std::vector<double> dev((size_t)ttk::kNosSlices * n_sino);
         TT_HIP(hipMemcpyAsync(dev.data(), part, sizeof(double) * dev.size(),
