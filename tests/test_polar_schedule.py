@@ -5,8 +5,9 @@ The library runs X <- X q(X^T X) with the table's q; these tests read the
 table from the source and check what the schedule promises, independently
 of the GPU: the scalar map drives every singular value in [1e-8, 1.02] to 1
 without crossing 0 (a crossing would flip a sign of the polar factor), and
-the same matrix iteration in numpy float64 reproduces numpy's SVD polar
-factor U V^T for an ill-conditioned matrix (kappa 3e6, the Radon sinogram
+the same matrix iteration in numpy float64 (the first kPolarTallSteps steps
+on X, the rest on X^T X, as the library runs them) reproduces numpy's SVD
+polar factor U V^T for an ill-conditioned matrix (kappa 3e6, the Radon sinogram
 of the 1024 blob workload is 3.8e6).
 """
 import os
@@ -24,6 +25,10 @@ def table():
     body = re.search(r"kPolarSteps\[\] = \{(.*?)\n\};", text, re.S).group(1)
     rows = re.findall(r"\{\{([^}]*)\}\}", body)
     return [np.array([float(v) for v in r.split(",")]) for r in rows]
+
+
+def tall_steps():
+    return int(re.search(r"kPolarTallSteps = (\d+);", open(SRC).read()).group(1))
 
 
 def p(c, x):
@@ -63,12 +68,24 @@ def test_matrix_iteration_matches_svd_polar(kappa):
     a = (u * s) @ v.T
     g = a.T @ a
     x = a / np.sqrt(np.linalg.norm(g))         # X_0 = A ||A^T A||_F^(-1/2)
-    for c in table():
-        sm = x.T @ x
+
+    def q_of(c, sm):
         s2 = sm @ sm
-        q = (c[0] * np.eye(n) + c[1] * sm + c[2] * s2) + s2 @ (c[3] * sm + c[4] * s2)
-        x = x @ q
-    assert np.abs(x.T @ x - np.eye(n)).max() <= 1e-10   # the library's test
+        return (c[0] * np.eye(n) + c[1] * sm + c[2] * s2) + s2 @ (c[3] * sm + c[4] * s2)
+
+    # as the library: the first steps on X, the rest on S = X^T X
+    steps = table()
+    t = tall_steps()
+    for c in steps[:t]:
+        x = x @ q_of(c, x.T @ x)
+    sm, m_acc = x.T @ x, np.eye(n)
+    for c in steps[t:]:
+        q = q_of(c, sm)
+        m_acc = m_acc @ q
+        sm = sm @ (q @ q)
+    x = x @ m_acc
+    assert np.abs(sm - np.eye(n)).max() <= 1e-10       # the library's test
+    assert np.abs(x.T @ x - np.eye(n)).max() <= 1e-10
     ref = u @ v.T
     err = np.linalg.norm(x - ref) / np.linalg.norm(ref)
     # rounding perturbs the factor by about eps * kappa
