@@ -116,7 +116,8 @@ struct SortTParams {
     int32_t *t7_src;      // T7 source row (stable order of the tail), or null
     const int *bbox;      // non-zero pixels' col/row bounds {x0, x1, y0, y1}, or null
     // (line, which) queues: a pass reads defer_in (all lines when NULL) and
-    // queues the entries it cannot hold in defer_out (2 * A * n entries each)
+    // queues the line items (line << 2 | T6/T7 mask) it cannot hold in
+    // defer_out (room for A * n items each)
     const int32_t *defer_in;
     const unsigned int *defer_in_count;
     int32_t *defer_out;

@@ -1366,7 +1366,12 @@ __device__ __forceinline__ void sort_prefix(Key (&k)[KMAX], int K, int lane)
             return;
         }
     if constexpr (KMAX >= 8)
-        wave_bitonic_sort<8>(reinterpret_cast<Key(&)[8]>(k), lane);
+        if (K <= 8 || KMAX == 8) {
+            wave_bitonic_sort<8>(reinterpret_cast<Key(&)[8]>(k), lane);
+            return;
+        }
+    if constexpr (KMAX >= 16)
+        wave_bitonic_sort<16>(reinterpret_cast<Key(&)[16]>(k), lane);
 }
 
 // The largest key of a tail whose first Ls samples are in xs (the rest of
@@ -1613,29 +1618,29 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
 }
 
 // Persistent waves take work items from the counter p.work (zeroed ahead of
-// the launch), p.chunk at a time: a line (T6 then T7) or, with p.defer_in
-// set, a queued (line, which) entry.  One counter serialises its atomics
-// across the device (tens of nanoseconds each), so the cheap items of the
-// first pass go in chunks of several lines.  With DEFER, entries whose
-// sorted part needs more than KMAX keys per lane are queued in p.defer_out
-// for a wider pass (a wide network's VGPR budget would otherwise set the
-// occupancy of every wave); a wave collects a chunk's deferrals in one
-// register (lane i holds the i-th) and queues them with one atomic.
+// the launch), p.chunk at a time: a line (T6 and T7) or, with p.defer_in
+// set, a queued line item (line << 2 | mask, mask bit 0 for T6 and bit 1
+// for T7).  One counter serialises its atomics across the device (tens of
+// nanoseconds each), so the cheap items of the first pass go in chunks of
+// several lines.  With DEFER, tails whose sorted part needs more than KMAX
+// keys per lane are queued in p.defer_out for a wider pass (a wide network's
+// VGPR budget would otherwise set the occupancy of every wave); a wave
+// collects a chunk's queued items in one register (lane i holds the i-th)
+// and queues them with one atomic.  Kernels of at most 16 keys per lane take
+// the two tails of a line together when both are requested.
 template <int KMAX, bool DEFER, bool IDX7>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KMAX <= 8 ? 5 : KMAX <= 16 ? 4 : 1)))
 void sino_sort_kernel(SortTParams p)
 {
     constexpr int M = kWave * KMAX;
-    // first-pass kernels (at most 8 keys per lane) take T6 and T7 of a line
-    // together and hold two slices per wave
-    constexpr bool PAIR = KMAX <= 8;
+    // pairing kernels hold two slices per wave
+    constexpr bool PAIR = KMAX <= 16;
     constexpr int W = PAIR ? 2 * M : M;
     __shared__ __align__(16) float s_all[(kBlock / kWave) * W];
     const int lane = threadIdx.x & (kWave - 1);
     float *xs = s_all + wave_id() * W;
-    const bool pair = PAIR && !p.defer_in && p.slot6 >= 0 && p.slot7 >= 0;
     const int total = p.defer_in ? (int)*p.defer_in_count : p.A * p.n;
-    const int chunk = p.chunk;  // at most 32: two entries per line fit a wave
+    const int chunk = p.chunk;  // at most 64: one queued item per line fits a wave
     for (;;) {
         int j0 = 0;
         if (lane == 0)
@@ -1646,29 +1651,28 @@ void sino_sort_kernel(SortTParams p)
         const int j1 = j0 + chunk < total ? j0 + chunk : total;
         int queued = 0, nq = 0;
         for (int j = j0; j < j1; j++) {
-            if (pair) {
-                const int d = sort_pair<KMAX, DEFER, IDX7>(p, j, xs, xs + M, lane);
-                for (int which = 0; which < 2; which++)
-                    if ((d >> which) & 1) {
-                        queued = lane == nq ? j * 2 + which : queued;
-                        nq++;
-                    }
+            int line = j, mask = 3;
+            if (p.defer_in) {
+                const int e = p.defer_in[j];
+                line = e >> 2;
+                mask = e & 3;
+            }
+            mask &= (p.slot6 >= 0 ? 1 : 0) | (p.slot7 >= 0 ? 2 : 0);
+            int d = 0;
+            if (PAIR && mask == 3) {
+                d = sort_pair<KMAX, DEFER, IDX7>(p, line, xs, xs + M, lane);
                 wave_lds_sync();
-            } else if (!p.defer_in) {
+            } else {
                 for (int which = 0; which < 2; which++) {
-                    if (sort_line<KMAX, DEFER, IDX7>(p, j, which, xs, lane)) {
-                        queued = lane == nq ? j * 2 + which : queued;
-                        nq++;
-                    }
+                    if (((mask >> which) & 1) &&
+                        sort_line<KMAX, DEFER, IDX7>(p, line, which, xs, lane))
+                        d |= 1 << which;
                     wave_lds_sync();
                 }
-            } else {
-                const int e = p.defer_in[j];
-                if (sort_line<KMAX, DEFER, IDX7>(p, e >> 1, e & 1, xs, lane)) {
-                    queued = lane == nq ? e : queued;
-                    nq++;
-                }
-                wave_lds_sync();
+            }
+            if (d) {
+                queued = lane == nq ? (line << 2) | d : queued;
+                nq++;
             }
         }
         if (DEFER && nq > 0) {
