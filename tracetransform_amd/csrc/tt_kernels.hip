@@ -1487,7 +1487,7 @@ __device__ __forceinline__ bool sort_line(const SortTParams &p, int line, int wh
 // tails [sm, n) and [m, n) share their rows from max(sm, m) on, so the wave
 // samples the union [min(sm, m), E) once into xs (E: the row after the
 // line's last possibly non-zero row), sorts both tails' keys, writes the T6
-// roots over xs and the T7 roots into xs2, and runs the two Eigen sums on
+// roots into xs2 and the T7 roots over xs, and runs the two Eigen sums on
 // lanes 0..7 and 8..15 and the two walks on lanes 0 and 1 together.
 // Returns the entries left for a wider pass (bit 0: T6, bit 1: T7; nothing
 // written for them).  A pair whose union exceeds the slice goes entry by
@@ -1562,48 +1562,48 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
     }
     using Key7 = TailKey<IDX7>;
     const int K6 = keys_per_lane<KMAX>(Le6), K7 = keys_per_lane<KMAX>(Le7);
+    // T6 first: keys, sort, its samples gathered by index, roots into xs2
+    // (the samples in xs stay for T7's keys, so the two key sets are never
+    // both being sorted)
     uint64_t key6[KMAX];
-    Key7 key7[KMAX];
-    int zin6 = 0, zin7 = 0;
+    int zin6 = 0;
 #pragma unroll
     for (int r = 0; r < KMAX; r++) {
         const int i = r * kWave + lane;  // < 64 * K: within the first K slots
         key6[r] = i < Le6 ? tail_key<true, false>(xs + o6, i, &zin6) : ~0ull;
+    }
+    const int ins6 = wave_sum_i(zin6);
+    sort_prefix<KMAX>(key6, K6, lane);
+#pragma unroll
+    for (int r = 0; r < KMAX; r++) {
+        const int p6 = lane * K6 + r;
+        if (r < K6 && p6 < Le6)
+            xs2[p6] = sqrtf(xs[o6 + (uint32_t)key6[r]]);
+    }
+    // T7: keys, sort, roots from the key values over xs (after every lane
+    // has read its samples)
+    Key7 key7[KMAX];
+    int zin7 = 0;
+#pragma unroll
+    for (int r = 0; r < KMAX; r++) {
+        const int i = r * kWave + lane;
         key7[r] = i < Le7 ? tail_key<false, IDX7>(xs + o7, i, &zin7) : (Key7)~(Key7)0;
     }
-    const int ins6 = wave_sum_i(zin6), ins7 = IDX7 ? wave_sum_i(zin7) : 0;
-    // T6: sort, gather its samples by index, then write the roots over xs
-    // (both key sets are built, so the samples are no longer needed)
-    sort_prefix<KMAX>(key6, K6, lane);
-    {
-        float sq6[KMAX];
-#pragma unroll
-        for (int r = 0; r < KMAX; r++) {
-            const int p6 = lane * K6 + r;
-            sq6[r] = (r < K6 && p6 < Le6) ? sqrtf(xs[o6 + (uint32_t)key6[r]]) : 0.0f;
-        }
-        wave_lds_sync();
-#pragma unroll
-        for (int r = 0; r < KMAX; r++) {
-            const int p6 = lane * K6 + r;
-            if (r < K6 && p6 < Le6)
-                xs[p6] = sq6[r];
-        }
-    }
-    // T7: sort, roots from the key values into xs2
+    const int ins7 = IDX7 ? wave_sum_i(zin7) : 0;
     sort_prefix<KMAX>(key7, K7, lane);
+    wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < KMAX; r++) {
         const int p7 = lane * K7 + r;
         if (r < K7 && p7 < Le7) {
             if constexpr (IDX7)
-                xs2[p7] = sqrtf(ord_val((uint32_t)(key7[r] >> 32)));
+                xs[p7] = sqrtf(ord_val((uint32_t)(key7[r] >> 32)));
             else
-                xs2[p7] = sqrtf(ord_val(key7[r]));
+                xs[p7] = sqrtf(ord_val(key7[r]));
         }
     }
     wave_lds_sync();
-    const TailView v6 = {xs, ins6, L6 - Le6, L6}, v7 = {xs2, ins7, L7 - Le7, L7};
+    const TailView v6 = {xs2, ins6, L6 - Le6, L6}, v7 = {xs, ins7, L7 - Le7, L7};
     const float sg = eigen_sum_group(lane >= 8 ? v7 : v6, lane, lane < 16);
     const float S6 = __shfl(sg, 0, kWave), S7 = __shfl(sg, 8, kWave);
     int kw = 0;
