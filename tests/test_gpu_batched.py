@@ -3,8 +3,10 @@ per stage over a batch of uneven padded images, captured into a HIP graph.
 
 Bar: sinograms equal the single-image path bit for bit (the same line
 function runs on the same inputs); P1/P2 circus functions equal too; P3
-within 1e-6 normwise (its twiddles come from double sincospi in the batch,
-from a host table in the single-image path).  A few images are also checked
+within 5e-6 normwise (the batch forms each column's spectrum by a float DFT
+from float twiddles, error ~1e-6 relative; the single-image path by an FP64
+GEMM with the DFT matrix; both are held to 3e-5 against the FP64 oracle
+elsewhere).  A few images are also checked
 against the oracle's medians via the single-image path's aux outputs.
 Graph replays are checked against the eager path after the inputs change.
 """
@@ -48,7 +50,7 @@ def check_against_eager(batch, A, step, codes, pf):
         cb, ce = circs[k].cpu().numpy(), c.cpu().numpy()
         for q, p in enumerate(pf):
             if p[0] == 3:
-                assert normwise_err(cb[:, q], ce[:, q]) <= 1e-6, (k, q)
+                assert normwise_err(cb[:, q], ce[:, q]) <= 5e-6, (k, q)
             else:
                 assert np.array_equal(cb[:, q], ce[:, q]), (k, q)
 

@@ -61,16 +61,19 @@ __device__ __forceinline__ float rot_sample(const float *__restrict__ src,
     float y = pxns + py * c;
     x = x + o;
     y = y + o;
-    // bitwise, not short-circuit: keeps the test branch-free
-    const bool inb = (x >= 0.0f) & (x < lim) & (y >= 0.0f) & (y < lim);
-    const float xi = __builtin_truncf(x);
-    const float yi = __builtin_truncf(y);
-    const float fx = x - xi;
-    const float fy = y - yi;
+    // 0 <= x < lim as one unsigned compare of the bit patterns: x is never
+    // -0 (a sum with o > 0 rounds an exact zero to +0), non-negative floats
+    // order as their bits, and negative values and NaN compare above lim
+    const bool inb = (__float_as_uint(x) < __float_as_uint(lim)) &
+                     (__float_as_uint(y) < __float_as_uint(lim));
+    // for 0 <= x the integer part is the truncating conversion and the
+    // fraction x - floor(x) is exact (modf's split)
+    const float fx = __builtin_amdgcn_fractf(x);
+    const float fy = __builtin_amdgcn_fractf(y);
     // unsigned 32-bit offsets from a wave-uniform base: global loads in
     // scalar-base + vector-offset form, no 64-bit address arithmetic
-    const uint32_t X = inb ? (uint32_t)xi : 0u;
-    const uint32_t Y = inb ? (uint32_t)yi : 0u;
+    const uint32_t X = inb ? (uint32_t)x : 0u;
+    const uint32_t Y = inb ? (uint32_t)y : 0u;
     const uint32_t un = (uint32_t)n;
     float s00, s01, s10, s11;
     if (!TR) {
