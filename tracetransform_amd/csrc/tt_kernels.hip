@@ -76,13 +76,14 @@ __device__ __forceinline__ float rot_sample(const float *__restrict__ src,
     const uint32_t Y = inb ? (uint32_t)y : 0u;
     const uint32_t un = (uint32_t)n;
     float s00, s01, s10, s11;
+    // X, Y < n <= 2730, so X * n fits the 24-bit multiplier
     if (!TR) {
-        const uint32_t o0 = (X * un + Y) * 4u;
+        const uint32_t o0 = (__umul24(X, un) + Y) * 4u;
         const f2u a = ld2b(src, o0);
         const f2u b = ld2b(src, o0 + 4u * un);
         s00 = a.x; s10 = a.y; s01 = b.x; s11 = b.y;
     } else {
-        const uint32_t o0 = (Y * un + X) * 4u;
+        const uint32_t o0 = (__umul24(Y, un) + X) * 4u;
         const f2u a = ld2b(src, o0);
         const f2u b = ld2b(src, o0 + 4u * un);
         s00 = a.x; s01 = a.y; s10 = b.x; s11 = b.y;
@@ -205,11 +206,14 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
         rs[j] = 0.0f;
     }
     const int a8_end = a8 < R1 ? a8 : R1;
+    // row coordinates py = row - o are integers or half-integers far below
+    // 2^23: stepping them by whole rows is exact
     for (int r0 = R0 & ~7; r0 < a8_end; r0 += 8) {
+        const float py0 = (float)r0 - o;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s,
-                                           (float)(r0 + j) - o);
+                                           py0 + (float)j);
             rx[j] += v;
             if (NEED_S)
                 rs[j] += sqrtf(v);
@@ -288,9 +292,9 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
         const int re = rb + 64 < R1 ? rb + 64 : R1;
         float f1 = 0.0f, f2 = 0.0f;
         float f3r = 0.0f, f3i = 0.0f, f4r = 0.0f, f4i = 0.0f, f5r = 0.0f, f5i = 0.0f;
-        for (int r = rb; r < re; r++) {
-            const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s,
-                                           (float)r - o);
+        float py = (float)rb - o;
+        for (int r = rb; r < re; r++, py += 1.0f) {
+            const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s, py);
             if (NEED_X) {
                 if (m < 0) {
                     Px += v;
@@ -1459,15 +1463,22 @@ __device__ __forceinline__ bool sort_line(const SortTParams &p, int line, int wh
     // consecutive lanes take consecutive rows (neighbouring taps)
     int last_nz = -1;
     bool bad = false;
-    for (int i = lane; i < Ls; i += kWave) {
-        const float pyo = (float)(base + i) - o;
-        const float x = tr ? rot_sample<true>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo)
-                           : rot_sample<false>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo);
-        if (i < M)  // a longer sorted part is queued (DEFER) or cannot occur
-            xs[i] = x;
-        last_nz = x != 0.0f ? i : last_nz;
-        bad |= !(x >= 0.0f);  // negative or NaN
-    }
+    // (a longer sampled part is queued before sampling, or cannot occur)
+    auto sample_tail = [&](auto trc) {
+        constexpr bool TRC = decltype(trc)::value;
+        float pyo = (float)(base + lane) - o;  // exact steps of 64 rows
+        for (int i = lane; i < Ls; i += kWave, pyo += (float)kWave) {
+            const float x = rot_sample<TRC>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo);
+            if (i < M)
+                xs[i] = x;
+            last_nz = x != 0.0f ? i : last_nz;
+            bad |= !(x >= 0.0f);  // negative or NaN
+        }
+    };
+    if (tr)
+        sample_tail(std::true_type{});
+    else
+        sample_tail(std::false_type{});
     int k = L - 1;
     uint64_t res;
     if (__any(bad)) {
@@ -1531,14 +1542,20 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
         return deferred;
     }
     int last_nz = -1, last_bad = -1;
-    for (int i = lane; i < Lu; i += kWave) {
-        const float pyo = (float)(b0 + i) - o;
-        const float x = tr ? rot_sample<true>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo)
-                           : rot_sample<false>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo);
-        xs[i] = x;
-        last_nz = x != 0.0f ? i : last_nz;
-        last_bad = !(x >= 0.0f) ? i : last_bad;  // negative or NaN
-    }
+    auto sample_union = [&](auto trc) {
+        constexpr bool TRC = decltype(trc)::value;
+        float pyo = (float)(b0 + lane) - o;  // exact steps of 64 rows
+        for (int i = lane; i < Lu; i += kWave, pyo += (float)kWave) {
+            const float x = rot_sample<TRC>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo);
+            xs[i] = x;
+            last_nz = x != 0.0f ? i : last_nz;
+            last_bad = !(x >= 0.0f) ? i : last_bad;  // negative or NaN
+        }
+    };
+    if (tr)
+        sample_union(std::true_type{});
+    else
+        sample_union(std::false_type{});
     const int o6 = sm - b0, o7 = m - b0;
     const int lnz = wave_max_i(last_nz), lbad = wave_max_i(last_bad);
     const int Le6 = lnz + 1 - o6 > 0 ? lnz + 1 - o6 : 0;
