@@ -1821,8 +1821,8 @@ __device__ __forceinline__ float record_g(const LineRecord &r, int row, const fl
 // the line record, into LDS.  Each pixel then visits the rotated samples
 // within sqrt(2) of its inverse image (those whose 2x2 footprint can cover
 // it) column by column, keeps those whose taps include it and accumulates
-// G * weight in double.  The visiting order is fixed, so the result is
-// deterministic and needs no atomics.
+// G * weight (an angle's hits in float, the angles in double).  The visiting
+// order is fixed, so the result is deterministic and needs no atomics.
 constexpr int kTile = 16;
 constexpr int kBoxCap = 32;  // box side bound: a tile's candidates span <= 25
 
@@ -1891,19 +1891,22 @@ __device__ __forceinline__ float gather_tile(const LineRecord *recs, const float
             float y = px * (-s) + py * c;
             x = x + o;
             y = y + o;
-            const bool inb = (x >= 0.0f) & (x < lim) & (y >= 0.0f) & (y < lim);
-            const float xi = __builtin_truncf(x), yi = __builtin_truncf(y);
+            // bounds, taps and fractions as rot_sample computes them
+            const bool inb = (__float_as_uint(x) < __float_as_uint(lim)) &
+                             (__float_as_uint(y) < __float_as_uint(lim));
             float4 e;
-            e.x = __int_as_float(inb ? ((int)xi | ((int)yi << 16)) : -1);
+            e.x = __int_as_float(inb ? ((int)(uint32_t)x | ((int)(uint32_t)y << 16)) : -1);
             e.y = record_g(L.rec[col - bc0], row, s_tab, tabn, do345);
-            e.z = x - xi;
-            e.w = y - yi;
+            e.z = __builtin_amdgcn_fractf(x);
+            e.w = __builtin_amdgcn_fractf(y);
             L.smp[j] = e;
         }
         __syncthreads();
         if (valid) {
             // the pixel's candidates, column by column (as the samples were
-            // laid out); those outside [c0, c1] x [r0, r1] are skipped
+            // laid out); those outside [c0, c1] x [r0, r1] are skipped.  An
+            // angle's few hits are summed in float, the angles in double.
+            float part = 0.0f;
 #pragma unroll
             for (int dc = 0; dc < 3; dc++) {
                 const int col = c0 + dc;
@@ -1923,10 +1926,10 @@ __device__ __forceinline__ float gather_tile(const LineRecord *recs, const float
                                      ((Yi == Y) | (Yi == Y - 1));
                     const float wx = Xi == X ? 1.0f - e.z : e.z;
                     const float wy = Yi == Y ? 1.0f - e.w : e.w;
-                    const double t = fma((double)e.y, (double)wx * (double)wy, acc);
-                    acc = hit ? t : acc;
+                    part = fmaf(e.y, hit ? wx * wy : 0.0f, part);
                 }
             }
+            acc += (double)part;
         }
         __syncthreads();
     }
