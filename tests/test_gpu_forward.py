@@ -230,6 +230,17 @@ def test_circus_sizes(oracle, gpu):
         check_circus(oracle, sino, [(1,), (2,), (3,), (0, 3, n // 3)])
 
 
+def test_hermite_tables_follow_height_and_centre(oracle, gpu):
+    # the same (order, centre) at two heights, then alternating centres: the
+    # weight tables depend on the height through the z grid, and the cached
+    # tables must follow both
+    rng = np.random.default_rng(14)
+    cases = [(91, 30), (121, 30), (91, 30), (121, 40), (121, 30)]
+    for n, c in cases:
+        sino = rng.random((20, n), dtype=np.float32)
+        check_circus(oracle, sino, [(0, 2, c), (0, 5, c)])
+
+
 def test_circus_batched_sinograms(oracle, gpu):
     rng = np.random.default_rng(13)
     sino = rng.random((3, 50, 91), dtype=np.float32)

@@ -270,7 +270,7 @@ struct tt_context {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
-    DevBuf imgT, trig, tab345, dft_tw, herm_w, io_a, io_b, med_x, med_s;
+    DevBuf imgT, trig, tab345, dft_tw, io_a, io_b, med_x, med_s;
     DevBuf dft_w, p3_x, p3_spec;
     DevBuf bw_sino, bw_dsino, bw_z, bw_rec, bw_t6src, bw_t7src;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
@@ -280,7 +280,6 @@ struct tt_context {
     int trig_A = -1;
     float trig_step = 0.0f;
     int tab_n = -1, dft_n = -1, dftw_n = -1;
-    std::vector<int32_t> herm_key;
     std::vector<float2> h_trig, h_tab, h_tw;
     std::vector<double> h_herm;
 
@@ -330,22 +329,55 @@ struct tt_context {
         return TT_OK;
     }
 
-    tt_status ensure_hermite(int n, const std::vector<int32_t> &key)
+    // Hermite weight tables of (n, (order, center) list) keys, the most
+    // recently used kHermCache kept on the device: a transform's orthonormal
+    // sinograms have their own heights and centres, so one table per call
+    // would be rebuilt (and the stream synchronised) on every call.
+    static constexpr size_t kHermCache = 16;
+    struct HermEntry {
+        std::vector<int32_t> key;
+        DevBuf buf;
+        uint64_t used = 0;
+    };
+    std::vector<HermEntry> herm_cache;
+    uint64_t herm_clock = 0;
+
+    tt_status ensure_hermite(int n, const std::vector<int32_t> &pairs, const double **out)
     {
-        if (herm_key == key)
-            return TT_OK;
-        const size_t nh = key.size() / 2;
+        std::vector<int32_t> key;
+        key.reserve(pairs.size() + 1);
+        key.push_back(n);
+        key.insert(key.end(), pairs.begin(), pairs.end());
+        for (HermEntry &e : herm_cache)
+            if (e.key == key) {
+                e.used = ++herm_clock;
+                *out = e.buf.as<double>();
+                return TT_OK;
+            }
+        HermEntry *slot = nullptr;
+        if (herm_cache.size() < kHermCache) {
+            herm_cache.emplace_back();
+            slot = &herm_cache.back();
+        } else {
+            slot = &herm_cache[0];
+            for (HermEntry &e : herm_cache)
+                slot = e.used < slot->used ? &e : slot;
+            slot->buf.release();  // hipFree waits for the kernels that read it
+        }
+        const size_t nh = pairs.size() / 2;
         h_herm.assign(nh * (size_t)n, 0.0f);
         for (size_t h = 0; h < nh; h++)
-            make_hermite(n, (unsigned)key[2 * h], key[2 * h + 1],
+            make_hermite(n, (unsigned)pairs[2 * h], pairs[2 * h + 1],
                          h_herm.data() + h * (size_t)n);
-        TT_HIP(herm_w.ensure(sizeof(double) * (nh ? nh : 1) * (size_t)n));
+        TT_HIP(slot->buf.ensure(sizeof(double) * (nh ? nh : 1) * (size_t)n));
         if (nh)
-            TT_HIP(hipMemcpyAsync(herm_w.ptr, h_herm.data(),
+            TT_HIP(hipMemcpyAsync(slot->buf.ptr, h_herm.data(),
                                   sizeof(double) * nh * (size_t)n,
                                   hipMemcpyHostToDevice, stream));
-        TT_HIP(hipStreamSynchronize(stream));
-        herm_key = key;
+        TT_HIP(hipStreamSynchronize(stream));  // h_herm is reused
+        slot->key = std::move(key);
+        slot->used = ++herm_clock;
+        *out = slot->buf.as<double>();
         return TT_OK;
     }
 };
@@ -436,7 +468,7 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->dft_w, &ctx->p3_x, &ctx->p3_spec,
                       &ctx->imgT, &ctx->trig, &ctx->tab345, &ctx->dft_tw,
-                      &ctx->herm_w, &ctx->io_a, &ctx->io_b, &ctx->med_x,
+                      &ctx->io_a, &ctx->io_b, &ctx->med_x,
                       &ctx->med_s, &ctx->bw_sino, &ctx->bw_dsino, &ctx->bw_z,
                       &ctx->bw_rec, &ctx->bw_t6src, &ctx->bw_t7src, &ctx->nos_med, &ctx->nos_a, &ctx->nos_u,
                       &ctx->nos_v, &ctx->nos_s, &ctx->nos_e, &ctx->nos_c,
@@ -446,6 +478,9 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
                       &ctx->sort_qb, &ctx->sort_count, &ctx->bbox};
     for (DevBuf *b : bufs)
         b->release();
+    for (auto &e : ctx->herm_cache)
+        e.buf.release();
+    ctx->herm_cache.clear();
     ctx->timer.release();
     ctx->batch.release();
     ctx->bwd.g.release();
@@ -702,10 +737,8 @@ TT_API tt_status tt_circus(tt_context *ctx, const float *d_sino, int n,
         }
         p3 = p3 || pf[q].code == TT_P3;
     }
-    if (!hkey.empty()) {
-        TT_TRY(ctx->ensure_hermite(n, hkey));
-        p.herm_w = ctx->herm_w.as<double>();
-    }
+    if (!hkey.empty())
+        TT_TRY(ctx->ensure_hermite(n, hkey, &p.herm_w));
     hipStream_t st = ctx->stream;
     const int h = n / 2 + 1;
     const size_t cols = (size_t)n_sino * n_cols;
@@ -1018,7 +1051,7 @@ namespace {
 
 // Hermite weights of the P list into the context (rows in list order).
 tt_status prepare_hermite(tt_context *ctx, const tt_pfunc *pf, int np, int n,
-                          int *herm_row)
+                          int *herm_row, const double **herm_w)
 {
     std::vector<int32_t> hkey;
     for (int q = 0; q < np; q++) {
@@ -1029,8 +1062,9 @@ tt_status prepare_hermite(tt_context *ctx, const tt_pfunc *pf, int np, int n,
             hkey.push_back(pf[q].center);
         }
     }
+    *herm_w = nullptr;
     if (!hkey.empty())
-        TT_TRY(ctx->ensure_hermite(n, hkey));
+        TT_TRY(ctx->ensure_hermite(n, hkey, herm_w));
     return TT_OK;
 }
 
@@ -1048,8 +1082,7 @@ tt_status circus_vjp(tt_context *ctx, const float *d_sino, int n, int n_cols,
     p.np = np;
     for (int q = 0; q < np; q++)
         p.code[q] = pf[q].code;
-    TT_TRY(prepare_hermite(ctx, pf, np, n, p.herm));
-    p.herm_w = ctx->herm_w.as<double>();
+    TT_TRY(prepare_hermite(ctx, pf, np, n, p.herm, &p.herm_w));
     TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS_VJP, ctx->stream));
     TT_HIP(ttk::launch_circus_vjp(p, ctx->stream));
     TT_HIP(ctx->timer.end(ctx->stream));
