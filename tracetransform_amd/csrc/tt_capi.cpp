@@ -275,6 +275,8 @@ struct tt_context {
     DevBuf bw_sino, bw_dsino, bw_z, bw_rec, bw_t6src, bw_t7src;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
     DevBuf nos_w, nos_m, nos_q, nos_r, nos_dev, nos_sp, nos_ptrs, nos_h;
+    double *h_dev = nullptr;  // pinned host copy of the NOS convergence partials
+    size_t h_dev_cap = 0;
     DevBuf sort_qa, sort_qb, sort_count, bbox;
     rocblas_handle blas = nullptr;
     int trig_A = -1;
@@ -481,6 +483,10 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
     for (auto &e : ctx->herm_cache)
         e.buf.release();
     ctx->herm_cache.clear();
+    if (ctx->h_dev)
+        (void)hipHostFree(ctx->h_dev);
+    ctx->h_dev = nullptr;
+    ctx->h_dev_cap = 0;
     ctx->timer.release();
     ctx->batch.release();
     ctx->bwd.g.release();
@@ -1358,8 +1364,7 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
     for (int t = 0; t < n_sino; t++) {
         int mn = 1 << 30, mx = -(1 << 30);
         for (int c = 0; c < A; c++) {
-            int32_t &o = med[(size_t)t * A + c];
-            o -= sc;
+            const int32_t o = med[(size_t)t * A + c] - sc;
             mn = o < mn ? o : mn;
             mx = o > mx ? o : mx;
         }
@@ -1373,8 +1378,6 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
         maxoff[t] = mx;
         rows_max = rows > rows_max ? rows : rows_max;
     }
-    TT_HIP(hipMemcpyAsync(ctx->nos_med.ptr, med.data(), sizeof(int32_t) * cols_all,
-                          hipMemcpyHostToDevice, st));
     // Every aligned matrix is zero-padded to rows_max rows so one batched
     // solve covers them all; zero rows leave the polar factor's other rows
     // unchanged and stay exactly zero in it.  rows_max is rounded up to
@@ -1400,9 +1403,11 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
     TT_HIP(ctx->nos_r.ensure(sizeof(double) * 2 * n_sino));
     TT_HIP(ctx->nos_dev.ensure(sizeof(double) * ttk::kNosSlices * n_sino));
     TT_HIP(ctx->timer.begin(TT_KERNEL_NOS, st));
+    // row r of aligned column c reads row r - (maxoff - (med - sc)) of the
+    // sinogram column: the raw medians with maxoff + sc
     for (int t = 0; t < n_sino; t++)
         TT_HIP(ttk::launch_align(d_sino + (size_t)t * A * n, n, A,
-                                 ctx->nos_med.as<int32_t>() + (size_t)t * A, maxoff[t],
+                                 ctx->nos_med.as<int32_t>() + (size_t)t * A, maxoff[t] + sc,
                                  rows_max, ctx->nos_a.as<double>() + mA * t, st));
     if (!ctx->blas)
         TT_BLAS(rocblas_create_handle(&ctx->blas));
@@ -1514,15 +1519,31 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
         x = xa;
         // convergence: S (= X^T X up to rounding) against I
         TT_HIP(ttk::launch_nos_deviation(S, A, n_sino, part, st));
-        std::vector<double> dev((size_t)ttk::kNosSlices * n_sino);
-        TT_HIP(hipMemcpyAsync(dev.data(), part, sizeof(double) * dev.size(),
+        // the check's partials into pinned host memory and the float output
+        // written ahead of the check: one synchronisation, and none more
+        // unless a matrix falls back
+        const size_t ndev = (size_t)ttk::kNosSlices * n_sino;
+        if (ctx->h_dev_cap < ndev) {
+            if (ctx->h_dev)
+                TT_HIP(hipHostFree(ctx->h_dev));
+            ctx->h_dev = nullptr;
+            ctx->h_dev_cap = 0;
+            TT_HIP(hipHostMalloc((void **)&ctx->h_dev, sizeof(double) * ndev));
+            ctx->h_dev_cap = ndev;
+        }
+        TT_HIP(hipMemcpyAsync(ctx->h_dev, part, sizeof(double) * ndev,
                               hipMemcpyDeviceToHost, st));
+        for (int t = 0; t < n_sino; t++)
+            TT_HIP(ttk::launch_to_float(x + mA * t, rows_max, out_rows[t], A,
+                                        d_out + (size_t)t * out_capacity_rows * A, st));
+        TT_HIP(ctx->timer.end(st));
         TT_HIP(hipStreamSynchronize(st));
         bool converged = true;
-        for (double d : dev)
-            converged = converged && d <= 1e-10;  // false for NaN
+        for (size_t i = 0; i < ndev; i++)
+            converged = converged && ctx->h_dev[i] <= 1e-10;  // false for NaN
         if (converged)
-            result = const_cast<double *>(x);
+            return TT_OK;
+        TT_HIP(ctx->timer.begin(TT_KERNEL_NOS, st));
     }
     std::vector<rocblas_int> info((size_t)n_sino);
     bool gram_ok = true;
