@@ -1207,16 +1207,24 @@ using TailKey = typename std::conditional<IDX, uint64_t, uint32_t>::type;
 // when its source row is asked for (IDX7), else alone (equal values are
 // interchangeable).  Counts the zero keys of the index-carrying kinds.
 template <bool T6, bool IDX7>
-__device__ __forceinline__ TailKey<T6 || IDX7> tail_key(const float *xs, int i, int *zin)
+__device__ __forceinline__ TailKey<T6 || IDX7> tail_key(const float *xs, int i, bool *zero)
 {
     const float x = xs[i];
     const float kv = T6 ? (float)i * x : x;
-    if constexpr (T6 || IDX7) {
-        *zin += kv == 0.0f ? 1 : 0;
+    *zero = kv == 0.0f;
+    if constexpr (T6 || IDX7)
         return ((uint64_t)ord_key(kv) << 32) | (uint32_t)i;
-    } else {
+    else
         return ord_key(kv);
-    }
+}
+
+// Highest lane index whose flag is set plus base, or `prev` when none is:
+// the largest row index of a 64-row step (consecutive lanes on consecutive
+// rows) from one ballot, no cross-lane reduction.
+__device__ __forceinline__ int last_set(bool flag, int base, int prev)
+{
+    const unsigned long long b = __ballot(flag);
+    return b ? base + 63 - __clzll(b) : prev;
 }
 
 // The key at position k of a tail's full sequence (L = Le + zt entries,
@@ -1302,18 +1310,21 @@ __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le,
     constexpr bool IDX = T6 || IDX7;
     using Key = TailKey<IDX>;
     Key key[K];
-    int zin = 0;
+    int zin = 0;  // zero keys of the sorted part (a popcount per slot)
 #pragma unroll
     for (int r = 0; r < K; r++) {
         const int i = r * kWave + lane;  // any placement: the sort orders them
         key[r] = (Key)~(Key)0;
+        bool z = false;
         if (i < Le)
-            key[r] = tail_key<T6, IDX7>(xs, i, &zin);
+            key[r] = tail_key<T6, IDX7>(xs, i, &z);
+        if constexpr (IDX)
+            zin += __popcll(__ballot(z));
     }
     wave_bitonic_sort<K>(key, lane);
     // where the trailing zeros go: after the sorted part's zero keys in the
     // stable (index) order, or first when the order of equal values is free
-    const int ins = IDX ? wave_sum_i(zin) : 0;
+    const int ins = IDX ? zin : 0;
     return tail_finish<K, T6, IDX7>(key, K, ins, xs, L, Le, lane, k_out);
 }
 
@@ -1471,7 +1482,7 @@ __device__ __forceinline__ bool sort_line(const SortTParams &p, int line, int wh
             const float x = rot_sample<TRC>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo);
             if (i < M)
                 xs[i] = x;
-            last_nz = x != 0.0f ? i : last_nz;
+            last_nz = last_set(x != 0.0f, i - lane, last_nz);
             bad |= !(x >= 0.0f);  // negative or NaN
         }
     };
@@ -1485,7 +1496,8 @@ __device__ __forceinline__ bool sort_line(const SortTParams &p, int line, int wh
         wave_lds_sync();
         res = tail_max_key(xs, Ls < M ? Ls : M, L, t6, lane);
     } else {
-        const int Le = wave_max_i(last_nz) + 1;
+        // lane 0 took part in every step of the loop: its copy is complete
+        const int Le = __builtin_amdgcn_readfirstlane(last_nz) + 1;
         const int kneed = (Le + kWave - 1) / kWave;
         if (DEFER && kneed > KMAX)
             return true;
@@ -1548,8 +1560,8 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
         for (int i = lane; i < Lu; i += kWave, pyo += (float)kWave) {
             const float x = rot_sample<TRC>(src, n, lim, o, pxc, pxns, cs.x, cs.y, pyo);
             xs[i] = x;
-            last_nz = x != 0.0f ? i : last_nz;
-            last_bad = !(x >= 0.0f) ? i : last_bad;  // negative or NaN
+            last_nz = last_set(x != 0.0f, i - lane, last_nz);
+            last_bad = last_set(!(x >= 0.0f), i - lane, last_bad);  // negative or NaN
         }
     };
     if (tr)
@@ -1557,7 +1569,9 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
     else
         sample_union(std::false_type{});
     const int o6 = sm - b0, o7 = m - b0;
-    const int lnz = wave_max_i(last_nz), lbad = wave_max_i(last_bad);
+    // lane 0 took part in every step of the loop: its copies are complete
+    const int lnz = __builtin_amdgcn_readfirstlane(last_nz);
+    const int lbad = __builtin_amdgcn_readfirstlane(last_bad);
     const int Le6 = lnz + 1 - o6 > 0 ? lnz + 1 - o6 : 0;
     const int Le7 = lnz + 1 - o7 > 0 ? lnz + 1 - o7 : 0;
     wave_lds_sync();
@@ -1586,13 +1600,16 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
     // (the samples in xs stay for T7's keys, so the two key sets are never
     // both being sorted)
     uint64_t key6[KMAX];
-    int zin6 = 0;
+    int ins6 = 0;  // zero keys (a popcount per slot)
 #pragma unroll
     for (int r = 0; r < KMAX; r++) {
         const int i = r * kWave + lane;  // < 64 * K: within the first K slots
-        key6[r] = i < Le6 ? tail_key<true, false>(xs + o6, i, &zin6) : ~0ull;
+        bool z = false;
+        key6[r] = ~0ull;
+        if (i < Le6)
+            key6[r] = tail_key<true, false>(xs + o6, i, &z);
+        ins6 += __popcll(__ballot(z));
     }
-    const int ins6 = wave_sum_i(zin6);
     sort_prefix<KMAX>(key6, K6, lane);
 #pragma unroll
     for (int r = 0; r < KMAX; r++) {
@@ -1603,13 +1620,17 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
     // T7: keys, sort, roots from the key values over xs (after every lane
     // has read its samples)
     Key7 key7[KMAX];
-    int zin7 = 0;
+    int ins7 = 0;  // zero keys when the order of equal values matters
 #pragma unroll
     for (int r = 0; r < KMAX; r++) {
         const int i = r * kWave + lane;
-        key7[r] = i < Le7 ? tail_key<false, IDX7>(xs + o7, i, &zin7) : (Key7)~(Key7)0;
+        bool z = false;
+        key7[r] = (Key7)~(Key7)0;
+        if (i < Le7)
+            key7[r] = tail_key<false, IDX7>(xs + o7, i, &z);
+        if constexpr (IDX7)
+            ins7 += __popcll(__ballot(z));
     }
-    const int ins7 = IDX7 ? wave_sum_i(zin7) : 0;
     sort_prefix<KMAX>(key7, K7, lane);
     wave_lds_sync();
 #pragma unroll
