@@ -64,8 +64,8 @@ __device__ __forceinline__ float rot_sample(const float *__restrict__ src,
     // 0 <= x < lim as one unsigned compare of the bit patterns: x is never
     // -0 (a sum with o > 0 rounds an exact zero to +0), non-negative floats
     // order as their bits, and negative values and NaN compare above lim
-    const bool inb = (__float_as_uint(x) < __float_as_uint(lim)) &
-                     (__float_as_uint(y) < __float_as_uint(lim));
+    const bool inb = __float_as_uint(x) < __float_as_uint(lim) &&
+                     __float_as_uint(y) < __float_as_uint(lim);
     // for 0 <= x the integer part is the truncating conversion and the
     // fraction x - floor(x) is exact (modf's split)
     const float fx = __builtin_amdgcn_fractf(x);
@@ -1028,14 +1028,6 @@ __device__ __forceinline__ void wave_bitonic_sort(Key (&k)[KPL], int lane)
 }
 
 // Wave-wide reductions.
-__device__ __forceinline__ int wave_sum_i(int v)
-{
-#pragma unroll
-    for (int m = kWave / 2; m > 0; m >>= 1)
-        v += __shfl_xor(v, m, kWave);
-    return v;
-}
-
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v)
 {
 #pragma unroll
@@ -1845,12 +1837,11 @@ __device__ __forceinline__ float record_g(const LineRecord &r, int row, const fl
 // G * weight (an angle's hits in float, the angles in double).  The visiting
 // order is fixed, so the result is deterministic and needs no atomics.
 constexpr int kTile = 16;
-constexpr int kBoxCap = 32;  // box side bound: a tile's candidates span <= 25
+constexpr int kBoxCap = 32;  // box side bound: a tile's box spans <= 29 samples
 
 struct GatherLds {
     LineRecord rec[kBoxCap];
     float4 smp[kBoxCap * kBoxCap];  // (taps X | Y << 16, or -1), G, fx, fy
-    int box[4][kBlock / kWave];
 };
 
 __device__ __forceinline__ float gather_tile(const LineRecord *recs, const float2 *trig,
@@ -1864,7 +1855,6 @@ __device__ __forceinline__ float gather_tile(const LineRecord *recs, const float
     const float lim = (float)(n - 1);
     const float u = (float)X - o, v = (float)Y - o;
     const float R = 1.4163f;
-    const int wv = tid / kWave, lane = tid & (kWave - 1);
     double acc = 0.0;
     for (int a = 0; a < A; a++) {
         const float2 cs = trig[a];
@@ -1879,21 +1869,28 @@ __device__ __forceinline__ float gather_tile(const LineRecord *recs, const float
         r0 = r0 < 0 ? 0 : r0;
         c1 = c1 > n - 1 ? n - 1 : c1;
         r1 = r1 > n - 1 ? n - 1 : r1;
-        int bc0 = wave_min_i(valid ? c0 : n), bc1 = wave_max_i(valid ? c1 : -1);
-        int br0 = wave_min_i(valid ? r0 : n), br1 = wave_max_i(valid ? r1 : -1);
-        if (lane == 0) {
-            L.box[0][wv] = bc0;
-            L.box[1][wv] = bc1;
-            L.box[2][wv] = br0;
-            L.box[3][wv] = br1;
+        // The tile's union of those boxes, from its four corners: the exact
+        // coordinates are linear in the pixel, so the corners bound them,
+        // and the rounding of any pixel's float coordinates (< 1e-3 at
+        // n <= 2048) moves its bounds by at most one sample: a margin of 1
+        // contains every pixel's box (no reduction, no barrier).
+        int bc0 = n, bc1 = -1, br0 = n, br1 = -1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float uu = (float)(X0 + ((k & 1) ? kTile - 1 : 0)) - o;
+            const float vv = (float)(Y0 + ((k & 2) ? kTile - 1 : 0)) - o;
+            const float qx = uu * c - vv * s, qy = uu * s + vv * c;
+            const int k0 = (int)ceilf(qx - R + o) - 1, k1 = (int)floorf(qx + R + o) + 1;
+            const int l0 = (int)ceilf(qy - R + o) - 1, l1 = (int)floorf(qy + R + o) + 1;
+            bc0 = k0 < bc0 ? k0 : bc0;
+            bc1 = k1 > bc1 ? k1 : bc1;
+            br0 = l0 < br0 ? l0 : br0;
+            br1 = l1 > br1 ? l1 : br1;
         }
-        __syncthreads();
-        for (int w = 0; w < kBlock / kWave; w++) {
-            bc0 = L.box[0][w] < bc0 ? L.box[0][w] : bc0;
-            bc1 = L.box[1][w] > bc1 ? L.box[1][w] : bc1;
-            br0 = L.box[2][w] < br0 ? L.box[2][w] : br0;
-            br1 = L.box[3][w] > br1 ? L.box[3][w] : br1;
-        }
+        bc0 = bc0 < 0 ? 0 : bc0;
+        br0 = br0 < 0 ? 0 : br0;
+        bc1 = bc1 > n - 1 ? n - 1 : bc1;
+        br1 = br1 > n - 1 ? n - 1 : br1;
         int bw = bc1 - bc0 + 1, bh = br1 - br0 + 1;
         bw = bw < 0 ? 0 : (bw > kBoxCap ? kBoxCap : bw);  // bounded by geometry
         bh = bh < 0 ? 0 : (bh > kBoxCap ? kBoxCap : bh);
@@ -1913,8 +1910,8 @@ __device__ __forceinline__ float gather_tile(const LineRecord *recs, const float
             x = x + o;
             y = y + o;
             // bounds, taps and fractions as rot_sample computes them
-            const bool inb = (__float_as_uint(x) < __float_as_uint(lim)) &
-                             (__float_as_uint(y) < __float_as_uint(lim));
+            const bool inb = __float_as_uint(x) < __float_as_uint(lim) &&
+                             __float_as_uint(y) < __float_as_uint(lim);
             float4 e;
             e.x = __int_as_float(inb ? ((int)(uint32_t)x | ((int)(uint32_t)y << 16)) : -1);
             e.y = record_g(L.rec[col - bc0], row, s_tab, tabn, do345);
