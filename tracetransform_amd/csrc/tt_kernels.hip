@@ -944,11 +944,23 @@ __device__ void bitonic_lds(K *s, int m)
 }
 
 // 64-bit broadcast from one lane.
-__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src_lane)
+// Values of one lane given a wave-uniform lane index (v_readlane: no LDS
+// round trip, unlike a shuffle).
+__device__ __forceinline__ int lane_int(int v, int l)
 {
-    const int lo = __shfl((int)(uint32_t)v, src_lane, kWave);
-    const int hi = __shfl((int)(uint32_t)(v >> 32), src_lane, kWave);
-    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+    return __builtin_amdgcn_readlane(v, l);
+}
+
+__device__ __forceinline__ float lane_float(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int l)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // 64-bit shuffle (two 32-bit halves).
@@ -1074,15 +1086,16 @@ __device__ __forceinline__ float chain_sum(const float *c, int i0, int i1, float
 }
 
 // Eigen packet-order sum of a TailView's full sequence (as eigen_sum_block),
-// for up to 8 views at once: lanes 8g .. 8g + 7 run the stride-8 chains of
-// the view they pass (group g), lane 8g combines them; the sum of group g is
-// returned on lane 8g (other lanes: unspecified).  Lanes of no group pass
-// active = false.  Adding the zero run's +0 leaves a chain of non-negative
+// for G = 1 or 2 views at once: lanes 8g .. 8g + 7 run the stride-8 chains
+// of the view they pass (group g), lane 8g combines them; the sum of group g
+// is returned on lane 8g (other lanes: unspecified).  Adding the zero run's +0 leaves a chain of non-negative
 // values unchanged, so chain r sums the compact entries before the run
 // (positions r, r + 8, ... below ins) and after it (positions congruent to r
 // from ins + zt on, entry = position - zt) and skips the run.
-__device__ __forceinline__ float eigen_sum_group(const TailView &v, int lane, bool active)
+template <int G>
+__device__ __forceinline__ float eigen_sum_group(const TailView &v, int lane)
 {
+    const bool active = lane < 8 * G;
     const int n = v.L;
     const int a4 = n & ~3, a8 = n & ~7;
     const int ch_i = lane & 7;
@@ -1097,8 +1110,11 @@ __device__ __forceinline__ float eigen_sum_group(const TailView &v, int lane, bo
     }
     float ch[8];
 #pragma unroll
-    for (int j = 0; j < 8; j++)
-        ch[j] = __shfl(r, (lane & ~7) + j, kWave);
+    for (int j = 0; j < 8; j++) {
+        ch[j] = lane_float(r, j);
+        if (G == 2 && lane >= 8)
+            ch[j] = lane_float(r, 8 + j);
+    }
     float res = 0.0f;
     if (active && ch_i == 0) {
         if (a4 == 0) {
@@ -1123,7 +1139,7 @@ __device__ __forceinline__ float eigen_sum_group(const TailView &v, int lane, bo
 // The same for one view (lanes 0..7), the sum returned to every lane.
 __device__ __forceinline__ float eigen_sum_view(const TailView &v, int lane)
 {
-    return __shfl(eigen_sum_group(v, lane, lane < 8), 0, kWave);
+    return lane_float(eigen_sum_group<1>(v, lane), 0);
 }
 
 // findWeightedMedian's sequential walk over a TailView (one lane).  Adding
@@ -1236,9 +1252,9 @@ __device__ __forceinline__ uint64_t tail_key_at(const TailKey<IDX> (&key)[KR], i
     for (int r = 0; r < KR; r++)
         kk = (r == pr) ? key[r] : kk;
     if constexpr (IDX)
-        return shfl64(kk, pos / KO);
+        return lane_u64(kk, pos / KO);
     else  // value key in the high word (no index asked for)
-        return (uint64_t)(uint32_t)__shfl((int)kk, pos / KO, kWave) << 32;
+        return (uint64_t)(uint32_t)lane_int((int)kk, pos / KO) << 32;
 }
 
 // Second half of a sorted tail's median: the sorted part's keys are in
@@ -1282,7 +1298,7 @@ __device__ __forceinline__ uint64_t tail_finish(const TailKey<T6 || IDX7> (&key)
     int k = 0;
     if (lane == 0)
         k = weighted_median_walk_view(view, Le, S);
-    k = __shfl(k, 0, kWave);
+    k = lane_int(k, 0);
     *k_out = k;
     return tail_key_at<KR, IDX>(key, KO, k, ins, zt, Le);
 }
@@ -1637,12 +1653,12 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
     }
     wave_lds_sync();
     const TailView v6 = {xs2, ins6, L6 - Le6, L6}, v7 = {xs, ins7, L7 - Le7, L7};
-    const float sg = eigen_sum_group(lane >= 8 ? v7 : v6, lane, lane < 16);
-    const float S6 = __shfl(sg, 0, kWave), S7 = __shfl(sg, 8, kWave);
+    const float sg = eigen_sum_group<2>(lane >= 8 ? v7 : v6, lane);
+    const float S6 = lane_float(sg, 0), S7 = lane_float(sg, 8);
     int kw = 0;
     if (lane < 2)  // lane 0 walks T6, lane 1 T7
         kw = weighted_median_walk_view(lane ? v7 : v6, lane ? Le7 : Le6, lane ? S7 : S6);
-    const int k6 = __shfl(kw, 0, kWave), k7 = __shfl(kw, 1, kWave);
+    const int k6 = lane_int(kw, 0), k7 = lane_int(kw, 1);
     const uint64_t r6 = tail_key_at<KMAX, true>(key6, K6, k6, ins6, L6 - Le6, Le6);
     const uint64_t r7 = tail_key_at<KMAX, IDX7>(key7, K7, k7, ins7, L7 - Le7, Le7);
     store_tail_result<IDX7>(p, line, 0, r6, k6, sm, lane);
