@@ -54,6 +54,28 @@ def test_nos_matches_fp64_polar_factor(oracle, gpu):
         assert np.abs(gram - np.eye(A)).max() <= 1e-5
 
 
+@pytest.mark.parametrize("eps", [1e-3, 1e-4, 1e-5])
+def test_nos_ill_conditioned(oracle, gpu, eps):
+    # smooth bumps plus a small random part: condition numbers ~2e4, ~2e5,
+    # ~2e6.  The polar iteration's error is ~1e-16 * kappa, so the float
+    # output matches the FP64 SVD polar factor to its own rounding; a route
+    # through the Gram matrix (error ~1e-16 * kappa^2, 6e-4 at 2e6) fails.
+    rng = np.random.default_rng(21)
+    n, A = 301, 90
+    rows_idx = np.arange(n)
+    sino = np.empty((A, n), np.float32)
+    for a in range(A):
+        w = 20 + 8 * np.sin(2 * np.pi * a / A)
+        sino[a] = np.exp(-0.5 * ((rows_idx - n // 2) / w) ** 2) + eps * rng.random(n)
+    nos, center = core.nearest_orthonormal_sinogram(dev(sino))
+    ref, rc = oracle.nearest_orthonormal_sinogram(sino)
+    s = np.linalg.svd(np.asarray(sino, np.float64).T, compute_uv=False)  # offsets are 0
+    err = normwise_err(nos.cpu().numpy(), ref)
+    print(f"[nos ill] eps={eps} kappa~{s[0] / s[-1]:.1e} err={err:.2e}", flush=True)
+    assert center == rc and tuple(nos.shape) == ref.shape
+    assert err <= 2e-7, err
+
+
 def test_nos_batched_matches_single_and_fp64(oracle, gpu):
     # sinograms whose columns peak at different positions, so each has its
     # own offsets, aligned height and centre
