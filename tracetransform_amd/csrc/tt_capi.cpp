@@ -668,7 +668,7 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
         TT_HIP(ctx->sort_count.ensure(ttk::kSortCounters * sizeof(unsigned int)));
         unsigned int *counts = ctx->sort_count.as<unsigned int>();
         if (n > 512) {
-            const size_t qbytes = sizeof(int32_t) * 2 * (size_t)n * n_angles;
+            const size_t qbytes = sizeof(int32_t) * (size_t)n * n_angles;  // a line item per line
             TT_HIP(ctx->sort_qa.ensure(qbytes));
             TT_HIP(ctx->sort_qb.ensure(qbytes));
             qa = ctx->sort_qa.as<int32_t>();
@@ -1156,8 +1156,8 @@ tt_status forward_for_adjoint(tt_context *ctx, const float *d_image, int n, int 
         TT_HIP(ctx->sort_count.ensure(ttk::kSortCounters * sizeof(unsigned int)));
         unsigned int *counts = ctx->sort_count.as<unsigned int>();
         if (n > 512) {
-            TT_HIP(ctx->sort_qa.ensure(sizeof(int32_t) * 2 * lines));
-            TT_HIP(ctx->sort_qb.ensure(sizeof(int32_t) * 2 * lines));
+            TT_HIP(ctx->sort_qa.ensure(sizeof(int32_t) * lines));  // a line item per line
+            TT_HIP(ctx->sort_qb.ensure(sizeof(int32_t) * lines));
             qa = ctx->sort_qa.as<int32_t>();
             qb = ctx->sort_qb.as<int32_t>();
         }

@@ -236,7 +236,7 @@ hipError_t launch_image_bbox_batched(const BatchImage *images, int nimg, int nma
 hipError_t launch_sino_t05(const SinoParams &p, bool need_x, bool need_s,
                            bool do345, hipStream_t s);
 // counts: kSortCounters counters (always); n > 512 also needs two queues of
-// 2 * A * n entries.
+// A * n line items.
 constexpr int kSortCounters = 5;
 hipError_t launch_sino_sort(const SortTParams &p, int32_t *queue_a, int32_t *queue_b,
                             unsigned int *counts, hipStream_t s);
