@@ -1237,7 +1237,7 @@ __device__ __forceinline__ int last_set(bool flag, int base, int prev)
 
 // The key at position k of a tail's full sequence (L = Le + zt entries,
 // the zt trailing zeros at ins .. ins + zt - 1), the sorted part's keys in
-// blocked layout with KO per lane; returned to every lane as
+// blocked layout with KO per lane (a power of two); returned to every lane as
 // (orderable value << 32) | index (index 0 for value-only keys).
 template <int KR, bool IDX>
 __device__ __forceinline__ uint64_t tail_key_at(const TailKey<IDX> (&key)[KR], int KO, int k,
@@ -1246,15 +1246,17 @@ __device__ __forceinline__ uint64_t tail_key_at(const TailKey<IDX> (&key)[KR], i
     if (k >= ins && k < ins + zt)  // a trailing zero: w (or x) = +0, index Le + ...
         return ((uint64_t)ord_key(0.0f) << 32) | (uint32_t)(Le + (k - ins));
     const int pos = k < ins ? k : k - zt;
-    const int pr = pos % KO;
+    // KO is a power of two (a network width): shift and mask, no division
+    const int sh = __builtin_ctz((unsigned)KO);
+    const int pr = pos & (KO - 1), pl = pos >> sh;
     TailKey<IDX> kk = 0;
 #pragma unroll
     for (int r = 0; r < KR; r++)
         kk = (r == pr) ? key[r] : kk;
     if constexpr (IDX)
-        return lane_u64(kk, pos / KO);
+        return lane_u64(kk, pl);
     else  // value key in the high word (no index asked for)
-        return (uint64_t)(uint32_t)lane_int((int)kk, pos / KO) << 32;
+        return (uint64_t)(uint32_t)lane_int((int)kk, pl) << 32;
 }
 
 // Second half of a sorted tail's median: the sorted part's keys are in
