@@ -168,6 +168,25 @@ def test_negative_and_zero_images(oracle, gpu):
     assert not g.any()
 
 
+def test_walk_checkpoints_mixed_signs(oracle, gpu):
+    # n = 285: lines of several 64-row blocks, where the line kernel resumes
+    # the median walks from stored prefixes.  The left half is non-negative
+    # (monotone prefixes: resumed walks), the right half holds negative
+    # samples (those waves walk every row), and a zero band gives lines whose
+    # medians sit right after a run of zeros.
+    side = 200
+    rng = np.random.default_rng(31)
+    img = rng.random((side, side), dtype=np.float32)       # [col][row]
+    img[side // 2:, :] -= 0.25
+    img[:, 60:90] = 0.0
+    img, n = padded(oracle, np.ascontiguousarray(img).reshape(-1), side, side)
+    assert n == 285
+    # both walks, the x walk alone (negatives flagged by the smallest
+    # sample) and the square-root walk alone (by a NaN sum)
+    for codes in (T05, [0, 1], [4, 0]):
+        check_forward(oracle, img, n, 45, 4.0, codes, f64=False)
+
+
 def test_fractional_step(oracle, gpu):
     side = 40
     img, n = padded(oracle, inputs.blob_image(side, 6, nblobs=4, sigma=(3, 10)),

@@ -270,7 +270,7 @@ struct tt_context {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
-    DevBuf imgT, trig, tab345, dft_tw, io_a, io_b, med_x, med_s;
+    DevBuf imgT, trig, tab345, dft_tw, io_a, io_b, med_x, med_s, sino_ck;
     DevBuf dft_w, p3_x, p3_spec;
     DevBuf bw_sino, bw_dsino, bw_z, bw_rec, bw_t6src, bw_t7src;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
@@ -471,7 +471,7 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
     DevBuf *bufs[] = {&ctx->dft_w, &ctx->p3_x, &ctx->p3_spec,
                       &ctx->imgT, &ctx->trig, &ctx->tab345, &ctx->dft_tw,
                       &ctx->io_a, &ctx->io_b, &ctx->med_x,
-                      &ctx->med_s, &ctx->bw_sino, &ctx->bw_dsino, &ctx->bw_z,
+                      &ctx->med_s, &ctx->sino_ck, &ctx->bw_sino, &ctx->bw_dsino, &ctx->bw_z,
                       &ctx->bw_rec, &ctx->bw_t6src, &ctx->bw_t7src, &ctx->nos_med, &ctx->nos_a, &ctx->nos_u,
                       &ctx->nos_v, &ctx->nos_s, &ctx->nos_e, &ctx->nos_c,
                       &ctx->nos_info, &ctx->nos_sweeps, &ctx->nos_w, &ctx->nos_m,
@@ -638,6 +638,9 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
     p.med_x = med_x;
     p.med_s = med_s;
     p.bbox = ctx->bbox.as<int>();
+    TT_HIP(ctx->sino_ck.ensure(sizeof(float2) * ttk::sino_ckpt_blocks(n) * (size_t)n *
+                               n_angles));
+    p.ckpt = ctx->sino_ck.as<float2>();
     p.n = n;
     p.A = n_angles;
     p.nb = (n + ttk::kWave - 1) / ttk::kWave;
@@ -1124,6 +1127,8 @@ tt_status forward_for_adjoint(tt_context *ctx, const float *d_image, int n, int 
     p.med_x = ctx->med_x.as<int32_t>();
     p.med_s = ctx->med_s.as<int32_t>();
     p.z345 = ctx->bw_z.as<float2>();
+    TT_HIP(ctx->sino_ck.ensure(sizeof(float2) * ttk::sino_ckpt_blocks(n) * (size_t)n * A));
+    p.ckpt = ctx->sino_ck.as<float2>();
     p.n = n;
     p.A = A;
     p.nb = (n + ttk::kWave - 1) / ttk::kWave;

@@ -29,6 +29,7 @@ struct SinoParams {
     int32_t *med_s;       // [A][n] or null
     float2 *z345;         // [3][A][n] complex T3..T5 integrals, or null
     const int *bbox;      // non-zero pixels' col/row bounds {x0, x1, y0, y1}, or null
+    float2 *ckpt;         // [sino_ckpt_blocks(n)][A][n] walk prefixes (workspace)
     int n;
     int A;
     int nb;               // column blocks per angle: ceil(n / 64)
@@ -233,6 +234,14 @@ hipError_t launch_image_bbox(const float *img, int n, int *bbox, hipStream_t s);
 // The same for every image of a batch (bbox of image k at images[k].bbox).
 hipError_t launch_image_bbox_batched(const BatchImage *images, int nimg, int nmax,
                                      hipStream_t s);
+// Checkpoints of the median walks the T0..T5 line kernel keeps per line:
+// the prefixes at the starts of its 64-row blocks after the first.
+inline int sino_ckpt_blocks(int n)
+{
+    const int b = (n + 63) / 64 - 1;
+    return b > 1 ? b : 1;
+}
+
 hipError_t launch_sino_t05(const SinoParams &p, bool need_x, bool need_s,
                            bool do345, hipStream_t s);
 // counts: kSortCounters counters (always); n > 512 also needs two queues of

@@ -176,13 +176,24 @@ struct LineDst {
     int32_t *med_x;
     int32_t *med_s;
     float2 *z345;
+    float2 *ckpt;  // walk checkpoints (with CK), see sino_line
     int n;
     int A;
     int tabn;
     Slots slot;
 };
 
-template <bool TR, bool NEED_X, bool NEED_S, bool DO345>
+// With CK, pass 1 also runs the median walks' sequential prefixes (the
+// same float additions in the same row order as the walk) and stores them
+// at the start of every 64-row block after the first, counted from
+// base = R0 & ~7 (rows below R0 are zero).  When every sample of the wave's
+// lines is non-negative the prefixes never decrease, so a block whose
+// starting prefix fails a lane's test 2 * prefix >= S holds none of that
+// lane's earlier rows' passes: pass 2 starts at the last block start where
+// no lane has passed yet, from the stored prefixes, and samples only the
+// rows from there on.  A negative or NaN sample (S of the square roots NaN,
+// or a negative minimum) keeps the full walk.
+template <bool TR, bool NEED_X, bool NEED_S, bool DO345, bool CK>
 __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
                                           const float2 *s_tab, int a, int col,
                                           bool active, float c, float s, int R0, int R1)
@@ -206,17 +217,31 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
         rs[j] = 0.0f;
     }
     const int a8_end = a8 < R1 ? a8 : R1;
+    const size_t line = (size_t)a * n + col;
+    const size_t plane = (size_t)p.A * n;
+    const int base = R0 & ~7;
+    constexpr bool WALK = CK && (NEED_X || NEED_S);
+    float cx = 0.0f, cs = 0.0f, vmin = 0.0f;  // walk prefixes, smallest sample
     // row coordinates py = row - o are integers or half-integers far below
     // 2^23: stepping them by whole rows is exact
-    for (int r0 = R0 & ~7; r0 < a8_end; r0 += 8) {
+    for (int r0 = base; r0 < a8_end; r0 += 8) {
+        if (WALK && r0 > base && ((r0 - base) & 63) == 0 && active)
+            p.ckpt[(size_t)((r0 - base) / 64 - 1) * plane + line] = float2{cx, cs};
         const float py0 = (float)r0 - o;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s,
                                            py0 + (float)j);
             rx[j] += v;
+            const float sv = NEED_S ? sqrtf(v) : 0.0f;
             if (NEED_S)
-                rs[j] += sqrtf(v);
+                rs[j] += sv;
+            if (WALK && NEED_X)
+                cx += v;
+            if (WALK && NEED_S)
+                cs += sv;
+            if (WALK && !NEED_S)
+                vmin = fminf(vmin, v);
         }
     }
     float Sx, Ss;
@@ -264,19 +289,61 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
         }
     }
 
-    const size_t line = (size_t)a * n + col;
-    const size_t tstride = (size_t)p.A * n;
+    const size_t tstride = plane;
     if (active && p.slot.s[0] >= 0)
         p.sino[p.slot.s[0] * tstride + line] = Sx;
     if (!NEED_X && !NEED_S)
         return;
+
+    // ---- where pass 2 starts: the last checkpoint no lane passes --------
+    int start = R0;
+    float Px = 0.0f, Ps = 0.0f;
+    if constexpr (WALK) {
+        // NaN sums flag negative (square root NaN) or NaN samples
+        const bool bad = (NEED_S && Ss != Ss) || Sx != Sx || vmin < 0.0f;
+        const int nck = a8_end > base ? (a8_end - base + 63) / 64 - 1 : 0;
+        int b0 = 0;
+        float2 cb = {0.0f, 0.0f};
+        if (nck > 0 && !__ballot(active && bad)) {
+            for (int b = 1; b <= nck; b += 4) {
+                float2 ck[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (b + u <= nck)
+                        ck[u] = p.ckpt[(size_t)(b + u - 1) * plane + line];
+                bool stop = false;
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (stop || b + u > nck)
+                        break;
+                    const bool pass = (NEED_X && 2.0f * ck[u].x >= Sx) ||
+                                      (NEED_S && 2.0f * ck[u].y >= Ss);
+                    if (__ballot(active && pass)) {
+                        stop = true;
+                    } else {
+                        b0 = b + u;
+                        cb = ck[u];
+                    }
+                }
+                if (stop)
+                    break;
+            }
+        }
+        if (b0 > 0) {
+            start = base + 64 * b0;
+            Px = cb.x;
+            Ps = cb.y;
+        }
+    }
 
     // ---- pass 2: median walk + integrals -----------------------------
     // The walk is the reference's float prefix.  The integrals only feed
     // values: they accumulate with float FMAs over blocks of 64 rows, each
     // block flushed into a double, so they land within ~1e-7 of the FP64
     // oracle instead of inheriting the reference's sequential float error.
-    float Px = 0.0f, Ps = 0.0f;
+    // The blocks are the 64-row grid from base whatever row pass 2 starts
+    // at (every integral starts at or after it), so the integrals do not
+    // depend on the checkpoints.
     double t1 = 0.0, t2 = 0.0;
     double r3 = 0.0, i3 = 0.0, r4 = 0.0, i4 = 0.0, r5 = 0.0, i5 = 0.0;
     int m = -1, sm = -1;
@@ -288,8 +355,9 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
         if (NEED_S && 2.0f * 0.0f >= Ss)
             sm = 0;
     }
-    for (int rb = R0; rb < R1; rb += 64) {
-        const int re = rb + 64 < R1 ? rb + 64 : R1;
+    for (int rb = start, re; rb < R1; rb = re) {
+        re = base + ((rb - base) / 64 + 1) * 64;
+        re = re < R1 ? re : R1;
         float f1 = 0.0f, f2 = 0.0f;
         float f3r = 0.0f, f3i = 0.0f, f4r = 0.0f, f4i = 0.0f, f5r = 0.0f, f5i = 0.0f;
         float py = (float)rb - o;
@@ -406,16 +474,16 @@ __global__ __launch_bounds__(kBlock) void sino_t05_kernel(SinoParams p)
     if (!active)
         col = p.n - 1;
     const float2 cs = p.trig[a];
-    const LineDst dst = {p.sino, p.med_x, p.med_s, p.z345, p.n, p.A, p.n, p.slot};
+    const LineDst dst = {p.sino, p.med_x, p.med_s, p.z345, p.ckpt, p.n, p.A, p.n, p.slot};
     const int2 rows = line_rows(p.bbox, p.n, col, cs.x, cs.y);
     const int R0 = wave_min_i(rows.x), R1 = wave_max_i(rows.y) + 1;
     // Lanes step (dx, dy) = (c, -s) per column: read the copy whose fast axis
     // is the one the lanes mostly move along.
     if (fabsf(cs.x) > fabsf(cs.y))
-        sino_line<true, NEED_X, NEED_S, DO345>(dst, p.imgT, s_tab, a, col, active,
+        sino_line<true, NEED_X, NEED_S, DO345, true>(dst, p.imgT, s_tab, a, col, active,
                                               cs.x, cs.y, R0, R1);
     else
-        sino_line<false, NEED_X, NEED_S, DO345>(dst, p.img, s_tab, a, col, active,
+        sino_line<false, NEED_X, NEED_S, DO345, true>(dst, p.img, s_tab, a, col, active,
                                                cs.x, cs.y, R0, R1);
 }
 
@@ -460,15 +528,15 @@ __global__ __launch_bounds__(kBlock) void sino_t05_batched_kernel(BatchSinoParam
         if (!active)
             col = im.n - 1;
         const float2 cs = p.trig[a];
-        const LineDst dst = {im.sino, im.med_x, im.med_s, im.z345, im.n, p.A, p.tabn,
+        const LineDst dst = {im.sino, im.med_x, im.med_s, im.z345, nullptr, im.n, p.A, p.tabn,
                              p.slot};
         const int2 rows = line_rows(im.bbox, im.n, col, cs.x, cs.y);
         const int R0 = wave_min_i(rows.x), R1 = wave_max_i(rows.y) + 1;
         if (fabsf(cs.x) > fabsf(cs.y))
-            sino_line<true, NEED_X, NEED_S, DO345>(dst, im.imgT, s_tab, a, col,
+            sino_line<true, NEED_X, NEED_S, DO345, false>(dst, im.imgT, s_tab, a, col,
                                                   active, cs.x, cs.y, R0, R1);
         else
-            sino_line<false, NEED_X, NEED_S, DO345>(dst, im.img, s_tab, a, col,
+            sino_line<false, NEED_X, NEED_S, DO345, false>(dst, im.img, s_tab, a, col,
                                                    active, cs.x, cs.y, R0, R1);
     }
 }
