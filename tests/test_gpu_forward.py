@@ -187,6 +187,18 @@ def test_walk_checkpoints_mixed_signs(oracle, gpu):
         check_forward(oracle, img, n, 45, 4.0, codes, f64=False)
 
 
+@pytest.mark.parametrize("scale", [1e-12, 1e-30])
+def test_tiny_pixel_scales(oracle, gpu, scale):
+    # Square roots of samples: 1e-12 keeps every pixel above 2^-45, where the
+    # line kernel's short square root is exact for every sample; 1e-30 flags
+    # the image (samples can fall below 2^-96) and takes the full sequence.
+    side = 150
+    img, n = padded(oracle, inputs.uniform_image(side, 13), side, side)
+    img = (img * np.float32(scale)).astype(np.float32)
+    check_forward(oracle, img, n, 30, 6.0, T05, f64=False)
+    check_forward(oracle, img, n, 30, 6.0, [6, 7, 0], f64=False)
+
+
 def test_fractional_step(oracle, gpu):
     side = 40
     img, n = padded(oracle, inputs.blob_image(side, 6, nblobs=4, sigma=(3, 10)),

@@ -623,7 +623,7 @@ TT_API tt_status tt_sinograms(tt_context *ctx, const float *d_image, int n,
     if (do345)
         TT_TRY(ctx->ensure_tab345(n));
     TT_HIP(ctx->imgT.ensure(sizeof(float) * (size_t)n * n));
-    TT_HIP(ctx->bbox.ensure(4 * sizeof(int)));
+    TT_HIP(ctx->bbox.ensure(ttk::kBboxInts * sizeof(int)));
     TT_HIP(ctx->timer.begin(TT_KERNEL_AUX, st));
     TT_HIP(ttk::launch_transpose(d_image, ctx->imgT.as<float>(), n, st));
     TT_HIP(ttk::launch_image_bbox(d_image, n, ctx->bbox.as<int>(), st));
@@ -938,14 +938,14 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         for (int i = 0; i < nimg; i++)
             tot += (size_t)images[i].n * images[i].n;
         TT_HIP(B.imgT.ensure(sizeof(float) * tot));
-        TT_HIP(B.bbox.ensure(4 * sizeof(int) * (size_t)nimg));
+        TT_HIP(B.bbox.ensure(ttk::kBboxInts * sizeof(int) * (size_t)nimg));
         size_t off = 0;
         int items = 0, cols = 0;
         for (int k = 0; k < nimg; k++) {
             const tt_image &im = images[order[k]];
             ttk::BatchImage &b = d[k];
             b.img = im.d_image;
-            b.bbox = B.bbox.as<int>() + 4 * k;
+            b.bbox = B.bbox.as<int>() + ttk::kBboxInts * k;
             b.imgT = B.imgT.as<float>() + off;
             off += (size_t)im.n * im.n;
             b.sino = im.d_sino;
@@ -1111,7 +1111,7 @@ tt_status forward_for_adjoint(tt_context *ctx, const float *d_image, int n, int 
     TT_TRY(ctx->ensure_trig(A, step));
     TT_TRY(ctx->ensure_tab345(n));
     TT_HIP(ctx->imgT.ensure(sizeof(float) * (size_t)n * n));
-    TT_HIP(ctx->bbox.ensure(4 * sizeof(int)));
+    TT_HIP(ctx->bbox.ensure(ttk::kBboxInts * sizeof(int)));
     hipStream_t st = ctx->stream;
     TT_HIP(ctx->timer.begin(TT_KERNEL_AUX, st));
     TT_HIP(ttk::launch_transpose(d_image, ctx->imgT.as<float>(), n, st));
@@ -1703,14 +1703,14 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         TT_HIP(P.medx.ensure(sizeof(int32_t) * ln));
         TT_HIP(P.meds.ensure(sizeof(int32_t) * ln));
         TT_HIP(P.rec.ensure(sizeof(ttk::LineRecord) * ln));
-        TT_HIP(B.bbox.ensure(4 * sizeof(int) * (size_t)nimg));
+        TT_HIP(B.bbox.ensure(ttk::kBboxInts * sizeof(int) * (size_t)nimg));
         std::vector<ttk::BatchImage> d(nimg);
         size_t po = 0, lo = 0, tiles = 0;
         int items = 0, cols = 0;
         for (int k = 0; k < nimg; k++) {
             const tt_image_grad &im = images[order[k]];
             ttk::BatchImage &b = d[k];
-            b.bbox = B.bbox.as<int>() + 4 * k;
+            b.bbox = B.bbox.as<int>() + ttk::kBboxInts * k;
             const size_t l = (size_t)im.n * n_angles;
             b.img = im.d_image;
             b.imgT = B.imgT.as<float>() + po;

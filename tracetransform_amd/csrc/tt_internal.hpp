@@ -28,7 +28,7 @@ struct SinoParams {
     int32_t *med_x;       // [A][n] or null
     int32_t *med_s;       // [A][n] or null
     float2 *z345;         // [3][A][n] complex T3..T5 integrals, or null
-    const int *bbox;      // non-zero pixels' col/row bounds {x0, x1, y0, y1}, or null
+    const int *bbox;      // non-zero pixels' bounds {x0, x1, y0, y1} + flag, or null
     float2 *ckpt;         // [sino_ckpt_blocks(n)][A][n] walk prefixes (workspace)
     int n;
     int A;
@@ -60,7 +60,7 @@ struct BatchImage {
     float *sino;          // [nt][A][n]
     int32_t *med_x;
     int32_t *med_s;
-    const int *bbox;      // non-zero pixels' col/row bounds {x0, x1, y0, y1}, or null
+    const int *bbox;      // non-zero pixels' bounds {x0, x1, y0, y1} + flag, or null
     int n;
     int nb;               // column blocks per angle
     int item0;            // first wave item of this image
@@ -115,7 +115,7 @@ struct SortTParams {
     int32_t *t6_src;
     int32_t *t7_rank;
     int32_t *t7_src;      // T7 source row (stable order of the tail), or null
-    const int *bbox;      // non-zero pixels' col/row bounds {x0, x1, y0, y1}, or null
+    const int *bbox;      // non-zero pixels' bounds {x0, x1, y0, y1} + flag, or null
     // (line, which) queues: a pass reads defer_in (all lines when NULL) and
     // queues the line items (line << 2 | T6/T7 mask) it cannot hold in
     // defer_out (room for A * n items each)
@@ -229,7 +229,10 @@ hipError_t launch_pad(const float *in, int rows, int cols, float *out, int n,
                       int dfx, int dfy, hipStream_t s);
 // Bounding box {x0, x1, y0, y1} (columns, rows) of the non-zero pixels of an
 // n x n column-major image, for the line kernels' row ranges; an all-zero
-// image gives x0 > x1.
+// image gives x0 > x1.  Word 4 is 1 when some pixel is negative or in
+// (0, 2^-45) (then a bilinear sample can be a negative or a positive value
+// below 2^-96), else 0.  kBboxInts words per image.
+constexpr int kBboxInts = 8;
 hipError_t launch_image_bbox(const float *img, int n, int *bbox, hipStream_t s);
 // The same for every image of a batch (bbox of image k at images[k].bbox).
 hipError_t launch_image_bbox_batched(const BatchImage *images, int nimg, int nmax,

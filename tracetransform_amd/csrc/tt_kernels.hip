@@ -95,6 +95,28 @@ __device__ __forceinline__ float rot_sample(const float *__restrict__ src,
     return inb ? v : 0.0f;
 }
 
+// Square root of a line sample, correctly rounded.  FS: v_sqrt_f32 and one
+// residual test per neighbour (x - s' s against 0 picks the float nearest
+// the true root), without the correctly rounded sequence's scaling of small
+// inputs and its zero/infinity select.  Equal to that sequence on every
+// float except x in (0, 2^-96) and the negative denormals (checked on the
+// GPU over all 2^32 inputs); a bilinear sample falls there only if some
+// pixel is negative or in (0, 2^-45) (each non-zero tap weight is a product
+// of two factors >= 2^-25), which the image's bounding-box pass flags.
+template <bool FS>
+__device__ __forceinline__ float line_sqrt(float x)
+{
+    if constexpr (!FS) {
+        return sqrtf(x);
+    } else {
+        const float s = __builtin_amdgcn_sqrtf(x);
+        const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+        const float su = __uint_as_float(__float_as_uint(s) + 1u);
+        const float r = fmaf(-sd, s, x) <= 0.0f ? sd : s;
+        return fmaf(-su, s, x) > 0.0f ? su : r;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // T0..T5 line kernel.  Lane = one column of the rotated image of one angle.
 // Pass 1 streams the column and forms Eigen's packet-order sum (8 stride
@@ -193,7 +215,7 @@ struct LineDst {
 // no lane has passed yet, from the stored prefixes, and samples only the
 // rows from there on.  A negative or NaN sample (S of the square roots NaN,
 // or a negative minimum) keeps the full walk.
-template <bool TR, bool NEED_X, bool NEED_S, bool DO345, bool CK>
+template <bool TR, bool NEED_X, bool NEED_S, bool DO345, bool CK, bool FS>
 __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
                                           const float2 *s_tab, int a, int col,
                                           bool active, float c, float s, int R0, int R1)
@@ -233,7 +255,7 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
             const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s,
                                            py0 + (float)j);
             rx[j] += v;
-            const float sv = NEED_S ? sqrtf(v) : 0.0f;
+            const float sv = NEED_S ? line_sqrt<FS>(v) : 0.0f;
             if (NEED_S)
                 rs[j] += sv;
             if (WALK && NEED_X)
@@ -253,7 +275,7 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
                                            (float)r - o);
             Sx = r == 0 ? v : Sx + v;
             if (NEED_S)
-                Ss = r == 0 ? sqrtf(v) : Ss + sqrtf(v);
+                Ss = r == 0 ? line_sqrt<FS>(v) : Ss + line_sqrt<FS>(v);
         }
     } else {
         float ax[4], as[4];
@@ -267,7 +289,7 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
             for (int j = 0; j < 4; j++) {
                 const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s,
                                                (float)(a8 + j) - o);
-                const float sv = NEED_S ? sqrtf(v) : 0.0f;
+                const float sv = NEED_S ? line_sqrt<FS>(v) : 0.0f;
                 if (n >= 8) {
                     ax[j] += v;
                     as[j] += sv;
@@ -285,7 +307,7 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
                                            (float)r - o);
             Sx += v;
             if (NEED_S)
-                Ss += sqrtf(v);
+                Ss += line_sqrt<FS>(v);
         }
     }
 
@@ -378,7 +400,7 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
             }
             if (NEED_S) {
                 if (sm < 0) {
-                    Ps += sqrtf(v);
+                    Ps += line_sqrt<FS>(v);
                     if (2.0f * Ps >= Ss)
                         sm = r;
                 } else if (DO345) {
@@ -441,6 +463,31 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
     }
 }
 
+// One line, with the image copy whose fast axis the lanes step along (TR)
+// and the square root the image's bounding-box flag allows (FS).
+template <bool NEED_X, bool NEED_S, bool DO345, bool CK>
+__device__ __forceinline__ void sino_line_any(const LineDst &dst, const float *img,
+                                              const float *imgT, const int *bbox,
+                                              const float2 *s_tab, int a, int col,
+                                              bool active, float2 cs, int R0, int R1)
+{
+    // Lanes step (dx, dy) = (c, -s) per column
+    const bool tr = fabsf(cs.x) > fabsf(cs.y);
+    const bool fs = bbox && bbox[4] == 0;
+    if (tr && fs)
+        sino_line<true, NEED_X, NEED_S, DO345, CK, true>(dst, imgT, s_tab, a, col, active,
+                                                         cs.x, cs.y, R0, R1);
+    else if (tr)
+        sino_line<true, NEED_X, NEED_S, DO345, CK, false>(dst, imgT, s_tab, a, col, active,
+                                                          cs.x, cs.y, R0, R1);
+    else if (fs)
+        sino_line<false, NEED_X, NEED_S, DO345, CK, true>(dst, img, s_tab, a, col, active,
+                                                          cs.x, cs.y, R0, R1);
+    else
+        sino_line<false, NEED_X, NEED_S, DO345, CK, false>(dst, img, s_tab, a, col, active,
+                                                           cs.x, cs.y, R0, R1);
+}
+
 // LDS copy of the T3..T5 tables ([3][tabn] float2 in global memory) with the
 // T3 and T4 entries of a row interleaved as one float4 (rows 0..tabn-1) and
 // T5 after them (float2 at 2*tabn + r).
@@ -477,14 +524,8 @@ __global__ __launch_bounds__(kBlock) void sino_t05_kernel(SinoParams p)
     const LineDst dst = {p.sino, p.med_x, p.med_s, p.z345, p.ckpt, p.n, p.A, p.n, p.slot};
     const int2 rows = line_rows(p.bbox, p.n, col, cs.x, cs.y);
     const int R0 = wave_min_i(rows.x), R1 = wave_max_i(rows.y) + 1;
-    // Lanes step (dx, dy) = (c, -s) per column: read the copy whose fast axis
-    // is the one the lanes mostly move along.
-    if (fabsf(cs.x) > fabsf(cs.y))
-        sino_line<true, NEED_X, NEED_S, DO345, true>(dst, p.imgT, s_tab, a, col, active,
-                                              cs.x, cs.y, R0, R1);
-    else
-        sino_line<false, NEED_X, NEED_S, DO345, true>(dst, p.img, s_tab, a, col, active,
-                                               cs.x, cs.y, R0, R1);
+    sino_line_any<NEED_X, NEED_S, DO345, true>(dst, p.img, p.imgT, p.bbox, s_tab, a, col,
+                                               active, cs, R0, R1);
 }
 
 // ---------------------------------------------------------------------------
@@ -532,12 +573,8 @@ __global__ __launch_bounds__(kBlock) void sino_t05_batched_kernel(BatchSinoParam
                              p.slot};
         const int2 rows = line_rows(im.bbox, im.n, col, cs.x, cs.y);
         const int R0 = wave_min_i(rows.x), R1 = wave_max_i(rows.y) + 1;
-        if (fabsf(cs.x) > fabsf(cs.y))
-            sino_line<true, NEED_X, NEED_S, DO345, false>(dst, im.imgT, s_tab, a, col,
-                                                  active, cs.x, cs.y, R0, R1);
-        else
-            sino_line<false, NEED_X, NEED_S, DO345, false>(dst, im.img, s_tab, a, col,
-                                                   active, cs.x, cs.y, R0, R1);
+        sino_line_any<NEED_X, NEED_S, DO345, false>(dst, im.img, im.imgT, im.bbox, s_tab, a,
+                                                    col, active, cs, R0, R1);
     }
 }
 
@@ -2393,12 +2430,15 @@ __global__ __launch_bounds__(kBlock) void to_float_kernel(const double *in, int 
 __device__ __forceinline__ void bbox_accumulate(const float *img, int n, int block,
                                                 int nblocks, int *bb)
 {
-    __shared__ int s_b[4][kBlock / kWave];
+    __shared__ int s_b[5][kBlock / kWave];
     int x0 = n, x1 = -1, y0 = n, y1 = -1;
+    bool narrow = false;  // a pixel below 0 or in (0, 2^-45)
     for (int X = block; X < n; X += nblocks) {
         const float *col = img + (size_t)X * n;
         for (int Y = threadIdx.x; Y < n; Y += kBlock) {
-            if (col[Y] != 0.0f) {
+            const float v = col[Y];
+            narrow = narrow || v < 0.0f || (v > 0.0f && v < 0x1p-45f);
+            if (v != 0.0f) {
                 x0 = X < x0 ? X : x0;
                 x1 = X > x1 ? X : x1;
                 y0 = Y < y0 ? Y : y0;
@@ -2410,21 +2450,27 @@ __device__ __forceinline__ void bbox_accumulate(const float *img, int n, int blo
     x1 = wave_max_i(x1);
     y0 = wave_min_i(y0);
     y1 = wave_max_i(y1);
+    const int nw = __ballot(narrow) != 0;
     const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
     if (lane == 0) {
         s_b[0][w] = x0;
         s_b[1][w] = x1;
         s_b[2][w] = y0;
         s_b[3][w] = y1;
+        s_b[4][w] = nw;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        int nr = s_b[4][0];
         for (int k = 1; k < kBlock / kWave; k++) {
             x0 = s_b[0][k] < x0 ? s_b[0][k] : x0;
             x1 = s_b[1][k] > x1 ? s_b[1][k] : x1;
             y0 = s_b[2][k] < y0 ? s_b[2][k] : y0;
             y1 = s_b[3][k] > y1 ? s_b[3][k] : y1;
+            nr |= s_b[4][k];
         }
+        if (nr)
+            atomicOr(bb + 4, 1);
         if (x1 >= 0) {
             atomicMin(bb + 0, x0);
             atomicMax(bb + 1, x1);
@@ -2441,6 +2487,7 @@ __global__ void bbox_init_kernel(int *bb, int n)
         bb[1] = -1;
         bb[2] = n;
         bb[3] = -1;
+        bb[4] = 0;
     }
 }
 
@@ -2459,6 +2506,7 @@ __global__ void bbox_init_batched_kernel(const BatchImage *images, int nimg)
         bb[1] = -1;
         bb[2] = n;
         bb[3] = -1;
+        bb[4] = 0;
     }
 }
 
