@@ -238,10 +238,11 @@ hipError_t launch_image_bbox(const float *img, int n, int *bbox, hipStream_t s);
 hipError_t launch_image_bbox_batched(const BatchImage *images, int nimg, int nmax,
                                      hipStream_t s);
 // Checkpoints of the median walks the T0..T5 line kernel keeps per line:
-// the prefixes at the starts of its 64-row blocks after the first.
+// the prefixes at the starts of its 64-step blocks after the first (a wave's
+// first pass spans at most n + 63 steps with its row skew).
 inline int sino_ckpt_blocks(int n)
 {
-    const int b = (n + 63) / 64 - 1;
+    const int b = (n + 62) / 64;
     return b > 1 ? b : 1;
 }
 

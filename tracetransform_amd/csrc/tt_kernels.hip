@@ -205,65 +205,118 @@ struct LineDst {
     Slots slot;
 };
 
+// Row schedule of a wave.  At step v lane l handles row v - d_l of its
+// line, d_l = rint(l t) - (its smallest value), t = -s/c with the row-major
+// copy (TR, |c| > |s|) and c/s with the column-major one, so |t| <= 1: the
+// 64 samples of one step then lie along one row of the copy the wave reads
+// (TR: y constant, else x), a few cache lines per load, instead of spanning
+// one image row per sample when the lanes all take the same row.  Each line
+// still sees its rows in increasing order, so every sum, prefix and walk is
+// the unskewed one.  Steps run from G, the smallest first row plus skew in
+// the wave, in groups of 8: slot j of the stride-8 accumulators takes the
+// rows of steps G + 8k + j, i.e. Eigen's accumulator (j + G - d_l) mod 8,
+// and the slots are rotated back after the pass.  Rows outside [0, n) (and
+// outside [0, a8) for the stride-8 accumulators) are masked; a group all of
+// whose lanes are inside skips the masks.
+//
 // With CK, pass 1 also runs the median walks' sequential prefixes (the
 // same float additions in the same row order as the walk) and stores them
-// at the start of every 64-row block after the first, counted from
-// base = R0 & ~7 (rows below R0 are zero).  When every sample of the wave's
-// lines is non-negative the prefixes never decrease, so a block whose
-// starting prefix fails a lane's test 2 * prefix >= S holds none of that
-// lane's earlier rows' passes: pass 2 starts at the last block start where
-// no lane has passed yet, from the stored prefixes, and samples only the
-// rows from there on.  A negative or NaN sample (S of the square roots NaN,
-// or a negative minimum) keeps the full walk.
+// at the start of every 64-step block after the first.  When every sample
+// a line has before a block is non-negative its prefixes never decrease, so
+// a block whose starting prefix fails a lane's test 2 * prefix >= S holds
+// none of that lane's earlier passes: pass 2 starts at the last block start
+// where no lane has passed yet, from the stored prefixes, and samples only
+// the steps from there on.  A negative or NaN sample (S of the square roots
+// NaN, or a negative minimum) keeps the full walk.  [lo, hi] is the lane's
+// row range that can hold non-zero samples (lo > hi: none).
 template <bool TR, bool NEED_X, bool NEED_S, bool DO345, bool CK, bool FS>
 __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
                                           const float2 *s_tab, int a, int col,
-                                          bool active, float c, float s, int R0, int R1)
+                                          bool active, float c, float s, int lo, int hi)
 {
-    // Rows outside [R0, R1) sample exactly 0 for every lane of the wave: they
-    // add nothing to any sum, prefix or integral, so the loops skip them.
     const int n = p.n;
     const float o = (float)((n - 1) / 2.0);
     const float lim = (float)(n - 1);
     const float px = (float)col - o;
     const float pxc = px * c;
     const float pxns = px * (-s);
+    const int lane = threadIdx.x & (kWave - 1);
+    const size_t line = (size_t)a * n + col;
+    const size_t plane = (size_t)p.A * n;
+    constexpr int kNone = 1 << 30;
 
-    // ---- pass 1: Eigen-order sums -------------------------------------
+    // ---- schedule --------------------------------------------------------
+    int dl = (int)rintf((float)lane * (TR ? -s / c : c / s));
+    dl -= wave_min_i(dl);
+    const int dmax = wave_max_i(dl);
     const int a8 = n & ~7;
     const int a4 = n & ~3;
+    const int r0 = lo > 0 ? lo : 0;
+    const int r1 = hi + 1 < n ? hi + 1 : n;
+    const bool any = active && r1 > r0;  // the lanes past the last column follow it
+    const int G = wave_min_i(any ? r0 + dl : kNone);
+    const int V1 = wave_max_i(any ? (r1 < a8 ? r1 : a8) + dl : -kNone);
+    const int V2 = wave_max_i(any ? r1 + dl : -kNone);
+
+    // ---- pass 1: Eigen-order sums -------------------------------------
     float rx[8], rs[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         rx[j] = 0.0f;
         rs[j] = 0.0f;
     }
-    const int a8_end = a8 < R1 ? a8 : R1;
-    const size_t line = (size_t)a * n + col;
-    const size_t plane = (size_t)p.A * n;
-    const int base = R0 & ~7;
     constexpr bool WALK = CK && (NEED_X || NEED_S);
     float cx = 0.0f, cs = 0.0f, vmin = 0.0f;  // walk prefixes, smallest sample
-    // row coordinates py = row - o are integers or half-integers far below
-    // 2^23: stepping them by whole rows is exact
-    for (int r0 = base; r0 < a8_end; r0 += 8) {
-        if (WALK && r0 > base && ((r0 - base) & 63) == 0 && active)
-            p.ckpt[(size_t)((r0 - base) / 64 - 1) * plane + line] = float2{cx, cs};
-        const float py0 = (float)r0 - o;
+    // one group of 8 steps from step v (MASK: some lane has a row outside
+    // [0, a8)); row coordinates py = row - o are integers or half-integers
+    // far below 2^23, so stepping them by whole rows is exact
+    auto group = [&](auto mask_c, int v) {
+        constexpr bool MASK = decltype(mask_c)::value;
+        const int rb = v - dl;
+        const float py0 = (float)rb - o;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s,
-                                           py0 + (float)j);
-            rx[j] += v;
-            const float sv = NEED_S ? line_sqrt<FS>(v) : 0.0f;
+            float x = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s, py0 + (float)j);
+            bool chain = true;
+            if constexpr (MASK) {
+                x = (unsigned)(rb + j) < (unsigned)n ? x : 0.0f;
+                chain = (unsigned)(rb + j) < (unsigned)a8;
+            }
+            const float sv = NEED_S ? line_sqrt<FS>(x) : 0.0f;
+            rx[j] += chain ? x : 0.0f;
             if (NEED_S)
-                rs[j] += sv;
+                rs[j] += chain ? sv : 0.0f;
             if (WALK && NEED_X)
-                cx += v;
+                cx += x;
             if (WALK && NEED_S)
                 cs += sv;
             if (WALK && !NEED_S)
-                vmin = fminf(vmin, v);
+                vmin = fminf(vmin, x);
+        }
+    };
+    for (int v = G; v < V1; v += 8) {
+        if (WALK && v > G && ((v - G) & 63) == 0 && active)
+            p.ckpt[(size_t)((v - G) / 64 - 1) * plane + line] = float2{cx, cs};
+        if (v >= dmax && v + 8 <= a8)
+            group(std::false_type{}, v);
+        else
+            group(std::true_type{}, v);
+    }
+    // slot j holds accumulator (j + d) mod 8: rotate it back
+    const int d = (G - dl) & 7;
+#pragma unroll
+    for (int b = 1; b < 8; b <<= 1) {
+        const bool on = (d & b) != 0;
+        float tx[8], ts[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            tx[m] = on ? rx[(m - b) & 7] : rx[m];
+            ts[m] = on ? rs[(m - b) & 7] : rs[m];
+        }
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            rx[m] = tx[m];
+            rs[m] = ts[m];
         }
     }
     float Sx, Ss;
@@ -301,8 +354,7 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
         }
         Sx = (ax[0] + ax[2]) + (ax[1] + ax[3]);
         Ss = (as[0] + as[2]) + (as[1] + as[3]);
-        const int tail_end = n < R1 ? n : R1;
-        for (int r = a4; r < tail_end; r++) {
+        for (int r = a4; r < n; r++) {
             const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s,
                                            (float)r - o);
             Sx += v;
@@ -318,12 +370,12 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
         return;
 
     // ---- where pass 2 starts: the last checkpoint no lane passes --------
-    int start = R0;
+    int start = G;
     float Px = 0.0f, Ps = 0.0f;
     if constexpr (WALK) {
         // NaN sums flag negative (square root NaN) or NaN samples
         const bool bad = (NEED_S && Ss != Ss) || Sx != Sx || vmin < 0.0f;
-        const int nck = a8_end > base ? (a8_end - base + 63) / 64 - 1 : 0;
+        const int nck = V1 > G ? (V1 - G - 1) / 64 : 0;
         int b0 = 0;
         float2 cb = {0.0f, 0.0f};
         if (nck > 0 && !__ballot(active && bad)) {
@@ -352,7 +404,7 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
             }
         }
         if (b0 > 0) {
-            start = base + 64 * b0;
+            start = G + 64 * b0;
             Px = cb.x;
             Ps = cb.y;
         }
@@ -360,66 +412,71 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
 
     // ---- pass 2: median walk + integrals -----------------------------
     // The walk is the reference's float prefix.  The integrals only feed
-    // values: they accumulate with float FMAs over blocks of 64 rows, each
+    // values: they accumulate with float FMAs over blocks of 64 steps, each
     // block flushed into a double, so they land within ~1e-7 of the FP64
     // oracle instead of inheriting the reference's sequential float error.
-    // The blocks are the 64-row grid from base whatever row pass 2 starts
-    // at (every integral starts at or after it), so the integrals do not
+    // The blocks are the 64-step grid from G whatever step pass 2 starts at
+    // (every integral starts at or after it), so the integrals do not
     // depend on the checkpoints.
     double t1 = 0.0, t2 = 0.0;
     double r3 = 0.0, i3 = 0.0, r4 = 0.0, i4 = 0.0, r5 = 0.0, i5 = 0.0;
     int m = -1, sm = -1;
     const bool do3 = p.slot.s[3] >= 0, do4 = p.slot.s[4] >= 0, do5 = p.slot.s[5] >= 0;
-    if (R0 > 0 || R0 >= R1) {
-        // the walk's first row is a skipped zero: it passes there iff S <= 0
+    if (lo > 0 || lo > hi) {
+        // the walk's first row is a zero: it passes there iff S <= 0
         if (NEED_X && 2.0f * 0.0f >= Sx)
             m = 0;
         if (NEED_S && 2.0f * 0.0f >= Ss)
             sm = 0;
     }
-    for (int rb = start, re; rb < R1; rb = re) {
-        re = base + ((rb - base) / 64 + 1) * 64;
-        re = re < R1 ? re : R1;
+    for (int vb = start, ve; vb < V2; vb = ve) {
+        ve = G + ((vb - G) / 64 + 1) * 64;
+        ve = ve < V2 ? ve : V2;
+        // every lane's rows of the block inside [0, n)
+        const bool full = vb >= dmax && ve <= n;
         float f1 = 0.0f, f2 = 0.0f;
         float f3r = 0.0f, f3i = 0.0f, f4r = 0.0f, f4i = 0.0f, f5r = 0.0f, f5i = 0.0f;
-        float py = (float)rb - o;
-        for (int r = rb; r < re; r++, py += 1.0f) {
-            const float v = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s, py);
+        int r = vb - dl;
+        float py = (float)r - o;
+        for (int v = vb; v < ve; v++, r++, py += 1.0f) {
+            if (!full && (unsigned)r >= (unsigned)n)
+                continue;
+            const float x = rot_sample<TR>(src, n, lim, o, pxc, pxns, c, s, py);
             if (NEED_X) {
                 if (m < 0) {
-                    Px += v;
+                    Px += x;
                     if (2.0f * Px >= Sx)
                         m = r;
                 }
                 if (m >= 0) {
                     const float rr = (float)(r - m);
-                    const float vr = v * rr;
+                    const float vr = x * rr;
                     f1 += vr;
                     f2 = fmaf(vr, rr, f2);
                 }
             }
             if (NEED_S) {
                 if (sm < 0) {
-                    Ps += line_sqrt<FS>(v);
+                    Ps += line_sqrt<FS>(x);
                     if (2.0f * Ps >= Ss)
                         sm = r;
                 } else if (DO345) {
                     // one 16-byte and one 8-byte LDS read per row for all three
                     // tables (rows interleaved by the kernel prologue)
-                    const int r1 = r - sm;
-                    const float4 w34 = reinterpret_cast<const float4 *>(s_tab)[r1];
-                    const float2 w5 = s_tab[2 * p.tabn + r1];
+                    const int q = r - sm;
+                    const float4 w34 = reinterpret_cast<const float4 *>(s_tab)[q];
+                    const float2 w5 = s_tab[2 * p.tabn + q];
                     if (do3) {
-                        f3r = fmaf(w34.x, v, f3r);
-                        f3i = fmaf(w34.y, v, f3i);
+                        f3r = fmaf(w34.x, x, f3r);
+                        f3i = fmaf(w34.y, x, f3i);
                     }
                     if (do4) {
-                        f4r = fmaf(w34.z, v, f4r);
-                        f4i = fmaf(w34.w, v, f4i);
+                        f4r = fmaf(w34.z, x, f4r);
+                        f4i = fmaf(w34.w, x, f4i);
                     }
                     if (do5) {
-                        f5r = fmaf(w5.x, v, f5r);
-                        f5i = fmaf(w5.y, v, f5i);
+                        f5r = fmaf(w5.x, x, f5r);
+                        f5i = fmaf(w5.y, x, f5i);
                     }
                 }
             }
@@ -469,23 +526,23 @@ template <bool NEED_X, bool NEED_S, bool DO345, bool CK>
 __device__ __forceinline__ void sino_line_any(const LineDst &dst, const float *img,
                                               const float *imgT, const int *bbox,
                                               const float2 *s_tab, int a, int col,
-                                              bool active, float2 cs, int R0, int R1)
+                                              bool active, float2 cs, int lo, int hi)
 {
     // Lanes step (dx, dy) = (c, -s) per column
     const bool tr = fabsf(cs.x) > fabsf(cs.y);
     const bool fs = bbox && bbox[4] == 0;
     if (tr && fs)
         sino_line<true, NEED_X, NEED_S, DO345, CK, true>(dst, imgT, s_tab, a, col, active,
-                                                         cs.x, cs.y, R0, R1);
+                                                         cs.x, cs.y, lo, hi);
     else if (tr)
         sino_line<true, NEED_X, NEED_S, DO345, CK, false>(dst, imgT, s_tab, a, col, active,
-                                                          cs.x, cs.y, R0, R1);
+                                                          cs.x, cs.y, lo, hi);
     else if (fs)
         sino_line<false, NEED_X, NEED_S, DO345, CK, true>(dst, img, s_tab, a, col, active,
-                                                          cs.x, cs.y, R0, R1);
+                                                          cs.x, cs.y, lo, hi);
     else
         sino_line<false, NEED_X, NEED_S, DO345, CK, false>(dst, img, s_tab, a, col, active,
-                                                           cs.x, cs.y, R0, R1);
+                                                           cs.x, cs.y, lo, hi);
 }
 
 // LDS copy of the T3..T5 tables ([3][tabn] float2 in global memory) with the
@@ -523,9 +580,8 @@ __global__ __launch_bounds__(kBlock) void sino_t05_kernel(SinoParams p)
     const float2 cs = p.trig[a];
     const LineDst dst = {p.sino, p.med_x, p.med_s, p.z345, p.ckpt, p.n, p.A, p.n, p.slot};
     const int2 rows = line_rows(p.bbox, p.n, col, cs.x, cs.y);
-    const int R0 = wave_min_i(rows.x), R1 = wave_max_i(rows.y) + 1;
     sino_line_any<NEED_X, NEED_S, DO345, true>(dst, p.img, p.imgT, p.bbox, s_tab, a, col,
-                                               active, cs, R0, R1);
+                                               active, cs, rows.x, rows.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -572,9 +628,8 @@ __global__ __launch_bounds__(kBlock) void sino_t05_batched_kernel(BatchSinoParam
         const LineDst dst = {im.sino, im.med_x, im.med_s, im.z345, nullptr, im.n, p.A, p.tabn,
                              p.slot};
         const int2 rows = line_rows(im.bbox, im.n, col, cs.x, cs.y);
-        const int R0 = wave_min_i(rows.x), R1 = wave_max_i(rows.y) + 1;
         sino_line_any<NEED_X, NEED_S, DO345, false>(dst, im.img, im.imgT, im.bbox, s_tab, a,
-                                                    col, active, cs, R0, R1);
+                                                    col, active, cs, rows.x, rows.y);
     }
 }
 
