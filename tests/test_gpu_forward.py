@@ -255,10 +255,15 @@ def test_circus_on_oracle_sinograms(oracle, gpu):
 
 
 def test_circus_sizes(oracle, gpu):
+    # P2 sorts columns of up to 2048 rows in one wave's registers (1, 2, 16
+    # and 32 keys per lane here); 2049 rows take the workgroup sort
     rng = np.random.default_rng(12)
-    for n, A in ((7, 3), (91, 180), (725, 40), (1449, 16)):
+    for n, A in ((7, 3), (91, 180), (725, 40), (1449, 16), (2048, 3), (2049, 2)):
         sino = rng.random((A, n), dtype=np.float32)
         check_circus(oracle, sino, [(1,), (2,), (3,), (0, 3, n // 3)])
+    # ties and negative values (P2's order of equal values is free)
+    sino = (rng.integers(-4, 9, (6, 1449)) / 4.0).astype(np.float32)
+    check_circus(oracle, sino, [(2,), (1,)])
 
 
 def test_hermite_tables_follow_height_and_centre(oracle, gpu):

@@ -146,6 +146,7 @@ struct CircusParams {
     int np;
     int code[kMaxP];
     int herm[kMaxP];      // row of herm_w for Hermite entries
+    int p2_wave;          // P2 entries written by circus_p2_kernel (launcher's)
 };
 
 struct RecordParams {

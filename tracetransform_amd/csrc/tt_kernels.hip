@@ -871,7 +871,8 @@ __device__ __forceinline__ CircusLds circus_lds(float *smem, int nmax)
 __device__ void circus_column(const CircusLds &L, int n, int np, const int *code,
                               const int *herm, const double *herm_w, float *out,
                               size_t out_stride, int32_t *p2rank,
-                              const double *spec = nullptr, int spec_h = 0)
+                              const double *spec = nullptr, int spec_h = 0,
+                              bool skip_p2 = false)
 {
     __shared__ float s_red[kBlock / kWave];
     __shared__ double s_red_d[kBlock / kWave];
@@ -883,6 +884,8 @@ __device__ void circus_column(const CircusLds &L, int n, int np, const int *code
         m2 <<= 1;
     for (int q = 0; q < np; q++) {
         const int cd = code[q];
+        if (cd == 2 && skip_p2)
+            continue;  // written by circus_p2_kernel
         float res = 0.0f;
         if (cd == 1) {
             float acc = 0.0f;
@@ -994,7 +997,8 @@ __global__ __launch_bounds__(kBlock) void circus_kernel(CircusParams p)
     const size_t o = (size_t)sg * p.np * p.ncols + col;
     circus_column(L, n, p.np, p.code, p.herm, p.herm_w, p.out + o, p.ncols,
                   p.p2rank ? p.p2rank + o : nullptr,
-                  p.p3_spec ? p.p3_spec + (size_t)colg * 2 * p.p3_h : nullptr, p.p3_h);
+                  p.p3_spec ? p.p3_spec + (size_t)colg * 2 * p.p3_h : nullptr, p.p3_h,
+                  p.p2_wave != 0);
 }
 
 __global__ __launch_bounds__(kBlock) void dft_matrix_kernel(int n, int h, double *w)
@@ -1560,6 +1564,58 @@ __device__ __forceinline__ void sort_prefix(Key (&k)[KMAX], int K, int lane)
         }
     if constexpr (KMAX >= 16)
         wave_bitonic_sort<16>(reinterpret_cast<Key(&)[16]>(k), lane);
+}
+
+// P2 of one sinogram column per wave, columns of up to kP2Max rows
+// (functionals.cpp:303-311, the weighted median of the sorted column with
+// the values as weights, :21-30): the column sorted in the wave's registers
+// (32-bit order keys: equal values are interchangeable), the sorted values
+// into the wave's LDS row, the Eigen-order sum on lanes 0..7 and the
+// sequential walk on lane 0.  Entry q of the P list; four columns per
+// workgroup.
+constexpr int kP2Max = 2048;
+
+__global__ __launch_bounds__(kBlock) void circus_p2_kernel(CircusParams p, int q)
+{
+    __shared__ float s_x[kBlock / kWave][kP2Max];
+    const int w = wave_id(), lane = threadIdx.x & (kWave - 1);
+    const int colg = blockIdx.x * (kBlock / kWave) + w;
+    if (colg >= p.ncols * p.nsino)
+        return;
+    const int n = p.n;
+    const float *x = p.sino + (size_t)colg * n;
+    float *xs = s_x[w];
+    const int K = keys_per_lane<32>(n);
+    uint32_t key[32];
+#pragma unroll
+    for (int r = 0; r < 32; r++) {
+        const int i = r * kWave + lane;
+        key[r] = r < K && i < n ? ord_key(x[i]) : ~0u;
+    }
+    if (K <= 16)
+        sort_prefix<16>(reinterpret_cast<uint32_t(&)[16]>(key), K, lane);
+    else
+        wave_bitonic_sort<32>(key, lane);
+#pragma unroll
+    for (int r = 0; r < 32; r++) {
+        const int pos = lane * K + r;
+        if (r < K && pos < n)
+            xs[pos] = ord_val(key[r]);
+    }
+    wave_lds_sync();
+    const TailView v = {xs, 0, 0, n};
+    const float S = eigen_sum_view(v, lane);
+    int k = 0;
+    if (lane == 0)
+        k = weighted_median_walk_view(v, n, S);
+    k = lane_int(k, 0);
+    if (lane == 0) {
+        const int sg = colg / p.ncols, col = colg - sg * p.ncols;
+        const size_t o = ((size_t)sg * p.np + q) * p.ncols + col;
+        p.out[o] = xs[k];
+        if (p.p2rank)
+            p.p2rank[o] = k;
+    }
 }
 
 // The largest key of a tail whose first Ls samples are in xs (the rest of
@@ -2759,7 +2815,22 @@ static size_t circus_shm(int nmax)
 
 hipError_t launch_circus(const CircusParams &p, hipStream_t s)
 {
-    circus_kernel<<<p.ncols * p.nsino, kBlock, circus_shm(p.n), s>>>(p);
+    // P2 of columns up to kP2Max rows: one wave per column
+    CircusParams c = p;
+    c.p2_wave = 0;
+    bool other = false;
+    const int cols = p.ncols * p.nsino;
+    const int wpb = kBlock / kWave;
+    for (int q = 0; q < p.np; q++) {
+        if (p.code[q] == 2 && p.n <= kP2Max) {
+            circus_p2_kernel<<<(cols + wpb - 1) / wpb, kBlock, 0, s>>>(p, q);
+            c.p2_wave = 1;
+        } else {
+            other = true;
+        }
+    }
+    if (other)
+        circus_kernel<<<cols, kBlock, circus_shm(p.n), s>>>(c);
     return hipGetLastError();
 }
 
