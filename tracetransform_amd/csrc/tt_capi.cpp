@@ -271,7 +271,7 @@ struct tt_context {
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     DevBuf imgT, trig, tab345, dft_tw, io_a, io_b, med_x, med_s, sino_ck;
-    DevBuf dft_w, p3_x, p3_spec;
+    DevBuf dft_w, p3_spec;
     DevBuf bw_sino, bw_dsino, bw_z, bw_rec, bw_t6src, bw_t7src;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
     DevBuf nos_w, nos_m, nos_q, nos_r, nos_dev, nos_sp, nos_ptrs, nos_h;
@@ -468,7 +468,7 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
         return TT_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->dft_w, &ctx->p3_x, &ctx->p3_spec,
+    DevBuf *bufs[] = {&ctx->dft_w, &ctx->p3_spec,
                       &ctx->imgT, &ctx->trig, &ctx->tab345, &ctx->dft_tw,
                       &ctx->io_a, &ctx->io_b, &ctx->med_x,
                       &ctx->med_s, &ctx->sino_ck, &ctx->bw_sino, &ctx->bw_dsino, &ctx->bw_z,
@@ -752,30 +752,28 @@ TT_API tt_status tt_circus(tt_context *ctx, const float *d_sino, int n,
     const int h = n / 2 + 1;
     const size_t cols = (size_t)n_sino * n_cols;
     if (p3) {
-        // P3's half spectra of every column at once: one FP64 GEMM with the
-        // DFT matrix, spec (2h x cols) = W^T X, on the matrix cores
+        // P3's half spectra of every column at once: one FP32 GEMM with the
+        // DFT matrix (twiddles rounded from double), spec (2h x cols) = W^T X,
+        // on the matrix cores
         if (ctx->dftw_n != n) {
-            TT_HIP(ctx->dft_w.ensure(sizeof(double) * (size_t)n * 2 * h));
-            TT_HIP(ttk::launch_dft_matrix(n, h, ctx->dft_w.as<double>(), st));
+            TT_HIP(ctx->dft_w.ensure(sizeof(float) * (size_t)n * 2 * h));
+            TT_HIP(ttk::launch_dft_matrix(n, h, ctx->dft_w.as<float>(), st));
             ctx->dftw_n = n;
         }
-        TT_HIP(ctx->p3_x.ensure(sizeof(double) * cols * n));
-        TT_HIP(ctx->p3_spec.ensure(sizeof(double) * cols * 2 * h));
+        TT_HIP(ctx->p3_spec.ensure(sizeof(float) * cols * 2 * h));
         if (!ctx->blas)
             TT_BLAS(rocblas_create_handle(&ctx->blas));
         TT_BLAS(rocblas_set_stream(ctx->blas, st));
         TT_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
-        p.p3_spec = ctx->p3_spec.as<double>();
+        p.p3_spec = ctx->p3_spec.as<float>();
         p.p3_h = h;
     }
     TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS, st));
     if (p3) {
-        const double one = 1.0, zero = 0.0;
-        TT_HIP(ttk::launch_to_double(d_sino, cols * n, ctx->p3_x.as<double>(), st));
-        TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
-                              2 * h, (int)cols, n, &one, ctx->dft_w.as<double>(), n,
-                              ctx->p3_x.as<double>(), n, &zero, ctx->p3_spec.as<double>(),
-                              2 * h));
+        const float one = 1.0f, zero = 0.0f;
+        TT_BLAS(rocblas_sgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
+                              2 * h, (int)cols, n, &one, ctx->dft_w.as<float>(), n, d_sino, n,
+                              &zero, ctx->p3_spec.as<float>(), 2 * h));
     }
     TT_HIP(ttk::launch_circus(p, st));
     TT_HIP(ctx->timer.end(st));

@@ -137,7 +137,7 @@ struct CircusParams {
     float *out;           // [nsino][np][ncols]
     int32_t *p2rank;      // [nsino][np][ncols] or null
     const float2 *dft_tw; // [n] (cos, sin)(2 pi k / n)
-    const double *p3_spec; // [nsino * ncols][2 * p3_h] half spectra (re, then im), or null
+    const float *p3_spec; // [nsino * ncols][2 * p3_h] half spectra (re, then im), or null
     int p3_h;             // n / 2 + 1
     const double *herm_w; // [nherm][n] Hermite weights
     int n;
@@ -257,8 +257,7 @@ hipError_t launch_sino_sort(const SortTParams &p, int32_t *queue_a, int32_t *que
 hipError_t launch_circus(const CircusParams &p, hipStream_t s);
 // W (n x 2h column-major, h = n / 2 + 1): W(j, k) = cos(2 pi j k / n) and
 // W(j, h + k) = -sin(2 pi j k / n) for k < h, from the exact phase jk mod n.
-hipError_t launch_dft_matrix(int n, int h, double *w, hipStream_t s);
-hipError_t launch_to_double(const float *in, size_t count, double *out, hipStream_t s);
+hipError_t launch_dft_matrix(int n, int h, float *w, hipStream_t s);
 
 // Batched path; `waves_per_block` in {1, 2, 4} and `blocks` (persistent grid)
 // are the autotuned launch shape.

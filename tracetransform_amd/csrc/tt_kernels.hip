@@ -871,7 +871,7 @@ __device__ __forceinline__ CircusLds circus_lds(float *smem, int nmax)
 __device__ void circus_column(const CircusLds &L, int n, int np, const int *code,
                               const int *herm, const double *herm_w, float *out,
                               size_t out_stride, int32_t *p2rank,
-                              const double *spec = nullptr, int spec_h = 0,
+                              const float *spec = nullptr, int spec_h = 0,
                               bool skip_p2 = false)
 {
     __shared__ float s_red[kBlock / kWave];
@@ -911,11 +911,12 @@ __device__ void circus_column(const CircusLds &L, int n, int np, const int *code
             if (p2rank && threadIdx.x == 0)
                 p2rank[q * out_stride] = s_rank;
         } else if (spec) {
-            // P3 from the column's FP64 half spectrum (a GEMM with the DFT
-            // matrix): pow(hypot(Re/N, Im/N), 4) and the trapz sum below
+            // P3 from the column's half spectrum (a GEMM with the DFT
+            // matrix): pow(hypot(Re/N, Im/N), 4) in double and the trapz
+            // sum below
             const int half = n / 2;
             for (int k = threadIdx.x; k <= half; k += kBlock) {
-                const double h = hypot(spec[k] / n, spec[spec_h + k] / n);
+                const double h = hypot((double)spec[k] / n, (double)spec[spec_h + k] / n);
                 const double h2 = h * h;
                 L.mod[k] = (float)(h2 * h2);
             }
@@ -1001,7 +1002,7 @@ __global__ __launch_bounds__(kBlock) void circus_kernel(CircusParams p)
                   p.p2_wave != 0);
 }
 
-__global__ __launch_bounds__(kBlock) void dft_matrix_kernel(int n, int h, double *w)
+__global__ __launch_bounds__(kBlock) void dft_matrix_kernel(int n, int h, float *w)
 {
     const size_t total = (size_t)n * 2 * h;
     for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < total;
@@ -1011,16 +1012,8 @@ __global__ __launch_bounds__(kBlock) void dft_matrix_kernel(int n, int h, double
         const long long t = ((long long)j * k) % n;
         double sn, cn;
         sincospi(2.0 * (double)t / (double)n, &sn, &cn);
-        w[e] = c < h ? cn : -sn;
+        w[e] = (float)(c < h ? cn : -sn);
     }
-}
-
-__global__ __launch_bounds__(kBlock) void to_double_kernel(const float *in, size_t count,
-                                                          double *out)
-{
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < count;
-         i += (size_t)gridDim.x * kBlock)
-        out[i] = (double)in[i];
 }
 
 // Every sinogram column of a batch; the image is found by binary search over
@@ -3128,15 +3121,9 @@ hipError_t launch_nos_poly(const double *S, const double *S2, int k, int count,
     return hipGetLastError();
 }
 
-hipError_t launch_dft_matrix(int n, int h, double *w, hipStream_t s)
+hipError_t launch_dft_matrix(int n, int h, float *w, hipStream_t s)
 {
     dft_matrix_kernel<<<grid_for((size_t)n * 2 * h), kBlock, 0, s>>>(n, h, w);
-    return hipGetLastError();
-}
-
-hipError_t launch_to_double(const float *in, size_t count, double *out, hipStream_t s)
-{
-    to_double_kernel<<<grid_for(count), kBlock, 0, s>>>(in, count, out);
     return hipGetLastError();
 }
 
