@@ -1425,25 +1425,32 @@ __device__ __forceinline__ uint64_t tail_key_at(const TailKey<IDX> (&key)[KR], i
 template <int KR, bool T6, bool IDX7>
 __device__ __forceinline__ uint64_t tail_finish(const TailKey<T6 || IDX7> (&key)[KR], int KO,
                                                 int ins, float *xs, int L, int Le, int lane,
-                                                int *k_out)
+                                                int *k_out, bool fs)
 {
     constexpr bool IDX = T6 || IDX7;
     const int zt = L - Le;
     // gather the sorted samples before overwriting the slice with roots
     float sq[KR];
+    auto roots = [&](auto fsc) {
+        constexpr bool FS = decltype(fsc)::value;
 #pragma unroll
-    for (int r = 0; r < KR; r++) {
-        const int pos = lane * KO + r;
-        sq[r] = 0.0f;
-        if (r < KO && pos < Le) {
-            if constexpr (T6)
-                sq[r] = sqrtf(xs[(uint32_t)key[r]]);
-            else if constexpr (IDX7)
-                sq[r] = sqrtf(ord_val((uint32_t)(key[r] >> 32)));
-            else
-                sq[r] = sqrtf(ord_val(key[r]));
+        for (int r = 0; r < KR; r++) {
+            const int pos = lane * KO + r;
+            sq[r] = 0.0f;
+            if (r < KO && pos < Le) {
+                if constexpr (T6)
+                    sq[r] = line_sqrt<FS>(xs[(uint32_t)key[r]]);
+                else if constexpr (IDX7)
+                    sq[r] = line_sqrt<FS>(ord_val((uint32_t)(key[r] >> 32)));
+                else
+                    sq[r] = line_sqrt<FS>(ord_val(key[r]));
+            }
         }
-    }
+    };
+    if (fs)
+        roots(std::true_type{});
+    else
+        roots(std::false_type{});
     wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < KR; r++) {
@@ -1472,7 +1479,7 @@ __device__ __forceinline__ uint64_t tail_finish(const TailKey<T6 || IDX7> (&key)
 // order: ties by ascending index).
 template <int K, bool T6, bool IDX7>
 __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le, int lane,
-                                                       int *k_out)
+                                                       int *k_out, bool fs)
 {
     constexpr bool IDX = T6 || IDX7;
     using Key = TailKey<IDX>;
@@ -1492,32 +1499,32 @@ __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le,
     // where the trailing zeros go: after the sorted part's zero keys in the
     // stable (index) order, or first when the order of equal values is free
     const int ins = IDX ? zin : 0;
-    return tail_finish<K, T6, IDX7>(key, K, ins, xs, L, Le, lane, k_out);
+    return tail_finish<K, T6, IDX7>(key, K, ins, xs, L, Le, lane, k_out, fs);
 }
 
 // The sorted tail with the smallest register network that holds its Le
 // sorted keys (at most KMAX per lane).
 template <int KMAX, bool T6, bool IDX7>
 __device__ __forceinline__ uint64_t sorted_tail_dispatch(float *xs, int L, int Le, int lane,
-                                                         int *k)
+                                                         int *k, bool fs)
 {
     const int kneed = (Le + kWave - 1) / kWave;
     if (kneed <= 1 || KMAX == 1)
-        return sorted_tail_median<1, T6, IDX7>(xs, L, Le, lane, k);
+        return sorted_tail_median<1, T6, IDX7>(xs, L, Le, lane, k, fs);
     if constexpr (KMAX >= 2)
         if (kneed <= 2 || KMAX == 2)
-            return sorted_tail_median<2, T6, IDX7>(xs, L, Le, lane, k);
+            return sorted_tail_median<2, T6, IDX7>(xs, L, Le, lane, k, fs);
     if constexpr (KMAX >= 4)
         if (kneed <= 4 || KMAX == 4)
-            return sorted_tail_median<4, T6, IDX7>(xs, L, Le, lane, k);
+            return sorted_tail_median<4, T6, IDX7>(xs, L, Le, lane, k, fs);
     if constexpr (KMAX >= 8)
         if (kneed <= 8 || KMAX == 8)
-            return sorted_tail_median<8, T6, IDX7>(xs, L, Le, lane, k);
+            return sorted_tail_median<8, T6, IDX7>(xs, L, Le, lane, k, fs);
     if constexpr (KMAX >= 16)
         if (kneed <= 16 || KMAX == 16)
-            return sorted_tail_median<16, T6, IDX7>(xs, L, Le, lane, k);
+            return sorted_tail_median<16, T6, IDX7>(xs, L, Le, lane, k, fs);
     if constexpr (KMAX >= 32)
-        return sorted_tail_median<32, T6, IDX7>(xs, L, Le, lane, k);
+        return sorted_tail_median<32, T6, IDX7>(xs, L, Le, lane, k, fs);
     return 0;  // not reached: kneed <= KMAX
 }
 
@@ -1721,8 +1728,9 @@ __device__ __forceinline__ bool sort_line(const SortTParams &p, int line, int wh
         if (DEFER && kneed > KMAX)
             return true;
         wave_lds_sync();
-        res = t6 ? sorted_tail_dispatch<KMAX, true, false>(xs, L, Le, lane, &k)
-                 : sorted_tail_dispatch<KMAX, false, IDX7>(xs, L, Le, lane, &k);
+        const bool fs = p.bbox && p.bbox[4] == 0;
+        res = t6 ? sorted_tail_dispatch<KMAX, true, false>(xs, L, Le, lane, &k, fs)
+                 : sorted_tail_dispatch<KMAX, false, IDX7>(xs, L, Le, lane, &k, fs);
     }
     store_tail_result<IDX7>(p, line, which, res, k, base, lane);
     return false;
@@ -1754,6 +1762,7 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
     const bool tr = fabsf(cs.y) > fabsf(cs.x);
     const float *src = tr ? p.imgT : p.img;
     const int sm = p.med_s[line], m = p.med_x[line];
+    const bool fs = p.bbox && p.bbox[4] == 0;  // short square roots (line_sqrt)
     const int L6 = n - sm, L7 = n - m;
     const int2 rows = line_rows(p.bbox, n, col, cs.x, cs.y);
     const int E = rows.y + 1 < n ? rows.y + 1 : n;
@@ -1806,9 +1815,9 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
             r7 = tail_max_key(xs + o7, Ls7, L7, false, lane);
         wave_lds_sync();
         if (!bad6)
-            r6 = sorted_tail_dispatch<KMAX, true, false>(xs + o6, L6, Le6, lane, &k6);
+            r6 = sorted_tail_dispatch<KMAX, true, false>(xs + o6, L6, Le6, lane, &k6, fs);
         if (!bad7)
-            r7 = sorted_tail_dispatch<KMAX, false, IDX7>(xs + o7, L7, Le7, lane, &k7);
+            r7 = sorted_tail_dispatch<KMAX, false, IDX7>(xs + o7, L7, Le7, lane, &k7, fs);
         store_tail_result<IDX7>(p, line, 0, r6, k6, sm, lane);
         store_tail_result<IDX7>(p, line, 1, r7, k7, m, lane);
         return 0;
@@ -1830,12 +1839,19 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
         ins6 += __popcll(__ballot(z));
     }
     sort_prefix<KMAX>(key6, K6, lane);
+    auto roots6 = [&](auto fsc) {
+        constexpr bool FS = decltype(fsc)::value;
 #pragma unroll
-    for (int r = 0; r < KMAX; r++) {
-        const int p6 = lane * K6 + r;
-        if (r < K6 && p6 < Le6)
-            xs2[p6] = sqrtf(xs[o6 + (uint32_t)key6[r]]);
-    }
+        for (int r = 0; r < KMAX; r++) {
+            const int p6 = lane * K6 + r;
+            if (r < K6 && p6 < Le6)
+                xs2[p6] = line_sqrt<FS>(xs[o6 + (uint32_t)key6[r]]);
+        }
+    };
+    if (fs)
+        roots6(std::true_type{});
+    else
+        roots6(std::false_type{});
     // T7: keys, sort, roots from the key values over xs (after every lane
     // has read its samples)
     Key7 key7[KMAX];
@@ -1852,16 +1868,23 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
     }
     sort_prefix<KMAX>(key7, K7, lane);
     wave_lds_sync();
+    auto roots7 = [&](auto fsc) {
+        constexpr bool FS = decltype(fsc)::value;
 #pragma unroll
-    for (int r = 0; r < KMAX; r++) {
-        const int p7 = lane * K7 + r;
-        if (r < K7 && p7 < Le7) {
-            if constexpr (IDX7)
-                xs[p7] = sqrtf(ord_val((uint32_t)(key7[r] >> 32)));
-            else
-                xs[p7] = sqrtf(ord_val(key7[r]));
+        for (int r = 0; r < KMAX; r++) {
+            const int p7 = lane * K7 + r;
+            if (r < K7 && p7 < Le7) {
+                if constexpr (IDX7)
+                    xs[p7] = line_sqrt<FS>(ord_val((uint32_t)(key7[r] >> 32)));
+                else
+                    xs[p7] = line_sqrt<FS>(ord_val(key7[r]));
+            }
         }
-    }
+    };
+    if (fs)
+        roots7(std::true_type{});
+    else
+        roots7(std::false_type{});
     wave_lds_sync();
     const TailView v6 = {xs2, ins6, L6 - Le6, L6}, v7 = {xs, ins7, L7 - Le7, L7};
     const float sg = eigen_sum_group<2>(lane >= 8 ? v7 : v6, lane);
