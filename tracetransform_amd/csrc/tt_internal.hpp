@@ -100,6 +100,7 @@ struct BatchCircusParams {
     int nmax;
     int np;
     int code[kMaxP];
+    int p2_wave;          // P2 entries written by circus_p2_batched_kernel (launcher's)
 };
 
 // Sort-based T-functionals T6/T7 for one padded image: one workgroup owns one

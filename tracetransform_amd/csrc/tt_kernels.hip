@@ -1053,7 +1053,8 @@ __global__ __launch_bounds__(kBlock) void circus_batched_kernel(BatchCircusParam
     __syncthreads();
     const int herm_none[kMaxP] = {};
     circus_column(L, n, p.np, p.code, herm_none, nullptr,
-                  im.circus + (size_t)t * p.np * p.A + a, p.A, nullptr);
+                  im.circus + (size_t)t * p.np * p.A + a, p.A, nullptr, nullptr, 0,
+                  p.p2_wave != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1575,16 +1576,11 @@ __device__ __forceinline__ void sort_prefix(Key (&k)[KMAX], int K, int lane)
 // workgroup.
 constexpr int kP2Max = 2048;
 
-__global__ __launch_bounds__(kBlock) void circus_p2_kernel(CircusParams p, int q)
+// P2 of the column x[0..n) (n <= kP2Max) by one wave, sorted values in xs;
+// the value and its rank are valid on lane 0.
+__device__ __forceinline__ void p2_wave(const float *x, int n, float *xs, int lane,
+                                        float *res, int *rank)
 {
-    __shared__ float s_x[kBlock / kWave][kP2Max];
-    const int w = wave_id(), lane = threadIdx.x & (kWave - 1);
-    const int colg = blockIdx.x * (kBlock / kWave) + w;
-    if (colg >= p.ncols * p.nsino)
-        return;
-    const int n = p.n;
-    const float *x = p.sino + (size_t)colg * n;
-    float *xs = s_x[w];
     const int K = keys_per_lane<32>(n);
     uint32_t key[32];
 #pragma unroll
@@ -1609,13 +1605,56 @@ __global__ __launch_bounds__(kBlock) void circus_p2_kernel(CircusParams p, int q
     if (lane == 0)
         k = weighted_median_walk_view(v, n, S);
     k = lane_int(k, 0);
+    *res = xs[k];
+    *rank = k;
+}
+
+__global__ __launch_bounds__(kBlock) void circus_p2_kernel(CircusParams p, int q)
+{
+    __shared__ float s_x[kBlock / kWave][kP2Max];
+    const int w = wave_id(), lane = threadIdx.x & (kWave - 1);
+    const int colg = blockIdx.x * (kBlock / kWave) + w;
+    if (colg >= p.ncols * p.nsino)
+        return;
+    float res;
+    int k;
+    p2_wave(p.sino + (size_t)colg * p.n, p.n, s_x[w], lane, &res, &k);
     if (lane == 0) {
         const int sg = colg / p.ncols, col = colg - sg * p.ncols;
         const size_t o = ((size_t)sg * p.np + q) * p.ncols + col;
-        p.out[o] = xs[k];
+        p.out[o] = res;
         if (p.p2rank)
             p.p2rank[o] = k;
     }
+}
+
+// The same over every sinogram column of a batch (image found by binary
+// search over the descriptors' first-column offsets; every n <= kP2Max).
+__global__ __launch_bounds__(kBlock) void circus_p2_batched_kernel(BatchCircusParams p, int q)
+{
+    __shared__ float s_x[kBlock / kWave][kP2Max];
+    const int w = wave_id(), lane = threadIdx.x & (kWave - 1);
+    const int colg = blockIdx.x * (kBlock / kWave) + w;
+    if (colg >= p.cols)
+        return;
+    int lo = 0, hi = p.nimg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (p.images[mid].col0 <= colg)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BatchImage &im = p.images[lo];
+    const int n = im.n;
+    const int lc = colg - im.col0;
+    const int t = lc / p.A;
+    const int a = lc - t * p.A;
+    float res;
+    int k;
+    p2_wave(im.sino + ((size_t)t * p.A + a) * n, n, s_x[w], lane, &res, &k);
+    if (lane == 0)
+        im.circus[((size_t)t * p.np + q) * p.A + a] = res;
 }
 
 // The largest key of a tail whose first Ls samples are in xs (the rest of
@@ -2853,7 +2892,21 @@ hipError_t launch_circus(const CircusParams &p, hipStream_t s)
 
 hipError_t launch_circus_batched(const BatchCircusParams &p, hipStream_t s)
 {
-    circus_batched_kernel<<<p.cols, kBlock, circus_shm(p.nmax), s>>>(p);
+    // P2 of columns up to kP2Max rows: one wave per column
+    BatchCircusParams c = p;
+    c.p2_wave = 0;
+    bool other = false;
+    const int wpb = kBlock / kWave;
+    for (int q = 0; q < p.np; q++) {
+        if (p.code[q] == 2 && p.nmax <= kP2Max) {
+            circus_p2_batched_kernel<<<(p.cols + wpb - 1) / wpb, kBlock, 0, s>>>(p, q);
+            c.p2_wave = 1;
+        } else {
+            other = true;
+        }
+    }
+    if (other)
+        circus_batched_kernel<<<p.cols, kBlock, circus_shm(p.nmax), s>>>(c);
     return hipGetLastError();
 }
 
