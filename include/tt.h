@@ -73,8 +73,10 @@ typedef struct tt_context tt_context;
 /* ---- context ----------------------------------------------------------- */
 
 /* One context per host thread and device: owns a HIP stream, the device
- * workspaces (trig and T3-T5 tables, transposed image, Hermite weights) and
- * the cached graph of the batched path.  No reference counterpart: the
+ * workspaces (trig and T3-T5 tables, transposed image, Hermite weights, the
+ * line kernel's median-walk checkpoints: 8 * ceil((n - 1) / 64) * A * n
+ * bytes, ~100 MB at n = 1449 and 360 angles) and the cached graph of the
+ * batched path.  No reference counterpart: the
  * reference keeps no state between getSinograms calls. */
 TT_API tt_status tt_context_create(int device, tt_context **out);
 TT_API tt_status tt_context_destroy(tt_context *ctx);
