@@ -1657,6 +1657,28 @@ __global__ __launch_bounds__(kBlock) void circus_p2_batched_kernel(BatchCircusPa
         im.circus[((size_t)t * p.np + q) * p.A + a] = res;
 }
 
+// findWeightedMedian of one column per wave (columns up to kP2Max rows, in
+// their own order, c++/src/circus.cpp:74-78): the column into the wave's LDS
+// row, the Eigen-order sum on lanes 0..7 and the walk on lane 0.
+__global__ __launch_bounds__(kBlock) void column_median_wave_kernel(const float *in, int n,
+                                                                    int cols, int32_t *med)
+{
+    __shared__ float s_x[kBlock / kWave][kP2Max];
+    const int w = wave_id(), lane = threadIdx.x & (kWave - 1);
+    const int col = blockIdx.x * (kBlock / kWave) + w;
+    if (col >= cols)
+        return;
+    const float *x = in + (size_t)col * n;
+    float *xs = s_x[w];
+    for (int i = lane; i < n; i += kWave)
+        xs[i] = x[i];
+    wave_lds_sync();
+    const TailView v = {xs, 0, 0, n};
+    const float S = eigen_sum_view(v, lane);
+    if (lane == 0)
+        med[col] = weighted_median_walk_view(v, n, S);
+}
+
 // The largest key of a tail whose first Ls samples are in xs (the rest of
 // its L are zero): with a negative or NaN sample, sqrt makes the Eigen sum
 // NaN, no prefix passes the test and findWeightedMedian returns the last
@@ -3120,6 +3142,11 @@ hipError_t launch_zscore(const float *in, int n, int count, float *out, hipStrea
 hipError_t launch_column_medians(const float *in, int n, int cols, int32_t *med,
                                  hipStream_t s)
 {
+    if (n <= kP2Max) {
+        const int wpb = kBlock / kWave;
+        column_median_wave_kernel<<<(cols + wpb - 1) / wpb, kBlock, 0, s>>>(in, n, cols, med);
+        return hipGetLastError();
+    }
     const size_t shm = sizeof(float) * (((size_t)n + 3) & ~(size_t)3);
     column_median_kernel<<<cols, kBlock, shm, s>>>(in, n, med);
     return hipGetLastError();
