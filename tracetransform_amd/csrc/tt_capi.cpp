@@ -1433,6 +1433,24 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
             tab[(size_t)w * nb + b] = xbuf[w] + off;
         tab[(size_t)3 * nb + b] = sp + AA * b;
     }
+    // The Gram-space steps after the first form M Q and Q Q in one batched
+    // call of 2 n_sino products.  Their buffers: Q in nos_h's first quarter,
+    // Q Q in its third, and M alternating between nos_q and the second
+    // quarter (see the loop below); one table of A, B and C pointers for
+    // each parity of the step.
+    double *hq = ctx->nos_h.as<double>(), *hm = hq + AA * n_sino, *hq2 = hq + 2 * AA * n_sino;
+    double *qb = ctx->nos_q.as<double>();
+    const size_t mq0 = tab.size();
+    for (int par = 0; par < 2; par++) {
+        double *mcur = par == 0 ? qb : hm, *mnext = par == 0 ? hm : qb;
+        for (int w = 0; w < 3; w++)
+            for (int b = 0; b < 2 * n_sino; b++) {
+                const int i = b % n_sino;
+                const bool mq = b < n_sino;  // M Q, else Q Q
+                double *base = w == 0 ? (mq ? mcur : hq) : w == 1 ? hq : (mq ? mnext : hq2);
+                tab.push_back(base + AA * i);
+            }
+    }
     TT_HIP(ctx->nos_ptrs.ensure(sizeof(double *) * tab.size()));
     TT_HIP(hipMemcpyAsync(ctx->nos_ptrs.ptr, tab.data(), sizeof(double *) * tab.size(),
                           hipMemcpyHostToDevice, st));
@@ -1507,12 +1525,20 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
             }
             if (step == kPolarTallSteps) {
                 std::swap(M, Q);  // M = Q (Q gets M's buffer)
+                TT_BLAS(mul(M, M, 0.0, Q2));
             } else {
-                TT_BLAS(mul(M, Q, 0.0, Mn));
+                // M Q and Q Q together from the parity's pointer table
+                const int par = (step - kPolarTallSteps - 1) & 1;
+                double *const *t = tab_d + mq0 + (size_t)par * 6 * n_sino;
+                if (Q != hq || Q2 != hq2 || M != (par == 0 ? qb : hm) ||
+                    Mn != (par == 0 ? hm : qb))
+                    return fail(TT_ERR_INVALID, "internal: NOS buffer rotation");
+                TT_BLAS(rocblas_dgemm_batched(ctx->blas, rocblas_operation_none,
+                                              rocblas_operation_none, A, A, A, &one, t, A,
+                                              t + 2 * n_sino, A, &zero, t + 4 * n_sino, A,
+                                              2 * n_sino));
                 std::swap(M, Mn);
             }
-            const double *Qs = step == kPolarTallSteps ? M : Q;
-            TT_BLAS(mul(Qs, Qs, 0.0, Q2));
             TT_BLAS(mul(S, Q2, 0.0, Sn));
             std::swap(S, Sn);
         }
