@@ -225,7 +225,7 @@ struct KernelTimer {
 // rewrite the context's tables) cannot change what the graph reads.
 struct BatchPlan {
     std::vector<int64_t> key;
-    DevBuf desc, imgT, trig, tab, counter, bbox;
+    DevBuf desc, imgT, trig, tab, counter, bbox, ckpt;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int waves_per_block = 4, blocks = 0;
@@ -250,6 +250,7 @@ struct BatchPlan {
         tab.release();
         counter.release();
         bbox.release();
+        ckpt.release();
         key.clear();
     }
 };
@@ -937,13 +938,19 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
             tot += (size_t)images[i].n * images[i].n;
         TT_HIP(B.imgT.ensure(sizeof(float) * tot));
         TT_HIP(B.bbox.ensure(ttk::kBboxInts * sizeof(int) * (size_t)nimg));
-        size_t off = 0;
+        size_t ck = 0;  // the line kernel's walk checkpoints, image after image
+        for (int i = 0; i < nimg; i++)
+            ck += (size_t)ttk::sino_ckpt_blocks(images[i].n) * images[i].n * n_angles;
+        TT_HIP(B.ckpt.ensure(sizeof(float2) * ck));
+        size_t off = 0, cko = 0;
         int items = 0, cols = 0;
         for (int k = 0; k < nimg; k++) {
             const tt_image &im = images[order[k]];
             ttk::BatchImage &b = d[k];
             b.img = im.d_image;
             b.bbox = B.bbox.as<int>() + ttk::kBboxInts * k;
+            b.ckpt = B.ckpt.as<float2>() + cko;
+            cko += (size_t)ttk::sino_ckpt_blocks(im.n) * im.n * n_angles;
             b.imgT = B.imgT.as<float>() + off;
             off += (size_t)im.n * im.n;
             b.sino = im.d_sino;
@@ -1728,13 +1735,19 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         TT_HIP(P.meds.ensure(sizeof(int32_t) * ln));
         TT_HIP(P.rec.ensure(sizeof(ttk::LineRecord) * ln));
         TT_HIP(B.bbox.ensure(ttk::kBboxInts * sizeof(int) * (size_t)nimg));
+        size_t ck = 0;  // the line kernel's walk checkpoints, image after image
+        for (int i = 0; i < nimg; i++)
+            ck += (size_t)ttk::sino_ckpt_blocks(images[i].n) * images[i].n * n_angles;
+        TT_HIP(B.ckpt.ensure(sizeof(float2) * ck));
         std::vector<ttk::BatchImage> d(nimg);
-        size_t po = 0, lo = 0, tiles = 0;
+        size_t po = 0, lo = 0, tiles = 0, cko = 0;
         int items = 0, cols = 0;
         for (int k = 0; k < nimg; k++) {
             const tt_image_grad &im = images[order[k]];
             ttk::BatchImage &b = d[k];
             b.bbox = B.bbox.as<int>() + ttk::kBboxInts * k;
+            b.ckpt = B.ckpt.as<float2>() + cko;
+            cko += (size_t)ttk::sino_ckpt_blocks(im.n) * im.n * n_angles;
             const size_t l = (size_t)im.n * n_angles;
             b.img = im.d_image;
             b.imgT = B.imgT.as<float>() + po;

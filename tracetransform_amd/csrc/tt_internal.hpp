@@ -61,6 +61,7 @@ struct BatchImage {
     int32_t *med_x;
     int32_t *med_s;
     const int *bbox;      // non-zero pixels' bounds {x0, x1, y0, y1} + flag, or null
+    float2 *ckpt;         // [sino_ckpt_blocks(n)][A][n] walk prefixes (workspace)
     int n;
     int nb;               // column blocks per angle
     int item0;            // first wave item of this image

@@ -625,11 +625,11 @@ __global__ __launch_bounds__(kBlock) void sino_t05_batched_kernel(BatchSinoParam
         if (!active)
             col = im.n - 1;
         const float2 cs = p.trig[a];
-        const LineDst dst = {im.sino, im.med_x, im.med_s, im.z345, nullptr, im.n, p.A, p.tabn,
+        const LineDst dst = {im.sino, im.med_x, im.med_s, im.z345, im.ckpt, im.n, p.A, p.tabn,
                              p.slot};
         const int2 rows = line_rows(im.bbox, im.n, col, cs.x, cs.y);
-        sino_line_any<NEED_X, NEED_S, DO345, false>(dst, im.img, im.imgT, im.bbox, s_tab, a,
-                                                    col, active, cs, rows.x, rows.y);
+        sino_line_any<NEED_X, NEED_S, DO345, true>(dst, im.img, im.imgT, im.bbox, s_tab, a,
+                                                   col, active, cs, rows.x, rows.y);
     }
 }
 
