@@ -54,6 +54,30 @@ def test_nos_matches_fp64_polar_factor(oracle, gpu):
         assert np.abs(gram - np.eye(A)).max() <= 1e-5
 
 
+def test_nos_of_mixed_sign_sinograms(oracle, gpu):
+    # the column medians that align the sinogram walk raw columns of any
+    # sign: the T0 sinogram of an image with negative pixels (negative and
+    # mixed column sums) must give the reference's offsets, hence its aligned
+    # height and Hermite centre
+    side = 40
+    rng = np.random.default_rng(17)
+    img_cm = (rng.random(side * side, dtype=np.float32) - 0.45).astype(np.float32)
+    img, n = padded(oracle, img_cm, side, side)
+    A = 24
+    sino, _ = oracle.sinograms(img, n, A, 15.0, [0], nthreads=8)
+    cols = sino[0]
+    offs = np.array([oracle.weighted_median(cols[a]) for a in range(A)])
+    print(f"[nos signed] n={n} offsets {offs.min()}..{offs.max()}", flush=True)
+    nos, center = core.nearest_orthonormal_sinogram(dev(cols))
+    ref, rc = oracle.nearest_orthonormal_sinogram(cols)
+    assert center == rc and tuple(nos.shape) == ref.shape
+    q = nos.double()
+    assert np.abs((q @ q.t()).cpu().numpy() - np.eye(A)).max() <= 1e-5
+    err = normwise_err(nos.cpu().numpy(), ref)
+    print(f"[nos signed] rows={ref.shape[1]} center={rc} err={err:.2e}", flush=True)
+    assert err <= 1e-5, err
+
+
 @pytest.mark.parametrize("eps", [1e-3, 1e-4, 1e-5])
 def test_nos_ill_conditioned(oracle, gpu, eps):
     # smooth bumps plus a small random part: condition numbers ~2e4, ~2e5,

@@ -266,6 +266,23 @@ def test_circus_sizes(oracle, gpu):
     check_circus(oracle, sino, [(2,), (1,)])
 
 
+def test_p2_signed_columns(oracle, gpu):
+    # P2 walks the sorted column, whose prefix falls while the values are
+    # negative: negative and zero column sums, all-negative columns and the
+    # sorted column [-3, -1 x 8] (findWeightedMedian returns 0 there), at
+    # heights that take 1, 2, 16 and 32 sort keys per lane
+    rng = np.random.default_rng(41)
+    for n, A in ((9, 4), (91, 64), (725, 16), (2048, 4)):
+        sino = np.empty((A, n), np.float32)
+        for a in range(A):
+            shift = (0.5, 0.9, 1.5, 0.3)[a % 4]
+            sino[a] = (rng.random(n) - shift).astype(np.float32)
+        sino[0, :] = -1.0
+        sino[0, min(3, n - 1)] = -3.0
+        sino[1] = np.where(np.arange(n) % 2 == 0, 1.0, -1.0).astype(np.float32)
+        check_circus(oracle, sino, [(2,), (1,)])
+
+
 def test_hermite_tables_follow_height_and_centre(oracle, gpu):
     # the same (order, centre) at two heights, then alternating centres: the
     # weight tables depend on the height through the z grid, and the cached

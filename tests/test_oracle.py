@@ -35,6 +35,82 @@ def test_pad_nonsquare_places_image(oracle):
     assert abs(dx - (n - cols - dx)) <= 1 and abs(dy - (n - rows - dy)) <= 1
 
 
+# ------------------------------------------------------------------- resize
+# c++/src/auxiliary.cpp:198-221: out(row, col) = interpolate(in, p) with
+# p = ((col, row) + 0.5) * diag(in_rows / rows, in_cols / cols) - 0.5 on the
+# interior (borders stay 0).  p.x (the column coordinate) is scaled by the
+# row ratio and p.y by the column ratio, as the reference's diagonal
+# transform does.
+
+def _resize_grid(in_rows, in_cols, rows, cols):
+    """Exact (double) sample coordinates of resize's interior, and the mask
+    of the samples whose four taps lie inside the input."""
+    col = np.arange(cols)[None, :].astype(np.float64)
+    row = np.arange(rows)[:, None].astype(np.float64)
+    x = (col + 0.5) * (in_rows / rows) - 0.5 + 0 * row
+    y = (row + 0.5) * (in_cols / cols) - 0.5 + 0 * col
+    inner = np.zeros((rows, cols), bool)
+    inner[1:rows - 1, 1:cols - 1] = True
+    inside = (np.floor(x) >= 0) & (np.floor(x) + 1 <= in_cols - 1) & \
+             (np.floor(y) >= 0) & (np.floor(y) + 1 <= in_rows - 1)
+    return x, y, inner, inner & inside
+
+
+@pytest.mark.parametrize("side", [7, 40, 255])
+def test_resize_same_size_is_identity_inside(oracle, side):
+    # scale 1: p = (col, row) exactly, fractions 0, so each interior sample
+    # is its own pixel bit for bit; the one-pixel border is 0
+    img = np.random.default_rng(side).random(side * side, dtype=np.float32)
+    out = oracle.resize(img, side, side, side, side).reshape(side, side)  # [col, row]
+    ref = img.reshape(side, side)
+    assert np.array_equal(out[1:-1, 1:-1], ref[1:-1, 1:-1])
+    assert not out[0, :].any() and not out[-1, :].any()
+    assert not out[:, 0].any() and not out[:, -1].any()
+
+
+@pytest.mark.parametrize("src,dst", [((64, 64), (32, 32)), ((32, 32), (64, 64)),
+                                     ((64, 64), (32, 64)), ((40, 40), (80, 20))])
+def test_resize_reproduces_dyadic_ramps(oracle, src, dst):
+    # f(row, col) = 3 row + 5 col - 7: bilinear interpolation reproduces an
+    # affine function, and with the dyadic scales 2 and 1/2 every coordinate,
+    # fraction and product is exact in float, so the float result equals
+    # 3 y + 5 x - 7 exactly wherever the four taps are inside the input.  A
+    # swapped axis or scale, or a dropped 0.5 shift, moves x or y.
+    in_rows, in_cols = src
+    rows, cols = dst
+    r = np.arange(in_rows)[:, None]
+    c = np.arange(in_cols)[None, :]
+    f = (3.0 * r + 5.0 * c - 7.0).astype(np.float32)          # [row, col]
+    img = np.ascontiguousarray(f.T).ravel()                   # column-major
+    out = oracle.resize(img, in_rows, in_cols, rows, cols).reshape(cols, rows).T
+    x, y, inner, ok = _resize_grid(in_rows, in_cols, rows, cols)
+    assert ok.sum() >= 0.2 * inner.sum()
+    want = 3.0 * y + 5.0 * x - 7.0
+    assert np.array_equal(out[ok].astype(np.float64), want[ok])
+    assert not out[~inner].any()
+
+
+def test_resize_constant_and_brute_force(oracle):
+    # a constant image gives a constant interior (to float rounding of the
+    # tap weights); random images match a double evaluation of the bilinear
+    # taps at the exact coordinates (float coordinates differ by < 1e-5)
+    in_rows = in_cols = 60
+    rows = cols = 37
+    const = np.full(in_rows * in_cols, 2.5, np.float32)
+    out = oracle.resize(const, in_rows, in_cols, rows, cols).reshape(cols, rows).T
+    x, y, inner, ok = _resize_grid(in_rows, in_cols, rows, cols)
+    assert np.abs(out[ok] - 2.5).max() <= 4e-7 * 2.5
+    img = np.random.default_rng(4).random((in_cols, in_rows), dtype=np.float32)
+    out = oracle.resize(img.ravel(), in_rows, in_cols, rows, cols).reshape(cols, rows).T
+    f = img.T.astype(np.float64)                               # [row, col]
+    xi, yi = np.floor(x).astype(int), np.floor(y).astype(int)
+    fx, fy = x - xi, y - yi
+    xi, yi = np.clip(xi, 0, in_cols - 2), np.clip(yi, 0, in_rows - 2)
+    want = (f[yi, xi] * (1 - fx) * (1 - fy) + f[yi, xi + 1] * fx * (1 - fy)
+            + f[yi + 1, xi] * (1 - fx) * fy + f[yi + 1, xi + 1] * fx * fy)
+    assert np.abs(out[ok] - want[ok]).max() <= 2e-5
+
+
 # ------------------------------------------------------------------- rotate
 
 def _rot(oracle, img, n, a_step, step=1.0):

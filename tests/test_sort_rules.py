@@ -91,3 +91,58 @@ def test_trimmed_tail_rules_match_full_sort(oracle):
             assert got_v == v or (np.isnan(got_v) and np.isnan(v)), (trial, t6)
             if t6:
                 assert got_i == d["src"], (trial, t6)
+
+
+def block_walk(c, S, sign_test=True):
+    """findWeightedMedian's walk as walk_segment runs it: blocks of 8 summed
+    first and skipped when their last prefix fails the test, which is exact
+    only when no value of the block is negative or NaN (sign_test keeps the
+    element walk for such blocks)."""
+    f = np.float32
+    acc = f(0)
+    j = 0
+    while j + 8 <= c.size:
+        t = acc
+        for u in range(8):
+            t = f(t + c[j + u])
+        mono = bool(np.all(c[j:j + 8] >= 0)) if sign_test else True
+        if mono and not (f(2) * t >= S):
+            acc = t
+            j += 8
+            continue
+        for u in range(8):
+            acc = f(acc + c[j + u])
+            if f(2) * acc >= S:
+                return j + u
+        j += 8
+    for k in range(j, c.size):
+        acc = f(acc + c[k])
+        if f(2) * acc >= S:
+            return k
+    return c.size - 1
+
+
+def test_block_walk_exact_for_signed_columns(oracle):
+    # P2 walks its sorted column and the orthonormal step its raw columns,
+    # both of any sign; the block skip must give findWeightedMedian's index
+    # (c++/src/functionals.cpp:21-30) on every one of them
+    rng = np.random.default_rng(3)
+    cols = [np.array([-3.0] + [-1.0] * 8, np.float32),
+            np.array([-1.0] * 16 + [5.0] * 3, np.float32),
+            np.array([2.0, -1.0, -1.0, -1.0, 0.5, 0.5, 0.5, 0.5, 0.5, 0.5], np.float32),
+            np.full(24, np.nan, np.float32), np.zeros(17, np.float32)]
+    for trial in range(1500):
+        n = int(rng.integers(1, 200))
+        x = (rng.random(n) - (0.3, 0.5, 0.9)[trial % 3]).astype(np.float32)
+        cols.append(np.sort(x) if trial % 2 else x)
+    fails_without = 0
+    for c in cols:
+        S = oracle.eigen_sum(c)
+        want = oracle.weighted_median(c)
+        assert block_walk(c, S) == want, c
+        fails_without += block_walk(c, S, sign_test=False) != want
+    # the unguarded skip (monotone prefixes assumed) misses some of them,
+    # starting with the sorted column [-3, -1 x 8]: index 0, not 8
+    assert block_walk(cols[0], oracle.eigen_sum(cols[0]), sign_test=False) == 8
+    assert oracle.weighted_median(cols[0]) == 0
+    assert fails_without > 50

@@ -25,6 +25,9 @@ namespace {
 
 thread_local std::string g_err;
 
+// Dynamic LDS of one workgroup kept within 64 KiB.
+constexpr size_t kLdsLimit = 64 * 1024;
+
 tt_status fail(tt_status st, const char *fmt, ...)
 {
     char buf[512];
@@ -260,6 +263,7 @@ struct BatchPlan {
 struct BackwardPlan {
     BatchPlan g;                  // descriptors, tables, graph
     DevBuf sino, dsino, z, medx, meds, rec;
+    int gather_tile = 32, gather_waves = 4;
 };
 
 struct tt_context {
@@ -273,7 +277,7 @@ struct tt_context {
     hipStream_t stream = nullptr;
     DevBuf imgT, trig, tab345, dft_tw, io_a, io_b, med_x, med_s, sino_ck;
     DevBuf dft_w, p3_spec;
-    DevBuf bw_sino, bw_dsino, bw_z, bw_rec, bw_t6src, bw_t7src;
+    DevBuf bw_sino, bw_dsino, bw_z, bw_rec, bw_rec67, bw_part, bw_t6src, bw_t7src;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
     DevBuf nos_w, nos_m, nos_q, nos_r, nos_dev, nos_sp, nos_ptrs, nos_h;
     double *h_dev = nullptr;  // pinned host copy of the NOS convergence partials
@@ -1179,11 +1183,55 @@ tt_status forward_for_adjoint(tt_context *ctx, const float *d_image, int n, int 
     return TT_OK;
 }
 
+// Launch shape of the transposed-rotation gather: the largest tile (32,
+// else 16 pixels) whose workgroup of `waves` waves fits the LDS limit with
+// the T3..T5 tables, and for a single image enough angle groups that the
+// tiles x groups waves cover every SIMD of the GPU twice.
+struct GatherLaunch {
+    int tile, waves, groups;
+};
+
+GatherLaunch choose_gather(int tiles32, int tiles16, int A, int cus, int tabn, bool do345,
+                           bool do12, bool do67, bool split_angles)
+{
+    GatherLaunch g = {16, 1, 1};
+    bool found = false;
+    for (int tile : {32, 16}) {
+        for (int waves : {4, 2, 1}) {
+            if (ttk::gather_lds_bytes(tile, waves, tabn, do345, do12, do67) <= kLdsLimit) {
+                g = {tile, waves, 1};
+                found = true;
+                break;
+            }
+        }
+        if (found)
+            break;
+    }
+    if (split_angles) {
+        const int tiles = g.tile == 32 ? tiles32 : tiles16;
+        const int want = 8 * cus;
+        int groups = (want + tiles - 1) / tiles;
+        groups = groups > 32 ? 32 : groups;
+        groups = groups > A ? A : groups;
+        g.groups = groups < 1 ? 1 : groups;
+    }
+    return g;
+}
+
+int tile_count(int n, int tile)
+{
+    const int t = (n + tile - 1) / tile;
+    return t * t;
+}
+
 tt_status image_vjp(tt_context *ctx, int n, int A, const int *slot,
                     const float *d_dsino, float *d_dimage)
 {
     const size_t lines = (size_t)n * A;
+    const bool do67 = slot[6] >= 0 || slot[7] >= 0;
     TT_HIP(ctx->bw_rec.ensure(sizeof(ttk::LineRecord) * lines));
+    if (do67)
+        TT_HIP(ctx->bw_rec67.ensure(sizeof(ttk::LineRecord67) * lines));
     hipStream_t st = ctx->stream;
     ttk::RecordParams r = {};
     r.dsino = d_dsino;
@@ -1193,18 +1241,31 @@ tt_status image_vjp(tt_context *ctx, int n, int A, const int *slot,
     r.t7_src = slot[7] >= 0 ? ctx->bw_t7src.as<int32_t>() : nullptr;
     r.z345 = ctx->bw_z.as<float2>();
     r.rec = ctx->bw_rec.as<ttk::LineRecord>();
+    r.rec67 = do67 ? ctx->bw_rec67.as<ttk::LineRecord67>() : nullptr;
     r.n = n;
     r.A = A;
     for (int k = 0; k < 8; k++)
         r.slot.s[k] = slot[k];
     ttk::GatherParams g = {};
     g.rec = r.rec;
+    g.rec67 = r.rec67;
     g.trig = ctx->trig.as<float2>();
     g.tab345 = ctx->tab345.as<float2>();
     g.dimg = d_dimage;
     g.n = n;
     g.A = A;
     g.do345 = slot[3] >= 0 || slot[4] >= 0 || slot[5] >= 0;
+    g.do12 = slot[1] >= 0 || slot[2] >= 0;
+    const GatherLaunch gl =
+        choose_gather(tile_count(n, 32), tile_count(n, 16), A, device_cus(ctx->device), n,
+                      g.do345, g.do12, do67, true);
+    g.tile = gl.tile;
+    g.waves = gl.waves;
+    g.groups = gl.groups;
+    if (g.groups > 1) {
+        TT_HIP(ctx->bw_part.ensure(sizeof(float) * (size_t)g.groups * n * n));
+        g.part = ctx->bw_part.as<float>();
+    }
     TT_HIP(ctx->timer.begin(TT_KERNEL_GATHER, st));
     TT_HIP(ttk::launch_records(r, st));
     TT_HIP(ttk::launch_gather(g, st));
@@ -1739,6 +1800,16 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         for (int i = 0; i < nimg; i++)
             ck += (size_t)ttk::sino_ckpt_blocks(images[i].n) * images[i].n * n_angles;
         TT_HIP(B.ckpt.ensure(sizeof(float2) * ck));
+        int tiles32 = 0, tiles16 = 0;
+        for (int i = 0; i < nimg; i++) {
+            tiles32 += tile_count(images[i].n, 32);
+            tiles16 += tile_count(images[i].n, 16);
+        }
+        const GatherLaunch gl = choose_gather(tiles32, tiles16, n_angles,
+                                              device_cus(ctx->device), nmax, do345,
+                                              slot[1] >= 0 || slot[2] >= 0, false, false);
+        P.gather_tile = gl.tile;
+        P.gather_waves = gl.waves;
         std::vector<ttk::BatchImage> d(nimg);
         size_t po = 0, lo = 0, tiles = 0, cko = 0;
         int items = 0, cols = 0;
@@ -1767,8 +1838,7 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
             b.line0 = (int)lo;
             b.pix0 = (int)po;
             b.tile0 = (int)tiles;
-            const size_t tn = (size_t)(im.n + 15) / 16;  // 16x16 gather tiles
-            tiles += tn * tn;
+            tiles += (size_t)tile_count(im.n, gl.tile);
             items += b.nb * n_angles;
             cols += nt * n_angles;
             po += (size_t)im.n * im.n;
@@ -1835,7 +1905,7 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         if (ce == hipSuccess)
             ce = ttk::launch_records_batched(ap, own);
         if (ce == hipSuccess)
-            ce = ttk::launch_gather_batched(ap, own);
+            ce = ttk::launch_gather_batched(ap, P.gather_tile, P.gather_waves, own);
         hipGraph_t g = nullptr;
         hipError_t ee = hipStreamEndCapture(own, &g);
         TT_HIP(ce);

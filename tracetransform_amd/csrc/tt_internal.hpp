@@ -40,16 +40,21 @@ struct SinoParams {
 // Adjoint.  Per-line record of the rotated-domain gradient of one angle's
 // column: G(row) = g0 + [row >= m] (g1 (row-m) + g2 (row-m)^2)
 //                + [row > sm] sum_t (alpha_t W_t.re[row-sm] + beta_t W_t.im[row-sm]).
-//          + [row == r6] c6 + [row == r7] c7
-// (T6 = q x[sm + q] and T7 = x[src7] with the sort held fixed: c6 = dT6 * q
-// at row r6 = sm + q, c7 = dT7 at row r7 = src7; rows -1 when absent).
+// The first 32 bytes carry what T0 and T3..T5 need.
 struct LineRecord {
-    float g0, g1, g2;
-    int m, sm;
+    float g0;
+    int sm;
     float ab[6];          // alpha3, beta3, alpha4, beta4, alpha5, beta5
+    float g1, g2;
+    int m;
+    int pad;
+};
+// T6 = q x[sm + q] and T7 = x[src7] with the sort held fixed add
+// [row == r6] c6 + [row == r7] c7 (c6 = dT6 * q at row r6 = sm + q, c7 = dT7
+// at row r7 = src7; rows -1 when absent).
+struct LineRecord67 {
     int r6, r7;
     float c6, c7;
-    float pad[3];
 };
 
 // One image of a batch (device array, ordered by first item).
@@ -75,7 +80,7 @@ struct BatchImage {
     float *dimg;          // n x n
     int line0;            // first line (angle*n + col) over the batch
     int pix0;             // first pixel over the batch
-    int tile0;            // first 16x16 gather tile over the batch
+    int tile0;            // first gather tile over the batch (tiles of the launch's side)
 };
 
 struct BatchSinoParams {
@@ -159,20 +164,32 @@ struct RecordParams {
     const int32_t *t7_src; // [A][n] T7 source row, or null
     const float2 *z345;   // [3][A][n]
     LineRecord *rec;      // [A][n]
+    LineRecord67 *rec67;  // [A][n] (T6/T7 requested) or null
     int n;
     int A;
     Slots slot;
 };
 
+// Transposed rotation: one wave per (tile x tile pixel tile, angle group);
+// with groups > 1 each group writes its partial image into part and
+// gather_reduce sums them in group order.
 struct GatherParams {
-    const LineRecord *rec;  // [A][n]
-    const float2 *trig;     // [A]
-    const float2 *tab345;   // [3][n]
-    float *dimg;            // n x n column-major
+    const LineRecord *rec;      // [A][n]
+    const LineRecord67 *rec67;  // [A][n] or null
+    const float2 *trig;         // [A]
+    const float2 *tab345;       // [3][n]
+    float *dimg;                // n x n column-major
+    float *part;                // [groups][n][n] (groups > 1)
     int n;
     int A;
     int do345;
+    int do12;                   // T1 or T2 requested
+    int tile;                   // 16 or 32
+    int groups;                 // angle groups
+    int waves;                  // waves per workgroup (1..8)
 };
+// Dynamic LDS bytes of one gather workgroup.
+size_t gather_lds_bytes(int tile, int waves, int tabn, bool do345, bool do12, bool do67);
 
 // Circus vector-Jacobian product: one workgroup per sinogram column.
 struct CircusVjpParams {
@@ -281,7 +298,7 @@ struct BatchAdjointParams {
     int nmax;
     int lines;            // total lines (sum of A * n)
     int pixels;           // total pixels (sum of n * n)
-    int tiles;            // total 16x16 gather tiles
+    int tiles;            // total gather tiles (of the launch's tile side)
     int cols;             // total sinogram columns (sum of nt * A)
     const float2 *trig;
     const float2 *tab345; // [3][tabn]
@@ -292,6 +309,9 @@ struct BatchAdjointParams {
 };
 hipError_t launch_circus_vjp_batched(const BatchAdjointParams &p, hipStream_t s);
 hipError_t launch_records_batched(const BatchAdjointParams &p, hipStream_t s);
-hipError_t launch_gather_batched(const BatchAdjointParams &p, hipStream_t s);
+// Gather of a batch with tiles of side `tile` (16 or 32; images[].tile0 count
+// such tiles) and `waves` waves per workgroup.
+hipError_t launch_gather_batched(const BatchAdjointParams &p, int tile, int waves,
+                                 hipStream_t s);
 
 }  // namespace ttk

@@ -1305,10 +1305,14 @@ __device__ __forceinline__ float eigen_sum_view(const TailView &v, int lane)
 // only at its first position (when nothing before it passed, that is only
 // possible with ins == 0 and S <= 0).
 // Walks c[j0..j1); returns the first passing j + off, or -1 with the prefix
-// carried in *integral.  Every c is >= 0, so the prefix never decreases and
-// the test, once passed, stays passed: a block of 8 is summed first and
-// walked element by element only when its last prefix passes (the same
-// additions in the same order, so the same prefixes).
+// carried in *integral.  A block of 8 is summed first (the same additions in
+// the same order as the element walk, so the same prefixes); when none of its
+// values is negative or NaN the prefix does not decrease inside it, so a
+// block whose last prefix fails the test holds no passing position and is
+// skipped.  Any other block is walked element by element.  NONNEG: every c
+// is known to be >= 0 (square roots of non-negative samples), which drops
+// the per-block sign test.
+template <bool NONNEG>
 __device__ __forceinline__ int walk_segment(const float *c, int j0, int j1, int off, float S,
                                             float *integral)
 {
@@ -1320,19 +1324,23 @@ __device__ __forceinline__ int walk_segment(const float *c, int j0, int j1, int 
         for (int u = 0; u < 8; u++)
             v[u] = c[j + u];
         float t = acc;
+        bool mono = true;
 #pragma unroll
-        for (int u = 0; u < 8; u++)
+        for (int u = 0; u < 8; u++) {
             t += v[u];
-        if (2.0f * t >= S) {
-#pragma unroll
-            for (int u = 0; u < 7; u++) {
-                acc += v[u];
-                if (2.0f * acc >= S)
-                    return j + u + off;
-            }
-            return j + 7 + off;
+            if (!NONNEG)
+                mono = mono & (v[u] >= 0.0f);
         }
-        acc = t;
+        if (mono && !(2.0f * t >= S)) {
+            acc = t;
+            continue;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            acc += v[u];
+            if (2.0f * acc >= S)
+                return j + u + off;
+        }
     }
     for (; j < j1; j++) {
         acc += c[j];
@@ -1343,15 +1351,16 @@ __device__ __forceinline__ int walk_segment(const float *c, int j0, int j1, int 
     return -1;
 }
 
+template <bool NONNEG>
 __device__ __forceinline__ int weighted_median_walk_view(const TailView &v, int Le, float S)
 {
     float integral = 0.0f;
-    int k = walk_segment(v.c, 0, v.ins, 0, S, &integral);
+    int k = walk_segment<NONNEG>(v.c, 0, v.ins, 0, S, &integral);
     if (k >= 0)
         return k;
     if (v.zt > 0 && 2.0f * integral >= S)
         return v.ins;
-    k = walk_segment(v.c, v.ins, Le, v.zt, S, &integral);
+    k = walk_segment<NONNEG>(v.c, v.ins, Le, v.zt, S, &integral);
     return k >= 0 ? k : v.L - 1;
 }
 
@@ -1464,7 +1473,7 @@ __device__ __forceinline__ uint64_t tail_finish(const TailKey<T6 || IDX7> (&key)
     const float S = eigen_sum_view(view, lane);
     int k = 0;
     if (lane == 0)
-        k = weighted_median_walk_view(view, Le, S);
+        k = weighted_median_walk_view<true>(view, Le, S);
     k = lane_int(k, 0);
     *k_out = k;
     return tail_key_at<KR, IDX>(key, KO, k, ins, zt, Le);
@@ -1603,7 +1612,7 @@ __device__ __forceinline__ void p2_wave(const float *x, int n, float *xs, int la
     const float S = eigen_sum_view(v, lane);
     int k = 0;
     if (lane == 0)
-        k = weighted_median_walk_view(v, n, S);
+        k = weighted_median_walk_view<false>(v, n, S);
     k = lane_int(k, 0);
     *res = xs[k];
     *rank = k;
@@ -1676,7 +1685,7 @@ __global__ __launch_bounds__(kBlock) void column_median_wave_kernel(const float 
     const TailView v = {xs, 0, 0, n};
     const float S = eigen_sum_view(v, lane);
     if (lane == 0)
-        med[col] = weighted_median_walk_view(v, n, S);
+        med[col] = weighted_median_walk_view<false>(v, n, S);
 }
 
 // The largest key of a tail whose first Ls samples are in xs (the rest of
@@ -1952,7 +1961,7 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
     const float S6 = lane_float(sg, 0), S7 = lane_float(sg, 8);
     int kw = 0;
     if (lane < 2)  // lane 0 walks T6, lane 1 T7
-        kw = weighted_median_walk_view(lane ? v7 : v6, lane ? Le7 : Le6, lane ? S7 : S6);
+        kw = weighted_median_walk_view<true>(lane ? v7 : v6, lane ? Le7 : Le6, lane ? S7 : S6);
     const int k6 = lane_int(kw, 0), k7 = lane_int(kw, 1);
     const uint64_t r6 = tail_key_at<KMAX, true>(key6, K6, k6, ins6, L6 - Le6, Le6);
     const uint64_t r7 = tail_key_at<KMAX, IDX7>(key7, K7, k7, ins7, L7 - Le7, Le7);
@@ -2037,26 +2046,17 @@ void sino_sort_kernel(SortTParams p)
 // ---------------------------------------------------------------------------
 
 // One LineRecord per (angle, column): folds the sinogram gradient of every
-// requested T0..T5 into the coefficients of G(row).
-__device__ __forceinline__ LineRecord make_record(const float *dsino, const int32_t *med_x,
-                                                  const int32_t *med_s, const int32_t *t6_src,
-                                                  const int32_t *t7_src, const float2 *z345,
-                                                  size_t plane, size_t line, Slots slot)
+// requested T0..T5 into the coefficients of G(row); the T6/T7 one-sample
+// terms go to a LineRecord67 when those functionals are requested.
+__device__ __forceinline__ void make_record(const float *dsino, const int32_t *med_x,
+                                            const int32_t *med_s, const int32_t *t6_src,
+                                            const int32_t *t7_src, const float2 *z345,
+                                            size_t plane, size_t line, Slots slot,
+                                            LineRecord *out, LineRecord67 *out67)
 {
     LineRecord r = {};
     r.m = med_x ? med_x[line] : 0;
     r.sm = med_s ? med_s[line] : 0;
-    r.r6 = -1;
-    r.r7 = -1;
-    if (slot.s[6] >= 0 && t6_src) {
-        const int q = t6_src[line];
-        r.r6 = r.sm + q;
-        r.c6 = dsino[slot.s[6] * plane + line] * (float)q;
-    }
-    if (slot.s[7] >= 0 && t7_src) {
-        r.r7 = t7_src[line];
-        r.c7 = dsino[slot.s[7] * plane + line];
-    }
     if (slot.s[0] >= 0)
         r.g0 = dsino[slot.s[0] * plane + line];
     if (slot.s[1] >= 0)
@@ -2075,7 +2075,20 @@ __device__ __forceinline__ LineRecord make_record(const float *dsino, const int3
             r.ab[2 * k + 1] = g * z.y;
         }
     }
-    return r;
+    out[line] = r;
+    if (out67) {
+        LineRecord67 q = {-1, -1, 0.0f, 0.0f};
+        if (slot.s[6] >= 0 && t6_src) {
+            const int s6 = t6_src[line];
+            q.r6 = r.sm + s6;
+            q.c6 = dsino[slot.s[6] * plane + line] * (float)s6;
+        }
+        if (slot.s[7] >= 0 && t7_src) {
+            q.r7 = t7_src[line];
+            q.c7 = dsino[slot.s[7] * plane + line];
+        }
+        out67[line] = q;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void records_kernel(RecordParams p)
@@ -2083,8 +2096,8 @@ __global__ __launch_bounds__(kBlock) void records_kernel(RecordParams p)
     const size_t lines = (size_t)p.A * p.n;
     for (size_t line = (size_t)blockIdx.x * kBlock + threadIdx.x; line < lines;
          line += (size_t)gridDim.x * kBlock)
-        p.rec[line] = make_record(p.dsino, p.med_x, p.med_s, p.t6_src, p.t7_src, p.z345,
-                                  lines, line, p.slot);
+        make_record(p.dsino, p.med_x, p.med_s, p.t6_src, p.t7_src, p.z345, lines, line,
+                    p.slot, p.rec, p.rec67);
 }
 
 template <class F>
@@ -2109,201 +2122,333 @@ __global__ __launch_bounds__(kBlock) void records_batched_kernel(BatchAdjointPar
         const BatchImage &im = p.images[k];
         const size_t plane = (size_t)p.A * im.n;
         const size_t line = (size_t)(g - im.line0);
-        im.rec[line] = make_record(im.dsino, im.med_x, im.med_s, nullptr, nullptr, im.z345,
-                                   plane, line, p.slot);
+        make_record(im.dsino, im.med_x, im.med_s, nullptr, nullptr, im.z345, plane, line,
+                    p.slot, im.rec, nullptr);
     }
 }
 
-// G(row) of one line record (the rotated-domain gradient at that row).
-__device__ __forceinline__ float record_g(const LineRecord &r, int row, const float2 *s_tab,
-                                          int tabn, bool do345)
-{
-    float G = r.g0;
-    G += row == r.r6 ? r.c6 : 0.0f;
-    G += row == r.r7 ? r.c7 : 0.0f;
-    if (row >= r.m) {
-        const float d = (float)(row - r.m);
-        G = fmaf(d, fmaf(r.g2, d, r.g1), G);
-    }
-    if (do345 && row > r.sm) {
-        const int q = row - r.sm;
-        const float2 w3 = s_tab[q], w4 = s_tab[tabn + q], w5 = s_tab[2 * tabn + q];
-        G = fmaf(r.ab[0], w3.x, G);
-        G = fmaf(r.ab[1], w3.y, G);
-        G = fmaf(r.ab[2], w4.x, G);
-        G = fmaf(r.ab[3], w4.y, G);
-        G = fmaf(r.ab[4], w5.x, G);
-        G = fmaf(r.ab[5], w5.y, G);
-    }
-    return G;
-}
+// ---------------------------------------------------------------------------
+// Transposed bilinear rotation (the adjoint of rotate + interpolate,
+// c++/src/auxiliary.cpp:223-255 and :182-196) as a scatter into a pixel tile
+// held in LDS, one wave per tile.
+//
+// For every angle the wave visits the rotated samples (col, row) whose taps
+// can touch its T x T pixel tile: those whose exact source position lies in
+// the tile grown by one pixel, a square of side T + 1 turned by the angle.
+// They are enumerated as a parallelogram: the columns the square spans, and
+// in each column the rows of the narrower of the square's two slabs (one
+// slab is ignored, so up to twice the square's area at 45 degrees, 1.45x on
+// average).  Each sample is evaluated exactly as the forward evaluates it
+// (rot_sample's float operations: coordinates, strict bounds, truncated taps
+// and modf fractions), its rotated-domain gradient G(row) comes from its
+// column's line record, and G times each of the four tap weights is added
+// into the tile's accumulators (LDS, with a one-pixel halo ring that is
+// discarded: those pixels belong to the neighbouring tiles).
+//
+// Determinism without ordering races: two samples whose lattice positions
+// agree modulo 2 in both coordinates are at least 2 apart, so they never
+// share a tap cell.  The samples are visited one parity class (column mod 2,
+// row mod 2) at a time, so the 64 samples of a step lie in 64 distinct
+// cells and each of the step's four LDS additions (one per tap corner) has
+// 64 distinct targets; a pixel's additions follow program order.  Only the
+// owning wave touches a tile's accumulators, so each pixel's sum is formed
+// in one fixed order.  Every kGatherFlush angles the float accumulators are
+// added into per-lane doubles and cleared.
+// ---------------------------------------------------------------------------
+constexpr int kGatherFlush = 4;
 
-// Transposed bilinear rotation as a gather over 16x16 pixel tiles.  Per
-// angle the workgroup evaluates every rotated sample that any of its pixels
-// can receive from once: the sample's forward float coordinates (the exact
-// operations of rot_sample), its taps (X, Y) and fractions, and G(row) from
-// the line record, into LDS.  Each pixel then visits the rotated samples
-// within sqrt(2) of its inverse image (those whose 2x2 footprint can cover
-// it) column by column, keeps those whose taps include it and accumulates
-// G * weight (an angle's hits in float, the angles in double).  The visiting
-// order is fixed, so the result is deterministic and needs no atomics.
-constexpr int kTile = 16;
-constexpr int kBoxCap = 32;  // box side bound: a tile's box spans <= 29 samples
-
-struct GatherLds {
-    LineRecord rec[kBoxCap];
-    float4 smp[kBoxCap * kBoxCap];  // (taps X | Y << 16, or -1), G, fx, fy
+template <int T>
+struct GatherShape {
+    static constexpr int P = T + 2;                    // accumulator side, with halo
+    static constexpr int ACC = ((P * P + 3) / 4) * 4;  // floats, padded to float4
+    static constexpr int KMAX = T == 16 ? 32 : 64;     // footprint columns per angle
+    static constexpr int NPIX = T * T / kWave;         // interior pixels per lane
 };
 
-__device__ __forceinline__ float gather_tile(const LineRecord *recs, const float2 *trig,
-                                             const float2 *s_tab, int tabn, bool do345,
-                                             int n, int A, int X0, int Y0, GatherLds &L)
+// LDS words of one wave: accumulators, then the staged column data
+// (per column: (g0, sm, rlo, rhi), (ab0..ab3), (ab4, ab5), and with T1/T2
+// (g1, g2, m, -), with T6/T7 (r6, r7, c6, c7)).
+__host__ __device__ constexpr int gather_wave_words(int T, bool do12, bool do67)
 {
-    const int tid = threadIdx.x;
-    const int X = X0 + tid / kTile, Y = Y0 + tid % kTile;
-    const bool valid = X < n && Y < n;
+    return (((T + 2) * (T + 2) + 3) / 4) * 4 +
+           (T == 16 ? 32 : 64) * (4 + 4 + 2 + (do12 ? 4 : 0) + (do67 ? 4 : 0));
+}
+
+struct GatherImg {
+    const LineRecord *rec;      // [A][n]
+    const LineRecord67 *rec67;  // [A][n] or null
+    float *out;                 // n x n (or this angle group's partial)
+    int n;
+};
+
+template <int T, bool DO345>
+__device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const float2 *trig,
+                                                 const float2 *s_tab, int tabn, int a0,
+                                                 int a1, int X0, int Y0, float *wl,
+                                                 bool do12)
+{
+    using S = GatherShape<T>;
+    constexpr int P = S::P;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int n = im.n;
+    const bool do67 = im.rec67 != nullptr;
+    float *acc = wl;
+    float4 *sA = reinterpret_cast<float4 *>(wl + S::ACC);
+    float4 *sB = sA + S::KMAX;
+    float2 *sC = reinterpret_cast<float2 *>(sB + S::KMAX);
+    float4 *sD = reinterpret_cast<float4 *>(sC + S::KMAX);
+    float4 *sE = sD + (do12 ? S::KMAX : 0);
     const float o = (float)((n - 1) / 2.0);
     const float lim = (float)(n - 1);
-    const float u = (float)X - o, v = (float)Y - o;
-    const float R = 1.4163f;
-    double acc = 0.0;
-    for (int a = 0; a < A; a++) {
-        const float2 cs = trig[a];
-        const float c = cs.x, s = cs.y;
-        // this pixel's candidate box (the rotated samples within R of its
-        // inverse image), and the tile's union of them
-        const float pxc = u * c - v * s;
-        const float pyc = u * s + v * c;
-        int c0 = (int)ceilf(pxc - R + o), c1 = (int)floorf(pxc + R + o);
-        int r0 = (int)ceilf(pyc - R + o), r1 = (int)floorf(pyc + R + o);
-        c0 = c0 < 0 ? 0 : c0;
-        r0 = r0 < 0 ? 0 : r0;
-        c1 = c1 > n - 1 ? n - 1 : c1;
-        r1 = r1 > n - 1 ? n - 1 : r1;
-        // The tile's union of those boxes, from its four corners: the exact
-        // coordinates are linear in the pixel, so the corners bound them,
-        // and the rounding of any pixel's float coordinates (< 1e-3 at
-        // n <= 2048) moves its bounds by at most one sample: a margin of 1
-        // contains every pixel's box (no reduction, no barrier).
-        int bc0 = n, bc1 = -1, br0 = n, br1 = -1;
+    // the tile grown by one pixel, plus a margin far above the float
+    // rounding of any sample position (< 1e-3 for n <= 2730), relative to o
+    const float dm = 0.01f;
+    const float xa = (float)(X0 - 1) - o - dm, xb = (float)(X0 + T) - o + dm;
+    const float ya = (float)(Y0 - 1) - o - dm, yb = (float)(Y0 + T) - o + dm;
+    const float4 z4 = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = lane; i < S::ACC / 4; i += kWave)
+        reinterpret_cast<float4 *>(acc)[i] = z4;
+    double accd[S::NPIX];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const float uu = (float)(X0 + ((k & 1) ? kTile - 1 : 0)) - o;
-            const float vv = (float)(Y0 + ((k & 2) ? kTile - 1 : 0)) - o;
-            const float qx = uu * c - vv * s, qy = uu * s + vv * c;
-            const int k0 = (int)ceilf(qx - R + o) - 1, k1 = (int)floorf(qx + R + o) + 1;
-            const int l0 = (int)ceilf(qy - R + o) - 1, l1 = (int)floorf(qy + R + o) + 1;
-            bc0 = k0 < bc0 ? k0 : bc0;
-            bc1 = k1 > bc1 ? k1 : bc1;
-            br0 = l0 < br0 ? l0 : br0;
-            br1 = l1 > br1 ? l1 : br1;
-        }
-        bc0 = bc0 < 0 ? 0 : bc0;
-        br0 = br0 < 0 ? 0 : br0;
-        bc1 = bc1 > n - 1 ? n - 1 : bc1;
-        br1 = br1 > n - 1 ? n - 1 : br1;
-        int bw = bc1 - bc0 + 1, bh = br1 - br0 + 1;
-        bw = bw < 0 ? 0 : (bw > kBoxCap ? kBoxCap : bw);  // bounded by geometry
-        bh = bh < 0 ? 0 : (bh > kBoxCap ? kBoxCap : bh);
-        // the box's line records
-        const LineRecord *rec = recs + (size_t)a * n;
-        constexpr int kRecWords = (int)(sizeof(LineRecord) / 4);
-        for (int j = tid; j < bw * kRecWords; j += kBlock)
-            reinterpret_cast<float *>(L.rec)[j] =
-                reinterpret_cast<const float *>(rec + bc0)[j];
-        __syncthreads();
-        // every rotated sample of the box, once
-        for (int j = tid; j < bw * bh; j += kBlock) {
-            const int col = bc0 + j % bw, row = br0 + j / bw;
-            const float px = (float)col - o, py = (float)row - o;
-            float x = px * c + py * s;
-            float y = px * (-s) + py * c;
-            x = x + o;
-            y = y + o;
-            // bounds, taps and fractions as rot_sample computes them
-            const bool inb = __float_as_uint(x) < __float_as_uint(lim) &&
-                             __float_as_uint(y) < __float_as_uint(lim);
-            float4 e;
-            e.x = __int_as_float(inb ? ((int)(uint32_t)x | ((int)(uint32_t)y << 16)) : -1);
-            e.y = record_g(L.rec[col - bc0], row, s_tab, tabn, do345);
-            e.z = __builtin_amdgcn_fractf(x);
-            e.w = __builtin_amdgcn_fractf(y);
-            L.smp[j] = e;
-        }
-        __syncthreads();
-        if (valid) {
-            // the pixel's candidates, column by column (as the samples were
-            // laid out); those outside [c0, c1] x [r0, r1] are skipped.  An
-            // angle's few hits are summed in float, the angles in double.
-            float part = 0.0f;
-#pragma unroll
-            for (int dc = 0; dc < 3; dc++) {
-                const int col = c0 + dc;
-#pragma unroll
-                for (int dr = 0; dr < 3; dr++) {
-                    const int row = r0 + dr;
-                    // inside the box by construction; the test keeps every
-                    // LDS index in range regardless
-                    const bool cand = (col <= c1) & (row <= r1) &
-                                      ((unsigned)(col - bc0) < (unsigned)bw) &
-                                      ((unsigned)(row - br0) < (unsigned)bh);
-                    const int j = (row - br0) * bw + (col - bc0);
-                    const float4 e = L.smp[cand ? j : 0];
-                    const int taps = __float_as_int(e.x);
-                    const int Xi = taps & 0xFFFF, Yi = taps >> 16;
-                    const bool hit = cand & (taps >= 0) & ((Xi == X) | (Xi == X - 1)) &
-                                     ((Yi == Y) | (Yi == Y - 1));
-                    const float wx = Xi == X ? 1.0f - e.z : e.z;
-                    const float wy = Yi == Y ? 1.0f - e.w : e.w;
-                    part = fmaf(e.y, hit ? wx * wy : 0.0f, part);
+    for (int i = 0; i < S::NPIX; i++)
+        accd[i] = 0.0;
+    wave_lds_sync();
+
+    for (int ab = a0; ab < a1; ab += kGatherFlush) {
+        const int ae = ab + kGatherFlush < a1 ? ab + kGatherFlush : a1;
+        for (int a = ab; a < ae; a++) {
+            const float2 cs = trig[a];
+            const float c = cs.x, s = cs.y;
+            // columns spanned by the grown tile: px = c (x - o) - s (y - o)
+            const float p1 = c * xa - s * ya, p2 = c * xb - s * ya;
+            const float p3 = c * xa - s * yb, p4 = c * xb - s * yb;
+            const float pmin = fminf(fminf(p1, p2), fminf(p3, p4));
+            const float pmax = fmaxf(fmaxf(p1, p2), fmaxf(p3, p4));
+            int c_lo = (int)ceilf(pmin + o), c_hi = (int)floorf(pmax + o);
+            c_lo = c_lo < 0 ? 0 : c_lo;
+            c_hi = c_hi > n - 1 ? n - 1 : c_hi;
+            // (T + 1.02) (|c| + |s|) bounds the span: at most 25 / 47 columns
+            // for T = 16 / 32, so the clamp never drops one
+            int K = c_hi - c_lo + 1;
+            K = K > S::KMAX ? S::KMAX : K;
+            if (K <= 0)
+                continue;
+            // rows: the slab kp px + kq py in [lo, hi] with |kq| >= 1/sqrt 2
+            // (y - o = -s px + c py, or x - o = c px + s py)
+            const bool usey = fabsf(c) >= fabsf(s);
+            const float kp = usey ? -s : c, kq = usey ? c : s;
+            const float lo = usey ? ya : xa, hi = usey ? yb : xb;
+            const float inv = 1.0f / kq;
+            const int W = (int)((hi - lo) * fabsf(inv)) + 2;
+            if (lane < K) {
+                const int col = c_lo + lane;
+                const float px = (float)col - o;
+                const float pa = (lo - kp * px) * inv, pb = (hi - kp * px) * inv;
+                int rlo = (int)ceilf(fminf(pa, pb) + o), rhi = (int)floorf(fmaxf(pa, pb) + o);
+                rlo = rlo < 0 ? 0 : rlo;
+                rhi = rhi > n - 1 ? n - 1 : rhi;
+                const LineRecord &r = im.rec[(size_t)a * n + col];
+                sA[lane] = float4{r.g0, __int_as_float(r.sm), __int_as_float(rlo),
+                                  __int_as_float(rhi)};
+                if (DO345) {
+                    sB[lane] = float4{r.ab[0], r.ab[1], r.ab[2], r.ab[3]};
+                    sC[lane] = float2{r.ab[4], r.ab[5]};
+                }
+                if (do12)
+                    sD[lane] = float4{r.g1, r.g2, __int_as_float(r.m), 0.0f};
+                if (do67) {
+                    const LineRecord67 &q = im.rec67[(size_t)a * n + col];
+                    sE[lane] = float4{__int_as_float(q.r6), __int_as_float(q.r7), q.c6, q.c7};
                 }
             }
-            acc += (double)part;
+            wave_lds_sync();
+            const float pxc0 = (float)c_lo - o;
+            const int W2 = (W + 1) >> 1;
+            const float invW2 = 1.0f / (float)W2;
+            // one parity class (column, row mod 2) at a time, so that the 64
+            // samples of each step are pairwise at least 2 apart
+            for (int cls = 0; cls < 4; cls++) {
+                const int koff = ((cls & 1) - c_lo) & 1;  // first column of the class
+                const int K2 = (K - koff + 1) >> 1;
+                const int total = K2 * W2;
+                for (int t0 = 0; t0 < total; t0 += kWave) {
+                    const int t = t0 + lane;
+                    // t / W2 exactly: (t + 0.5) / W2 is at least 0.5 / W2 from
+                    // an integer, far above the product's rounding
+                    int k2 = (int)(((float)t + 0.5f) * invW2);
+                    k2 = t < total ? k2 : 0;
+                    const int k = 2 * k2 + koff;
+                    const float4 ea = sA[k];
+                    const int sm = __float_as_int(ea.y);
+                    const int rlo = __float_as_int(ea.z);
+                    const int j = 2 * (t - __mul24(k2, W2)) + (((cls >> 1) - rlo) & 1);
+                    const int row = rlo + j;
+                    const int col = c_lo + k;
+                    // the forward's sample position (rot_sample), bit for bit
+                    const float px = pxc0 + (float)k;
+                    const float pxc = px * c, pxns = px * (-s);
+                    const float py = (float)row - o;
+                    float x = pxc + py * s;
+                    float y = pxns + py * c;
+                    x = x + o;
+                    y = y + o;
+                    const bool inb = __float_as_uint(x) < __float_as_uint(lim) &&
+                                     __float_as_uint(y) < __float_as_uint(lim);
+                    const int Xb = inb ? (int)(uint32_t)x : -8;
+                    const int Yb = inb ? (int)(uint32_t)y : -8;
+                    const float fx = __builtin_amdgcn_fractf(x);
+                    const float fy = __builtin_amdgcn_fractf(y);
+                    const int xi = Xb - X0 + 1, yi = Yb - Y0 + 1;  // accumulator cell
+                    const bool use = t < total && j < W && row <= __float_as_int(ea.w) &&
+                                     (unsigned)xi <= (unsigned)T && (unsigned)yi <= (unsigned)T;
+                    (void)col;
+                    // G(row): g0 + [row >= m] (g1 d + g2 d^2)
+                    //   + [row > sm] sum_t (alpha_t W_t.re + beta_t W_t.im)[row - sm]
+                    //   + [row == r6] c6 + [row == r7] c7
+                    float G = ea.x;
+                    if (do12) {
+                        const float4 ed = sD[k];
+                        const int m = __float_as_int(ed.z);
+                        const float d = (float)(row - m);
+                        const float h = fmaf(d, fmaf(ed.y, d, ed.x), G);
+                        G = row >= m ? h : G;
+                    }
+                    if (DO345) {
+                        const float4 eb = sB[k];
+                        const float2 ec = sC[k];
+                        // table entry 0 is zero, so rows at or before sm add nothing
+                        int q = row - sm;
+                        q = q > 0 && q < tabn ? q : 0;
+                        const float2 w3 = s_tab[q], w4 = s_tab[tabn + q];
+                        const float2 w5 = s_tab[2 * tabn + q];
+                        G = fmaf(eb.x, w3.x, G);
+                        G = fmaf(eb.y, w3.y, G);
+                        G = fmaf(eb.z, w4.x, G);
+                        G = fmaf(eb.w, w4.y, G);
+                        G = fmaf(ec.x, w5.x, G);
+                        G = fmaf(ec.y, w5.y, G);
+                    }
+                    if (do67) {
+                        const float4 ee = sE[k];
+                        G += row == __float_as_int(ee.x) ? ee.z : 0.0f;
+                        G += row == __float_as_int(ee.y) ? ee.w : 0.0f;
+                    }
+                    if (use) {
+                        const float ofx = 1.0f - fx, ofy = 1.0f - fy;
+                        const float ga = G * ofx, gb = G * fx;
+                        float *cell = acc + __mul24(xi, P) + yi;
+                        // same class: every lane's cell is distinct, so each
+                        // of these four additions has distinct targets
+                        __hip_atomic_fetch_add(cell, ga * ofy, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        __hip_atomic_fetch_add(cell + P, gb * ofy, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        __hip_atomic_fetch_add(cell + 1, ga * fy, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        __hip_atomic_fetch_add(cell + P + 1, gb * fy, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    }
+                }
+            }
+            wave_lds_sync();  // staged columns are read; the next angle rewrites them
         }
-        __syncthreads();
+        // flush the float accumulators of these angles into the doubles
+#pragma unroll
+        for (int i = 0; i < S::NPIX; i++) {
+            const int p = lane + kWave * i;
+            accd[i] += (double)acc[(p / T + 1) * P + p % T + 1];
+        }
+        wave_lds_sync();
+        for (int i = lane; i < S::ACC / 4; i += kWave)
+            reinterpret_cast<float4 *>(acc)[i] = z4;
+        wave_lds_sync();
     }
-    return (float)acc;
+#pragma unroll
+    for (int i = 0; i < S::NPIX; i++) {
+        const int p = lane + kWave * i;
+        const int X = X0 + p / T, Y = Y0 + p % T;
+        if (X < n && Y < n)
+            im.out[(size_t)X * n + Y] = (float)accd[i];
+    }
 }
 
-__global__ __launch_bounds__(kBlock) void gather_kernel(GatherParams p)
+// Stages the T3..T5 tables into the workgroup's LDS (every wave helps),
+// then returns this wave's slice of the dynamic LDS.
+__device__ __forceinline__ float *gather_lds(float *smem, const float2 *tab345, int tabn,
+                                             bool do345, int wave_words)
 {
-    extern __shared__ float2 s_tab[];
-    __shared__ GatherLds L;
-    const int n = p.n;
-    if (p.do345) {
-        for (int i = threadIdx.x; i < 3 * n; i += kBlock)
-            s_tab[i] = p.tab345[i];
+    const int tab_words = do345 ? 6 * tabn : 0;
+    if (do345) {
+        float2 *s_tab = reinterpret_cast<float2 *>(smem);
+        for (int i = threadIdx.x; i < 3 * tabn; i += blockDim.x)
+            s_tab[i] = tab345[i];
     }
     __syncthreads();
-    const int tn = (n + kTile - 1) / kTile;
-    const int X0 = blockIdx.x / tn * kTile, Y0 = blockIdx.x % tn * kTile;
-    const float g = gather_tile(p.rec, p.trig, s_tab, n, p.do345, n, p.A, X0, Y0, L);
-    const int X = X0 + threadIdx.x / kTile, Y = Y0 + threadIdx.x % kTile;
-    if (X < n && Y < n)
-        p.dimg[(size_t)X * n + Y] = g;
+    return smem + ((tab_words + 3) & ~3) + wave_id() * wave_words;
 }
 
-__global__ __launch_bounds__(kBlock) void gather_batched_kernel(BatchAdjointParams p)
+template <int T, bool DO345>
+__global__ __launch_bounds__(512) void gather_kernel(GatherParams p)
 {
-    extern __shared__ float2 s_tab[];
-    __shared__ GatherLds L;
-    if (p.do345) {
-        for (int i = threadIdx.x; i < 3 * p.tabn; i += kBlock)
-            s_tab[i] = p.tab345[i];
+    extern __shared__ float4 smem4[];
+    float *smem = reinterpret_cast<float *>(smem4);
+    const int words = gather_wave_words(T, p.do12, p.rec67 != nullptr);
+    float *wl = gather_lds(smem, p.tab345, p.n, DO345, words);
+    const int waves = blockDim.x / kWave;
+    const int item = blockIdx.x * waves + wave_id();
+    const int tn = (p.n + T - 1) / T;
+    if (item >= tn * tn * p.groups)
+        return;
+    const int g = item / (tn * tn), tile = item - g * tn * tn;
+    const int a0 = (int)((long long)p.A * g / p.groups);
+    const int a1 = (int)((long long)p.A * (g + 1) / p.groups);
+    GatherImg im;
+    im.rec = p.rec;
+    im.rec67 = p.rec67;
+    im.out = p.groups > 1 ? p.part + (size_t)g * p.n * p.n : p.dimg;
+    im.n = p.n;
+    gather_tile_wave<T, DO345>(im, p.trig, reinterpret_cast<const float2 *>(smem), p.n, a0,
+                               a1, tile / tn * T, tile % tn * T, wl, p.do12);
+}
+
+// dimg = sum over angle groups of the partial images, in group order.
+__global__ __launch_bounds__(kBlock) void gather_reduce_kernel(const float *part, int groups,
+                                                               size_t npix, float *dimg)
+{
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < npix;
+         i += (size_t)gridDim.x * kBlock) {
+        double acc = 0.0;
+        for (int g = 0; g < groups; g++)
+            acc += (double)part[g * npix + i];
+        dimg[i] = (float)acc;
     }
-    __syncthreads();
-    const int t = blockIdx.x;
-    const int k = find_image(p.images, p.nimg, t,
+}
+
+template <int T, bool DO345>
+__global__ __launch_bounds__(512) void gather_batched_kernel(BatchAdjointParams p)
+{
+    extern __shared__ float4 smem4[];
+    float *smem = reinterpret_cast<float *>(smem4);
+    const bool do12 = p.slot.s[1] >= 0 || p.slot.s[2] >= 0;
+    const int words = gather_wave_words(T, do12, false);
+    float *wl = gather_lds(smem, p.tab345, p.tabn, DO345, words);
+    const int waves = blockDim.x / kWave;
+    const int item = blockIdx.x * waves + wave_id();
+    if (item >= p.tiles)
+        return;
+    const int k = find_image(p.images, p.nimg, item,
                              [](const BatchImage &b) { return b.tile0; });
-    const BatchImage &im = p.images[k];
-    const int n = im.n;
-    const int tn = (n + kTile - 1) / kTile;
-    const int local = t - im.tile0;
-    const int X0 = local / tn * kTile, Y0 = local % tn * kTile;
-    const float g = gather_tile(im.rec, p.trig, s_tab, p.tabn, p.do345, n, p.A, X0, Y0, L);
-    const int X = X0 + threadIdx.x / kTile, Y = Y0 + threadIdx.x % kTile;
-    if (X < n && Y < n)
-        im.dimg[(size_t)X * n + Y] = g;
+    const BatchImage &bi = p.images[k];
+    const int n = bi.n;
+    const int tn = (n + T - 1) / T;
+    const int tile = item - bi.tile0;
+    GatherImg im;
+    im.rec = bi.rec;
+    im.rec67 = nullptr;
+    im.out = bi.dimg;
+    im.n = n;
+    gather_tile_wave<T, DO345>(im, p.trig, reinterpret_cast<const float2 *>(smem), p.tabn, 0,
+                               p.A, tile / tn * T, tile % tn * T, wl, do12);
 }
 
 // dS = J_P^T dC for every sinogram column (P1: sign differences; P2: one-hot
@@ -3068,11 +3213,40 @@ hipError_t launch_records(const RecordParams &p, hipStream_t s)
     return hipGetLastError();
 }
 
+size_t gather_lds_bytes(int tile, int waves, int tabn, bool do345, bool do12, bool do67)
+{
+    const size_t tab = do345 ? (((size_t)6 * tabn + 3) & ~(size_t)3) : 0;
+    return sizeof(float) * (tab + (size_t)waves * gather_wave_words(tile, do12, do67));
+}
+
 hipError_t launch_gather(const GatherParams &p, hipStream_t s)
 {
-    const int tn = (p.n + kTile - 1) / kTile;
-    const size_t shm = p.do345 ? sizeof(float2) * 3 * (size_t)p.n : 0;
-    gather_kernel<<<tn * tn, kBlock, shm, s>>>(p);
+    if ((p.tile != 16 && p.tile != 32) || p.waves < 1 || p.waves > 8 || p.groups < 1 ||
+        (p.groups > 1 && !p.part))
+        return hipErrorInvalidValue;
+    const int tn = (p.n + p.tile - 1) / p.tile;
+    const int items = tn * tn * p.groups;
+    const int blocks = (items + p.waves - 1) / p.waves;
+    const size_t shm = gather_lds_bytes(p.tile, p.waves, p.n, p.do345, p.do12,
+                                        p.rec67 != nullptr);
+    const dim3 block(kWave * p.waves);
+    if (p.tile == 16) {
+        if (p.do345)
+            gather_kernel<16, true><<<blocks, block, shm, s>>>(p);
+        else
+            gather_kernel<16, false><<<blocks, block, shm, s>>>(p);
+    } else {
+        if (p.do345)
+            gather_kernel<32, true><<<blocks, block, shm, s>>>(p);
+        else
+            gather_kernel<32, false><<<blocks, block, shm, s>>>(p);
+    }
+    if (p.groups > 1) {
+        const size_t npix = (size_t)p.n * p.n;
+        size_t gb = (npix + kBlock - 1) / kBlock;
+        gb = gb > 4096 ? 4096 : gb;
+        gather_reduce_kernel<<<(int)gb, kBlock, 0, s>>>(p.part, p.groups, npix, p.dimg);
+    }
     return hipGetLastError();
 }
 
@@ -3108,10 +3282,26 @@ hipError_t launch_records_batched(const BatchAdjointParams &p, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_gather_batched(const BatchAdjointParams &p, hipStream_t s)
+hipError_t launch_gather_batched(const BatchAdjointParams &p, int tile, int waves,
+                                 hipStream_t s)
 {
-    const size_t shm = p.do345 ? sizeof(float2) * 3 * (size_t)p.tabn : 0;
-    gather_batched_kernel<<<p.tiles, kBlock, shm, s>>>(p);
+    if ((tile != 16 && tile != 32) || waves < 1 || waves > 8)
+        return hipErrorInvalidValue;
+    const bool do12 = p.slot.s[1] >= 0 || p.slot.s[2] >= 0;
+    const size_t shm = gather_lds_bytes(tile, waves, p.tabn, p.do345, do12, false);
+    const int blocks = (p.tiles + waves - 1) / waves;
+    const dim3 block(kWave * waves);
+    if (tile == 16) {
+        if (p.do345)
+            gather_batched_kernel<16, true><<<blocks, block, shm, s>>>(p);
+        else
+            gather_batched_kernel<16, false><<<blocks, block, shm, s>>>(p);
+    } else {
+        if (p.do345)
+            gather_batched_kernel<32, true><<<blocks, block, shm, s>>>(p);
+        else
+            gather_batched_kernel<32, false><<<blocks, block, shm, s>>>(p);
+    }
     return hipGetLastError();
 }
 
