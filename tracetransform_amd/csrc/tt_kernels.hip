@@ -2145,12 +2145,12 @@ __global__ __launch_bounds__(kBlock) void records_batched_kernel(BatchAdjointPar
 // into the tile's accumulators (LDS, with a one-pixel halo ring that is
 // discarded: those pixels belong to the neighbouring tiles).
 //
-// Determinism without ordering races: two samples whose lattice positions
-// agree modulo 2 in both coordinates are at least 2 apart, so they never
-// share a tap cell.  The samples are visited one parity class (column mod 2,
-// row mod 2) at a time, so the 64 samples of a step lie in 64 distinct
-// cells and each of the step's four LDS additions (one per tap corner) has
-// 64 distinct targets; a pixel's additions follow program order.  Only the
+// Determinism without atomics: two samples whose lattice positions agree
+// modulo 3 in both coordinates are at least 3 apart, more than the 2 sqrt 2
+// that two samples with overlapping 2x2 tap footprints can be apart.  The
+// samples are visited one residue class (column mod 3, row mod 3) at a
+// time, so the 64 lanes of a step read, add to and write back 256 distinct
+// accumulators, and a pixel's additions follow program order.  Only the
 // owning wave touches a tile's accumulators, so each pixel's sum is formed
 // in one fixed order.  Every kGatherFlush angles the float accumulators are
 // added into per-lane doubles and cleared.
@@ -2214,76 +2214,116 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
         accd[i] = 0.0;
     wave_lds_sync();
 
-    for (int ab = a0; ab < a1; ab += kGatherFlush) {
-        const int ae = ab + kGatherFlush < a1 ? ab + kGatherFlush : a1;
-        for (int a = ab; a < ae; a++) {
-            const float2 cs = trig[a];
-            const float c = cs.x, s = cs.y;
-            // columns spanned by the grown tile: px = c (x - o) - s (y - o)
-            const float p1 = c * xa - s * ya, p2 = c * xb - s * ya;
-            const float p3 = c * xa - s * yb, p4 = c * xb - s * yb;
-            const float pmin = fminf(fminf(p1, p2), fminf(p3, p4));
-            const float pmax = fmaxf(fmaxf(p1, p2), fmaxf(p3, p4));
-            int c_lo = (int)ceilf(pmin + o), c_hi = (int)floorf(pmax + o);
-            c_lo = c_lo < 0 ? 0 : c_lo;
-            c_hi = c_hi > n - 1 ? n - 1 : c_hi;
-            // (T + 1.02) (|c| + |s|) bounds the span: at most 25 / 47 columns
-            // for T = 16 / 32, so the clamp never drops one
-            int K = c_hi - c_lo + 1;
-            K = K > S::KMAX ? S::KMAX : K;
-            if (K <= 0)
-                continue;
-            // rows: the slab kp px + kq py in [lo, hi] with |kq| >= 1/sqrt 2
-            // (y - o = -s px + c py, or x - o = c px + s py)
-            const bool usey = fabsf(c) >= fabsf(s);
-            const float kp = usey ? -s : c, kq = usey ? c : s;
-            const float lo = usey ? ya : xa, hi = usey ? yb : xb;
-            const float inv = 1.0f / kq;
-            const int W = (int)((hi - lo) * fabsf(inv)) + 2;
-            if (lane < K) {
-                const int col = c_lo + lane;
-                const float px = (float)col - o;
-                const float pa = (lo - kp * px) * inv, pb = (hi - kp * px) * inv;
-                int rlo = (int)ceilf(fminf(pa, pb) + o), rhi = (int)floorf(fmaxf(pa, pb) + o);
-                rlo = rlo < 0 ? 0 : rlo;
-                rhi = rhi > n - 1 ? n - 1 : rhi;
-                const LineRecord &r = im.rec[(size_t)a * n + col];
-                sA[lane] = float4{r.g0, __int_as_float(r.sm), __int_as_float(rlo),
-                                  __int_as_float(rhi)};
-                if (DO345) {
-                    sB[lane] = float4{r.ab[0], r.ab[1], r.ab[2], r.ab[3]};
-                    sC[lane] = float2{r.ab[4], r.ab[5]};
-                }
-                if (do12)
-                    sD[lane] = float4{r.g1, r.g2, __int_as_float(r.m), 0.0f};
-                if (do67) {
-                    const LineRecord67 &q = im.rec67[(size_t)a * n + col];
-                    sE[lane] = float4{__int_as_float(q.r6), __int_as_float(q.r7), q.c6, q.c7};
-                }
+    // Footprint of angle a: the columns [c_lo, c_lo + K) the grown tile
+    // spans, px = c (x - o) - s (y - o) over its corners, and in each column
+    // the rows [rlo, rhi] of the slab kp px + kq py in [lo, hi] with
+    // |kq| >= 1/sqrt 2 (y - o = -s px + c py, or x - o = c px + s py), at
+    // most W of them.
+    struct Foot {
+        float c, s, kp, lo, hi, inv;
+        int c_lo, K, W;
+    };
+    auto footprint = [&](int a) {
+        Foot f;
+        const float2 cs = trig[a];
+        f.c = cs.x;
+        f.s = cs.y;
+        const float p1 = f.c * xa - f.s * ya, p2 = f.c * xb - f.s * ya;
+        const float p3 = f.c * xa - f.s * yb, p4 = f.c * xb - f.s * yb;
+        const float pmin = fminf(fminf(p1, p2), fminf(p3, p4));
+        const float pmax = fmaxf(fmaxf(p1, p2), fmaxf(p3, p4));
+        int c_lo = (int)ceilf(pmin + o), c_hi = (int)floorf(pmax + o);
+        c_lo = c_lo < 0 ? 0 : c_lo;
+        c_hi = c_hi > n - 1 ? n - 1 : c_hi;
+        // (T + 1.02) (|c| + |s|) bounds the span: at most 25 / 47 columns
+        // for T = 16 / 32, so the clamp never drops one
+        const int K = c_hi - c_lo + 1;
+        f.c_lo = c_lo;
+        f.K = K > S::KMAX ? S::KMAX : K;
+        const bool usey = fabsf(f.c) >= fabsf(f.s);
+        const float kq = usey ? f.c : f.s;
+        f.kp = usey ? -f.s : f.c;
+        f.lo = usey ? ya : xa;
+        f.hi = usey ? yb : xb;
+        f.inv = 1.0f / kq;
+        f.W = (int)((f.hi - f.lo) * fabsf(f.inv)) + 2;
+        return f;
+    };
+    // this lane's column record of angle a (lane < K), loaded ahead (global
+    // address space: the loads do not wait on LDS traffic)
+    typedef const float __attribute__((address_space(1))) *GF;
+    auto ld4 = [](GF q) { return float4{q[0], q[1], q[2], q[3]}; };
+    float4 pa4 = z4, pb4 = z4, pd4 = z4, pe4 = z4;
+    auto prefetch = [&](const Foot &f, int a) {
+        if (lane < f.K) {
+            const size_t line = (size_t)a * n + f.c_lo + lane;
+            GF r = (GF)(im.rec + line);
+            pa4 = ld4(r);
+            if (DO345 || do12)
+                pb4 = ld4(r + 4);
+            if (do12)
+                pd4 = ld4(r + 8);
+            if (do67)
+                pe4 = ld4((GF)(im.rec67 + line));
+        }
+    };
+    Foot fn = footprint(a0);
+    prefetch(fn, a0);
+    for (int a = a0; a < a1; a++) {
+        const Foot f = fn;
+        // stage this angle's columns: (g0, sm, rlo, rhi), (ab0..ab3),
+        // (ab4, ab5), (g1, g2, m, -), (r6, r7, c6, c7)
+        if (lane < f.K) {
+            const float px = (float)(f.c_lo + lane) - o;
+            const float pa = (f.lo - f.kp * px) * f.inv, pb = (f.hi - f.kp * px) * f.inv;
+            int rlo = (int)ceilf(fminf(pa, pb) + o), rhi = (int)floorf(fmaxf(pa, pb) + o);
+            rlo = rlo < 0 ? 0 : rlo;
+            rhi = rhi > n - 1 ? n - 1 : rhi;
+            sA[lane] = float4{pa4.x, pa4.y, __int_as_float(rlo), __int_as_float(rhi)};
+            if (DO345) {
+                sB[lane] = float4{pa4.z, pa4.w, pb4.x, pb4.y};
+                sC[lane] = float2{pb4.z, pb4.w};
             }
-            wave_lds_sync();
+            if (do12)
+                sD[lane] = pd4;
+            if (do67)
+                sE[lane] = pe4;
+        }
+        wave_lds_sync();
+        if (a + 1 < a1) {
+            fn = footprint(a + 1);
+            prefetch(fn, a + 1);
+        }
+        const float c = f.c, s = f.s;
+        const int c_lo = f.c_lo, K = f.K, W = f.W;
+        {
             const float pxc0 = (float)c_lo - o;
-            const int W2 = (W + 1) >> 1;
-            const float invW2 = 1.0f / (float)W2;
-            // one parity class (column, row mod 2) at a time, so that the 64
-            // samples of each step are pairwise at least 2 apart
-            for (int cls = 0; cls < 4; cls++) {
-                const int koff = ((cls & 1) - c_lo) & 1;  // first column of the class
-                const int K2 = (K - koff + 1) >> 1;
-                const int total = K2 * W2;
+            const int W3 = (W + 2) / 3;
+            const float invW3 = 1.0f / (float)W3;
+            const int clo3 = c_lo % 3;
+            // one residue class (column, row mod 3) at a time: the 64
+            // samples of a step are pairwise at least 3 apart, so their 2x2
+            // tap footprints are disjoint and the step's 256 read-add-writes
+            // have distinct targets
+            for (int cls = 0; cls < 9; cls++) {
+                const int cc = cls % 3, rc = cls / 3;
+                const int koff = cc >= clo3 ? cc - clo3 : cc - clo3 + 3;
+                const int K3 = (K - koff + 2) / 3;
+                const int total = K3 * W3;
                 for (int t0 = 0; t0 < total; t0 += kWave) {
                     const int t = t0 + lane;
-                    // t / W2 exactly: (t + 0.5) / W2 is at least 0.5 / W2 from
+                    // t / W3 exactly: (t + 0.5) / W3 is at least 0.5 / W3 from
                     // an integer, far above the product's rounding
-                    int k2 = (int)(((float)t + 0.5f) * invW2);
-                    k2 = t < total ? k2 : 0;
-                    const int k = 2 * k2 + koff;
+                    int k3 = (int)(((float)t + 0.5f) * invW3);
+                    k3 = t < total ? k3 : 0;
+                    const int k = 3 * k3 + koff;
                     const float4 ea = sA[k];
                     const int sm = __float_as_int(ea.y);
                     const int rlo = __float_as_int(ea.z);
-                    const int j = 2 * (t - __mul24(k2, W2)) + (((cls >> 1) - rlo) & 1);
+                    const int r3 = rlo - 3 * (int)(((uint32_t)rlo * 0xAAABu) >> 17);
+                    const int roff = rc >= r3 ? rc - r3 : rc - r3 + 3;
+                    const int j = 3 * (t - __mul24(k3, W3)) + roff;
                     const int row = rlo + j;
-                    const int col = c_lo + k;
                     // the forward's sample position (rot_sample), bit for bit
                     const float px = pxc0 + (float)k;
                     const float pxc = px * c, pxns = px * (-s);
@@ -2301,7 +2341,6 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                     const int xi = Xb - X0 + 1, yi = Yb - Y0 + 1;  // accumulator cell
                     const bool use = t < total && j < W && row <= __float_as_int(ea.w) &&
                                      (unsigned)xi <= (unsigned)T && (unsigned)yi <= (unsigned)T;
-                    (void)col;
                     // G(row): g0 + [row >= m] (g1 d + g2 d^2)
                     //   + [row > sm] sum_t (alpha_t W_t.re + beta_t W_t.im)[row - sm]
                     //   + [row == r6] c6 + [row == r7] c7
@@ -2337,31 +2376,29 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                         const float ofx = 1.0f - fx, ofy = 1.0f - fy;
                         const float ga = G * ofx, gb = G * fx;
                         float *cell = acc + __mul24(xi, P) + yi;
-                        // same class: every lane's cell is distinct, so each
-                        // of these four additions has distinct targets
-                        __hip_atomic_fetch_add(cell, ga * ofy, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
-                        __hip_atomic_fetch_add(cell + P, gb * ofy, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
-                        __hip_atomic_fetch_add(cell + 1, ga * fy, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
-                        __hip_atomic_fetch_add(cell + P + 1, gb * fy, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        const float v00 = cell[0], v01 = cell[1];
+                        const float v10 = cell[P], v11 = cell[P + 1];
+                        cell[0] = v00 + ga * ofy;
+                        cell[1] = v01 + ga * fy;
+                        cell[P] = v10 + gb * ofy;
+                        cell[P + 1] = v11 + gb * fy;
                     }
                 }
             }
-            wave_lds_sync();  // staged columns are read; the next angle rewrites them
         }
-        // flush the float accumulators of these angles into the doubles
+        wave_lds_sync();  // staged columns are read; the next angle rewrites them
+        if ((a - a0) % kGatherFlush == kGatherFlush - 1 || a == a1 - 1) {
+            // flush the float accumulators of these angles into the doubles
 #pragma unroll
-        for (int i = 0; i < S::NPIX; i++) {
-            const int p = lane + kWave * i;
-            accd[i] += (double)acc[(p / T + 1) * P + p % T + 1];
+            for (int i = 0; i < S::NPIX; i++) {
+                const int p = lane + kWave * i;
+                accd[i] += (double)acc[(p / T + 1) * P + p % T + 1];
+            }
+            wave_lds_sync();
+            for (int i = lane; i < S::ACC / 4; i += kWave)
+                reinterpret_cast<float4 *>(acc)[i] = z4;
+            wave_lds_sync();
         }
-        wave_lds_sync();
-        for (int i = lane; i < S::ACC / 4; i += kWave)
-            reinterpret_cast<float4 *>(acc)[i] = z4;
-        wave_lds_sync();
     }
 #pragma unroll
     for (int i = 0; i < S::NPIX; i++) {
