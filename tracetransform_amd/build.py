@@ -56,15 +56,19 @@ def build(verbose: bool = False) -> str:
     objs = []
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    for src in sources():
+    procs = []
+    for src in sources():  # one hipcc per source, run side by side
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         lang = ["-x", "hip"] if src.endswith((".hip", ".cpp")) else []
         cmd = [hipcc(), *lang, *DEVICE_FLAGS, "-I", os.path.join(ROOT, "include"),
                "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
         objs.append(obj)
+    failed = [cmd for cmd, pr in procs if pr.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
            "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]

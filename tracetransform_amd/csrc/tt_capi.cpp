@@ -113,15 +113,6 @@ void make_tab345(int n, std::vector<float2> &out)
     }
 }
 
-void make_dft_tw(int n, std::vector<float2> &out)
-{
-    out.resize(n);
-    for (int k = 0; k < n; k++) {
-        const double a = 2.0 * M_PI * (double)k / (double)n;
-        out[k] = float2{(float)cos(a), (float)sin(a)};
-    }
-}
-
 // Hermite function h_n(z) = H_n(z) / (sqrt(2^n n! sqrt(pi)) e^{z^2/2}) in
 // double, with the physicists' recursion H_k = 2z H_{k-1} - 2(k-1) H_{k-2}
 // (c++/src/functionals.cpp:62-84 evaluates the same recursion, rounding each
@@ -228,7 +219,8 @@ struct KernelTimer {
 // rewrite the context's tables) cannot change what the graph reads.
 struct BatchPlan {
     std::vector<int64_t> key;
-    DevBuf desc, imgT, trig, tab, counter, bbox, ckpt;
+    DevBuf desc, imgT, trig, tab, counter, bbox, ckpt, spec, jobs;
+    ttk::DftParams dft = {};      // P3 spectra of every image (jobs in `jobs`)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int waves_per_block = 4, blocks = 0;
@@ -254,6 +246,9 @@ struct BatchPlan {
         counter.release();
         bbox.release();
         ckpt.release();
+        spec.release();
+        jobs.release();
+        dft = {};
         key.clear();
     }
 };
@@ -262,7 +257,7 @@ struct BatchPlan {
 // medians, T3..T5 integrals, line records), laid out back to back.
 struct BackwardPlan {
     BatchPlan g;                  // descriptors, tables, graph
-    DevBuf sino, dsino, z, medx, meds, rec;
+    DevBuf sino, dsino, z, medx, meds, rec, spec, fjobs, bjobs;
     int gather_tile = 32, gather_waves = 4;
 };
 
@@ -275,8 +270,8 @@ struct tt_context {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
-    DevBuf imgT, trig, tab345, dft_tw, io_a, io_b, med_x, med_s, sino_ck;
-    DevBuf dft_w, p3_spec;
+    DevBuf imgT, trig, tab345, io_a, io_b, med_x, med_s, sino_ck;
+    DevBuf p3_spec;
     DevBuf bw_sino, bw_dsino, bw_z, bw_rec, bw_rec67, bw_part, bw_t6src, bw_t7src;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
     DevBuf nos_w, nos_m, nos_q, nos_r, nos_dev, nos_sp, nos_ptrs, nos_h;
@@ -286,8 +281,8 @@ struct tt_context {
     rocblas_handle blas = nullptr;
     int trig_A = -1;
     float trig_step = 0.0f;
-    int tab_n = -1, dft_n = -1, dftw_n = -1;
-    std::vector<float2> h_trig, h_tab, h_tw;
+    int tab_n = -1;
+    std::vector<float2> h_trig, h_tab;
     std::vector<double> h_herm;
 
     tt_status set_device()
@@ -320,19 +315,6 @@ struct tt_context {
                               hipMemcpyHostToDevice, stream));
         TT_HIP(hipStreamSynchronize(stream));
         tab_n = n;
-        return TT_OK;
-    }
-
-    tt_status ensure_dft(int n)
-    {
-        if (dft_n == n)
-            return TT_OK;
-        make_dft_tw(n, h_tw);
-        TT_HIP(dft_tw.ensure(sizeof(float2) * (size_t)n));
-        TT_HIP(hipMemcpyAsync(dft_tw.ptr, h_tw.data(), sizeof(float2) * (size_t)n,
-                              hipMemcpyHostToDevice, stream));
-        TT_HIP(hipStreamSynchronize(stream));
-        dft_n = n;
         return TT_OK;
     }
 
@@ -473,11 +455,12 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
         return TT_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->dft_w, &ctx->p3_spec,
-                      &ctx->imgT, &ctx->trig, &ctx->tab345, &ctx->dft_tw,
+    DevBuf *bufs[] = {&ctx->p3_spec,
+                      &ctx->imgT, &ctx->trig, &ctx->tab345,
                       &ctx->io_a, &ctx->io_b, &ctx->med_x,
                       &ctx->med_s, &ctx->sino_ck, &ctx->bw_sino, &ctx->bw_dsino, &ctx->bw_z,
-                      &ctx->bw_rec, &ctx->bw_t6src, &ctx->bw_t7src, &ctx->nos_med, &ctx->nos_a, &ctx->nos_u,
+                      &ctx->bw_rec, &ctx->bw_rec67, &ctx->bw_part, &ctx->bw_t6src,
+                      &ctx->bw_t7src, &ctx->nos_med, &ctx->nos_a, &ctx->nos_u,
                       &ctx->nos_v, &ctx->nos_s, &ctx->nos_e, &ctx->nos_c,
                       &ctx->nos_info, &ctx->nos_sweeps, &ctx->nos_w, &ctx->nos_m,
                       &ctx->nos_q, &ctx->nos_r, &ctx->nos_dev, &ctx->nos_sp,
@@ -496,7 +479,8 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
     ctx->batch.release();
     ctx->bwd.g.release();
     DevBuf *bb[] = {&ctx->bwd.sino, &ctx->bwd.dsino, &ctx->bwd.z, &ctx->bwd.medx,
-                    &ctx->bwd.meds, &ctx->bwd.rec};
+                    &ctx->bwd.meds, &ctx->bwd.rec, &ctx->bwd.spec, &ctx->bwd.fjobs,
+                    &ctx->bwd.bjobs};
     for (DevBuf *b : bb)
         b->release();
     if (ctx->blas)
@@ -756,30 +740,25 @@ TT_API tt_status tt_circus(tt_context *ctx, const float *d_sino, int n,
     hipStream_t st = ctx->stream;
     const int h = n / 2 + 1;
     const size_t cols = (size_t)n_sino * n_cols;
+    ttk::DftParams dp = {};
     if (p3) {
-        // P3's half spectra of every column at once: one FP32 GEMM with the
-        // DFT matrix (twiddles rounded from double), spec (2h x cols) = W^T X,
-        // on the matrix cores
-        if (ctx->dftw_n != n) {
-            TT_HIP(ctx->dft_w.ensure(sizeof(float) * (size_t)n * 2 * h));
-            TT_HIP(ttk::launch_dft_matrix(n, h, ctx->dft_w.as<float>(), st));
-            ctx->dftw_n = n;
-        }
+        // P3's half spectra of every column at once: the DFT as a GEMM on the
+        // matrix cores, spec (2h x cols) = W^T x (tt_dft.hip)
         TT_HIP(ctx->p3_spec.ensure(sizeof(float) * cols * 2 * h));
-        if (!ctx->blas)
-            TT_BLAS(rocblas_create_handle(&ctx->blas));
-        TT_BLAS(rocblas_set_stream(ctx->blas, st));
-        TT_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
         p.p3_spec = ctx->p3_spec.as<float>();
         p.p3_h = h;
+        dp.njobs = 1;
+        dp.job.in = d_sino;
+        dp.job.out = ctx->p3_spec.as<float>();
+        dp.job.n = n;
+        dp.job.cols = (int)cols;
+        dp.job.ld_in = n;
+        dp.job.ld_out = 2 * h;
+        dp.tiles = ttk::dft_job_tiles(n, (int)cols, true);
     }
     TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS, st));
-    if (p3) {
-        const float one = 1.0f, zero = 0.0f;
-        TT_BLAS(rocblas_sgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
-                              2 * h, (int)cols, n, &one, ctx->dft_w.as<float>(), n, d_sino, n,
-                              &zero, ctx->p3_spec.as<float>(), 2 * h));
-    }
+    if (p3)
+        TT_HIP(ttk::launch_dft_gemm(dp, true, n, st));
     TT_HIP(ttk::launch_circus(p, st));
     TT_HIP(ctx->timer.end(st));
     return TT_OK;
@@ -833,6 +812,8 @@ hipError_t enqueue_batch(tt_context *ctx, const ttk::BatchSinoParams &sp,
         e = ttk::launch_reset_counter(sp.counter, st);
     if (e == hipSuccess)
         e = ttk::launch_sino_t05_batched(sp, need_x, need_s, do345, wpb, blocks, st);
+    if (e == hipSuccess && with_circus && ctx->batch.dft.tiles > 0)
+        e = ttk::launch_dft_gemm(ctx->batch.dft, true, nmax, st);
     if (e == hipSuccess && with_circus && cp.np > 0)
         e = ttk::launch_circus_batched(cp, st);
     return e;
@@ -946,11 +927,37 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         for (int i = 0; i < nimg; i++)
             ck += (size_t)ttk::sino_ckpt_blocks(images[i].n) * images[i].n * n_angles;
         TT_HIP(B.ckpt.ensure(sizeof(float2) * ck));
-        size_t off = 0, cko = 0;
+        bool p3 = false;
+        for (int q = 0; q < np; q++)
+            p3 = p3 || pf[q].code == TT_P3;
+        size_t spec_tot = 0;
+        for (int i = 0; i < nimg && p3; i++)
+            spec_tot += (size_t)nt * n_angles * 2 * (images[i].n / 2 + 1);
+        if (p3)
+            TT_HIP(B.spec.ensure(sizeof(float) * spec_tot));
+        std::vector<ttk::DftJob> jobs;
+        int dft_tiles = 0;
+        size_t off = 0, cko = 0, so = 0;
         int items = 0, cols = 0;
         for (int k = 0; k < nimg; k++) {
             const tt_image &im = images[order[k]];
             ttk::BatchImage &b = d[k];
+            b.spec = nullptr;
+            if (p3) {
+                const int h2 = 2 * (im.n / 2 + 1);
+                b.spec = B.spec.as<float>() + so;
+                so += (size_t)nt * n_angles * h2;
+                ttk::DftJob j = {};
+                j.in = im.d_sino;
+                j.out = b.spec;
+                j.n = im.n;
+                j.cols = nt * n_angles;
+                j.ld_in = im.n;
+                j.ld_out = h2;
+                j.tile0 = dft_tiles;
+                dft_tiles += ttk::dft_job_tiles(im.n, j.cols, true);
+                jobs.push_back(j);
+            }
             b.img = im.d_image;
             b.bbox = B.bbox.as<int>() + ttk::kBboxInts * k;
             b.ckpt = B.ckpt.as<float2>() + cko;
@@ -971,6 +978,15 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         TT_HIP(B.desc.ensure(sizeof(ttk::BatchImage) * nimg));
         TT_HIP(hipMemcpy(B.desc.ptr, d.data(), sizeof(ttk::BatchImage) * nimg,
                          hipMemcpyHostToDevice));
+        B.dft = {};
+        if (p3) {
+            TT_HIP(B.jobs.ensure(sizeof(ttk::DftJob) * jobs.size()));
+            TT_HIP(hipMemcpy(B.jobs.ptr, jobs.data(), sizeof(ttk::DftJob) * jobs.size(),
+                             hipMemcpyHostToDevice));
+            B.dft.jobs = B.jobs.as<ttk::DftJob>();
+            B.dft.njobs = (int)jobs.size();
+            B.dft.tiles = dft_tiles;
+        }
         std::vector<float2> h;
         make_trig(n_angles, angle_step_deg, h);
         TT_HIP(B.trig.ensure(sizeof(float2) * n_angles));
@@ -1086,6 +1102,16 @@ tt_status prepare_hermite(tt_context *ctx, const tt_pfunc *pf, int np, int n,
     return TT_OK;
 }
 
+// P3's positions in a P list.
+int p3_positions(const tt_pfunc *pf, int np, int *q3)
+{
+    int k = 0;
+    for (int q = 0; q < np; q++)
+        if (pf[q].code == TT_P3)
+            q3[k++] = q;
+    return k;
+}
+
 tt_status circus_vjp(tt_context *ctx, const float *d_sino, int n, int n_cols,
                      int n_sino, const tt_pfunc *pf, int np, const float *d_dcircus,
                      float *d_dsino)
@@ -1101,9 +1127,49 @@ tt_status circus_vjp(tt_context *ctx, const float *d_sino, int n, int n_cols,
     for (int q = 0; q < np; q++)
         p.code[q] = pf[q].code;
     TT_TRY(prepare_hermite(ctx, pf, np, n, p.herm, &p.herm_w));
-    TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS_VJP, ctx->stream));
-    TT_HIP(ttk::launch_circus_vjp(p, ctx->stream));
-    TT_HIP(ctx->timer.end(ctx->stream));
+    hipStream_t st = ctx->stream;
+    ttk::P3ScaleParams sp = {};
+    sp.np3 = p3_positions(pf, np, sp.q3);
+    const int h = n / 2 + 1, cols = n_cols * n_sino;
+    ttk::DftJob j = {};
+    if (sp.np3 > 0) {
+        // P3's part: spectra (W^T x), scaled per column and frequency, then
+        // W y into d_dsino, which the VJP kernel adds the other parts to
+        TT_HIP(ctx->p3_spec.ensure(sizeof(float) * (size_t)cols * 2 * h));
+        j.in = d_sino;
+        j.out = ctx->p3_spec.as<float>();
+        j.n = n;
+        j.cols = cols;
+        j.ld_in = n;
+        j.ld_out = 2 * h;
+        j.g = d_dcircus;
+        j.g_cols = n_cols;
+        j.g_stride = np * n_cols;
+        sp.job = j;
+        sp.njobs = 1;
+        p.p3_done = 1;
+    }
+    TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS_VJP, st));
+    if (sp.np3 > 0) {
+        ttk::DftParams f = {};
+        f.njobs = 1;
+        f.job = j;
+        f.tiles = ttk::dft_job_tiles(n, cols, true);
+        TT_HIP(ttk::launch_dft_gemm(f, true, n, st));
+        TT_HIP(ttk::launch_p3_vjp_scale(sp, cols, st));
+        ttk::DftParams b = {};
+        b.njobs = 1;
+        b.job.in = j.out;
+        b.job.out = d_dsino;
+        b.job.n = n;
+        b.job.cols = cols;
+        b.job.ld_in = 2 * h;
+        b.job.ld_out = n;
+        b.tiles = ttk::dft_job_tiles(n, cols, false);
+        TT_HIP(ttk::launch_dft_gemm(b, false, n, st));
+    }
+    TT_HIP(ttk::launch_circus_vjp(p, st));
+    TT_HIP(ctx->timer.end(st));
     return TT_OK;
 }
 
@@ -1810,12 +1876,49 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
                                               slot[1] >= 0 || slot[2] >= 0, false, false);
         P.gather_tile = gl.tile;
         P.gather_waves = gl.waves;
+        ttk::P3ScaleParams p3s = {};
+        p3s.np3 = p3_positions(pf, np, p3s.q3);
+        size_t spec_tot = 0;
+        for (int i = 0; i < nimg && p3s.np3 > 0; i++)
+            spec_tot += (size_t)nt * n_angles * 2 * (images[i].n / 2 + 1);
+        if (p3s.np3 > 0)
+            TT_HIP(P.spec.ensure(sizeof(float) * spec_tot));
+        std::vector<ttk::DftJob> fj, bj;
+        int ftiles = 0, btiles = 0;
+        size_t so = 0;
         std::vector<ttk::BatchImage> d(nimg);
         size_t po = 0, lo = 0, tiles = 0, cko = 0;
         int items = 0, cols = 0;
         for (int k = 0; k < nimg; k++) {
             const tt_image_grad &im = images[order[k]];
             ttk::BatchImage &b = d[k];
+            if (p3s.np3 > 0) {
+                // P3's adjoint: spectra of the recomputed sinograms, scaled in
+                // place, then W y into the sinogram gradient
+                const int h2 = 2 * (im.n / 2 + 1);
+                ttk::DftJob j = {};
+                j.in = P.sino.as<float>() + nt * lo;
+                j.out = P.spec.as<float>() + so;
+                j.n = im.n;
+                j.cols = nt * n_angles;
+                j.ld_in = im.n;
+                j.ld_out = h2;
+                j.tile0 = ftiles;
+                j.g = im.d_dcircus;
+                j.g_cols = n_angles;
+                j.g_stride = np * n_angles;
+                ftiles += ttk::dft_job_tiles(im.n, j.cols, true);
+                fj.push_back(j);
+                ttk::DftJob jb = j;
+                jb.in = j.out;
+                jb.out = P.dsino.as<float>() + nt * lo;
+                jb.ld_in = h2;
+                jb.ld_out = im.n;
+                jb.tile0 = btiles;
+                btiles += ttk::dft_job_tiles(im.n, j.cols, false);
+                bj.push_back(jb);
+                so += (size_t)nt * n_angles * h2;
+            }
             b.bbox = B.bbox.as<int>() + ttk::kBboxInts * k;
             b.ckpt = B.ckpt.as<float2>() + cko;
             cko += (size_t)ttk::sino_ckpt_blocks(im.n) * im.n * n_angles;
@@ -1848,6 +1951,23 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         TT_HIP(B.desc.ensure(sizeof(ttk::BatchImage) * nimg));
         TT_HIP(hipMemcpy(B.desc.ptr, d.data(), sizeof(ttk::BatchImage) * nimg,
                          hipMemcpyHostToDevice));
+        ttk::DftParams fdp = {}, bdp = {};
+        if (p3s.np3 > 0) {
+            TT_HIP(P.fjobs.ensure(sizeof(ttk::DftJob) * fj.size()));
+            TT_HIP(P.bjobs.ensure(sizeof(ttk::DftJob) * bj.size()));
+            TT_HIP(hipMemcpy(P.fjobs.ptr, fj.data(), sizeof(ttk::DftJob) * fj.size(),
+                             hipMemcpyHostToDevice));
+            TT_HIP(hipMemcpy(P.bjobs.ptr, bj.data(), sizeof(ttk::DftJob) * bj.size(),
+                             hipMemcpyHostToDevice));
+            fdp.jobs = P.fjobs.as<ttk::DftJob>();
+            fdp.njobs = (int)fj.size();
+            fdp.tiles = ftiles;
+            bdp.jobs = P.bjobs.as<ttk::DftJob>();
+            bdp.njobs = (int)bj.size();
+            bdp.tiles = btiles;
+            p3s.jobs = fdp.jobs;
+            p3s.njobs = fdp.njobs;
+        }
         std::vector<float2> h;
         make_trig(n_angles, angle_step_deg, h);
         TT_HIP(B.trig.ensure(sizeof(float2) * n_angles));
@@ -1888,6 +2008,7 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         ap.slot = sp.slot;
         for (int q = 0; q < np; q++)
             ap.code[q] = pf[q].code;
+        ap.p3_done = p3s.np3 > 0;
         const int cus = device_cus(ctx->device);
         B.waves_per_block = 4;
         B.blocks = cus * 4;
@@ -1900,6 +2021,12 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         if (ce == hipSuccess)
             ce = ttk::launch_sino_t05_batched(sp, true, true, do345, B.waves_per_block,
                                               B.blocks, own);
+        if (ce == hipSuccess && p3s.np3 > 0)
+            ce = ttk::launch_dft_gemm(fdp, true, nmax, own);
+        if (ce == hipSuccess && p3s.np3 > 0)
+            ce = ttk::launch_p3_vjp_scale(p3s, nt * n_angles, own);
+        if (ce == hipSuccess && p3s.np3 > 0)
+            ce = ttk::launch_dft_gemm(bdp, false, nmax, own);
         if (ce == hipSuccess)
             ce = ttk::launch_circus_vjp_batched(ap, own);
         if (ce == hipSuccess)

@@ -1,0 +1,209 @@
+// tt_dft.hip -- P3's discrete Fourier transforms as GEMMs on the CDNA4
+// matrix cores (v_mfma_f32_32x32x2_f32), with the DFT matrix generated tile
+// by tile from a twiddle table in LDS instead of being read from memory.
+//
+// P3 (c++/src/functionals.cpp:332-360) needs the half spectrum
+// X_k = sum_j x_j e^{-2 pi i jk/n}, k = 0..n/2, of every sinogram column:
+//   spec (2h x cols) = W^T x (n x cols),   h = n/2 + 1,
+// with W (n x 2h) = [cos(2 pi jk/n) | -sin(2 pi jk/n)].  Its adjoint applies
+// the same matrix the other way: ds (n x cols) = W y (2h x cols), y the
+// spectrum scaled per column and frequency.  Both are one kernel over a list
+// of jobs (one per image of a batch), so the single-image and the batched
+// path form their spectra with the same operations in the same order and
+// give the same P3 bit for bit.
+#include "tt_internal.hpp"
+
+#include <math.h>
+
+namespace ttk {
+
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kDftBM = 64;  // output rows per workgroup
+constexpr int kDftBN = 64;  // output columns per workgroup
+constexpr int kDftBK = 16;  // reduction chunk
+
+// W(row, kk) of the GEMM: the twiddle of phase (u v mod n) / n, its cosine
+// or minus its sine.  FWD: out = W^T in, so the output row m is a frequency
+// (m < h: real part, else imaginary part of frequency m - h) and kk a
+// position; otherwise out = W in, the output row is a position and kk runs
+// over the 2h spectrum entries.
+template <bool FWD>
+__device__ __forceinline__ float dft_weight(const float2 *tw, int n, int h, float inv_n,
+                                            int row, int kk)
+{
+    int u, v;
+    bool sn;
+    if (FWD) {
+        sn = row >= h;
+        u = sn ? row - h : row;
+        v = kk;
+    } else {
+        sn = kk >= h;
+        u = row;
+        v = sn ? kk - h : kk;
+    }
+    // u v < 2730^2 < 2^23: exact in float; the quotient is off by at most one
+    const int uv = (int)__umul24((uint32_t)u, (uint32_t)v);
+    int t = uv - (int)__umul24((uint32_t)(int)((float)uv * inv_n), (uint32_t)n);
+    t = t < 0 ? t + n : t;
+    t = t >= n ? t - n : t;
+    const float2 w = tw[t];
+    return sn ? -w.y : w.x;
+}
+
+__device__ __forceinline__ int find_job(const DftJob *jobs, int njobs, int tile)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile0 <= tile)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// One 64 x 64 output tile per workgroup of four waves (2 x 2 wave tiles of
+// 32 x 32, one MFMA accumulator each), reduction in chunks of 16: the W
+// chunk is generated into LDS from the twiddles, the input chunk loaded.
+// Twiddles (cos, sin)(2 pi t / n) are rounded from double sincospi.
+template <bool FWD>
+__global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
+{
+    __shared__ float sA[kDftBK][kDftBM];
+    __shared__ float sB[kDftBK][kDftBN];
+    extern __shared__ float2 s_tw[];
+    const DftJob &jb = p.jobs ? p.jobs[find_job(p.jobs, p.njobs, blockIdx.x)] : p.job;
+    const int n = jb.n, h = n / 2 + 1;
+    const int M = FWD ? 2 * h : n, Kd = FWD ? n : 2 * h;
+    const int local = blockIdx.x - jb.tile0;
+    const int mtiles = (M + kDftBM - 1) / kDftBM;
+    const int m0 = (local % mtiles) * kDftBM, c0 = (local / mtiles) * kDftBN;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < n; i += kBlock) {
+        double sn, cn;
+        sincospi(2.0 * (double)i / (double)n, &sn, &cn);
+        s_tw[i] = float2{(float)cn, (float)sn};
+    }
+    const float inv_n = 1.0f / (float)n;
+    const int lane = tid & (kWave - 1), wave = tid >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    // loaders: row/column r of the tile, 4 consecutive reduction indices
+    const int lr = tid & 63, lk = (tid >> 6) * 4;
+    const int col = c0 + lr;
+    const float *src = jb.in + (size_t)(col < jb.cols ? col : 0) * jb.ld_in;
+    f32x16 acc = {};
+    __syncthreads();
+    for (int k0 = 0; k0 < Kd; k0 += kDftBK) {
+        const int row = m0 + lr;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int kk = k0 + lk + e;
+            sA[lk + e][lr] =
+                row < M && kk < Kd ? dft_weight<FWD>(s_tw, n, h, inv_n, row, kk) : 0.0f;
+            sB[lk + e][lr] = col < jb.cols && kk < Kd ? src[kk] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < kDftBK / 2; s++) {
+            const int kr = 2 * s + (lane >> 5);
+            const float a = sA[kr][wm + (lane & 31)];
+            const float b = sB[kr][wn + (lane & 31)];
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // accumulator register r of this lane: column wn + lane % 32, row
+    // wm + 8 (r / 4) + 4 (lane / 32) + r % 4
+    const int oc = c0 + wn + (lane & 31);
+    if (oc < jb.cols) {
+        float *dst = jb.out + (size_t)oc * jb.ld_out;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int om = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (om < M)
+                dst[om] = acc[r];
+        }
+    }
+}
+
+// Scales a column's half spectrum into the input of P3's adjoint GEMM:
+// y_k = g c_k Re X_k, y_{h+k} = g c_k Im X_k, with g the upstream gradient of
+// the column's P3 (summed over the P list's P3 entries) and c_k = w_k 4 |X_k|^2 / n^4 (the derivative of
+// pow(hypot(Re/n, Im/n), 4) = |X_k|^4 / n^4, times the trapz weight of the
+// reference's float abscissae lin[p] = -1 + 2p/(n-1), folded onto k from
+// p = k and its mirror p = n - k).  In place over spec.
+__global__ __launch_bounds__(kBlock) void p3_vjp_scale_kernel(P3ScaleParams p)
+{
+    const DftJob &jb = p.jobs ? p.jobs[blockIdx.y] : p.job;
+    const int n = jb.n, h = n / 2 + 1;
+    const int cols = jb.cols;
+    for (int c = blockIdx.x; c < cols; c += gridDim.x) {
+        // every P3 entry of the list adds the same derivative times its own
+        // upstream gradient
+        float g = 0.0f;
+        const size_t gb = (size_t)(c / jb.g_cols) * jb.g_stride + c % jb.g_cols;
+        for (int i = 0; i < p.np3; i++)
+            g += jb.g[gb + (size_t)p.q3[i] * jb.g_cols];
+        float *y = jb.out + (size_t)c * jb.ld_out;
+        for (int k = threadIdx.x; k < h; k += kBlock) {
+            double w = 0.0;
+            for (int side = 0; side < 2; side++) {
+                const int q = side == 0 ? k : n - k;
+                if (q >= n || (side == 1 && (q == k || k == 0)))
+                    continue;
+                const double l = (float)(-1 + q * 2.0 / (n - 1));
+                if (q >= 1)
+                    w += 0.5 * (l - (double)(float)(-1 + (q - 1) * 2.0 / (n - 1)));
+                if (q <= n - 2)
+                    w += 0.5 * ((double)(float)(-1 + (q + 1) * 2.0 / (n - 1)) - l);
+            }
+            const double re = y[k], im = y[h + k];
+            const double n4 = (double)n * n * (double)n * n;
+            const double ck = (double)g * w * 4.0 * (re * re + im * im) / n4;
+            y[k] = (float)(ck * re);
+            y[h + k] = (float)(ck * im);
+        }
+    }
+}
+
+}  // namespace
+
+size_t dft_lds_bytes(int nmax)
+{
+    return sizeof(float2) * (size_t)nmax;
+}
+
+hipError_t launch_dft_gemm(const DftParams &p, bool forward, int nmax, hipStream_t s)
+{
+    if (p.tiles <= 0)
+        return hipSuccess;
+    if (forward)
+        dft_gemm_kernel<true><<<p.tiles, kBlock, dft_lds_bytes(nmax), s>>>(p);
+    else
+        dft_gemm_kernel<false><<<p.tiles, kBlock, dft_lds_bytes(nmax), s>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_p3_vjp_scale(const P3ScaleParams &p, int max_cols, hipStream_t s)
+{
+    if (p.njobs <= 0)
+        return hipSuccess;
+    int gx = max_cols < 2048 ? max_cols : 2048;
+    gx = gx < 1 ? 1 : gx;
+    p3_vjp_scale_kernel<<<dim3(gx, p.njobs), kBlock, 0, s>>>(p);
+    return hipGetLastError();
+}
+
+int dft_job_tiles(int n, int cols, bool forward)
+{
+    const int h = n / 2 + 1;
+    const int M = forward ? 2 * h : n;
+    return ((M + kDftBM - 1) / kDftBM) * ((cols + kDftBN - 1) / kDftBN);
+}
+
+}  // namespace ttk

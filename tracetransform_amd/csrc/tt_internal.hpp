@@ -72,6 +72,7 @@ struct BatchImage {
     int item0;            // first wave item of this image
     int col0;             // first sinogram column (over all T) of this image
     float *circus;        // [nt][np][A]
+    float *spec;          // [nt][A][2h] P3 half spectra (workspace), or null
     // adjoint (tt_backward_batched); null in the forward-only batch
     float2 *z345;         // [3][A][n] T3..T5 integrals
     const float *dcircus; // [nt][np][A]
@@ -143,7 +144,6 @@ struct CircusParams {
     const float *sino;    // [nsino][ncols][n]
     float *out;           // [nsino][np][ncols]
     int32_t *p2rank;      // [nsino][np][ncols] or null
-    const float2 *dft_tw; // [n] (cos, sin)(2 pi k / n)
     const float *p3_spec; // [nsino * ncols][2 * p3_h] half spectra (re, then im), or null
     int p3_h;             // n / 2 + 1
     const double *herm_w; // [nherm][n] Hermite weights
@@ -195,7 +195,8 @@ size_t gather_lds_bytes(int tile, int waves, int tabn, bool do345, bool do12, bo
 struct CircusVjpParams {
     const float *sino;    // [nsino][ncols][n]
     const float *dcircus; // [nsino][np][ncols]
-    float *dsino;         // [nsino][ncols][n]
+    float *dsino;         // [nsino][ncols][n] (holds P3's part when p3_done)
+    int p3_done;
     const double *herm_w; // [nherm][n]
     int n;
     int ncols;
@@ -204,6 +205,43 @@ struct CircusVjpParams {
     int code[kMaxP];
     int herm[kMaxP];
 };
+
+// P3's transforms as GEMMs on the matrix cores (tt_dft.hip).  One job per
+// image: forward, out (2h x cols, ld_out) = W^T in (n x cols, ld_in), the
+// half spectra (real parts, then imaginary parts); adjoint, out (n x cols)
+// = W in (2h x cols).  W (n x 2h) = [cos(2 pi jk/n) | -sin(2 pi jk/n)],
+// h = n / 2 + 1.
+struct DftJob {
+    const float *in;
+    float *out;
+    int n;
+    int cols;
+    int ld_in;
+    int ld_out;
+    int tile0;            // first workgroup of this job (dft_job_tiles each)
+    // p3_vjp_scale: the columns' upstream circus gradients, P-functional q's
+    // of column c at g[(c / g_cols) * g_stride + q * g_cols + c % g_cols]
+    const float *g;
+    int g_cols;
+    int g_stride;
+};
+struct DftParams {
+    const DftJob *jobs;   // device array of njobs, or null: job is the one
+    int njobs;
+    int tiles;            // total workgroups
+    DftJob job;
+};
+struct P3ScaleParams {
+    const DftJob *jobs;   // device array of njobs, or null: job is the one
+    int njobs;            // (out = the spectra, scaled in place)
+    DftJob job;
+    int np3;              // P3 entries of the P list, at positions q3
+    int q3[kMaxP];
+};
+int dft_job_tiles(int n, int cols, bool forward);
+size_t dft_lds_bytes(int nmax);
+hipError_t launch_dft_gemm(const DftParams &p, bool forward, int nmax, hipStream_t s);
+hipError_t launch_p3_vjp_scale(const P3ScaleParams &p, int max_cols, hipStream_t s);
 
 // Steps either side of the path and the orthonormal step.
 hipError_t launch_resize(const float *in, int in_rows, int in_cols, float *out,
@@ -274,9 +312,7 @@ constexpr int kSortCounters = 5;
 hipError_t launch_sino_sort(const SortTParams &p, int32_t *queue_a, int32_t *queue_b,
                             unsigned int *counts, hipStream_t s);
 hipError_t launch_circus(const CircusParams &p, hipStream_t s);
-// W (n x 2h column-major, h = n / 2 + 1): W(j, k) = cos(2 pi j k / n) and
-// W(j, h + k) = -sin(2 pi j k / n) for k < h, from the exact phase jk mod n.
-hipError_t launch_dft_matrix(int n, int h, float *w, hipStream_t s);
+
 
 // Batched path; `waves_per_block` in {1, 2, 4} and `blocks` (persistent grid)
 // are the autotuned launch shape.
@@ -306,6 +342,7 @@ struct BatchAdjointParams {
     int do345;
     Slots slot;
     int code[kMaxP];
+    int p3_done;          // dsino holds P3's part (the DFT GEMMs ran first)
 };
 hipError_t launch_circus_vjp_batched(const BatchAdjointParams &p, hipStream_t s);
 hipError_t launch_records_batched(const BatchAdjointParams &p, hipStream_t s);

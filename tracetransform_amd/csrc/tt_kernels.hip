@@ -849,7 +849,6 @@ struct CircusLds {
     float *x;      // nmax4
     float *sort;   // pow2 >= nmax
     float *mod;    // nmax4
-    float2 *tw;    // nmax
 };
 
 __device__ __forceinline__ CircusLds circus_lds(float *smem, int nmax)
@@ -862,17 +861,16 @@ __device__ __forceinline__ CircusLds circus_lds(float *smem, int nmax)
     l.x = smem;
     l.sort = l.x + n4;
     l.mod = l.sort + m2;
-    l.tw = reinterpret_cast<float2 *>(l.mod + n4);
     return l;
 }
 
-// All P-functionals of one sinogram column x[0..n) (already in L.x, twiddles
-// in L.tw when P3 is requested).  Result q goes to out[q * out_stride].
+// All P-functionals of one sinogram column x[0..n) (already in L.x; P3 reads
+// the column's half spectrum spec: real parts, then imaginary parts from
+// spec_h on).  Result q goes to out[q * out_stride].
 __device__ void circus_column(const CircusLds &L, int n, int np, const int *code,
                               const int *herm, const double *herm_w, float *out,
                               size_t out_stride, int32_t *p2rank,
-                              const float *spec = nullptr, int spec_h = 0,
-                              bool skip_p2 = false)
+                              const float *spec, int spec_h, bool skip_p2)
 {
     __shared__ float s_red[kBlock / kWave];
     __shared__ double s_red_d[kBlock / kWave];
@@ -910,56 +908,15 @@ __device__ void circus_column(const CircusLds &L, int n, int np, const int *code
             res = L.sort[s_rank];
             if (p2rank && threadIdx.x == 0)
                 p2rank[q * out_stride] = s_rank;
-        } else if (spec) {
-            // P3 from the column's half spectrum (a GEMM with the DFT
-            // matrix): pow(hypot(Re/N, Im/N), 4) in double and the trapz
-            // sum below
-            const int half = n / 2;
-            for (int k = threadIdx.x; k <= half; k += kBlock) {
-                const double h = hypot((double)spec[k] / n, (double)spec[spec_h + k] / n);
-                const double h2 = h * h;
-                L.mod[k] = (float)(h2 * h2);
-            }
-            __syncthreads();
-            for (int k = half + 1 + threadIdx.x; k < n; k += kBlock)
-                L.mod[k] = L.mod[n - k];
-            __syncthreads();
-            float acc = 0.0f;
-            for (int i = threadIdx.x; i < n - 1; i += kBlock) {
-                const float l0 = (float)(-1 + i * 2.0 / (n - 1));
-                const float l1 = (float)(-1 + (i + 1) * 2.0 / (n - 1));
-                acc += (l1 - l0) * (L.mod[i + 1] + L.mod[i]);
-            }
-            res = (float)(block_sum(acc, s_red) * 0.5);
         } else {
-            // P3: half spectrum by direct DFT (twiddles from LDS), then
-            // pow(hypot(Re/N, Im/N), 4) in double and the trapz sum.
+            // P3 from the column's half spectrum (tt_dft.hip):
+            // pow(hypot(Re/N, Im/N), 4) in double, mirrored onto k > N/2,
+            // and the trapz sum over the reference's float abscissae
+            // lin[p] = -1 + 2p/(N-1) (c++/src/functionals.cpp:53-60, 332-360)
             const int half = n / 2;
-            const float fn = (float)n;
             for (int k = threadIdx.x; k <= half; k += kBlock) {
-                float re0 = 0.0f, im0 = 0.0f, re1 = 0.0f, im1 = 0.0f;
-                int idx = 0;
-                int j = 0;
-                for (; j + 2 <= n; j += 2) {
-                    const float2 t0 = L.tw[idx];
-                    idx += k;
-                    idx -= idx >= n ? n : 0;
-                    const float2 t1 = L.tw[idx];
-                    idx += k;
-                    idx -= idx >= n ? n : 0;
-                    re0 = fmaf(L.x[j], t0.x, re0);
-                    im0 = fmaf(-L.x[j], t0.y, im0);
-                    re1 = fmaf(L.x[j + 1], t1.x, re1);
-                    im1 = fmaf(-L.x[j + 1], t1.y, im1);
-                }
-                if (j < n) {
-                    const float2 t0 = L.tw[idx];
-                    re0 = fmaf(L.x[j], t0.x, re0);
-                    im0 = fmaf(-L.x[j], t0.y, im0);
-                }
-                const float re = re0 + re1, im = im0 + im1;
-                const double h = hypot((double)(re / fn), (double)(im / fn));
-                const double h2 = h * h;
+                const double hy = hypot((double)spec[k] / n, (double)spec[spec_h + k] / n);
+                const double h2 = hy * hy;
                 L.mod[k] = (float)(h2 * h2);
             }
             __syncthreads();
@@ -991,9 +948,6 @@ __global__ __launch_bounds__(kBlock) void circus_kernel(CircusParams p)
     const float *x = p.sino + (size_t)colg * n;
     for (int i = threadIdx.x; i < n; i += kBlock)
         L.x[i] = x[i];
-    if (p.dft_tw)
-        for (int i = threadIdx.x; i < n; i += kBlock)
-            L.tw[i] = p.dft_tw[i];
     __syncthreads();
     const size_t o = (size_t)sg * p.np * p.ncols + col;
     circus_column(L, n, p.np, p.code, p.herm, p.herm_w, p.out + o, p.ncols,
@@ -1002,24 +956,9 @@ __global__ __launch_bounds__(kBlock) void circus_kernel(CircusParams p)
                   p.p2_wave != 0);
 }
 
-__global__ __launch_bounds__(kBlock) void dft_matrix_kernel(int n, int h, float *w)
-{
-    const size_t total = (size_t)n * 2 * h;
-    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < total;
-         e += (size_t)gridDim.x * kBlock) {
-        const int c = (int)(e / n), j = (int)(e - (size_t)c * n);
-        const int k = c < h ? c : c - h;
-        const long long t = ((long long)j * k) % n;
-        double sn, cn;
-        sincospi(2.0 * (double)t / (double)n, &sn, &cn);
-        w[e] = (float)(c < h ? cn : -sn);
-    }
-}
-
 // Every sinogram column of a batch; the image is found by binary search over
-// the descriptors' first-column offsets.  Twiddles of P3 are formed per
-// column in LDS from double sincospi (the single-image path reads a host
-// table; both are the float rounding of the exact twiddle to within 1 ulp).
+// the descriptors' first-column offsets.  P3 reads the image's half spectra
+// (im.spec, formed by the DFT GEMM as in the single-image path).
 __global__ __launch_bounds__(kBlock) void circus_batched_kernel(BatchCircusParams p)
 {
     extern __shared__ __align__(16) float smem[];
@@ -1041,20 +980,12 @@ __global__ __launch_bounds__(kBlock) void circus_batched_kernel(BatchCircusParam
     const float *x = im.sino + ((size_t)t * p.A + a) * n;
     for (int i = threadIdx.x; i < n; i += kBlock)
         L.x[i] = x[i];
-    bool p3 = false;
-    for (int q = 0; q < p.np; q++)
-        p3 = p3 || p.code[q] == 3;
-    if (p3)
-        for (int i = threadIdx.x; i < n; i += kBlock) {
-            double sn, cn;
-            sincospi(2.0 * (double)i / (double)n, &sn, &cn);
-            L.tw[i] = float2{(float)cn, (float)sn};
-        }
     __syncthreads();
     const int herm_none[kMaxP] = {};
+    const int h = n / 2 + 1;
     circus_column(L, n, p.np, p.code, herm_none, nullptr,
-                  im.circus + (size_t)t * p.np * p.A + a, p.A, nullptr, nullptr, 0,
-                  p.p2_wave != 0);
+                  im.circus + (size_t)t * p.np * p.A + a, p.A, nullptr,
+                  im.spec ? im.spec + (size_t)lc * 2 * h : nullptr, h, p.p2_wave != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -2498,9 +2429,6 @@ struct VjpLds {
     float *x;       // nmax4
     float *sq;      // nmax4
     float *ds;      // nmax4
-    float2 *X;      // nmax/2+1
-    float *ck;      // (nmax/2+2) & ~1
-    float2 *tw;     // nmax
 };
 
 __device__ __forceinline__ VjpLds vjp_lds(unsigned char *smem, int nmax)
@@ -2514,17 +2442,17 @@ __device__ __forceinline__ VjpLds vjp_lds(unsigned char *smem, int nmax)
     l.x = reinterpret_cast<float *>(l.key + m2);
     l.sq = l.x + n4;
     l.ds = l.sq + n4;
-    l.X = reinterpret_cast<float2 *>(l.ds + n4);
-    l.ck = reinterpret_cast<float *>(l.X + (nmax / 2 + 1));
-    l.tw = reinterpret_cast<float2 *>(l.ck + ((nmax / 2 + 2) & ~1));
     return l;
 }
 
 // J_P^T dC for the column x[0..n) already in L.x; dc[q * dc_stride] is the
-// upstream gradient of P-functional q at this column; result written to out.
+// upstream gradient of P-functional q at this column.  P3's part comes from
+// the DFT GEMMs (tt_dft.hip) and is already in out when p3_done; the P1, P2
+// and Hermite parts are added to it and the sum written back to out.
 __device__ void circus_vjp_column(const VjpLds &L, int n, int np, const int *code,
                                   const int *herm, const double *herm_w,
-                                  const float *dc, size_t dc_stride, float *out)
+                                  const float *dc, size_t dc_stride, float *out,
+                                  bool p3_done)
 {
     __shared__ float s_chain[8];
     __shared__ float s_sum;
@@ -2532,17 +2460,8 @@ __device__ void circus_vjp_column(const VjpLds &L, int n, int np, const int *cod
     int m2 = 1;
     while (m2 < n)
         m2 <<= 1;
-    bool p3 = false;
-    for (int q = 0; q < np; q++)
-        p3 = p3 || code[q] == 3;
-    for (int i = threadIdx.x; i < n; i += kBlock) {
-        L.ds[i] = 0.0f;
-        if (p3) {
-            double sn, cn;
-            sincospi(2.0 * (double)i / (double)n, &sn, &cn);
-            L.tw[i] = float2{(float)cn, (float)sn};
-        }
-    }
+    for (int i = threadIdx.x; i < n; i += kBlock)
+        L.ds[i] = p3_done ? out[i] : 0.0f;
     __syncthreads();
     for (int q = 0; q < np; q++) {
         const float g = dc[q * dc_stride];
@@ -2581,48 +2500,6 @@ __device__ void circus_vjp_column(const VjpLds &L, int n, int np, const int *cod
             __syncthreads();
             if (threadIdx.x == 0)
                 L.ds[s_src] += g;
-        } else {
-            const int half = n / 2;
-            for (int k = threadIdx.x; k <= half; k += kBlock) {
-                double re = 0.0, im = 0.0;
-                int idx = 0;
-                for (int j = 0; j < n; j++) {
-                    const float2 t = L.tw[idx];
-                    idx += k;
-                    idx -= idx >= n ? n : 0;
-                    re = fma((double)L.x[j], (double)t.x, re);
-                    im = fma(-(double)L.x[j], (double)t.y, im);
-                }
-                L.X[k] = float2{(float)re, (float)im};
-                // trapz weights of the float abscissae, folded onto k
-                double w = 0.0;
-                for (int side = 0; side < 2; side++) {
-                    const int qq = side == 0 ? k : n - k;
-                    if (qq >= n || (side == 1 && (qq == k || k == 0)))
-                        continue;
-                    const double l = (float)(-1 + qq * 2.0 / (n - 1));
-                    if (qq >= 1)
-                        w += 0.5 * (l - (double)(float)(-1 + (qq - 1) * 2.0 / (n - 1)));
-                    if (qq <= n - 2)
-                        w += 0.5 * ((double)(float)(-1 + (qq + 1) * 2.0 / (n - 1)) - l);
-                }
-                const double n4d = (double)n * n * (double)n * n;
-                L.ck[k] = (float)(w * 4.0 * (re * re + im * im) / n4d);
-            }
-            __syncthreads();
-            for (int j = threadIdx.x; j < n; j += kBlock) {
-                double acc = 0.0;
-                int idx = 0;
-                for (int k = 0; k <= half; k++) {
-                    const float2 t = L.tw[idx];
-                    idx += j;
-                    idx -= idx >= n ? n : 0;
-                    const float2 X = L.X[k];
-                    acc = fma((double)L.ck[k],
-                              (double)X.x * (double)t.x - (double)X.y * (double)t.y, acc);
-                }
-                L.ds[j] += (float)((double)g * acc);
-            }
         }
         __syncthreads();
     }
@@ -2630,10 +2507,10 @@ __device__ void circus_vjp_column(const VjpLds &L, int n, int np, const int *cod
         out[i] = L.ds[i];
 }
 
-// dS = J_P^T dC for every sinogram column (P1: sign differences; P2: one-hot
-// at the source of the weighted-median rank of the stable sort; P3: the
-// derivative of sum_p w_p |X_q(p)|^4 / N^4, an inverse-DFT-shaped sum;
-// Hermite: its weights).
+// dS = J_P^T dC for every sinogram column: P1 sign differences, P2 one-hot
+// at the source of the weighted-median rank of the stable sort, Hermite its
+// weights, added to P3's part (the derivative of sum_p w_p |X_q(p)|^4 / N^4,
+// from the DFT GEMMs) when p3_done.
 __global__ __launch_bounds__(kBlock) void circus_vjp_kernel(CircusVjpParams p)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -2648,7 +2525,7 @@ __global__ __launch_bounds__(kBlock) void circus_vjp_kernel(CircusVjpParams p)
     __syncthreads();
     circus_vjp_column(L, n, p.np, p.code, p.herm, p.herm_w,
                       p.dcircus + (size_t)sg * p.np * p.ncols + col, p.ncols,
-                      p.dsino + (size_t)colg * n);
+                      p.dsino + (size_t)colg * n, p.p3_done != 0);
 }
 
 __global__ __launch_bounds__(kBlock) void circus_vjp_batched_kernel(BatchAdjointParams p)
@@ -2670,7 +2547,7 @@ __global__ __launch_bounds__(kBlock) void circus_vjp_batched_kernel(BatchAdjoint
     const int herm_none[kMaxP] = {};
     circus_vjp_column(L, n, p.np, p.code, herm_none, nullptr,
                       im.dcircus + (size_t)t * p.np * p.A + a, p.A,
-                      im.dsino + (size_t)lc * n);
+                      im.dsino + (size_t)lc * n, p.p3_done != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -3069,7 +2946,7 @@ static size_t circus_shm(int nmax)
     while (m2 < nmax)
         m2 <<= 1;
     const size_t n4 = ((size_t)nmax + 3) & ~(size_t)3;
-    return sizeof(float) * (2 * n4 + m2) + sizeof(float2) * (size_t)nmax;
+    return sizeof(float) * (2 * n4 + m2);
 }
 
 hipError_t launch_circus(const CircusParams &p, hipStream_t s)
@@ -3293,9 +3170,7 @@ static size_t vjp_shm(int nmax)
     while (m2 < nmax)
         m2 <<= 1;
     const size_t n4 = ((size_t)nmax + 3) & ~(size_t)3;
-    const size_t h = (size_t)nmax / 2 + 1;
-    return sizeof(uint64_t) * m2 + sizeof(float) * 3 * n4 + sizeof(float2) * h +
-           sizeof(float) * ((h + 1) & ~(size_t)1) + sizeof(float2) * (size_t)nmax;
+    return sizeof(uint64_t) * m2 + sizeof(float) * 3 * n4;
 }
 
 hipError_t launch_circus_vjp(const CircusVjpParams &p, hipStream_t s)
@@ -3451,11 +3326,6 @@ hipError_t launch_nos_poly(const double *S, const double *S2, int k, int count,
     return hipGetLastError();
 }
 
-hipError_t launch_dft_matrix(int n, int h, float *w, hipStream_t s)
-{
-    dft_matrix_kernel<<<grid_for((size_t)n * 2 * h), kBlock, 0, s>>>(n, h, w);
-    return hipGetLastError();
-}
 
 hipError_t launch_to_float(const double *in, int ld, int rows, int cols, float *out,
                            hipStream_t s)
