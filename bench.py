@@ -517,8 +517,8 @@ def run_batched_adjoint(args, w):
     target = torch.stack([core.circus(core.sinograms(t, A, w["step"], w["tcodes"]),
                                       w["pfuncs"]) for t in tgts])
     del tgts
-    fwd = core.Batch(imgs, A, w["step"], w["tcodes"], w["pfuncs"])
-    bwd = core.BatchBackward(imgs, A, w["step"], w["tcodes"], w["pfuncs"])
+    fwd = core.Batch(imgs, A, w["step"], w["tcodes"], w["pfuncs"], save_for_backward=True)
+    bwd = core.BatchBackward(imgs, A, w["step"], w["tcodes"], w["pfuncs"], forward=fwd)
 
     def step():
         fwd.run()

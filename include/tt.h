@@ -183,7 +183,16 @@ typedef struct tt_image {
     int32_t n;             /* padded side, 3..2730                           */
     float *d_sino;         /* nt x A x n sinograms                           */
     float *d_circus;       /* nt x np x A circus functions (NULL if np == 0) */
+    void *d_saved;         /* optional, tt_saved_bytes(n, A, nt) bytes: the
+                              line data (medians, T3..T5 integrals, P3
+                              spectra, P2 sources) a later tt_backward_batched
+                              of this batch reads instead of recomputing the
+                              forward; NULL keeps nothing                    */
 } tt_image;
+
+/* Bytes of one image's d_saved block (0 for invalid sizes).  No reference
+ * counterpart: the reference has no adjoint. */
+TT_API size_t tt_saved_bytes(int n, int n_angles, int nt);
 
 /* Sinograms (T0..T5) and circus functions (P1..P3) of every image of a batch
  * in one persistent launch per stage, the whole pass captured into a HIP graph
@@ -238,12 +247,22 @@ typedef struct tt_image_grad {
     int32_t n;               /* padded side, 3..1800                         */
     const float *d_dcircus;  /* nt x np x A upstream gradient                */
     float *d_dimage;         /* n x n gradient out                           */
+    const float *d_sino;     /* with d_saved: the forward's sinograms        */
+    const void *d_saved;     /* optional: the block tt_forward_batched filled
+                                for this image (tt_image.d_saved), with the
+                                same angles and functionals and the image
+                                unchanged since; NULL recomputes the forward */
 } tt_image_grad;
 
 /* tt_backward for every image of a batch, each stage one launch over the
- * batch (forward recompute, circus VJP, line records, transposed-rotation
- * gather), captured into a HIP graph on the first call for a batch and
- * replayed for the same batch.  T0..T5, P1..P3. */
+ * batch (forward recompute unless d_saved, circus VJP, line records,
+ * transposed-rotation gather), captured into a HIP graph on the first call
+ * for a batch and replayed for the same batch.  T0..T5, P1..P3.  d_saved
+ * must be set for every image or for none; a block not written by a
+ * tt_forward_batched call on this context with the image's size, sinograms,
+ * angles and functionals is rejected (TT_ERR_INVALID).  The T6/T7 adjoints
+ * stay single-image only (tt_backward): the batched forward computes
+ * T0..T5. */
 TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *images,
                                      int nimg, int n_angles, float angle_step_deg,
                                      const int32_t *tcodes, int nt,

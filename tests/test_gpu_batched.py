@@ -139,3 +139,36 @@ def test_batched_backward_matches_single_image(oracle, gpu):
         for im in imgs:  # eager work between replays
             im.mul_(0.9)
         torch.cuda.synchronize()
+
+
+def test_batched_backward_from_saved_forward(oracle, gpu):
+    # the backward reading the forward's saved line data (medians, T3..T5
+    # integrals, P3 spectra, P2 sources) instead of recomputing the forward
+    # gives the same gradients bit for bit, replay after replay
+    sides = [60, 23, 41]
+    imgs = make_batch(oracle, sides, 700)
+    codes, pf, A, step = [0, 3, 4, 5], P123, 90, 4.0
+    fwd = core.Batch(imgs, A, step, codes, pf, save_for_backward=True)
+    saved = core.BatchBackward(imgs, A, step, codes, pf, forward=fwd)
+    plain = core.BatchBackward(imgs, A, step, codes, pf)
+    rng = np.random.default_rng(9)
+    for rep in range(3):
+        fwd.run()
+        g = torch.from_numpy(rng.standard_normal(tuple(saved.dcircus_all.shape))
+                             .astype(np.float32)).cuda()
+        saved.dcircus_all.copy_(g)
+        plain.dcircus_all.copy_(g)
+        a = [x.clone() for x in saved.run()]
+        b = plain.run()
+        torch.cuda.synchronize()
+        for k in range(len(imgs)):
+            assert torch.equal(a[k], b[k]), (rep, k)
+        for im in imgs:
+            im.mul_(1.1)
+    # a block written for other functionals is refused
+    other = core.Batch(imgs, A, step, [0, 3], pf, save_for_backward=True)
+    other.run()
+    with pytest.raises(ValueError):
+        core.BatchBackward(imgs, A, step, codes, pf, forward=other).run()
+    with pytest.raises(ValueError):
+        core.BatchBackward(imgs, A, step, codes, pf, forward=core.Batch(imgs, A, step, codes, pf))

@@ -27,7 +27,7 @@ EXPORTED = (
     "tt_forward_batched", "tt_context_set_autotune", "tt_batched_launch_shape",
     "tt_circus_backward", "tt_sinograms_backward", "tt_backward",
     "tt_resize", "tt_zscore", "tt_nearest_orthonormal_sinogram",
-    "tt_nearest_orthonormal_sinograms", "tt_backward_batched",
+    "tt_nearest_orthonormal_sinograms", "tt_backward_batched", "tt_saved_bytes",
 )
 
 
@@ -45,12 +45,12 @@ class tt_pfunc(C.Structure):
 
 class tt_image(C.Structure):
     _fields_ = [("d_image", C.c_void_p), ("n", C.c_int32), ("d_sino", C.c_void_p),
-                ("d_circus", C.c_void_p)]
+                ("d_circus", C.c_void_p), ("d_saved", C.c_void_p)]
 
 
 class tt_image_grad(C.Structure):
     _fields_ = [("d_image", C.c_void_p), ("n", C.c_int32), ("d_dcircus", C.c_void_p),
-                ("d_dimage", C.c_void_p)]
+                ("d_dimage", C.c_void_p), ("d_sino", C.c_void_p), ("d_saved", C.c_void_p)]
 
 
 class tt_sino_aux(C.Structure):
@@ -113,11 +113,13 @@ def load():
         L.tt_backward_batched.argtypes = [vp, C.POINTER(tt_image_grad), C.c_int,
                                           C.c_int, f32, C.POINTER(i32), C.c_int,
                                           C.POINTER(tt_pfunc), C.c_int]
+        L.tt_saved_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.tt_saved_bytes.restype = C.c_size_t
         L.tt_profile_enable.argtypes = [vp, C.c_int]
         L.tt_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_int64)]
         for name in EXPORTED:
-            if name not in ("tt_last_error", "tt_angle_count"):
+            if name not in ("tt_last_error", "tt_angle_count", "tt_saved_bytes"):
                 getattr(L, name).restype = C.c_int
         _lib = L
         return L

@@ -204,7 +204,7 @@ class Batch:
     captured HIP graph; only the image contents may change between calls."""
 
     def __init__(self, images, n_angles: int, step_deg: float, tfunctionals,
-                 pfunctionals=()):
+                 pfunctionals=(), save_for_backward: bool = False):
         self.images = [im for im in images]
         for k, im in enumerate(self.images):
             _require_cuda_f32(f"images[{k}]", im)
@@ -223,12 +223,20 @@ class Batch:
                                       device=dev) if npf else None
         self.circus = [self.circus_all[k] if npf else None
                        for k in range(len(self.images))]
+        # with save_for_backward, per image the line data a BatchBackward built
+        # from this batch reads instead of recomputing the forward
+        self.saved = None
+        if save_for_backward:
+            L = _lib.load()
+            self.saved = [torch.empty(int(L.tt_saved_bytes(im.shape[0], A, nt)),
+                                      dtype=torch.uint8, device=dev) for im in self.images]
         arr = (_lib.tt_image * len(self.images))()
         for k, im in enumerate(self.images):
             arr[k].d_image = im.data_ptr()
             arr[k].n = im.shape[0]
             arr[k].d_sino = self.sino[k].data_ptr()
             arr[k].d_circus = self.circus[k].data_ptr() if npf else None
+            arr[k].d_saved = self.saved[k].data_ptr() if self.saved else None
         self._arr = arr
         self._codes = (C.c_int32 * nt)(*self.codes)
         self._pf = (_lib.tt_pfunc * max(npf, 1))(*[_lib.tt_pfunc(*t) for t in self.pf])
@@ -419,7 +427,11 @@ class BatchBackward:
     the captured graph."""
 
     def __init__(self, images, n_angles: int, step_deg: float, tfunctionals,
-                 pfunctionals):
+                 pfunctionals, forward: Batch | None = None):
+        """``forward``: a Batch of the same images, angles and functionals
+        built with ``save_for_backward``; its line data and sinograms from its
+        last run replace the forward recompute (the images must not change
+        between that run and this one)."""
         self.images = list(images)
         self.n_angles = int(n_angles)
         self.step = float(step_deg)
@@ -432,6 +444,9 @@ class BatchBackward:
                                        dtype=torch.float32, device=dev)
         self.dcircus = [self.dcircus_all[k] for k in range(len(self.images))]
         self.dimage = [torch.empty_like(im) for im in self.images]
+        if forward is not None and forward.saved is None:
+            raise ValueError("the forward Batch was not built with save_for_backward")
+        self.forward = forward
         arr = (_lib.tt_image_grad * len(self.images))()
         for k, im in enumerate(self.images):
             _require_cuda_f32(f"images[{k}]", im)
@@ -439,6 +454,9 @@ class BatchBackward:
             arr[k].n = im.shape[0]
             arr[k].d_dcircus = self.dcircus[k].data_ptr()
             arr[k].d_dimage = self.dimage[k].data_ptr()
+            if forward is not None:
+                arr[k].d_sino = forward.sino[k].data_ptr()
+                arr[k].d_saved = forward.saved[k].data_ptr()
         self._arr = arr
         self._codes = (C.c_int32 * nt)(*self.codes)
         self._pf = (_lib.tt_pfunc * npf)(*[_lib.tt_pfunc(*t) for t in self.pf])

@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -148,6 +149,48 @@ void make_hermite(int n, unsigned order, int center, double *w)
 
 }  // namespace
 
+// Line data a batched forward keeps for a batched backward of the same batch
+// (per image, parts 256-byte aligned): the weighted medians m and sm
+// ([A][n] int32 each), the complex T3..T5 integrals ([3][A][n] float2), P3's
+// half spectra ([nt][A][2h] float, h = n / 2 + 1) and P2's source rows
+// ([nt][A] int32).
+struct SavedLayout {
+    size_t med_x, med_s, z345, spec, p2src, total;
+};
+
+SavedLayout saved_layout(int n, int A, int nt)
+{
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t ln = (size_t)n * A;
+    const size_t h2 = 2 * (size_t)(n / 2 + 1);
+    SavedLayout L;
+    L.med_x = 0;
+    L.med_s = al(L.med_x + 4 * ln);
+    L.z345 = al(L.med_s + 4 * ln);
+    L.spec = al(L.z345 + 24 * ln);
+    L.p2src = al(L.spec + 4 * (size_t)nt * A * h2);
+    L.total = al(L.p2src + 4 * (size_t)nt * A);
+    return L;
+}
+
+// What a saved block was written for: angles, step and functionals.
+std::vector<int64_t> saved_key(int A, float step, const int32_t *tcodes, int nt,
+                               const tt_pfunc *pf, int np)
+{
+    std::vector<int64_t> k = {A, (int64_t)(*(const uint32_t *)&step), nt, np};
+    for (int t = 0; t < nt; t++)
+        k.push_back(tcodes[t]);
+    for (int q = 0; q < np; q++)
+        k.push_back(pf[q].code);
+    return k;
+}
+
+struct SavedEntry {
+    int n;
+    std::vector<int64_t> key;
+    const float *sino;
+};
+
 // Event pairs bracketing kernel launches while profiling is on.
 struct KernelTimer {
     bool on = false;
@@ -257,7 +300,7 @@ struct BatchPlan {
 // medians, T3..T5 integrals, line records), laid out back to back.
 struct BackwardPlan {
     BatchPlan g;                  // descriptors, tables, graph
-    DevBuf sino, dsino, z, medx, meds, rec, spec, fjobs, bjobs;
+    DevBuf sino, dsino, z, medx, meds, rec, spec, y, p2src, fjobs, sjobs, bjobs;
     int gather_tile = 32, gather_waves = 4;
 };
 
@@ -267,12 +310,13 @@ struct tt_context {
     BatchPlan batch;
     BackwardPlan bwd;
     bool autotune = false;
+    std::map<const void *, SavedEntry> saved;  // d_saved blocks a batched forward writes
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     DevBuf imgT, trig, tab345, io_a, io_b, med_x, med_s, sino_ck;
     DevBuf p3_spec;
-    DevBuf bw_sino, bw_dsino, bw_z, bw_rec, bw_rec67, bw_part, bw_t6src, bw_t7src;
+    DevBuf bw_sino, bw_dsino, bw_z, bw_rec, bw_rec67, bw_part, bw_t6src, bw_t7src, bw_p2src;
     DevBuf nos_med, nos_a, nos_u, nos_v, nos_s, nos_e, nos_c, nos_info, nos_sweeps;
     DevBuf nos_w, nos_m, nos_q, nos_r, nos_dev, nos_sp, nos_ptrs, nos_h;
     double *h_dev = nullptr;  // pinned host copy of the NOS convergence partials
@@ -459,7 +503,7 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
                       &ctx->imgT, &ctx->trig, &ctx->tab345,
                       &ctx->io_a, &ctx->io_b, &ctx->med_x,
                       &ctx->med_s, &ctx->sino_ck, &ctx->bw_sino, &ctx->bw_dsino, &ctx->bw_z,
-                      &ctx->bw_rec, &ctx->bw_rec67, &ctx->bw_part, &ctx->bw_t6src,
+                      &ctx->bw_rec, &ctx->bw_rec67, &ctx->bw_part, &ctx->bw_p2src, &ctx->bw_t6src,
                       &ctx->bw_t7src, &ctx->nos_med, &ctx->nos_a, &ctx->nos_u,
                       &ctx->nos_v, &ctx->nos_s, &ctx->nos_e, &ctx->nos_c,
                       &ctx->nos_info, &ctx->nos_sweeps, &ctx->nos_w, &ctx->nos_m,
@@ -479,8 +523,8 @@ TT_API tt_status tt_context_destroy(tt_context *ctx)
     ctx->batch.release();
     ctx->bwd.g.release();
     DevBuf *bb[] = {&ctx->bwd.sino, &ctx->bwd.dsino, &ctx->bwd.z, &ctx->bwd.medx,
-                    &ctx->bwd.meds, &ctx->bwd.rec, &ctx->bwd.spec, &ctx->bwd.fjobs,
-                    &ctx->bwd.bjobs};
+                    &ctx->bwd.meds, &ctx->bwd.rec, &ctx->bwd.spec, &ctx->bwd.y,
+                    &ctx->bwd.p2src, &ctx->bwd.fjobs, &ctx->bwd.sjobs, &ctx->bwd.bjobs};
     for (DevBuf *b : bb)
         b->release();
     if (ctx->blas)
@@ -547,6 +591,13 @@ TT_API tt_status tt_pad(tt_context *ctx, const float *d_image, int rows,
     TT_HIP(ttk::launch_pad(d_image, rows, cols, d_padded, n, dfx, dfy, ctx->stream));
     TT_HIP(ctx->timer.end(ctx->stream));
     return TT_OK;
+}
+
+TT_API size_t tt_saved_bytes(int n, int n_angles, int nt)
+{
+    if (n < 3 || n_angles < 1 || nt < 1)
+        return 0;
+    return saved_layout(n, n_angles, nt).total;
 }
 
 TT_API int tt_angle_count(float angle_step_deg)
@@ -892,6 +943,7 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         key.push_back(images[i].n);
         key.push_back((int64_t)(intptr_t)images[i].d_sino);
         key.push_back((int64_t)(intptr_t)images[i].d_circus);
+        key.push_back((int64_t)(intptr_t)images[i].d_saved);
     }
     BatchPlan &B = ctx->batch;
     hipStream_t own = ctx->own;
@@ -939,13 +991,21 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         int dft_tiles = 0;
         size_t off = 0, cko = 0, so = 0;
         int items = 0, cols = 0;
+        bool p2 = false;
+        for (int q = 0; q < np; q++)
+            p2 = p2 || pf[q].code == TT_P2;
         for (int k = 0; k < nimg; k++) {
             const tt_image &im = images[order[k]];
             ttk::BatchImage &b = d[k];
             b.spec = nullptr;
+            b.p2src = nullptr;
+            unsigned char *sv = (unsigned char *)im.d_saved;
+            const SavedLayout SL = saved_layout(im.n, n_angles, nt);
+            if (sv && p2 && nmax <= 2048)
+                b.p2src = (int32_t *)(sv + SL.p2src);
             if (p3) {
                 const int h2 = 2 * (im.n / 2 + 1);
-                b.spec = B.spec.as<float>() + so;
+                b.spec = sv ? (float *)(sv + SL.spec) : B.spec.as<float>() + so;
                 so += (size_t)nt * n_angles * h2;
                 ttk::DftJob j = {};
                 j.in = im.d_sino;
@@ -965,8 +1025,10 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
             b.imgT = B.imgT.as<float>() + off;
             off += (size_t)im.n * im.n;
             b.sino = im.d_sino;
-            b.med_x = nullptr;
-            b.med_s = nullptr;
+            // with d_saved the medians and T3..T5 integrals a backward needs
+            b.med_x = sv && need_x ? (int32_t *)(sv + SL.med_x) : nullptr;
+            b.med_s = sv && need_s ? (int32_t *)(sv + SL.med_s) : nullptr;
+            b.z345 = sv && do345 ? (float2 *)(sv + SL.z345) : nullptr;
             b.n = im.n;
             b.nb = (im.n + ttk::kWave - 1) / ttk::kWave;
             b.item0 = items;
@@ -1017,6 +1079,7 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         cp.A = n_angles;
         cp.nmax = nmax;
         cp.np = np;
+        cp.with_circus = 1;
         for (int q = 0; q < np; q++)
             cp.code[q] = pf[q].code;
 
@@ -1070,6 +1133,11 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         TT_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
         B.key = key;
         B.valid = true;
+        const std::vector<int64_t> fkey =
+            saved_key(n_angles, angle_step_deg, tcodes, nt, pf, np);
+        for (int i = 0; i < nimg; i++)
+            if (images[i].d_saved)
+                ctx->saved[images[i].d_saved] = SavedEntry{images[i].n, fkey, images[i].d_sino};
     }
     TT_HIP(ctx->timer.begin(TT_KERNEL_BATCH, own));
     TT_HIP(hipGraphLaunch(B.exec, own));
@@ -1146,8 +1214,18 @@ tt_status circus_vjp(tt_context *ctx, const float *d_sino, int n, int n_cols,
         j.g_cols = n_cols;
         j.g_stride = np * n_cols;
         sp.job = j;
+        sp.job.in = j.out;  // scaled in place
+        sp.job.ld_in = 2 * h;
         sp.njobs = 1;
         p.p3_done = 1;
+    }
+    bool p2 = false;
+    for (int q = 0; q < np; q++)
+        p2 = p2 || pf[q].code == TT_P2;
+    if (p2 && n <= 2048) {
+        // P2's source rows by the one-wave register sort
+        TT_HIP(ctx->bw_p2src.ensure(sizeof(int32_t) * (size_t)cols));
+        p.p2src = ctx->bw_p2src.as<int32_t>();
     }
     TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS_VJP, st));
     if (sp.np3 > 0) {
@@ -1168,6 +1246,8 @@ tt_status circus_vjp(tt_context *ctx, const float *d_sino, int n, int n_cols,
         b.tiles = ttk::dft_job_tiles(n, cols, false);
         TT_HIP(ttk::launch_dft_gemm(b, false, n, st));
     }
+    if (p.p2src)
+        TT_HIP(ttk::launch_p2_source(d_sino, n, cols, ctx->bw_p2src.as<int32_t>(), st));
     TT_HIP(ttk::launch_circus_vjp(p, st));
     TT_HIP(ctx->timer.end(st));
     return TT_OK;
@@ -1809,24 +1889,42 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
     int slot[8];
     TT_TRY(adjoint_slots(tcodes, nt, slot, false));
     TT_CHECK(np >= 1 && np <= ttk::kMaxP && pf, "np must be in 1..%d", ttk::kMaxP);
+    bool p2 = false;
     for (int q = 0; q < np; q++) {
         if (pf[q].code == TT_P_HERMITE)
             return fail(TT_ERR_UNSUPPORTED,
                         "Hermite P-functionals are not available in the batched path");
         TT_CHECK(pf[q].code >= 1 && pf[q].code <= 3,
                  "invalid P-functional code %d at position %d", pf[q].code, q);
+        p2 = p2 || pf[q].code == TT_P2;
     }
-    int nmax = 0;
+    int nmax = 0, nsaved = 0;
     for (int i = 0; i < nimg; i++) {
         TT_CHECK(images[i].d_image && images[i].d_dcircus && images[i].d_dimage,
                  "image %d: NULL pointer", i);
         TT_CHECK(images[i].n >= 3 && images[i].n <= 1800,
                  "image %d: padded side %d outside 3..1800", i, images[i].n);
         nmax = images[i].n > nmax ? images[i].n : nmax;
+        nsaved += images[i].d_saved != nullptr;
+    }
+    TT_CHECK(nsaved == 0 || nsaved == nimg,
+             "d_saved is set for %d of %d images (all or none)", nsaved, nimg);
+    const bool saved = nsaved > 0;
+    if (saved) {
+        const std::vector<int64_t> fkey =
+            saved_key(n_angles, angle_step_deg, tcodes, nt, pf, np);
+        for (int i = 0; i < nimg; i++) {
+            TT_CHECK(images[i].d_sino, "image %d: d_saved without d_sino", i);
+            auto it = ctx->saved.find(images[i].d_saved);
+            TT_CHECK(it != ctx->saved.end() && it->second.n == images[i].n &&
+                         it->second.key == fkey && it->second.sino == images[i].d_sino,
+                     "image %d: d_saved was not written by tt_forward_batched with this "
+                     "image size, its sinograms, angles and functionals", i);
+        }
     }
     TT_TRY(ctx->set_device());
     std::vector<int64_t> key = {nimg, n_angles, (int64_t)(*(const uint32_t *)&angle_step_deg),
-                                nt, np};
+                                nt, np, saved};
     for (int t = 0; t < nt; t++)
         key.push_back(tcodes[t]);
     for (int q = 0; q < np; q++)
@@ -1836,11 +1934,14 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         key.push_back(images[i].n);
         key.push_back((int64_t)(intptr_t)images[i].d_dcircus);
         key.push_back((int64_t)(intptr_t)images[i].d_dimage);
+        key.push_back((int64_t)(intptr_t)images[i].d_sino);
+        key.push_back((int64_t)(intptr_t)images[i].d_saved);
     }
     BackwardPlan &P = ctx->bwd;
     BatchPlan &B = P.g;
     hipStream_t own = ctx->own;
     const bool do345 = slot[3] >= 0 || slot[4] >= 0 || slot[5] >= 0;
+    const bool do12 = slot[1] >= 0 || slot[2] >= 0;
     if (!B.valid || B.key != key) {
         B.drop_graph();
         std::vector<int> order(nimg);
@@ -1849,41 +1950,47 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
             return images[a].n > images[b].n;
         });
-        size_t sq = 0, ln = 0;
+        size_t sq = 0, ln = 0, spec_tot = 0;
         for (int i = 0; i < nimg; i++) {
             sq += (size_t)images[i].n * images[i].n;
             ln += (size_t)images[i].n * n_angles;
+            spec_tot += (size_t)nt * n_angles * 2 * (images[i].n / 2 + 1);
         }
-        TT_HIP(B.imgT.ensure(sizeof(float) * sq));
-        TT_HIP(P.sino.ensure(sizeof(float) * nt * ln));
+        ttk::P3ScaleParams p3s = {};
+        p3s.np3 = p3_positions(pf, np, p3s.q3);
+        const bool p2src = p2 && nmax <= 2048;
+        // without saved line data the forward is recomputed into workspaces
+        if (!saved) {
+            TT_HIP(B.imgT.ensure(sizeof(float) * sq));
+            TT_HIP(P.sino.ensure(sizeof(float) * nt * ln));
+            TT_HIP(P.z.ensure(sizeof(float2) * 3 * ln));
+            TT_HIP(P.medx.ensure(sizeof(int32_t) * ln));
+            TT_HIP(P.meds.ensure(sizeof(int32_t) * ln));
+            TT_HIP(B.bbox.ensure(ttk::kBboxInts * sizeof(int) * (size_t)nimg));
+            size_t ck = 0;  // the line kernel's walk checkpoints, image after image
+            for (int i = 0; i < nimg; i++)
+                ck += (size_t)ttk::sino_ckpt_blocks(images[i].n) * images[i].n * n_angles;
+            TT_HIP(B.ckpt.ensure(sizeof(float2) * ck));
+            if (p3s.np3 > 0)
+                TT_HIP(P.spec.ensure(sizeof(float) * spec_tot));
+            if (p2src)
+                TT_HIP(P.p2src.ensure(sizeof(int32_t) * nt * (size_t)n_angles * nimg));
+        }
         TT_HIP(P.dsino.ensure(sizeof(float) * nt * ln));
-        TT_HIP(P.z.ensure(sizeof(float2) * 3 * ln));
-        TT_HIP(P.medx.ensure(sizeof(int32_t) * ln));
-        TT_HIP(P.meds.ensure(sizeof(int32_t) * ln));
         TT_HIP(P.rec.ensure(sizeof(ttk::LineRecord) * ln));
-        TT_HIP(B.bbox.ensure(ttk::kBboxInts * sizeof(int) * (size_t)nimg));
-        size_t ck = 0;  // the line kernel's walk checkpoints, image after image
-        for (int i = 0; i < nimg; i++)
-            ck += (size_t)ttk::sino_ckpt_blocks(images[i].n) * images[i].n * n_angles;
-        TT_HIP(B.ckpt.ensure(sizeof(float2) * ck));
+        if (p3s.np3 > 0)
+            TT_HIP(P.y.ensure(sizeof(float) * spec_tot));
         int tiles32 = 0, tiles16 = 0;
         for (int i = 0; i < nimg; i++) {
             tiles32 += tile_count(images[i].n, 32);
             tiles16 += tile_count(images[i].n, 16);
         }
         const GatherLaunch gl = choose_gather(tiles32, tiles16, n_angles,
-                                              device_cus(ctx->device), nmax, do345,
-                                              slot[1] >= 0 || slot[2] >= 0, false, false);
+                                              device_cus(ctx->device), nmax, do345, do12,
+                                              false, false);
         P.gather_tile = gl.tile;
         P.gather_waves = gl.waves;
-        ttk::P3ScaleParams p3s = {};
-        p3s.np3 = p3_positions(pf, np, p3s.q3);
-        size_t spec_tot = 0;
-        for (int i = 0; i < nimg && p3s.np3 > 0; i++)
-            spec_tot += (size_t)nt * n_angles * 2 * (images[i].n / 2 + 1);
-        if (p3s.np3 > 0)
-            TT_HIP(P.spec.ensure(sizeof(float) * spec_tot));
-        std::vector<ttk::DftJob> fj, bj;
+        std::vector<ttk::DftJob> fj, sj, bj;
         int ftiles = 0, btiles = 0;
         size_t so = 0;
         std::vector<ttk::BatchImage> d(nimg);
@@ -1892,48 +1999,14 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         for (int k = 0; k < nimg; k++) {
             const tt_image_grad &im = images[order[k]];
             ttk::BatchImage &b = d[k];
-            if (p3s.np3 > 0) {
-                // P3's adjoint: spectra of the recomputed sinograms, scaled in
-                // place, then W y into the sinogram gradient
-                const int h2 = 2 * (im.n / 2 + 1);
-                ttk::DftJob j = {};
-                j.in = P.sino.as<float>() + nt * lo;
-                j.out = P.spec.as<float>() + so;
-                j.n = im.n;
-                j.cols = nt * n_angles;
-                j.ld_in = im.n;
-                j.ld_out = h2;
-                j.tile0 = ftiles;
-                j.g = im.d_dcircus;
-                j.g_cols = n_angles;
-                j.g_stride = np * n_angles;
-                ftiles += ttk::dft_job_tiles(im.n, j.cols, true);
-                fj.push_back(j);
-                ttk::DftJob jb = j;
-                jb.in = j.out;
-                jb.out = P.dsino.as<float>() + nt * lo;
-                jb.ld_in = h2;
-                jb.ld_out = im.n;
-                jb.tile0 = btiles;
-                btiles += ttk::dft_job_tiles(im.n, j.cols, false);
-                bj.push_back(jb);
-                so += (size_t)nt * n_angles * h2;
-            }
-            b.bbox = B.bbox.as<int>() + ttk::kBboxInts * k;
-            b.ckpt = B.ckpt.as<float2>() + cko;
-            cko += (size_t)ttk::sino_ckpt_blocks(im.n) * im.n * n_angles;
+            const int h2 = 2 * (im.n / 2 + 1);
             const size_t l = (size_t)im.n * n_angles;
             b.img = im.d_image;
-            b.imgT = B.imgT.as<float>() + po;
-            b.sino = P.sino.as<float>() + nt * lo;
-            b.med_x = P.medx.as<int32_t>() + lo;
-            b.med_s = P.meds.as<int32_t>() + lo;
             b.n = im.n;
             b.nb = (im.n + ttk::kWave - 1) / ttk::kWave;
             b.item0 = items;
             b.col0 = cols;
             b.circus = nullptr;
-            b.z345 = P.z.as<float2>() + 3 * lo;
             b.dcircus = im.d_dcircus;
             b.dsino = P.dsino.as<float>() + nt * lo;
             b.rec = P.rec.as<ttk::LineRecord>() + lo;
@@ -1941,6 +2014,58 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
             b.line0 = (int)lo;
             b.pix0 = (int)po;
             b.tile0 = (int)tiles;
+            if (saved) {
+                const SavedLayout L = saved_layout(im.n, n_angles, nt);
+                unsigned char *base = (unsigned char *)images[order[k]].d_saved;
+                b.sino = const_cast<float *>(im.d_sino);
+                b.med_x = (int32_t *)(base + L.med_x);
+                b.med_s = (int32_t *)(base + L.med_s);
+                b.z345 = (float2 *)(base + L.z345);
+                b.spec = (float *)(base + L.spec);
+                b.p2src = p2src ? (int32_t *)(base + L.p2src) : nullptr;
+            } else {
+                b.bbox = B.bbox.as<int>() + ttk::kBboxInts * k;
+                b.ckpt = B.ckpt.as<float2>() + cko;
+                cko += (size_t)ttk::sino_ckpt_blocks(im.n) * im.n * n_angles;
+                b.imgT = B.imgT.as<float>() + po;
+                b.sino = P.sino.as<float>() + nt * lo;
+                b.med_x = P.medx.as<int32_t>() + lo;
+                b.med_s = P.meds.as<int32_t>() + lo;
+                b.z345 = P.z.as<float2>() + 3 * lo;
+                b.spec = p3s.np3 > 0 ? P.spec.as<float>() + so : nullptr;
+                b.p2src = p2src ? P.p2src.as<int32_t>() + (size_t)nt * n_angles * k : nullptr;
+            }
+            if (p3s.np3 > 0) {
+                // P3's adjoint: spectra of the sinograms (unless saved),
+                // scaled per column into y, then W y into the sinogram gradient
+                ttk::DftJob j = {};
+                j.in = b.sino;
+                j.out = b.spec;
+                j.n = im.n;
+                j.cols = nt * n_angles;
+                j.ld_in = im.n;
+                j.ld_out = h2;
+                j.tile0 = ftiles;
+                ftiles += ttk::dft_job_tiles(im.n, j.cols, true);
+                fj.push_back(j);
+                ttk::DftJob js = j;
+                js.in = b.spec;
+                js.ld_in = h2;
+                js.out = P.y.as<float>() + so;
+                js.g = im.d_dcircus;
+                js.g_cols = n_angles;
+                js.g_stride = np * n_angles;
+                sj.push_back(js);
+                ttk::DftJob jb = j;
+                jb.in = js.out;
+                jb.out = b.dsino;
+                jb.ld_in = h2;
+                jb.ld_out = im.n;
+                jb.tile0 = btiles;
+                btiles += ttk::dft_job_tiles(im.n, j.cols, false);
+                bj.push_back(jb);
+            }
+            so += (size_t)nt * n_angles * h2;
             tiles += (size_t)tile_count(im.n, gl.tile);
             items += b.nb * n_angles;
             cols += nt * n_angles;
@@ -1954,8 +2079,11 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         ttk::DftParams fdp = {}, bdp = {};
         if (p3s.np3 > 0) {
             TT_HIP(P.fjobs.ensure(sizeof(ttk::DftJob) * fj.size()));
+            TT_HIP(P.sjobs.ensure(sizeof(ttk::DftJob) * sj.size()));
             TT_HIP(P.bjobs.ensure(sizeof(ttk::DftJob) * bj.size()));
             TT_HIP(hipMemcpy(P.fjobs.ptr, fj.data(), sizeof(ttk::DftJob) * fj.size(),
+                             hipMemcpyHostToDevice));
+            TT_HIP(hipMemcpy(P.sjobs.ptr, sj.data(), sizeof(ttk::DftJob) * sj.size(),
                              hipMemcpyHostToDevice));
             TT_HIP(hipMemcpy(P.bjobs.ptr, bj.data(), sizeof(ttk::DftJob) * bj.size(),
                              hipMemcpyHostToDevice));
@@ -1965,8 +2093,8 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
             bdp.jobs = P.bjobs.as<ttk::DftJob>();
             bdp.njobs = (int)bj.size();
             bdp.tiles = btiles;
-            p3s.jobs = fdp.jobs;
-            p3s.njobs = fdp.njobs;
+            p3s.jobs = P.sjobs.as<ttk::DftJob>();
+            p3s.njobs = (int)sj.size();
         }
         std::vector<float2> h;
         make_trig(n_angles, angle_step_deg, h);
@@ -1990,6 +2118,14 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         sp.counter = B.counter.as<unsigned int>();
         for (int k = 0; k < 8; k++)
             sp.slot.s[k] = slot[k];
+        ttk::BatchCircusParams cp = {};
+        cp.images = sp.images;
+        cp.nimg = nimg;
+        cp.cols = cols;
+        cp.nt = nt;
+        cp.A = n_angles;
+        cp.nmax = nmax;
+        cp.np = np;
         ttk::BatchAdjointParams ap = {};
         ap.images = sp.images;
         ap.nimg = nimg;
@@ -2013,16 +2149,21 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         B.waves_per_block = 4;
         B.blocks = cus * 4;
         TT_HIP(hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal));
-        hipError_t ce = ttk::launch_transpose_batched(sp.images, nimg, nmax, own);
-        if (ce == hipSuccess)
-            ce = ttk::launch_image_bbox_batched(sp.images, nimg, nmax, own);
-        if (ce == hipSuccess)
-            ce = ttk::launch_reset_counter(sp.counter, own);
-        if (ce == hipSuccess)
-            ce = ttk::launch_sino_t05_batched(sp, true, true, do345, B.waves_per_block,
-                                              B.blocks, own);
-        if (ce == hipSuccess && p3s.np3 > 0)
-            ce = ttk::launch_dft_gemm(fdp, true, nmax, own);
+        hipError_t ce = hipSuccess;
+        if (!saved) {
+            ce = ttk::launch_transpose_batched(sp.images, nimg, nmax, own);
+            if (ce == hipSuccess)
+                ce = ttk::launch_image_bbox_batched(sp.images, nimg, nmax, own);
+            if (ce == hipSuccess)
+                ce = ttk::launch_reset_counter(sp.counter, own);
+            if (ce == hipSuccess)
+                ce = ttk::launch_sino_t05_batched(sp, true, true, do345, B.waves_per_block,
+                                                  B.blocks, own);
+            if (ce == hipSuccess && p3s.np3 > 0)
+                ce = ttk::launch_dft_gemm(fdp, true, nmax, own);
+            if (ce == hipSuccess && p2src)
+                ce = ttk::launch_p2_source_batched(cp, own);
+        }
         if (ce == hipSuccess && p3s.np3 > 0)
             ce = ttk::launch_p3_vjp_scale(p3s, nt * n_angles, own);
         if (ce == hipSuccess && p3s.np3 > 0)

@@ -136,7 +136,8 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
 // the column's P3 (summed over the P list's P3 entries) and c_k = w_k 4 |X_k|^2 / n^4 (the derivative of
 // pow(hypot(Re/n, Im/n), 4) = |X_k|^4 / n^4, times the trapz weight of the
 // reference's float abscissae lin[p] = -1 + 2p/(n-1), folded onto k from
-// p = k and its mirror p = n - k).  In place over spec.
+// p = k and its mirror p = n - k).  x (the spectra, ld_in) and y (ld_out)
+// may be the same array.
 __global__ __launch_bounds__(kBlock) void p3_vjp_scale_kernel(P3ScaleParams p)
 {
     const DftJob &jb = p.jobs ? p.jobs[blockIdx.y] : p.job;
@@ -149,6 +150,7 @@ __global__ __launch_bounds__(kBlock) void p3_vjp_scale_kernel(P3ScaleParams p)
         const size_t gb = (size_t)(c / jb.g_cols) * jb.g_stride + c % jb.g_cols;
         for (int i = 0; i < p.np3; i++)
             g += jb.g[gb + (size_t)p.q3[i] * jb.g_cols];
+        const float *x = jb.in + (size_t)c * jb.ld_in;
         float *y = jb.out + (size_t)c * jb.ld_out;
         for (int k = threadIdx.x; k < h; k += kBlock) {
             double w = 0.0;
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void p3_vjp_scale_kernel(P3ScaleParams p)
                 if (q <= n - 2)
                     w += 0.5 * ((double)(float)(-1 + (q + 1) * 2.0 / (n - 1)) - l);
             }
-            const double re = y[k], im = y[h + k];
+            const double re = x[k], im = x[h + k];
             const double n4 = (double)n * n * (double)n * n;
             const double ck = (double)g * w * 4.0 * (re * re + im * im) / n4;
             y[k] = (float)(ck * re);

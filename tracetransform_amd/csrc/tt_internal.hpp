@@ -72,7 +72,8 @@ struct BatchImage {
     int item0;            // first wave item of this image
     int col0;             // first sinogram column (over all T) of this image
     float *circus;        // [nt][np][A]
-    float *spec;          // [nt][A][2h] P3 half spectra (workspace), or null
+    float *spec;          // [nt][A][2h] P3 half spectra, or null
+    int32_t *p2src;       // [nt][A] P2 source rows (the adjoint's one-hot), or null
     // adjoint (tt_backward_batched); null in the forward-only batch
     float2 *z345;         // [3][A][n] T3..T5 integrals
     const float *dcircus; // [nt][np][A]
@@ -108,6 +109,7 @@ struct BatchCircusParams {
     int np;
     int code[kMaxP];
     int p2_wave;          // P2 entries written by circus_p2_batched_kernel (launcher's)
+    int with_circus;      // circus_p2_batched_kernel writes circus values
 };
 
 // Sort-based T-functionals T6/T7 for one padded image: one workgroup owns one
@@ -197,6 +199,7 @@ struct CircusVjpParams {
     const float *dcircus; // [nsino][np][ncols]
     float *dsino;         // [nsino][ncols][n] (holds P3's part when p3_done)
     int p3_done;
+    const int32_t *p2src; // [nsino][ncols] P2 source rows, or null (then sorted here)
     const double *herm_w; // [nherm][n]
     int n;
     int ncols;
@@ -233,7 +236,7 @@ struct DftParams {
 };
 struct P3ScaleParams {
     const DftJob *jobs;   // device array of njobs, or null: job is the one
-    int njobs;            // (out = the spectra, scaled in place)
+    int njobs;            // (in: the spectra, out: the scaled spectra)
     DftJob job;
     int np3;              // P3 entries of the P list, at positions q3
     int q3[kMaxP];
@@ -323,6 +326,11 @@ hipError_t launch_sino_t05_batched(const BatchSinoParams &p, bool need_x,
                                    bool need_s, bool do345, int waves_per_block,
                                    int blocks, hipStream_t s);
 hipError_t launch_circus_batched(const BatchCircusParams &p, hipStream_t s);
+// P2's source rows (stable order) of n-row columns, n <= 2048: src[cols].
+hipError_t launch_p2_source(const float *sino, int n, int cols, int32_t *src,
+                            hipStream_t s);
+// The same for every column of a batch into images[].p2src (nmax <= 2048).
+hipError_t launch_p2_source_batched(const BatchCircusParams &p, hipStream_t s);
 
 // Batched adjoint stages (same descriptors as the forward batch).
 struct BatchAdjointParams {

@@ -1516,10 +1516,38 @@ __device__ __forceinline__ void sort_prefix(Key (&k)[KMAX], int K, int lane)
 // workgroup.
 constexpr int kP2Max = 2048;
 
+// Source row of P2's element (the adjoint's one-hot): the stable order of
+// compareFloat (c++/src/functionals.cpp:32-43, ties by ascending index, -0
+// equal to +0) puts the r-th occurrence in column order of the value v at
+// rank lo + r, lo the number of elements below v.  -1 when v occurs nowhere
+// (a NaN median).  Wave-uniform result.
+__device__ __forceinline__ int p2_source(const float *x, int n, float v, int k, int lane)
+{
+    int lo = 0;
+    for (int i0 = 0; i0 < n; i0 += kWave) {
+        const int i = i0 + lane;
+        lo += __popcll(__ballot(i < n && x[i] < v));
+    }
+    int r = k - lo;
+    for (int i0 = 0; i0 < n; i0 += kWave) {
+        const int i = i0 + lane;
+        uint64_t eq = __ballot(i < n && x[i] == v);
+        const int c = __popcll(eq);
+        if (r < c) {
+            for (int j = 0; j < r; j++)
+                eq &= eq - 1;
+            return i0 + __ffsll((unsigned long long)eq) - 1;
+        }
+        r -= c;
+    }
+    return -1;
+}
+
 // P2 of the column x[0..n) (n <= kP2Max) by one wave, sorted values in xs;
-// the value and its rank are valid on lane 0.
+// the value and its rank are valid on lane 0, and with src its source row
+// (every lane).
 __device__ __forceinline__ void p2_wave(const float *x, int n, float *xs, int lane,
-                                        float *res, int *rank)
+                                        float *res, int *rank, int *src = nullptr)
 {
     const int K = keys_per_lane<32>(n);
     uint32_t key[32];
@@ -1547,6 +1575,8 @@ __device__ __forceinline__ void p2_wave(const float *x, int n, float *xs, int la
     k = lane_int(k, 0);
     *res = xs[k];
     *rank = k;
+    if (src)
+        *src = p2_source(x, n, xs[k], k, lane);
 }
 
 __global__ __launch_bounds__(kBlock) void circus_p2_kernel(CircusParams p, int q)
@@ -1568,8 +1598,27 @@ __global__ __launch_bounds__(kBlock) void circus_p2_kernel(CircusParams p, int q
     }
 }
 
+// P2's source row of every sinogram column (the adjoint's one-hot), columns
+// up to kP2Max rows: src[colg] for the n-row columns of sino.
+__global__ __launch_bounds__(kBlock) void p2_source_kernel(const float *sino, int n, int cols,
+                                                           int32_t *src)
+{
+    __shared__ float s_x[kBlock / kWave][kP2Max];
+    const int w = wave_id(), lane = threadIdx.x & (kWave - 1);
+    const int colg = blockIdx.x * (kBlock / kWave) + w;
+    if (colg >= cols)
+        return;
+    float res;
+    int k, sr;
+    p2_wave(sino + (size_t)colg * n, n, s_x[w], lane, &res, &k, &sr);
+    if (lane == 0)
+        src[colg] = sr;
+}
+
 // The same over every sinogram column of a batch (image found by binary
 // search over the descriptors' first-column offsets; every n <= kP2Max).
+// Writes the circus value (with circus) and the source row (into the
+// image's p2src [nt][A], when set).
 __global__ __launch_bounds__(kBlock) void circus_p2_batched_kernel(BatchCircusParams p, int q)
 {
     __shared__ float s_x[kBlock / kWave][kP2Max];
@@ -1591,10 +1640,15 @@ __global__ __launch_bounds__(kBlock) void circus_p2_batched_kernel(BatchCircusPa
     const int t = lc / p.A;
     const int a = lc - t * p.A;
     float res;
-    int k;
-    p2_wave(im.sino + ((size_t)t * p.A + a) * n, n, s_x[w], lane, &res, &k);
-    if (lane == 0)
-        im.circus[((size_t)t * p.np + q) * p.A + a] = res;
+    int k, sr = -1;
+    p2_wave(im.sino + ((size_t)t * p.A + a) * n, n, s_x[w], lane, &res, &k,
+            im.p2src ? &sr : nullptr);
+    if (lane == 0) {
+        if (p.with_circus)
+            im.circus[((size_t)t * p.np + q) * p.A + a] = res;
+        if (im.p2src)
+            im.p2src[lc] = sr;
+    }
 }
 
 // findWeightedMedian of one column per wave (columns up to kP2Max rows, in
@@ -2452,7 +2506,7 @@ __device__ __forceinline__ VjpLds vjp_lds(unsigned char *smem, int nmax)
 __device__ void circus_vjp_column(const VjpLds &L, int n, int np, const int *code,
                                   const int *herm, const double *herm_w,
                                   const float *dc, size_t dc_stride, float *out,
-                                  bool p3_done)
+                                  bool p3_done, const int32_t *p2src)
 {
     __shared__ float s_chain[8];
     __shared__ float s_sum;
@@ -2483,6 +2537,11 @@ __device__ void circus_vjp_column(const VjpLds &L, int n, int np, const int *cod
             const double *w = herm_w + (size_t)herm[q] * n;
             for (int j = threadIdx.x; j < n; j += kBlock)
                 L.ds[j] += (float)((double)g * w[j]);
+        } else if (cd == 2 && p2src) {
+            // the source row found by the P2 wave kernel (stable order)
+            const int sr = *p2src;
+            if (threadIdx.x == 0 && sr >= 0)
+                L.ds[sr] += g;
         } else if (cd == 2) {
             for (int i = threadIdx.x; i < m2; i += kBlock)
                 L.key[i] = i < n ? (((uint64_t)ord_key(L.x[i]) << 32) | (uint32_t)i)
@@ -2525,7 +2584,8 @@ __global__ __launch_bounds__(kBlock) void circus_vjp_kernel(CircusVjpParams p)
     __syncthreads();
     circus_vjp_column(L, n, p.np, p.code, p.herm, p.herm_w,
                       p.dcircus + (size_t)sg * p.np * p.ncols + col, p.ncols,
-                      p.dsino + (size_t)colg * n, p.p3_done != 0);
+                      p.dsino + (size_t)colg * n, p.p3_done != 0,
+                      p.p2src ? p.p2src + colg : nullptr);
 }
 
 __global__ __launch_bounds__(kBlock) void circus_vjp_batched_kernel(BatchAdjointParams p)
@@ -2547,7 +2607,8 @@ __global__ __launch_bounds__(kBlock) void circus_vjp_batched_kernel(BatchAdjoint
     const int herm_none[kMaxP] = {};
     circus_vjp_column(L, n, p.np, p.code, herm_none, nullptr,
                       im.dcircus + (size_t)t * p.np * p.A + a, p.A,
-                      im.dsino + (size_t)lc * n, p.p3_done != 0);
+                      im.dsino + (size_t)lc * n, p.p3_done != 0,
+                      im.p2src ? im.p2src + lc : nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -2970,6 +3031,26 @@ hipError_t launch_circus(const CircusParams &p, hipStream_t s)
     return hipGetLastError();
 }
 
+
+hipError_t launch_p2_source(const float *sino, int n, int cols, int32_t *src, hipStream_t s)
+{
+    if (n > kP2Max)
+        return hipErrorInvalidValue;
+    const int wpb = kBlock / kWave;
+    p2_source_kernel<<<(cols + wpb - 1) / wpb, kBlock, 0, s>>>(sino, n, cols, src);
+    return hipGetLastError();
+}
+
+hipError_t launch_p2_source_batched(const BatchCircusParams &p, hipStream_t s)
+{
+    if (p.nmax > kP2Max)
+        return hipErrorInvalidValue;
+    BatchCircusParams c = p;
+    c.with_circus = 0;
+    const int wpb = kBlock / kWave;
+    circus_p2_batched_kernel<<<(p.cols + wpb - 1) / wpb, kBlock, 0, s>>>(c, 0);
+    return hipGetLastError();
+}
 
 hipError_t launch_circus_batched(const BatchCircusParams &p, hipStream_t s)
 {
