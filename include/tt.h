@@ -272,21 +272,26 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
 
 /* No reference counterpart (the reference times whole getTransform calls,
  * c++/src/demo.cpp:222-248).  When enabled, the context brackets each of its
- * kernel launches with HIP events on the stream the kernel runs on;
+ * kernel launches with HIP events on the stream the kernel runs on (the
+ * batched entry points then launch their stages directly instead of
+ * replaying their graphs, so that each stage is bracketed);
  * tt_profile_read synchronizes, returns the summed kernel time (ms) and
  * launch count of one kernel kind since the last read, and resets them. */
 enum {
     TT_KERNEL_SINO_T05 = 0,   /* T0..T5 line kernel                         */
     TT_KERNEL_SINO_SORT = 1,  /* T6/T7 line kernel                          */
-    TT_KERNEL_CIRCUS = 2,     /* P-functional kernel                        */
-    TT_KERNEL_AUX = 3,        /* transpose/pad helpers                      */
-    TT_KERNEL_BATCH = 4,      /* one replay of the batched graph            */
-    TT_KERNEL_GATHER = 5,     /* adjoint: transposed rotation (+ records)   */
-    TT_KERNEL_CIRCUS_VJP = 6, /* adjoint of the P-functionals               */
-    TT_KERNEL_BATCH_BACKWARD = 7, /* one replay of the batched backward graph */
+    TT_KERNEL_CIRCUS = 2,     /* P-functional kernels (P2 wave sort incl.)  */
+    TT_KERNEL_AUX = 3,        /* transpose/pad/bounding-box helpers         */
+    TT_KERNEL_BATCH = 4,      /* (unused: batched calls report per stage)   */
+    TT_KERNEL_GATHER = 5,     /* adjoint: transposed rotation               */
+    TT_KERNEL_CIRCUS_VJP = 6, /* adjoint of P1, P2, Hermite                 */
+    TT_KERNEL_BATCH_BACKWARD = 7, /* (unused: batched calls report per stage) */
     TT_KERNEL_NOS = 8,        /* nearest orthonormal sinogram after the
-                                 medians: align, gesvd, gemm, to float      */
-    TT_KERNEL_KINDS = 9
+                                 medians: align, polar iteration, to float  */
+    TT_KERNEL_P3_DFT = 9,     /* P3's DFT GEMMs (forward and adjoint) and the
+                                 adjoint's spectrum scaling                 */
+    TT_KERNEL_RECORDS = 10,   /* adjoint: per-line gradient records         */
+    TT_KERNEL_KINDS = 11
 };
 TT_API tt_status tt_profile_enable(tt_context *ctx, int enable);
 TT_API tt_status tt_profile_read(tt_context *ctx, int kind, double *total_ms,

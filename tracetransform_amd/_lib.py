@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "libtt_amd.so")
 TT_OK, TT_ERR_INVALID, TT_ERR_NOMEM, TT_ERR_DEVICE, TT_ERR_UNSUPPORTED = range(5)
 (TT_KERNEL_SINO_T05, TT_KERNEL_SINO_SORT, TT_KERNEL_CIRCUS, TT_KERNEL_AUX,
  TT_KERNEL_BATCH, TT_KERNEL_GATHER, TT_KERNEL_CIRCUS_VJP,
- TT_KERNEL_BATCH_BACKWARD, TT_KERNEL_NOS) = range(9)
+ TT_KERNEL_BATCH_BACKWARD, TT_KERNEL_NOS, TT_KERNEL_P3_DFT,
+ TT_KERNEL_RECORDS) = range(11)
 
 # Every symbol include/tt.h declares; tests check the library exports them.
 EXPORTED = (

@@ -26,8 +26,8 @@ namespace {
 
 thread_local std::string g_err;
 
-// Dynamic LDS of one workgroup kept within 64 KiB.
-constexpr size_t kLdsLimit = 64 * 1024;
+// Dynamic LDS of one workgroup: at most the 160 KiB of a gfx950 CU.
+constexpr size_t kLdsLimit = 160 * 1024;
 
 tt_status fail(tt_status st, const char *fmt, ...)
 {
@@ -264,6 +264,11 @@ struct BatchPlan {
     std::vector<int64_t> key;
     DevBuf desc, imgT, trig, tab, counter, bbox, ckpt, spec, jobs;
     ttk::DftParams dft = {};      // P3 spectra of every image (jobs in `jobs`)
+    // launch parameters of the captured stages (reused for timed eager runs)
+    ttk::BatchSinoParams sp = {};
+    ttk::BatchCircusParams cp = {};
+    int nimg = 0, nmax = 0;
+    bool need_x = false, need_s = false, do345 = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int waves_per_block = 4, blocks = 0;
@@ -302,6 +307,11 @@ struct BackwardPlan {
     BatchPlan g;                  // descriptors, tables, graph
     DevBuf sino, dsino, z, medx, meds, rec, spec, y, p2src, fjobs, sjobs, bjobs;
     int gather_tile = 32, gather_waves = 4;
+    ttk::BatchAdjointParams ap = {};
+    ttk::DftParams fdp = {}, bdp = {};
+    ttk::P3ScaleParams p3s = {};
+    bool saved = false, p2src_on = false;
+    int cols = 0;
 };
 
 struct tt_context {
@@ -807,9 +817,12 @@ TT_API tt_status tt_circus(tt_context *ctx, const float *d_sino, int n,
         dp.job.ld_out = 2 * h;
         dp.tiles = ttk::dft_job_tiles(n, (int)cols, true);
     }
-    TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS, st));
-    if (p3)
+    if (p3) {
+        TT_HIP(ctx->timer.begin(TT_KERNEL_P3_DFT, st));
         TT_HIP(ttk::launch_dft_gemm(dp, true, n, st));
+        TT_HIP(ctx->timer.end(st));
+    }
+    TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS, st));
     TT_HIP(ttk::launch_circus(p, st));
     TT_HIP(ctx->timer.end(st));
     return TT_OK;
@@ -849,24 +862,92 @@ int device_cus(int device)
 }
 
 // Enqueue the batched pass (used for capture and for autotuning).
+// The batched forward's stages on st.  timed: each stage bracketed by the
+// context's kernel timer (never while a graph is being captured).
 hipError_t enqueue_batch(tt_context *ctx, const ttk::BatchSinoParams &sp,
                          const ttk::BatchCircusParams &cp, int nimg, int nmax,
                          bool need_x, bool need_s, bool do345, int wpb, int blocks,
-                         bool with_transpose, bool with_circus, hipStream_t st)
+                         bool with_transpose, bool with_circus, hipStream_t st,
+                         bool timed = false)
 {
+    KernelTimer *T = timed ? &ctx->timer : nullptr;
     hipError_t e = hipSuccess;
+    auto stage = [&](int kind, auto launch) {
+        if (e != hipSuccess)
+            return;
+        if (T)
+            e = T->begin(kind, st);
+        if (e == hipSuccess)
+            e = launch();
+        if (T && e == hipSuccess)
+            e = T->end(st);
+    };
     if (with_transpose)
-        e = ttk::launch_transpose_batched(sp.images, nimg, nmax, st);
-    if (with_transpose && e == hipSuccess)
-        e = ttk::launch_image_bbox_batched(sp.images, nimg, nmax, st);
-    if (e == hipSuccess)
-        e = ttk::launch_reset_counter(sp.counter, st);
-    if (e == hipSuccess)
-        e = ttk::launch_sino_t05_batched(sp, need_x, need_s, do345, wpb, blocks, st);
-    if (e == hipSuccess && with_circus && ctx->batch.dft.tiles > 0)
-        e = ttk::launch_dft_gemm(ctx->batch.dft, true, nmax, st);
-    if (e == hipSuccess && with_circus && cp.np > 0)
-        e = ttk::launch_circus_batched(cp, st);
+        stage(TT_KERNEL_AUX, [&] {
+            hipError_t r = ttk::launch_transpose_batched(sp.images, nimg, nmax, st);
+            return r == hipSuccess ? ttk::launch_image_bbox_batched(sp.images, nimg, nmax, st)
+                                   : r;
+        });
+    stage(TT_KERNEL_SINO_T05, [&] {
+        hipError_t r = ttk::launch_reset_counter(sp.counter, st);
+        return r == hipSuccess
+                   ? ttk::launch_sino_t05_batched(sp, need_x, need_s, do345, wpb, blocks, st)
+                   : r;
+    });
+    if (with_circus && ctx->batch.dft.tiles > 0)
+        stage(TT_KERNEL_P3_DFT, [&] { return ttk::launch_dft_gemm(ctx->batch.dft, true, nmax, st); });
+    if (with_circus && cp.np > 0)
+        stage(TT_KERNEL_CIRCUS, [&] { return ttk::launch_circus_batched(cp, st); });
+    return e;
+}
+
+// The batched backward's stages on st (timed as enqueue_batch).
+hipError_t enqueue_backward(tt_context *ctx, const BackwardPlan &P, hipStream_t st,
+                            bool timed)
+{
+    const BatchPlan &B = P.g;
+    KernelTimer *T = timed ? &ctx->timer : nullptr;
+    hipError_t e = hipSuccess;
+    auto stage = [&](int kind, auto launch) {
+        if (e != hipSuccess)
+            return;
+        if (T)
+            e = T->begin(kind, st);
+        if (e == hipSuccess)
+            e = launch();
+        if (T && e == hipSuccess)
+            e = T->end(st);
+    };
+    const bool p3 = P.p3s.np3 > 0;
+    if (!P.saved) {
+        stage(TT_KERNEL_AUX, [&] {
+            hipError_t r = ttk::launch_transpose_batched(B.sp.images, B.nimg, B.nmax, st);
+            return r == hipSuccess
+                       ? ttk::launch_image_bbox_batched(B.sp.images, B.nimg, B.nmax, st)
+                       : r;
+        });
+        stage(TT_KERNEL_SINO_T05, [&] {
+            hipError_t r = ttk::launch_reset_counter(B.sp.counter, st);
+            return r == hipSuccess ? ttk::launch_sino_t05_batched(B.sp, true, true, B.do345,
+                                                                  B.waves_per_block, B.blocks,
+                                                                  st)
+                                   : r;
+        });
+        if (p3)
+            stage(TT_KERNEL_P3_DFT, [&] { return ttk::launch_dft_gemm(P.fdp, true, B.nmax, st); });
+        if (P.p2src_on)
+            stage(TT_KERNEL_CIRCUS, [&] { return ttk::launch_p2_source_batched(B.cp, st); });
+    }
+    if (p3)
+        stage(TT_KERNEL_P3_DFT, [&] {
+            hipError_t r = ttk::launch_p3_vjp_scale(P.p3s, P.cols, st);
+            return r == hipSuccess ? ttk::launch_dft_gemm(P.bdp, false, B.nmax, st) : r;
+        });
+    stage(TT_KERNEL_CIRCUS_VJP, [&] { return ttk::launch_circus_vjp_batched(P.ap, st); });
+    stage(TT_KERNEL_RECORDS, [&] { return ttk::launch_records_batched(P.ap, st); });
+    stage(TT_KERNEL_GATHER, [&] {
+        return ttk::launch_gather_batched(P.ap, P.gather_tile, P.gather_waves, st);
+    });
     return e;
 }
 
@@ -1121,6 +1202,13 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         }
         B.waves_per_block = best_w;
         B.blocks = best_b;
+        B.sp = sp;
+        B.cp = cp;
+        B.nimg = nimg;
+        B.nmax = nmax;
+        B.need_x = need_x;
+        B.need_s = need_s;
+        B.do345 = do345;
 
         TT_HIP(hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal));
         hipError_t ce = enqueue_batch(ctx, sp, cp, nimg, nmax, need_x, need_s, do345,
@@ -1139,9 +1227,14 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
             if (images[i].d_saved)
                 ctx->saved[images[i].d_saved] = SavedEntry{images[i].n, fkey, images[i].d_sino};
     }
-    TT_HIP(ctx->timer.begin(TT_KERNEL_BATCH, own));
-    TT_HIP(hipGraphLaunch(B.exec, own));
-    TT_HIP(ctx->timer.end(own));
+    if (ctx->timer.on) {
+        // per-stage kernel times: the captured stages launched directly, each
+        // bracketed by HIP events
+        TT_HIP(enqueue_batch(ctx, B.sp, B.cp, B.nimg, B.nmax, B.need_x, B.need_s, B.do345,
+                             B.waves_per_block, B.blocks, true, true, own, true));
+    } else {
+        TT_HIP(hipGraphLaunch(B.exec, own));
+    }
     if (ctx->stream != own) {
         TT_HIP(hipEventRecord(ctx->ev_out, own));
         TT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_out, 0));
@@ -1227,8 +1320,8 @@ tt_status circus_vjp(tt_context *ctx, const float *d_sino, int n, int n_cols,
         TT_HIP(ctx->bw_p2src.ensure(sizeof(int32_t) * (size_t)cols));
         p.p2src = ctx->bw_p2src.as<int32_t>();
     }
-    TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS_VJP, st));
     if (sp.np3 > 0) {
+        TT_HIP(ctx->timer.begin(TT_KERNEL_P3_DFT, st));
         ttk::DftParams f = {};
         f.njobs = 1;
         f.job = j;
@@ -1245,9 +1338,14 @@ tt_status circus_vjp(tt_context *ctx, const float *d_sino, int n, int n_cols,
         b.job.ld_out = n;
         b.tiles = ttk::dft_job_tiles(n, cols, false);
         TT_HIP(ttk::launch_dft_gemm(b, false, n, st));
+        TT_HIP(ctx->timer.end(st));
     }
-    if (p.p2src)
+    if (p.p2src) {
+        TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS, st));
         TT_HIP(ttk::launch_p2_source(d_sino, n, cols, ctx->bw_p2src.as<int32_t>(), st));
+        TT_HIP(ctx->timer.end(st));
+    }
+    TT_HIP(ctx->timer.begin(TT_KERNEL_CIRCUS_VJP, st));
     TT_HIP(ttk::launch_circus_vjp(p, st));
     TT_HIP(ctx->timer.end(st));
     return TT_OK;
@@ -1329,34 +1427,28 @@ tt_status forward_for_adjoint(tt_context *ctx, const float *d_image, int n, int 
     return TT_OK;
 }
 
-// Launch shape of the transposed-rotation gather: the largest tile (32,
-// else 16 pixels) whose workgroup of `waves` waves fits the LDS limit with
-// the T3..T5 tables, and for a single image enough angle groups that the
-// tiles x groups waves cover every SIMD of the GPU twice.
+// Launch shape of the transposed-rotation gather: 32 x 32 pixel tiles, the
+// most waves per workgroup (16, 8, 4, 2, 1) whose LDS (the T3..T5 tables once
+// plus each wave's accumulators and staged columns) fits the 160 KiB of a
+// CU, and for a single image enough angle groups that tiles x groups waves
+// fill every SIMD four times over.
 struct GatherLaunch {
     int tile, waves, groups;
 };
 
-GatherLaunch choose_gather(int tiles32, int tiles16, int A, int cus, int tabn, bool do345,
-                           bool do12, bool do67, bool split_angles)
+GatherLaunch choose_gather(int tiles32, int A, int cus, int tabn, bool do345, bool do12,
+                           bool do67, bool split_angles)
 {
-    GatherLaunch g = {16, 1, 1};
-    bool found = false;
-    for (int tile : {32, 16}) {
-        for (int waves : {4, 2, 1}) {
-            if (ttk::gather_lds_bytes(tile, waves, tabn, do345, do12, do67) <= kLdsLimit) {
-                g = {tile, waves, 1};
-                found = true;
-                break;
-            }
-        }
-        if (found)
+    GatherLaunch g = {32, 1, 1};
+    for (int waves : {16, 8, 4, 2, 1}) {
+        if (ttk::gather_lds_bytes(32, waves, tabn, do345, do12, do67) <= kLdsLimit) {
+            g.waves = waves;
             break;
+        }
     }
     if (split_angles) {
-        const int tiles = g.tile == 32 ? tiles32 : tiles16;
-        const int want = 8 * cus;
-        int groups = (want + tiles - 1) / tiles;
+        const int want = 16 * cus;
+        int groups = (want + tiles32 - 1) / tiles32;
         groups = groups > 32 ? 32 : groups;
         groups = groups > A ? A : groups;
         g.groups = groups < 1 ? 1 : groups;
@@ -1402,9 +1494,8 @@ tt_status image_vjp(tt_context *ctx, int n, int A, const int *slot,
     g.A = A;
     g.do345 = slot[3] >= 0 || slot[4] >= 0 || slot[5] >= 0;
     g.do12 = slot[1] >= 0 || slot[2] >= 0;
-    const GatherLaunch gl =
-        choose_gather(tile_count(n, 32), tile_count(n, 16), A, device_cus(ctx->device), n,
-                      g.do345, g.do12, do67, true);
+    const GatherLaunch gl = choose_gather(tile_count(n, 32), A, device_cus(ctx->device), n,
+                                          g.do345, g.do12, do67, true);
     g.tile = gl.tile;
     g.waves = gl.waves;
     g.groups = gl.groups;
@@ -1412,8 +1503,10 @@ tt_status image_vjp(tt_context *ctx, int n, int A, const int *slot,
         TT_HIP(ctx->bw_part.ensure(sizeof(float) * (size_t)g.groups * n * n));
         g.part = ctx->bw_part.as<float>();
     }
-    TT_HIP(ctx->timer.begin(TT_KERNEL_GATHER, st));
+    TT_HIP(ctx->timer.begin(TT_KERNEL_RECORDS, st));
     TT_HIP(ttk::launch_records(r, st));
+    TT_HIP(ctx->timer.end(st));
+    TT_HIP(ctx->timer.begin(TT_KERNEL_GATHER, st));
     TT_HIP(ttk::launch_gather(g, st));
     TT_HIP(ctx->timer.end(st));
     return TT_OK;
@@ -1980,14 +2073,11 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         TT_HIP(P.rec.ensure(sizeof(ttk::LineRecord) * ln));
         if (p3s.np3 > 0)
             TT_HIP(P.y.ensure(sizeof(float) * spec_tot));
-        int tiles32 = 0, tiles16 = 0;
-        for (int i = 0; i < nimg; i++) {
+        int tiles32 = 0;
+        for (int i = 0; i < nimg; i++)
             tiles32 += tile_count(images[i].n, 32);
-            tiles16 += tile_count(images[i].n, 16);
-        }
-        const GatherLaunch gl = choose_gather(tiles32, tiles16, n_angles,
-                                              device_cus(ctx->device), nmax, do345, do12,
-                                              false, false);
+        const GatherLaunch gl = choose_gather(tiles32, n_angles, device_cus(ctx->device), nmax,
+                                              do345, do12, false, false);
         P.gather_tile = gl.tile;
         P.gather_waves = gl.waves;
         std::vector<ttk::DftJob> fj, sj, bj;
@@ -2148,32 +2238,20 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         const int cus = device_cus(ctx->device);
         B.waves_per_block = 4;
         B.blocks = cus * 4;
+        B.sp = sp;
+        B.cp = cp;
+        B.nimg = nimg;
+        B.nmax = nmax;
+        B.do345 = do345;
+        P.ap = ap;
+        P.fdp = fdp;
+        P.bdp = bdp;
+        P.p3s = p3s;
+        P.saved = saved;
+        P.p2src_on = p2src;
+        P.cols = nt * n_angles;
         TT_HIP(hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal));
-        hipError_t ce = hipSuccess;
-        if (!saved) {
-            ce = ttk::launch_transpose_batched(sp.images, nimg, nmax, own);
-            if (ce == hipSuccess)
-                ce = ttk::launch_image_bbox_batched(sp.images, nimg, nmax, own);
-            if (ce == hipSuccess)
-                ce = ttk::launch_reset_counter(sp.counter, own);
-            if (ce == hipSuccess)
-                ce = ttk::launch_sino_t05_batched(sp, true, true, do345, B.waves_per_block,
-                                                  B.blocks, own);
-            if (ce == hipSuccess && p3s.np3 > 0)
-                ce = ttk::launch_dft_gemm(fdp, true, nmax, own);
-            if (ce == hipSuccess && p2src)
-                ce = ttk::launch_p2_source_batched(cp, own);
-        }
-        if (ce == hipSuccess && p3s.np3 > 0)
-            ce = ttk::launch_p3_vjp_scale(p3s, nt * n_angles, own);
-        if (ce == hipSuccess && p3s.np3 > 0)
-            ce = ttk::launch_dft_gemm(bdp, false, nmax, own);
-        if (ce == hipSuccess)
-            ce = ttk::launch_circus_vjp_batched(ap, own);
-        if (ce == hipSuccess)
-            ce = ttk::launch_records_batched(ap, own);
-        if (ce == hipSuccess)
-            ce = ttk::launch_gather_batched(ap, P.gather_tile, P.gather_waves, own);
+        hipError_t ce = enqueue_backward(ctx, P, own, false);
         hipGraph_t g = nullptr;
         hipError_t ee = hipStreamEndCapture(own, &g);
         TT_HIP(ce);
@@ -2191,9 +2269,10 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         TT_HIP(hipEventRecord(ctx->ev_in, ctx->stream));
         TT_HIP(hipStreamWaitEvent(own, ctx->ev_in, 0));
     }
-    TT_HIP(ctx->timer.begin(TT_KERNEL_BATCH_BACKWARD, own));
-    TT_HIP(hipGraphLaunch(B.exec, own));
-    TT_HIP(ctx->timer.end(own));
+    if (ctx->timer.on)
+        TT_HIP(enqueue_backward(ctx, P, own, true));  // per-stage kernel times
+    else
+        TT_HIP(hipGraphLaunch(B.exec, own));
     if (ctx->stream != own) {
         TT_HIP(hipEventRecord(ctx->ev_out, own));
         TT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_out, 0));

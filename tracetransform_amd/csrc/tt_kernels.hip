@@ -2141,12 +2141,13 @@ __global__ __launch_bounds__(kBlock) void records_batched_kernel(BatchAdjointPar
 // added into per-lane doubles and cleared.
 // ---------------------------------------------------------------------------
 constexpr int kGatherFlush = 4;
+constexpr int kGatherTile = 32;
 
 template <int T>
 struct GatherShape {
     static constexpr int P = T + 2;                    // accumulator side, with halo
     static constexpr int ACC = ((P * P + 3) / 4) * 4;  // floats, padded to float4
-    static constexpr int KMAX = T == 16 ? 32 : 64;     // footprint columns per angle
+    static constexpr int KMAX = T == 16 ? 32 : 48;     // footprint columns per angle
     static constexpr int NPIX = T * T / kWave;         // interior pixels per lane
 };
 
@@ -2156,7 +2157,7 @@ struct GatherShape {
 __host__ __device__ constexpr int gather_wave_words(int T, bool do12, bool do67)
 {
     return (((T + 2) * (T + 2) + 3) / 4) * 4 +
-           (T == 16 ? 32 : 64) * (4 + 4 + 2 + (do12 ? 4 : 0) + (do67 ? 4 : 0));
+           (T == 16 ? 32 : 48) * (4 + 4 + 2 + (do12 ? 4 : 0) + (do67 ? 4 : 0));
 }
 
 struct GatherImg {
@@ -2166,12 +2167,12 @@ struct GatherImg {
     int n;
 };
 
-template <int T, bool DO345>
+template <int T, bool DO345, bool DO12>
 __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const float2 *trig,
                                                  const float2 *s_tab, int tabn, int a0,
-                                                 int a1, int X0, int Y0, float *wl,
-                                                 bool do12)
+                                                 int a1, int X0, int Y0, float *wl)
 {
+    constexpr bool do12 = DO12;
     using S = GatherShape<T>;
     constexpr int P = S::P;
     const int lane = threadIdx.x & (kWave - 1);
@@ -2221,7 +2222,8 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
         c_lo = c_lo < 0 ? 0 : c_lo;
         c_hi = c_hi > n - 1 ? n - 1 : c_hi;
         // (T + 1.02) (|c| + |s|) bounds the span: at most 25 / 47 columns
-        // for T = 16 / 32, so the clamp never drops one
+        // for T = 16 / 32, so the clamp never drops one (and a class of
+        // every third column has at most 16)
         const int K = c_hi - c_lo + 1;
         f.c_lo = c_lo;
         f.K = K > S::KMAX ? S::KMAX : K;
@@ -2284,34 +2286,40 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
         {
             const float pxc0 = (float)c_lo - o;
             const int W3 = (W + 2) / 3;
-            const float invW3 = 1.0f / (float)W3;
             const int clo3 = c_lo % 3;
-            // one residue class (column, row mod 3) at a time: the 64
-            // samples of a step are pairwise at least 3 apart, so their 2x2
-            // tap footprints are disjoint and the step's 256 read-add-writes
-            // have distinct targets
+            // one residue class (column, row mod 3) at a time: samples of one
+            // class are pairwise at least 3 apart, so their 2x2 tap footprints
+            // are disjoint and every read-add-write of the class has its own
+            // four targets.  Lane l takes the class's column l % 16 (at most
+            // 16: K <= 47) and its rows 4 apart in the class from l / 16 on.
+            const int kl = lane & 15, jl = lane >> 4;
             for (int cls = 0; cls < 9; cls++) {
                 const int cc = cls % 3, rc = cls / 3;
                 const int koff = cc >= clo3 ? cc - clo3 : cc - clo3 + 3;
                 const int K3 = (K - koff + 2) / 3;
-                const int total = K3 * W3;
-                for (int t0 = 0; t0 < total; t0 += kWave) {
-                    const int t = t0 + lane;
-                    // t / W3 exactly: (t + 0.5) / W3 is at least 0.5 / W3 from
-                    // an integer, far above the product's rounding
-                    int k3 = (int)(((float)t + 0.5f) * invW3);
-                    k3 = t < total ? k3 : 0;
-                    const int k = 3 * k3 + koff;
-                    const float4 ea = sA[k];
-                    const int sm = __float_as_int(ea.y);
-                    const int rlo = __float_as_int(ea.z);
-                    const int r3 = rlo - 3 * (int)(((uint32_t)rlo * 0xAAABu) >> 17);
-                    const int roff = rc >= r3 ? rc - r3 : rc - r3 + 3;
-                    const int j = 3 * (t - __mul24(k3, W3)) + roff;
-                    const int row = rlo + j;
-                    // the forward's sample position (rot_sample), bit for bit
-                    const float px = pxc0 + (float)k;
-                    const float pxc = px * c, pxns = px * (-s);
+                const bool colok = kl < K3;
+                const int k = colok ? 3 * kl + koff : 0;
+                const float4 ea = sA[k];
+                const float g0 = ea.x;
+                const int sm = __float_as_int(ea.y);
+                const int rlo = __float_as_int(ea.z);
+                const int rhi = colok ? __float_as_int(ea.w) : -1;
+                const int r3 = rlo - 3 * (int)(((uint32_t)rlo * 0xAAABu) >> 17);
+                const int roff = rc >= r3 ? rc - r3 : rc - r3 + 3;
+                float4 eb = {}, ed = {}, ee = {};
+                float2 ec = {};
+                if (DO345) {
+                    eb = sB[k];
+                    ec = sC[k];
+                }
+                if (do12)
+                    ed = sD[k];
+                if (do67)
+                    ee = sE[k];
+                // the forward's sample position (rot_sample), bit for bit
+                const float px = pxc0 + (float)k;
+                const float pxc = px * c, pxns = px * (-s);
+                for (int row = rlo + roff + 3 * jl; row <= rhi; row += 12) {
                     const float py = (float)row - o;
                     float x = pxc + py * s;
                     float y = pxns + py * c;
@@ -2324,22 +2332,20 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                     const float fx = __builtin_amdgcn_fractf(x);
                     const float fy = __builtin_amdgcn_fractf(y);
                     const int xi = Xb - X0 + 1, yi = Yb - Y0 + 1;  // accumulator cell
-                    const bool use = t < total && j < W && row <= __float_as_int(ea.w) &&
-                                     (unsigned)xi <= (unsigned)T && (unsigned)yi <= (unsigned)T;
+                    const bool use = (unsigned)xi <= (unsigned)T && (unsigned)yi <= (unsigned)T;
+                    if (!use)
+                        continue;
                     // G(row): g0 + [row >= m] (g1 d + g2 d^2)
                     //   + [row > sm] sum_t (alpha_t W_t.re + beta_t W_t.im)[row - sm]
                     //   + [row == r6] c6 + [row == r7] c7
-                    float G = ea.x;
+                    float G = g0;
                     if (do12) {
-                        const float4 ed = sD[k];
                         const int m = __float_as_int(ed.z);
                         const float d = (float)(row - m);
                         const float h = fmaf(d, fmaf(ed.y, d, ed.x), G);
                         G = row >= m ? h : G;
                     }
                     if (DO345) {
-                        const float4 eb = sB[k];
-                        const float2 ec = sC[k];
                         // table entry 0 is zero, so rows at or before sm add nothing
                         int q = row - sm;
                         q = q > 0 && q < tabn ? q : 0;
@@ -2353,21 +2359,18 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                         G = fmaf(ec.y, w5.y, G);
                     }
                     if (do67) {
-                        const float4 ee = sE[k];
                         G += row == __float_as_int(ee.x) ? ee.z : 0.0f;
                         G += row == __float_as_int(ee.y) ? ee.w : 0.0f;
                     }
-                    if (use) {
-                        const float ofx = 1.0f - fx, ofy = 1.0f - fy;
-                        const float ga = G * ofx, gb = G * fx;
-                        float *cell = acc + __mul24(xi, P) + yi;
-                        const float v00 = cell[0], v01 = cell[1];
-                        const float v10 = cell[P], v11 = cell[P + 1];
-                        cell[0] = v00 + ga * ofy;
-                        cell[1] = v01 + ga * fy;
-                        cell[P] = v10 + gb * ofy;
-                        cell[P + 1] = v11 + gb * fy;
-                    }
+                    const float ofx = 1.0f - fx, ofy = 1.0f - fy;
+                    const float ga = G * ofx, gb = G * fx;
+                    float *cell = acc + __mul24(xi, P) + yi;
+                    const float v00 = cell[0], v01 = cell[1];
+                    const float v10 = cell[P], v11 = cell[P + 1];
+                    cell[0] = v00 + ga * ofy;
+                    cell[1] = v01 + ga * fy;
+                    cell[P] = v10 + gb * ofy;
+                    cell[P + 1] = v11 + gb * fy;
                 }
             }
         }
@@ -2409,12 +2412,13 @@ __device__ __forceinline__ float *gather_lds(float *smem, const float2 *tab345, 
     return smem + ((tab_words + 3) & ~3) + wave_id() * wave_words;
 }
 
-template <int T, bool DO345>
-__global__ __launch_bounds__(512) void gather_kernel(GatherParams p)
+template <bool DO345, bool DO12>
+__global__ __launch_bounds__(1024) void gather_kernel(GatherParams p)
 {
+    constexpr int T = kGatherTile;
     extern __shared__ float4 smem4[];
     float *smem = reinterpret_cast<float *>(smem4);
-    const int words = gather_wave_words(T, p.do12, p.rec67 != nullptr);
+    const int words = gather_wave_words(T, DO12, p.rec67 != nullptr);
     float *wl = gather_lds(smem, p.tab345, p.n, DO345, words);
     const int waves = blockDim.x / kWave;
     const int item = blockIdx.x * waves + wave_id();
@@ -2429,8 +2433,8 @@ __global__ __launch_bounds__(512) void gather_kernel(GatherParams p)
     im.rec67 = p.rec67;
     im.out = p.groups > 1 ? p.part + (size_t)g * p.n * p.n : p.dimg;
     im.n = p.n;
-    gather_tile_wave<T, DO345>(im, p.trig, reinterpret_cast<const float2 *>(smem), p.n, a0,
-                               a1, tile / tn * T, tile % tn * T, wl, p.do12);
+    gather_tile_wave<T, DO345, DO12>(im, p.trig, reinterpret_cast<const float2 *>(smem), p.n,
+                                     a0, a1, tile / tn * T, tile % tn * T, wl);
 }
 
 // dimg = sum over angle groups of the partial images, in group order.
@@ -2446,13 +2450,13 @@ __global__ __launch_bounds__(kBlock) void gather_reduce_kernel(const float *part
     }
 }
 
-template <int T, bool DO345>
-__global__ __launch_bounds__(512) void gather_batched_kernel(BatchAdjointParams p)
+template <bool DO345, bool DO12>
+__global__ __launch_bounds__(1024) void gather_batched_kernel(BatchAdjointParams p)
 {
+    constexpr int T = kGatherTile;
     extern __shared__ float4 smem4[];
     float *smem = reinterpret_cast<float *>(smem4);
-    const bool do12 = p.slot.s[1] >= 0 || p.slot.s[2] >= 0;
-    const int words = gather_wave_words(T, do12, false);
+    const int words = gather_wave_words(T, DO12, false);
     float *wl = gather_lds(smem, p.tab345, p.tabn, DO345, words);
     const int waves = blockDim.x / kWave;
     const int item = blockIdx.x * waves + wave_id();
@@ -2469,8 +2473,8 @@ __global__ __launch_bounds__(512) void gather_batched_kernel(BatchAdjointParams 
     im.rec67 = nullptr;
     im.out = bi.dimg;
     im.n = n;
-    gather_tile_wave<T, DO345>(im, p.trig, reinterpret_cast<const float2 *>(smem), p.tabn, 0,
-                               p.A, tile / tn * T, tile % tn * T, wl, do12);
+    gather_tile_wave<T, DO345, DO12>(im, p.trig, reinterpret_cast<const float2 *>(smem),
+                                     p.tabn, 0, p.A, tile / tn * T, tile % tn * T, wl);
 }
 
 // dS = J_P^T dC for every sinogram column (P1: sign differences; P2: one-hot
@@ -3216,7 +3220,7 @@ size_t gather_lds_bytes(int tile, int waves, int tabn, bool do345, bool do12, bo
 
 hipError_t launch_gather(const GatherParams &p, hipStream_t s)
 {
-    if ((p.tile != 16 && p.tile != 32) || p.waves < 1 || p.waves > 8 || p.groups < 1 ||
+    if (p.tile != kGatherTile || p.waves < 1 || p.waves > 16 || p.groups < 1 ||
         (p.groups > 1 && !p.part))
         return hipErrorInvalidValue;
     const int tn = (p.n + p.tile - 1) / p.tile;
@@ -3225,16 +3229,16 @@ hipError_t launch_gather(const GatherParams &p, hipStream_t s)
     const size_t shm = gather_lds_bytes(p.tile, p.waves, p.n, p.do345, p.do12,
                                         p.rec67 != nullptr);
     const dim3 block(kWave * p.waves);
-    if (p.tile == 16) {
-        if (p.do345)
-            gather_kernel<16, true><<<blocks, block, shm, s>>>(p);
+    if (p.do345) {
+        if (p.do12)
+            gather_kernel<true, true><<<blocks, block, shm, s>>>(p);
         else
-            gather_kernel<16, false><<<blocks, block, shm, s>>>(p);
+            gather_kernel<true, false><<<blocks, block, shm, s>>>(p);
     } else {
-        if (p.do345)
-            gather_kernel<32, true><<<blocks, block, shm, s>>>(p);
+        if (p.do12)
+            gather_kernel<false, true><<<blocks, block, shm, s>>>(p);
         else
-            gather_kernel<32, false><<<blocks, block, shm, s>>>(p);
+            gather_kernel<false, false><<<blocks, block, shm, s>>>(p);
     }
     if (p.groups > 1) {
         const size_t npix = (size_t)p.n * p.n;
@@ -3278,22 +3282,22 @@ hipError_t launch_records_batched(const BatchAdjointParams &p, hipStream_t s)
 hipError_t launch_gather_batched(const BatchAdjointParams &p, int tile, int waves,
                                  hipStream_t s)
 {
-    if ((tile != 16 && tile != 32) || waves < 1 || waves > 8)
+    if (tile != kGatherTile || waves < 1 || waves > 16)
         return hipErrorInvalidValue;
     const bool do12 = p.slot.s[1] >= 0 || p.slot.s[2] >= 0;
     const size_t shm = gather_lds_bytes(tile, waves, p.tabn, p.do345, do12, false);
     const int blocks = (p.tiles + waves - 1) / waves;
     const dim3 block(kWave * waves);
-    if (tile == 16) {
-        if (p.do345)
-            gather_batched_kernel<16, true><<<blocks, block, shm, s>>>(p);
+    if (p.do345) {
+        if (do12)
+            gather_batched_kernel<true, true><<<blocks, block, shm, s>>>(p);
         else
-            gather_batched_kernel<16, false><<<blocks, block, shm, s>>>(p);
+            gather_batched_kernel<true, false><<<blocks, block, shm, s>>>(p);
     } else {
-        if (p.do345)
-            gather_batched_kernel<32, true><<<blocks, block, shm, s>>>(p);
+        if (do12)
+            gather_batched_kernel<false, true><<<blocks, block, shm, s>>>(p);
         else
-            gather_batched_kernel<32, false><<<blocks, block, shm, s>>>(p);
+            gather_batched_kernel<false, false><<<blocks, block, shm, s>>>(p);
     }
     return hipGetLastError();
 }
