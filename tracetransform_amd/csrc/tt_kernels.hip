@@ -2282,10 +2282,9 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
             prefetch(fn, a + 1);
         }
         const float c = f.c, s = f.s;
-        const int c_lo = f.c_lo, K = f.K, W = f.W;
+        const int c_lo = f.c_lo, K = f.K;
         {
             const float pxc0 = (float)c_lo - o;
-            const int W3 = (W + 2) / 3;
             const int clo3 = c_lo % 3;
             // one residue class (column, row mod 3) at a time: samples of one
             // class are pairwise at least 3 apart, so their 2x2 tap footprints
