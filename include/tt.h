@@ -134,6 +134,18 @@ TT_API tt_status tt_circus_host(tt_context *ctx, const float *sino, int n,
                                 int n_cols, const tt_pfunc *pf, int np,
                                 float *circus);
 
+/* tt_circus without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  No reference
+ * counterpart.  P1, P2 and P3 only (Hermite needs allocated weight tables:
+ * TT_ERR_UNSUPPORTED).  P3's spectra go to a library-owned device array of
+ * 8 Mi floats (32 MiB) per GPU, so n_sino * n_cols * 2 * (n/2 + 1) may not
+ * exceed 8388608 (TT_ERR_INVALID).  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (different streams or contexts) race.  No
+ * per-kernel timing.  Results equal tt_circus's bit for bit. */
+TT_API tt_status tt_circus_noalloc(tt_context *ctx, const float *d_sino, int n,
+                                   int n_cols, int n_sino, const tt_pfunc *pf,
+                                   int np, float *d_circus, int32_t *d_p2_rank);
+
 /* ---- steps either side of the path -------------------------------------- */
 
 /* Replaces resize (c++/src/auxiliary.cpp:198-221), used by the orthonormal

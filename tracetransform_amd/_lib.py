@@ -29,6 +29,7 @@ EXPORTED = (
     "tt_circus_backward", "tt_sinograms_backward", "tt_backward",
     "tt_resize", "tt_zscore", "tt_nearest_orthonormal_sinogram",
     "tt_nearest_orthonormal_sinograms", "tt_backward_batched", "tt_saved_bytes",
+    "tt_circus_noalloc",
 )
 
 
@@ -96,6 +97,8 @@ def load():
                                 C.POINTER(tt_pfunc), C.c_int, vp, vp]
         L.tt_circus_host.argtypes = [vp, vp, C.c_int, C.c_int,
                                      C.POINTER(tt_pfunc), C.c_int, vp]
+        L.tt_circus_noalloc.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int,
+                                        C.POINTER(tt_pfunc), C.c_int, vp, vp]
         L.tt_forward_batched.argtypes = [vp, C.POINTER(tt_image), C.c_int, C.c_int,
                                          f32, C.POINTER(i32), C.c_int,
                                          C.POINTER(tt_pfunc), C.c_int]

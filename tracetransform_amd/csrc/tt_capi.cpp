@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -477,6 +478,51 @@ tt_status check_pfuncs(const tt_pfunc *pf, int np, int n)
 
 extern "C" {
 
+// ---- scratch of the allocation-free entry point -----------------------------
+// tt_circus_noalloc keeps P3's half spectra in this file-scope device array
+// (each GPU the library runs on holds its own copy until the process ends).
+// Host code never uses the array's name as an address: each device's copy is
+// looked up once, with that device current, when the first context is
+// created on it, and kernels receive that pointer.
+constexpr size_t kNoAllocSpecFloats = (size_t)8 << 20;  // 32 MiB per GPU
+static __device__ __align__(256) float scratch[kNoAllocSpecFloats];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+bool g_scratch_done[kMaxDevices];
+
+// Call with `device` current.
+void resolve_noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[device])
+        return;
+    void *ptr = nullptr;
+    g_scratch_err[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = g_scratch_err[device] == hipSuccess ? (float *)ptr : nullptr;
+    g_scratch_done[device] = true;
+}
+
+tt_status noalloc_scratch(int device, float **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices || !g_scratch_done[device])
+        return fail(TT_ERR_DEVICE, "no scratch address for device %d", device);
+    if (!g_scratch_ptr[device])
+        return fail(TT_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s",
+                    device, hipGetErrorString(g_scratch_err[device]));
+    *out = g_scratch_ptr[device];
+    return TT_OK;
+}
+
+}  // namespace
+
 TT_API const char *tt_last_error(void) { return g_err.c_str(); }
 
 TT_API tt_status tt_context_create(int device, tt_context **out)
@@ -494,6 +540,8 @@ TT_API tt_status tt_context_create(int device, tt_context **out)
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        resolve_noalloc_scratch(device);  // tt_circus_noalloc reports a failure
     if (e != hipSuccess) {
         delete ctx;
         return fail(TT_ERR_DEVICE, "context setup failed: %s", hipGetErrorString(e));
@@ -2306,3 +2354,69 @@ TT_API tt_status tt_profile_read(tt_context *ctx, int kind, double *total_ms,
 }
 
 }  // extern "C"
+
+TT_API tt_status tt_circus_noalloc(tt_context *ctx, const float *d_sino, int n,
+                                   int n_cols, int n_sino, const tt_pfunc *pf, int np,
+                                   float *d_circus, int32_t *d_p2_rank)
+{
+    TT_CHECK(ctx, "ctx is NULL");
+    TT_CHECK(n >= 2 && n <= 2730, "sinogram height %d outside 2..2730", n);
+    TT_CHECK(n_cols >= 1 && n_sino >= 1, "n_cols and n_sino must be >= 1");
+    TT_CHECK(np >= 0 && np <= ttk::kMaxP && (np == 0 || pf), "np must be in 0..%d",
+             ttk::kMaxP);
+    for (int q = 0; q < np; q++) {
+        if (pf[q].code == TT_P_HERMITE)
+            return fail(TT_ERR_UNSUPPORTED,
+                        "tt_circus_noalloc computes P1..P3 (Hermite needs its weight "
+                        "tables allocated: use tt_circus)");
+        TT_CHECK(pf[q].code >= 1 && pf[q].code <= 3,
+                 "invalid P-functional code %d at position %d", pf[q].code, q);
+    }
+    if (np == 0)
+        return TT_OK;
+    TT_CHECK(d_sino && d_circus, "NULL sinogram or circus pointer");
+    const int h = n / 2 + 1;
+    const size_t cols = (size_t)n_sino * n_cols;
+    bool p3 = false;
+    for (int q = 0; q < np; q++)
+        p3 = p3 || pf[q].code == TT_P3;
+    TT_CHECK(!p3 || cols * 2 * h <= kNoAllocSpecFloats,
+             "P3 spectra of %zu columns of height %d need %zu floats, above the %zu of "
+             "the allocation-free scratch",
+             cols, n, cols * 2 * h, kNoAllocSpecFloats);
+    TT_TRY(ctx->set_device());
+    float *spec = nullptr;
+    if (p3)
+        TT_TRY(noalloc_scratch(ctx->device, &spec));
+
+    ttk::CircusParams p = {};
+    p.sino = d_sino;
+    p.out = d_circus;
+    p.p2rank = d_p2_rank;
+    p.n = n;
+    p.ncols = n_cols;
+    p.nsino = n_sino;
+    p.np = np;
+    for (int q = 0; q < np; q++) {
+        p.code[q] = pf[q].code;
+        p.herm[q] = -1;
+    }
+    hipStream_t st = ctx->stream;
+    if (p3) {
+        // every spectrum entry is written before it is read: no clearing
+        p.p3_spec = spec;
+        p.p3_h = h;
+        ttk::DftParams dp = {};
+        dp.njobs = 1;
+        dp.job.in = d_sino;
+        dp.job.out = spec;
+        dp.job.n = n;
+        dp.job.cols = (int)cols;
+        dp.job.ld_in = n;
+        dp.job.ld_out = 2 * h;
+        dp.tiles = ttk::dft_job_tiles(n, (int)cols, true);
+        TT_HIP(ttk::launch_dft_gemm(dp, true, n, st));
+    }
+    TT_HIP(ttk::launch_circus(p, st));
+    return TT_OK;
+}
