@@ -205,6 +205,14 @@ typedef struct tt_image {
 /* Bytes of one image's d_saved block (0 for invalid sizes).  No reference
  * counterpart: the reference has no adjoint. */
 TT_API size_t tt_saved_bytes(int n, int n_angles, int nt);
+/* Byte offsets of the parts of a d_saved block: the weighted medians
+ * findWeightedMedian(x) (med_x, [A][n] int32, written when T1/T2 are
+ * requested) and findWeightedMedian(sqrt(x)) (med_s, with T3..T5), the
+ * complex T3..T5 integrals ([3][A][n] float pairs), P3's half spectra
+ * ([nt][A][2 (n/2 + 1)] float) and P2's source rows ([nt][A] int32). */
+TT_API tt_status tt_saved_offsets(int n, int n_angles, int nt, size_t *med_x,
+                                  size_t *med_s, size_t *z345, size_t *spec,
+                                  size_t *p2src);
 
 /* Sinograms (T0..T5) and circus functions (P1..P3) of every image of a batch
  * in one persistent launch per stage, the whole pass captured into a HIP graph

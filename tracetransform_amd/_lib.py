@@ -29,7 +29,7 @@ EXPORTED = (
     "tt_circus_backward", "tt_sinograms_backward", "tt_backward",
     "tt_resize", "tt_zscore", "tt_nearest_orthonormal_sinogram",
     "tt_nearest_orthonormal_sinograms", "tt_backward_batched", "tt_saved_bytes",
-    "tt_circus_noalloc",
+    "tt_circus_noalloc", "tt_saved_offsets",
 )
 
 
@@ -119,6 +119,8 @@ def load():
                                           C.POINTER(tt_pfunc), C.c_int]
         L.tt_saved_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
         L.tt_saved_bytes.restype = C.c_size_t
+        szp = C.POINTER(C.c_size_t)
+        L.tt_saved_offsets.argtypes = [C.c_int, C.c_int, C.c_int, szp, szp, szp, szp, szp]
         L.tt_profile_enable.argtypes = [vp, C.c_int]
         L.tt_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_int64)]

@@ -255,6 +255,26 @@ class Batch:
                                         len(self.codes), self._pf, len(self.pf)))
         return self.sino, self.circus
 
+    def saved_medians(self, k: int):
+        """Image k's weighted medians (findWeightedMedian of x and of
+        sqrt(x), [A, n] int32 tensors, None where the functionals did not
+        need them) from its saved line data, as of the last run."""
+        if self.saved is None:
+            raise ValueError("the Batch was not built with save_for_backward")
+        n, A, nt = self.images[k].shape[0], self.n_angles, len(self.codes)
+        offs = [C.c_size_t() for _ in range(5)]
+        _lib.check("tt_saved_offsets",
+                   _lib.load().tt_saved_offsets(n, A, nt, *[C.byref(o) for o in offs]))
+        raw = self.saved[k]
+
+        def part(off):
+            return raw[off:off + 4 * A * n].view(torch.int32).view(A, n)
+
+        need_x = any(c in (1, 2) for c in self.codes)
+        need_s = any(c in (3, 4, 5) for c in self.codes)
+        return (part(offs[0].value) if need_x else None,
+                part(offs[1].value) if need_s else None)
+
     def launch_shape(self):
         ctx = get_context(self.device)
         w, b = C.c_int(), C.c_int()

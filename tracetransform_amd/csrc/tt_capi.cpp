@@ -658,6 +658,21 @@ TT_API size_t tt_saved_bytes(int n, int n_angles, int nt)
     return saved_layout(n, n_angles, nt).total;
 }
 
+TT_API tt_status tt_saved_offsets(int n, int n_angles, int nt, size_t *med_x,
+                                  size_t *med_s, size_t *z345, size_t *spec,
+                                  size_t *p2src)
+{
+    TT_CHECK(n >= 3 && n_angles >= 1 && nt >= 1, "invalid sizes");
+    TT_CHECK(med_x && med_s && z345 && spec && p2src, "NULL argument");
+    const SavedLayout L = saved_layout(n, n_angles, nt);
+    *med_x = L.med_x;
+    *med_s = L.med_s;
+    *z345 = L.z345;
+    *spec = L.spec;
+    *p2src = L.p2src;
+    return TT_OK;
+}
+
 TT_API int tt_angle_count(float angle_step_deg)
 {
     if (!(angle_step_deg > 0.0f))
