@@ -15,7 +15,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "tt.h")).read()
+    """Entry points of include/tt.h and of the test hooks in
+    include/tt_testing.h."""
+    text = "".join(open(os.path.join(ROOT, "include", h)).read()
+                   for h in ("tt.h", "tt_testing.h"))
     return sorted(set(re.findall(r"TT_API\s+[\w\s\*]+?\b(tt_\w+)\s*\(", text)))
 
 

@@ -30,6 +30,8 @@ EXPORTED = (
     "tt_resize", "tt_zscore", "tt_nearest_orthonormal_sinogram",
     "tt_nearest_orthonormal_sinograms", "tt_backward_batched", "tt_saved_bytes",
     "tt_circus_noalloc", "tt_saved_offsets",
+    # test hooks (include/tt_testing.h)
+    "tt_testing_fail_allocations", "tt_testing_short_sqrt",
 )
 
 
@@ -121,6 +123,9 @@ def load():
         L.tt_saved_bytes.restype = C.c_size_t
         szp = C.POINTER(C.c_size_t)
         L.tt_saved_offsets.argtypes = [C.c_int, C.c_int, C.c_int, szp, szp, szp, szp, szp]
+        L.tt_testing_fail_allocations.argtypes = [C.c_longlong]
+        L.tt_testing_short_sqrt.argtypes = [vp, C.c_uint32, C.c_uint64,
+                                            C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]
         L.tt_profile_enable.argtypes = [vp, C.c_int]
         L.tt_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_int64)]

@@ -7,6 +7,7 @@
 // and Hermite weights at the reference's float sample positions.  All GPU
 // work is in tt_kernels.hip.
 #include "../../include/tt.h"
+#include "../../include/tt_testing.h"
 #include "tt_internal.hpp"
 
 #include <rocblas/rocblas.h>
@@ -18,6 +19,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -55,6 +57,22 @@ tt_status fail(tt_status st, const char *fmt, ...)
             return fail(TT_ERR_INVALID, __VA_ARGS__);                          \
     } while (0)
 
+// Allocation-failure hook (tt_testing_fail_allocations): the number of
+// workspace allocations still allowed to succeed, or -1 (no limit).
+std::atomic<long long> g_alloc_budget{-1};
+
+hipError_t workspace_malloc(void **ptr, size_t bytes)
+{
+    long long left = g_alloc_budget.load();
+    while (left >= 0) {
+        if (left == 0)
+            return hipErrorOutOfMemory;
+        if (g_alloc_budget.compare_exchange_weak(left, left - 1))
+            break;
+    }
+    return hipMalloc(ptr, bytes);
+}
+
 struct DevBuf {
     void *ptr = nullptr;
     size_t cap = 0;
@@ -69,7 +87,7 @@ struct DevBuf {
             ptr = nullptr;
             cap = 0;
         }
-        hipError_t e = hipMalloc(&ptr, bytes);
+        hipError_t e = workspace_malloc(&ptr, bytes);
         if (e == hipSuccess)
             cap = bytes;
         return e;
@@ -2433,5 +2451,45 @@ TT_API tt_status tt_circus_noalloc(tt_context *ctx, const float *d_sino, int n,
         TT_HIP(ttk::launch_dft_gemm(dp, true, n, st));
     }
     TT_HIP(ttk::launch_circus(p, st));
+    return TT_OK;
+}
+
+// ---- test hooks (include/tt_testing.h) --------------------------------------
+
+TT_API tt_status tt_testing_fail_allocations(long long allowed)
+{
+    g_alloc_budget.store(allowed < 0 ? -1 : allowed);
+    return TT_OK;
+}
+
+TT_API tt_status tt_testing_short_sqrt(tt_context *ctx, uint32_t lo, uint64_t count,
+                                       uint64_t *mismatches, uint32_t *first_bad)
+{
+    TT_CHECK(ctx && mismatches && first_bad, "NULL argument");
+    TT_CHECK(count <= ((uint64_t)1 << 32) - lo, "range past the last bit pattern");
+    TT_TRY(ctx->set_device());
+    unsigned long long *d_m = nullptr;
+    unsigned int *d_f = nullptr;
+    TT_HIP(hipMalloc(&d_m, sizeof(unsigned long long) + sizeof(unsigned int)));
+    d_f = reinterpret_cast<unsigned int *>(d_m + 1);
+    const unsigned long long zero = 0;
+    const unsigned int none = 0xFFFFFFFFu;
+    hipError_t e = hipMemcpy(d_m, &zero, sizeof zero, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(d_f, &none, sizeof none, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = ttk::launch_short_sqrt_check(lo, count, d_m, d_f, ctx->stream);
+    unsigned long long m = 0;
+    unsigned int f = 0;
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpy(&m, d_m, sizeof m, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpy(&f, d_f, sizeof f, hipMemcpyDeviceToHost);
+    (void)hipFree(d_m);
+    TT_HIP(e);
+    *mismatches = m;
+    *first_bad = f;
     return TT_OK;
 }

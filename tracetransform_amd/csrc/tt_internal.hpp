@@ -246,6 +246,12 @@ size_t dft_lds_bytes(int nmax);
 hipError_t launch_dft_gemm(const DftParams &p, bool forward, int nmax, hipStream_t s);
 hipError_t launch_p3_vjp_scale(const P3ScaleParams &p, int max_cols, hipStream_t s);
 
+// Test hook: mismatches of the line kernel's short square root against
+// sqrtf over float bit patterns [lo, lo + count) (mism and first in device
+// memory, first initialised to 0xFFFFFFFF by the caller).
+hipError_t launch_short_sqrt_check(uint32_t lo, uint64_t count, unsigned long long *mism,
+                                   unsigned int *first, hipStream_t s);
+
 // Steps either side of the path and the orthonormal step.
 hipError_t launch_resize(const float *in, int in_rows, int in_cols, float *out,
                          int rows, int cols, hipStream_t s);

@@ -117,6 +117,37 @@ __device__ __forceinline__ float line_sqrt(float x)
     }
 }
 
+// Test hook: line_sqrt<true> against the correctly rounded sqrtf on the
+// float bit patterns [lo, lo + count), outside its documented exceptions
+// ((0, 2^-96) and the negative denormals).  Counts the patterns whose
+// results differ (NaN results compare equal) and keeps the smallest one.
+__global__ __launch_bounds__(kBlock) void short_sqrt_check_kernel(uint32_t lo, uint64_t count,
+                                                                 unsigned long long *mism,
+                                                                 unsigned int *first)
+{
+    unsigned long long bad = 0;
+    unsigned int firstbad = 0xFFFFFFFFu;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
+        const uint32_t b = lo + (uint32_t)i;
+        const bool tiny = b > 0u && b < 0x0F800000u;           // (0, 2^-96)
+        const bool negden = b > 0x80000000u && b < 0x80800000u;  // negative denormals
+        if (tiny || negden)
+            continue;
+        const float x = __uint_as_float(b);
+        const float a = line_sqrt<true>(x), r = line_sqrt<false>(x);
+        const bool same = __float_as_uint(a) == __float_as_uint(r) || (a != a && r != r);
+        if (!same) {
+            bad++;
+            firstbad = b < firstbad ? b : firstbad;
+        }
+    }
+    if (bad) {
+        atomicAdd(mism, bad);
+        atomicMin(first, firstbad);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // T0..T5 line kernel.  Lane = one column of the rotated image of one angle.
 // Pass 1 streams the column and forms Eigen's packet-order sum (8 stride
@@ -3309,6 +3340,13 @@ static int grid_for(size_t total)
 {
     size_t b = (total + kBlock - 1) / kBlock;
     return (int)(b > 4096 ? 4096 : (b < 1 ? 1 : b));
+}
+
+hipError_t launch_short_sqrt_check(uint32_t lo, uint64_t count, unsigned long long *mism,
+                                   unsigned int *first, hipStream_t s)
+{
+    short_sqrt_check_kernel<<<8192, kBlock, 0, s>>>(lo, count, mism, first);
+    return hipGetLastError();
 }
 
 hipError_t launch_resize(const float *in, int in_rows, int in_cols, float *out,
