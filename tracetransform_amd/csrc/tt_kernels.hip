@@ -2323,8 +2323,9 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
             // four targets.  Lane l takes the class's column l % 16 (at most
             // 16: K <= 47) and its rows 4 apart in the class from l / 16 on.
             const int kl = lane & 15, jl = lane >> 4;
-            for (int cls = 0; cls < 9; cls++) {
-                const int cc = cls % 3, rc = cls / 3;
+            // one column class (col mod 3) at a time: the column's data
+            // serves its three row classes
+            for (int cc = 0; cc < 3; cc++) {
                 const int koff = cc >= clo3 ? cc - clo3 : cc - clo3 + 3;
                 const int K3 = (K - koff + 2) / 3;
                 const bool colok = kl < K3;
@@ -2335,7 +2336,6 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                 const int rlo = __float_as_int(ea.z);
                 const int rhi = colok ? __float_as_int(ea.w) : -1;
                 const int r3 = rlo - 3 * (int)(((uint32_t)rlo * 0xAAABu) >> 17);
-                const int roff = rc >= r3 ? rc - r3 : rc - r3 + 3;
                 float4 eb = {}, ed = {}, ee = {};
                 float2 ec = {};
                 if (DO345) {
@@ -2349,7 +2349,15 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                 // the forward's sample position (rot_sample), bit for bit
                 const float px = pxc0 + (float)k;
                 const float pxc = px * c, pxns = px * (-s);
-                for (int row = rlo + roff + 3 * jl; row <= rhi; row += 12) {
+                // One sample: its accumulator cell and the four weighted
+                // gradient values of its taps (use: its taps touch the tile).
+                struct Hit {
+                    int cell;
+                    float w00, w01, w10, w11;
+                    bool use;
+                };
+                auto sample = [&](int row) {
+                    Hit h;
                     const float py = (float)row - o;
                     float x = pxc + py * s;
                     float y = pxns + py * c;
@@ -2362,9 +2370,7 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                     const float fx = __builtin_amdgcn_fractf(x);
                     const float fy = __builtin_amdgcn_fractf(y);
                     const int xi = Xb - X0 + 1, yi = Yb - Y0 + 1;  // accumulator cell
-                    const bool use = (unsigned)xi <= (unsigned)T && (unsigned)yi <= (unsigned)T;
-                    if (!use)
-                        continue;
+                    h.use = (unsigned)xi <= (unsigned)T && (unsigned)yi <= (unsigned)T;
                     // G(row): g0 + [row >= m] (g1 d + g2 d^2)
                     //   + [row > sm] sum_t (alpha_t W_t.re + beta_t W_t.im)[row - sm]
                     //   + [row == r6] c6 + [row == r7] c7
@@ -2372,8 +2378,8 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                     if (do12) {
                         const int m = __float_as_int(ed.z);
                         const float d = (float)(row - m);
-                        const float h = fmaf(d, fmaf(ed.y, d, ed.x), G);
-                        G = row >= m ? h : G;
+                        const float hh = fmaf(d, fmaf(ed.y, d, ed.x), G);
+                        G = row >= m ? hh : G;
                     }
                     if (DO345) {
                         // table entry 0 is zero, so rows at or before sm add nothing
@@ -2394,13 +2400,30 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                     }
                     const float ofx = 1.0f - fx, ofy = 1.0f - fy;
                     const float ga = G * ofx, gb = G * fx;
-                    float *cell = acc + __mul24(xi, P) + yi;
-                    const float v00 = cell[0], v01 = cell[1];
-                    const float v10 = cell[P], v11 = cell[P + 1];
-                    cell[0] = v00 + ga * ofy;
-                    cell[1] = v01 + ga * fy;
-                    cell[P] = v10 + gb * ofy;
-                    cell[P + 1] = v11 + gb * fy;
+                    h.cell = h.use ? __mul24(xi, P) + yi : 0;
+                    h.w00 = ga * ofy;
+                    h.w01 = ga * fy;
+                    h.w10 = gb * ofy;
+                    h.w11 = gb * fy;
+                    return h;
+                };
+                for (int rc = 0; rc < 3; rc++) {
+                    // one residue class (column, row mod 3): its samples are
+                    // pairwise at least 3 apart, so their 2x2 tap footprints
+                    // are disjoint and every read-add-write of the class has
+                    // its own four targets
+                    const int roff = rc >= r3 ? rc - r3 : rc - r3 + 3;
+                    for (int row = rlo + roff + 3 * jl; row <= rhi; row += 12) {
+                        const Hit a = sample(row);
+                        if (a.use) {
+                            float *ca = acc + a.cell;
+                            const float a00 = ca[0], a01 = ca[1], a10 = ca[P], a11 = ca[P + 1];
+                            ca[0] = a00 + a.w00;
+                            ca[1] = a01 + a.w01;
+                            ca[P] = a10 + a.w10;
+                            ca[P + 1] = a11 + a.w11;
+                        }
+                    }
                 }
             }
         }
