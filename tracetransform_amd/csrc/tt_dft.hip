@@ -21,15 +21,26 @@ namespace {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kDftBM = 64;  // output rows per workgroup
-constexpr int kDftBN = 64;  // output columns per workgroup
-constexpr int kDftBK = 16;  // reduction chunk
+constexpr int kDftBM = 64;   // output rows per workgroup
+constexpr int kDftBN = 128;  // output columns per workgroup
+constexpr int kDftBK = 16;   // reduction chunk
 
-// W(row, kk) of the GEMM: the twiddle of phase (u v mod n) / n, its cosine
-// or minus its sine.  FWD: out = W^T in, so the output row m is a frequency
-// (m < h: real part, else imaginary part of frequency m - h) and kk a
-// position; otherwise out = W in, the output row is a position and kk runs
-// over the 2h spectrum entries.
+// Twiddle index (u v) mod n of W(row, kk) for the first of four consecutive
+// kk and the step between them: W(row, kk) is the cosine or minus the sine
+// of 2 pi t / n.  FWD: out = W^T in, the output row m is a frequency (m < h:
+// real part, else imaginary part of frequency m - h) and kk a position;
+// otherwise out = W in, the output row is a position and kk runs over the
+// 2h spectrum entries (consecutive kk share u, and v steps by one except
+// across kk = h, handled by the caller).
+__device__ __forceinline__ int phase_mod(int u, int v, int n, float inv_n)
+{
+    // u v < 2730^2 < 2^23: exact in float; the quotient is off by at most one
+    const int uv = (int)__umul24((uint32_t)u, (uint32_t)v);
+    int t = uv - (int)__umul24((uint32_t)(int)((float)uv * inv_n), (uint32_t)n);
+    t = t < 0 ? t + n : t;
+    return t >= n ? t - n : t;
+}
+
 template <bool FWD>
 __device__ __forceinline__ float dft_weight(const float2 *tw, int n, int h, float inv_n,
                                             int row, int kk)
@@ -45,12 +56,7 @@ __device__ __forceinline__ float dft_weight(const float2 *tw, int n, int h, floa
         u = row;
         v = sn ? kk - h : kk;
     }
-    // u v < 2730^2 < 2^23: exact in float; the quotient is off by at most one
-    const int uv = (int)__umul24((uint32_t)u, (uint32_t)v);
-    int t = uv - (int)__umul24((uint32_t)(int)((float)uv * inv_n), (uint32_t)n);
-    t = t < 0 ? t + n : t;
-    t = t >= n ? t - n : t;
-    const float2 w = tw[t];
+    const float2 w = tw[phase_mod(u, v, n, inv_n)];
     return sn ? -w.y : w.x;
 }
 
@@ -67,15 +73,17 @@ __device__ __forceinline__ int find_job(const DftJob *jobs, int njobs, int tile)
     return lo;
 }
 
-// One 64 x 64 output tile per workgroup of four waves (2 x 2 wave tiles of
-// 32 x 32, one MFMA accumulator each), reduction in chunks of 16: the W
-// chunk is generated into LDS from the twiddles, the input chunk loaded.
+// One 64 x 128 output tile per workgroup of four waves (2 x 2 wave tiles of
+// 32 x 64, two MFMA accumulators each), reduction in chunks of 16, double
+// buffered: while the matrix cores work on one chunk the waves generate the
+// next W chunk from the LDS twiddles (four consecutive entries per thread,
+// the phase stepped by u) and load the next input chunk into registers.
 // Twiddles (cos, sin)(2 pi t / n) are rounded from double sincospi.
 template <bool FWD>
 __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
 {
-    __shared__ float sA[kDftBK][kDftBM];
-    __shared__ float sB[kDftBK][kDftBN];
+    __shared__ float sA[2][kDftBK][kDftBM];
+    __shared__ float sB[2][kDftBK][kDftBN];
     extern __shared__ float2 s_tw[];
     const DftJob &jb = p.jobs ? p.jobs[find_job(p.jobs, p.njobs, blockIdx.x)] : p.job;
     const int n = jb.n, h = n / 2 + 1;
@@ -91,42 +99,100 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
     }
     const float inv_n = 1.0f / (float)n;
     const int lane = tid & (kWave - 1), wave = tid >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    // loaders: row/column r of the tile, 4 consecutive reduction indices
-    const int lr = tid & 63, lk = (tid >> 6) * 4;
-    const int col = c0 + lr;
-    const float *src = jb.in + (size_t)(col < jb.cols ? col : 0) * jb.ld_in;
-    f32x16 acc = {};
-    __syncthreads();
-    for (int k0 = 0; k0 < Kd; k0 += kDftBK) {
-        const int row = m0 + lr;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 64;
+    // W loader: row ar of the tile, reduction entries ak .. ak + 3
+    const int ar = tid & 63, ak = (tid >> 6) * 4;
+    const int arow = m0 + ar;
+    // input loader: columns bc and bc + 64 of the tile, entries bk .. bk + 3
+    const int bc = tid & 63, bk = (tid >> 6) * 4;
+    const int col0 = c0 + bc, col1 = c0 + 64 + bc;
+    const float *src0 = jb.in + (size_t)(col0 < jb.cols ? col0 : 0) * jb.ld_in;
+    const float *src1 = jb.in + (size_t)(col1 < jb.cols ? col1 : 0) * jb.ld_in;
+    float ra[4], rb0[4], rb1[4];
+    auto fetch = [&](int k0) {
+        const int kk0 = k0 + ak;
+        if (arow < M && kk0 + 3 < Kd && (FWD || kk0 + 3 < h || kk0 >= h)) {
+            // four consecutive kk of one half: one modular product, then steps
+            int u, v;
+            bool sn;
+            if (FWD) {
+                sn = arow >= h;
+                u = sn ? arow - h : arow;
+                v = kk0;
+            } else {
+                sn = kk0 >= h;
+                u = arow;
+                v = sn ? kk0 - h : kk0;
+            }
+            int t = phase_mod(u, v, n, inv_n);
+            const int du = u >= n ? u - n : u;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const float2 w = s_tw[t];
+                ra[e] = sn ? -w.y : w.x;
+                t += du;
+                t = t >= n ? t - n : t;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int kk = kk0 + e;
+                ra[e] = arow < M && kk < Kd ? dft_weight<FWD>(s_tw, n, h, inv_n, arow, kk)
+                                            : 0.0f;
+            }
+        }
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            const int kk = k0 + lk + e;
-            sA[lk + e][lr] =
-                row < M && kk < Kd ? dft_weight<FWD>(s_tw, n, h, inv_n, row, kk) : 0.0f;
-            sB[lk + e][lr] = col < jb.cols && kk < Kd ? src[kk] : 0.0f;
+            const int kk = k0 + bk + e;
+            rb0[e] = col0 < jb.cols && kk < Kd ? src0[kk] : 0.0f;
+            rb1[e] = col1 < jb.cols && kk < Kd ? src1[kk] : 0.0f;
         }
-        __syncthreads();
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            sA[buf][ak + e][ar] = ra[e];
+            sB[buf][bk + e][bc] = rb0[e];
+            sB[buf][bk + e][64 + bc] = rb1[e];
+        }
+    };
+    f32x16 acc0 = {}, acc1 = {};
+    __syncthreads();  // twiddles
+    fetch(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < Kd; k0 += kDftBK) {
+        const bool more = k0 + kDftBK < Kd;
+        if (more)
+            fetch(k0 + kDftBK);
 #pragma unroll
         for (int s = 0; s < kDftBK / 2; s++) {
             const int kr = 2 * s + (lane >> 5);
-            const float a = sA[kr][wm + (lane & 31)];
-            const float b = sB[kr][wn + (lane & 31)];
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+            const float a = sA[buf][kr][wm + (lane & 31)];
+            const float b0 = sB[buf][kr][wn + (lane & 31)];
+            const float b1 = sB[buf][kr][wn + 32 + (lane & 31)];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1, acc1, 0, 0, 0);
         }
+        if (more)
+            store(buf ^ 1);
         __syncthreads();
+        buf ^= 1;
     }
-    // accumulator register r of this lane: column wn + lane % 32, row
+    // accumulator register r of this lane: column wn (+ 32) + lane % 32, row
     // wm + 8 (r / 4) + 4 (lane / 32) + r % 4
-    const int oc = c0 + wn + (lane & 31);
-    if (oc < jb.cols) {
-        float *dst = jb.out + (size_t)oc * jb.ld_out;
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int om = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (om < M)
-                dst[om] = acc[r];
+    for (int half = 0; half < 2; half++) {
+        const int oc = c0 + wn + 32 * half + (lane & 31);
+        if (oc < jb.cols) {
+            float *dst = jb.out + (size_t)oc * jb.ld_out;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int om = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (om < M)
+                    dst[om] = half ? acc1[r] : acc0[r];
+            }
         }
     }
 }
