@@ -441,8 +441,8 @@ def run_batched(args, w):
         "sides": f"{int(min(sides))}..{int(max(sides))}",
         "padded_n": f"{min(ns)}..{max(ns)}", "angles": A, "angle_step_deg": w["step"],
         "tfunctionals": w["tcodes"], "pfunctionals": [p[0] for p in w["pfuncs"]],
-        "lines_per_step": lines, "launch_shape": list(shape),
-        "autotune": bool(args.autotune),
+        "lines_per_step": lines, "launch_shape": list(shape[:2]),
+        "tiles": core.batched_tiles(0), "autotune": bool(args.autotune),
         "step": "one batched forward (graph replay)"})
     out["roofline"] = roofline_of(kern, args.steps, models)
     if out["roofline"]:
@@ -631,7 +631,8 @@ def run_batched_adjoint(args, w):
         "lines_per_step": lines,
         "step": "batched forward (keeps the line data) + loss gradient + batched "
                 "backward (reads it: no forward recompute), graph replays",
-        "launch_shape": list(fwd.launch_shape())})
+        "launch_shape": list(fwd.launch_shape()[:2]), "tiles": core.batched_tiles(0),
+        "autotune": bool(args.autotune)})
     out["roofline"] = roofline_of(kern, args.steps, models)
     if out["roofline"]:
         out["roofline"]["traffic"] = load_traffic(args.workload, out["roofline"]["kind"])

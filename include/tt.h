@@ -226,11 +226,40 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
                                     int nimg, int n_angles, float angle_step_deg,
                                     const int32_t *tcodes, int nt,
                                     const tt_pfunc *pf, int np);
+/* Autotuning of the batched paths: on a batch's first call the forward
+ * times its line kernel with every candidate line tile and then every launch
+ * shape, the backward its transposed rotation with every candidate gather
+ * tile, and each keeps the fastest. */
 TT_API tt_status tt_context_set_autotune(tt_context *ctx, int enable);
 /* Launch shape chosen for the current batch: waves per workgroup and
  * persistent workgroups of the line kernel. */
 TT_API tt_status tt_batched_launch_shape(tt_context *ctx, int *waves_per_block,
                                          int *blocks);
+
+/* Work tiles of the batched paths; they tile the reference's loops over
+ * angles and columns (c++/src/sinogram.cpp:110-150) and over the rotated
+ * image's pixels (c++/src/auxiliary.cpp:223-255, transposed).
+ *   line_angles x line_columns: one wave item of the batched T0..T5 line
+ *     kernel: 64, 32 or 16 consecutive columns of line_angles consecutive
+ *     angles, a multiple of 64 / line_columns up to 16 (candidates 1x64,
+ *     2x64, 4x64, 2x32, 4x32, 4x16, 8x16);
+ *   gather_tile x gather_tile pixels (32 or 16), gather_waves waves per
+ *     workgroup (1..16), of the batched backward's transposed rotation.
+ * A line's results and a pixel's gradient are the same bits whatever the
+ * tiles.  Zero fields mean the default (1x64, 32 pixels, the most waves that
+ * fit the LDS) or, with autotuning, the tuned choice. */
+typedef struct tt_tiles {
+    int32_t line_angles;
+    int32_t line_columns;
+    int32_t gather_tile;
+    int32_t gather_waves;
+} tt_tiles;
+/* Forces tiles for the batched calls that follow (NULL: none forced).
+ * Invalid tiles: TT_ERR_INVALID. */
+TT_API tt_status tt_context_set_tiles(tt_context *ctx, const tt_tiles *tiles);
+/* Tiles of the last planned batched forward (line fields) and batched
+ * backward (gather fields, zero before any). */
+TT_API tt_status tt_batched_tiles(tt_context *ctx, tt_tiles *tiles);
 
 /* ---- adjoint (no reference counterpart) --------------------------------- */
 

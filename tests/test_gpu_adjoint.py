@@ -9,7 +9,8 @@ Bar: normwise relative error <= 1e-5 against the oracle's FP64 analytic
 vector-Jacobian products (reference geometry, same fixed indices), whose own
 correctness is pinned on the CPU against FP64 central differences
 (tests/test_oracle.py); and the dot-product identity <J x, y> = <x, J^T y>
-for the parts linear in the image (T0, T1, T2 with fixed medians) to 1e-5.
+for the parts linear in the image (T0, T1, T2 with fixed medians) to 2e-4,
+J x from a finite difference of float sinograms.
 """
 import numpy as np
 import pytest
@@ -61,9 +62,12 @@ def test_linear_parts_dot_product_identity(oracle, gpu):
     y = dev(rng.standard_normal((len(codes), A, n)))
     x = dev(rng.standard_normal((n, n)) * (img.reshape(n, n) != 0))
     # J x through the forward (medians fixed: evaluate at img + eps * x and
-    # subtract, both linear in the image for T0-T2 with the same medians)
+    # subtract, both linear in the image for T0-T2 with the same medians).
+    # The float rounding of T2 (up to ~3e4 here, an ulp ~2e-3) divided by
+    # eps is the noise of the difference: eps = 2e-2 keeps it near 1e-5 of
+    # the identity's terms; lines whose median moves are masked out.
     s_img, aux0 = core.sinograms(d_img, A, step, codes, with_aux=True)
-    eps = 1e-3
+    eps = 2e-2
     s_pert, aux1 = core.sinograms(d_img + eps * x, A, step, codes, with_aux=True)
     same = (aux0["median_x"] == aux1["median_x"])
     Jx = ((s_pert.double() - s_img.double()) / eps)

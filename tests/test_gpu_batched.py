@@ -79,9 +79,10 @@ def test_batched_autotune_and_subsets(oracle, gpu):
     for codes, pf in (([0], []), ([2, 4], [(2,)]), ([5, 0], [(3,), (1,)])):
         b = core.Batch(imgs, 45, 8.0, codes, pf)
         b.run(autotune=True)
-        w, nb = b.launch_shape()
-        print(f"[batched] autotuned shape: {w} waves/block, {nb} blocks", flush=True)
-        assert w in (1, 2, 4) and nb > 0
+        w, nb, ta, tc = b.launch_shape()
+        print(f"[batched] autotuned shape: {w} waves/block, {nb} blocks, "
+              f"tile {ta} angles x {tc} columns", flush=True)
+        assert w in (1, 2, 4) and nb > 0 and tc in (16, 32, 64) and ta % (64 // tc) == 0
         check_against_eager(b, 45, 8.0, codes, pf)
     core.get_context(0)  # leave autotune off for later tests
     b = core.Batch(imgs[:2], 4, 90.0, [0], [])
@@ -172,3 +173,70 @@ def test_batched_backward_from_saved_forward(oracle, gpu):
         core.BatchBackward(imgs, A, step, codes, pf, forward=other).run()
     with pytest.raises(ValueError):
         core.BatchBackward(imgs, A, step, codes, pf, forward=core.Batch(imgs, A, step, codes, pf))
+
+
+LINE_TILES = [(1, 64), (2, 64), (4, 64), (2, 32), (4, 32), (4, 16), (8, 16)]
+GATHER_TILES = [(32, 16), (32, 4), (16, 16), (16, 2)]
+
+
+def test_every_tile_gives_the_same_bits(oracle, gpu):
+    # every candidate line tile (angles x columns of a wave item) and gather
+    # tile (pixel side, waves per workgroup) gives sinograms, circus
+    # functions, saved medians and gradients (from the saved line data and
+    # from the recompute) bit-identical to the default tiles; 90 angles at 2
+    # degrees put the 45, 90 and 135 degree turns inside tiles
+    sides = [200, 23, 97, 64, 131]
+    imgs = make_batch(oracle, sides, 800)
+    codes, pf, A, step = T05, P123, 90, 2.0
+    fwd = core.Batch(imgs, A, step, codes, pf, save_for_backward=True)
+    saved = core.BatchBackward(imgs, A, step, codes, pf, forward=fwd)
+    plain = core.BatchBackward(imgs, A, step, codes, pf)
+    g = torch.from_numpy(np.random.default_rng(11).standard_normal(
+        tuple(saved.dcircus_all.shape)).astype(np.float32)).cuda()
+
+    def run_all():
+        fwd.run(autotune=False)
+        saved.dcircus_all.copy_(g)
+        plain.dcircus_all.copy_(g)
+        a = [x.clone() for x in saved.run()]
+        b = [x.clone() for x in plain.run()]
+        torch.cuda.synchronize()
+        meds = [tuple(m.clone() for m in fwd.saved_medians(k)) for k in range(len(imgs))]
+        return ([x.clone() for x in fwd.sino], [x.clone() for x in fwd.circus], meds, a, b)
+
+    def same(ref, got, what):
+        for k in range(len(imgs)):
+            assert torch.equal(ref[0][k], got[0][k]), (what, "sinograms", k)
+            assert torch.equal(ref[1][k], got[1][k]), (what, "circus", k)
+            for m0, m1 in zip(ref[2][k], got[2][k]):
+                assert torch.equal(m0, m1), (what, "medians", k)
+            assert torch.equal(ref[3][k], got[3][k]), (what, "saved-data gradient", k)
+            assert torch.equal(ref[4][k], got[4][k]), (what, "recompute gradient", k)
+
+    try:
+        core.set_tiles(0)
+        ref = run_all()
+        t = core.batched_tiles(0)
+        assert (t["line_angles"], t["line_columns"], t["gather_tile"]) == (1, 64, 32), t
+        for k in range(len(imgs)):   # the recompute equals the saved-data backward
+            assert torch.equal(ref[3][k], ref[4][k]), k
+        for lt in LINE_TILES:
+            core.set_tiles(0, line=lt)
+            got = run_all()
+            t = core.batched_tiles(0)
+            print(f"[tiles] line {lt}: reported {t}", flush=True)
+            assert (t["line_angles"], t["line_columns"]) == lt, t
+            same(ref, got, lt)
+        for gt in GATHER_TILES:
+            core.set_tiles(0, gather=gt)
+            got = run_all()
+            t = core.batched_tiles(0)
+            print(f"[tiles] gather {gt}: reported {t}", flush=True)
+            assert (t["gather_tile"], t["gather_waves"]) == gt, t
+            same(ref, got, gt)
+        with pytest.raises(Exception):
+            core.set_tiles(0, line=(3, 32))
+        with pytest.raises(Exception):
+            core.set_tiles(0, gather=(8, 4))
+    finally:
+        core.set_tiles(0)

@@ -29,7 +29,7 @@ EXPORTED = (
     "tt_circus_backward", "tt_sinograms_backward", "tt_backward",
     "tt_resize", "tt_zscore", "tt_nearest_orthonormal_sinogram",
     "tt_nearest_orthonormal_sinograms", "tt_backward_batched", "tt_saved_bytes",
-    "tt_circus_noalloc", "tt_saved_offsets",
+    "tt_circus_noalloc", "tt_saved_offsets", "tt_context_set_tiles", "tt_batched_tiles",
     # test hooks (include/tt_testing.h)
     "tt_testing_fail_allocations", "tt_testing_short_sqrt",
 )
@@ -55,6 +55,11 @@ class tt_image(C.Structure):
 class tt_image_grad(C.Structure):
     _fields_ = [("d_image", C.c_void_p), ("n", C.c_int32), ("d_dcircus", C.c_void_p),
                 ("d_dimage", C.c_void_p), ("d_sino", C.c_void_p), ("d_saved", C.c_void_p)]
+
+
+class tt_tiles(C.Structure):
+    _fields_ = [("line_angles", C.c_int32), ("line_columns", C.c_int32),
+                ("gather_tile", C.c_int32), ("gather_waves", C.c_int32)]
 
 
 class tt_sino_aux(C.Structure):
@@ -106,6 +111,8 @@ def load():
                                          C.POINTER(tt_pfunc), C.c_int]
         L.tt_context_set_autotune.argtypes = [vp, C.c_int]
         L.tt_batched_launch_shape.argtypes = [vp, ip, ip]
+        L.tt_context_set_tiles.argtypes = [vp, C.POINTER(tt_tiles)]
+        L.tt_batched_tiles.argtypes = [vp, C.POINTER(tt_tiles)]
         L.tt_circus_backward.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int,
                                          C.POINTER(tt_pfunc), C.c_int, vp, vp]
         L.tt_sinograms_backward.argtypes = [vp, vp, C.c_int, C.c_int, f32,

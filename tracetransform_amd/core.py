@@ -276,11 +276,37 @@ class Batch:
                 part(offs[1].value) if need_s else None)
 
     def launch_shape(self):
+        """(waves per workgroup, workgroups, tile angles, tile columns) of
+        the line kernel of the last planned batch."""
         ctx = get_context(self.device)
         w, b = C.c_int(), C.c_int()
         _lib.check("tt_batched_launch_shape",
                    _lib.load().tt_batched_launch_shape(ctx.handle, C.byref(w), C.byref(b)))
-        return w.value, b.value
+        t = batched_tiles(self.device)
+        return w.value, b.value, t["line_angles"], t["line_columns"]
+
+
+def set_tiles(device: int = 0, line=None, gather=None):
+    """Forces the batched paths' tiles on ``device``'s context:
+    ``line`` = (angles, columns) of a line-kernel wave item, ``gather`` =
+    (tile side, waves per workgroup) of the backward's transposed rotation;
+    None leaves the default (or the tuned choice)."""
+    t = _lib.tt_tiles()
+    if line is not None:
+        t.line_angles, t.line_columns = int(line[0]), int(line[1])
+    if gather is not None:
+        t.gather_tile, t.gather_waves = int(gather[0]), int(gather[1])
+    ctx = get_context(device)
+    _lib.check("tt_context_set_tiles", _lib.load().tt_context_set_tiles(ctx.handle, C.byref(t)))
+
+
+def batched_tiles(device: int = 0) -> dict:
+    """Tiles of the last planned batched forward (line_*) and backward
+    (gather_*, zero before any)."""
+    t = _lib.tt_tiles()
+    ctx = get_context(device)
+    _lib.check("tt_batched_tiles", _lib.load().tt_batched_tiles(ctx.handle, C.byref(t)))
+    return {f: getattr(t, f) for f, _ in t._fields_}
 
 
 # ---------------------------------------------------------------- adjoint

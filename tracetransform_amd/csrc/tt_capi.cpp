@@ -291,6 +291,7 @@ struct BatchPlan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int waves_per_block = 4, blocks = 0;
+    int tile_a = 1, tile_c = 64;  // line tile (angles x columns)
     bool valid = false;
 
     void drop_graph()
@@ -339,6 +340,7 @@ struct tt_context {
     BatchPlan batch;
     BackwardPlan bwd;
     bool autotune = false;
+    tt_tiles force = {};  // tiles forced by tt_context_set_tiles (0: not forced)
     std::map<const void *, SavedEntry> saved;  // d_saved blocks a batched forward writes
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipStream_t own = nullptr;
@@ -942,6 +944,53 @@ int device_cus(int device)
     return cus;
 }
 
+// Line tiles (angles x columns) the batched forward's tuner tries.
+struct LineTile {
+    int a, c;
+};
+constexpr LineTile kLineTiles[] = {{1, 64}, {2, 64}, {4, 64}, {2, 32},
+                                   {4, 32}, {4, 16}, {8, 16}};
+
+LineTile line_tile_of(const tt_context *ctx)
+{
+    if (ctx->force.line_angles > 0)
+        return {ctx->force.line_angles, ctx->force.line_columns};
+    return kLineTiles[0];
+}
+
+// Wave items of the batched line kernel with tile t: each image's column
+// tiles and first item (descriptors in launch order); returns the total.
+int layout_line_items(std::vector<ttk::BatchImage> &d, int A, LineTile t)
+{
+    int items = 0;
+    for (ttk::BatchImage &b : d) {
+        b.item0 = items;
+        items += ttk::line_tile_items(b.n, A, t.a, t.c, &b.nb);
+    }
+    return items;
+}
+
+// First gather tile of each image for tiles of side T; returns the total.
+int layout_gather_tiles(std::vector<ttk::BatchImage> &d, int T)
+{
+    int tiles = 0;
+    for (ttk::BatchImage &b : d) {
+        b.tile0 = tiles;
+        const int t = (b.n + T - 1) / T;
+        tiles += t * t;
+    }
+    return tiles;
+}
+
+// The most waves per workgroup (up to 16) whose gather LDS fits, or 0.
+int gather_waves_max(int T, int tabn, bool do345, bool do12)
+{
+    for (int waves : {16, 8, 4, 2, 1})
+        if (ttk::gather_lds_bytes(T, waves, tabn, do345, do12, false) <= kLdsLimit)
+            return waves;
+    return 0;
+}
+
 // Enqueue the batched pass (used for capture and for autotuning).
 // The batched forward's stages on st.  timed: each stage bracketed by the
 // context's kernel timer (never while a graph is being captured).
@@ -1037,9 +1086,46 @@ hipError_t enqueue_backward(tt_context *ctx, const BackwardPlan &P, hipStream_t 
 TT_API tt_status tt_context_set_autotune(tt_context *ctx, int enable)
 {
     TT_CHECK(ctx, "ctx is NULL");
-    if ((enable != 0) != ctx->autotune)
+    if ((enable != 0) != ctx->autotune) {
         ctx->batch.drop_graph();
+        ctx->bwd.g.drop_graph();
+    }
     ctx->autotune = enable != 0;
+    return TT_OK;
+}
+
+TT_API tt_status tt_context_set_tiles(tt_context *ctx, const tt_tiles *tiles)
+{
+    TT_CHECK(ctx, "ctx is NULL");
+    tt_tiles t = tiles ? *tiles : tt_tiles{};
+    TT_CHECK((t.line_angles == 0 && t.line_columns == 0) ||
+                 ttk::line_tile_ok(t.line_angles, t.line_columns),
+             "line tile %d x %d: columns must be 64, 32 or 16 and angles a multiple of "
+             "64 / columns up to 16",
+             t.line_angles, t.line_columns);
+    TT_CHECK(t.gather_tile == 0 || t.gather_tile == 16 || t.gather_tile == 32,
+             "gather tile %d: must be 16 or 32", t.gather_tile);
+    TT_CHECK(t.gather_waves >= 0 && t.gather_waves <= 16,
+             "gather waves %d: must be 1..16", t.gather_waves);
+    ctx->force = t;
+    ctx->batch.drop_graph();
+    ctx->bwd.g.drop_graph();
+    return TT_OK;
+}
+
+TT_API tt_status tt_batched_tiles(tt_context *ctx, tt_tiles *tiles)
+{
+    TT_CHECK(ctx && tiles, "NULL argument");
+    TT_CHECK(ctx->batch.valid || ctx->bwd.g.valid, "no batch has been planned on this context");
+    *tiles = tt_tiles{};
+    if (ctx->batch.valid) {
+        tiles->line_angles = ctx->batch.tile_a;
+        tiles->line_columns = ctx->batch.tile_c;
+    }
+    if (ctx->bwd.g.valid) {
+        tiles->gather_tile = ctx->bwd.gather_tile;
+        tiles->gather_waves = ctx->bwd.gather_waves;
+    }
     return TT_OK;
 }
 
@@ -1123,6 +1209,8 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
     const bool need_s = do345;
     if (!B.valid || B.key != key) {
         B.drop_graph();
+        // a replay of the previous plan may still read the descriptors
+        TT_HIP(hipStreamSynchronize(own));
         // largest images first: the dynamic counter then hands out the long
         // items early and the short ones fill the tail
         std::vector<int> order(nimg);
@@ -1152,7 +1240,7 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         std::vector<ttk::DftJob> jobs;
         int dft_tiles = 0;
         size_t off = 0, cko = 0, so = 0;
-        int items = 0, cols = 0;
+        int cols = 0;
         bool p2 = false;
         for (int q = 0; q < np; q++)
             p2 = p2 || pf[q].code == TT_P2;
@@ -1192,16 +1280,11 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
             b.med_s = sv && need_s ? (int32_t *)(sv + SL.med_s) : nullptr;
             b.z345 = sv && do345 ? (float2 *)(sv + SL.z345) : nullptr;
             b.n = im.n;
-            b.nb = (im.n + ttk::kWave - 1) / ttk::kWave;
-            b.item0 = items;
             b.col0 = cols;
             b.circus = im.d_circus;
-            items += b.nb * n_angles;
             cols += nt * n_angles;
         }
         TT_HIP(B.desc.ensure(sizeof(ttk::BatchImage) * nimg));
-        TT_HIP(hipMemcpy(B.desc.ptr, d.data(), sizeof(ttk::BatchImage) * nimg,
-                         hipMemcpyHostToDevice));
         B.dft = {};
         if (p3) {
             TT_HIP(B.jobs.ensure(sizeof(ttk::DftJob) * jobs.size()));
@@ -1225,7 +1308,6 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         ttk::BatchSinoParams sp = {};
         sp.images = B.desc.as<ttk::BatchImage>();
         sp.nimg = nimg;
-        sp.items = items;
         sp.trig = B.trig.as<float2>();
         sp.tab345 = B.tab.as<float2>();
         sp.tabn = nmax;
@@ -1233,6 +1315,16 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         sp.counter = B.counter.as<unsigned int>();
         for (int k = 0; k < 8; k++)
             sp.slot.s[k] = slot[k];
+        // descriptors with the line items of tile t
+        auto use_tile = [&](LineTile t) {
+            sp.items = layout_line_items(d, n_angles, t);
+            sp.tile_a = t.a;
+            sp.tile_c = t.c;
+            return hipMemcpy(B.desc.ptr, d.data(), sizeof(ttk::BatchImage) * nimg,
+                             hipMemcpyHostToDevice);
+        };
+        LineTile best_t = line_tile_of(ctx);
+        TT_HIP(use_tile(best_t));
         ttk::BatchCircusParams cp = {};
         cp.images = sp.images;
         cp.nimg = nimg;
@@ -1245,7 +1337,9 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         for (int q = 0; q < np; q++)
             cp.code[q] = pf[q].code;
 
-        // launch shape: default, or the fastest candidate on this batch
+        // line tile and launch shape: forced, the defaults, or the fastest
+        // candidates on this batch (tiles at the default shape, then shapes
+        // with the chosen tile)
         const int cus = device_cus(ctx->device);
         int best_w = 4, best_b = cus * 4;
         if (ctx->autotune) {
@@ -1254,26 +1348,53 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
             hipEvent_t e0, e1;
             TT_HIP(hipEventCreate(&e0));
             TT_HIP(hipEventCreate(&e1));
+            // line kernel time with w waves per workgroup, blocks workgroups
+            // (the mean of two runs after a first)
+            auto trial = [&](int w, int blocks, float *ms_out) {
+                float tsum = 0.0f;
+                for (int rep = 0; rep < 3; rep++) {
+                    hipError_t r = hipEventRecord(e0, own);
+                    if (r == hipSuccess)
+                        r = enqueue_batch(ctx, sp, cp, nimg, nmax, need_x, need_s, do345, w,
+                                          blocks, false, false, own);
+                    if (r == hipSuccess)
+                        r = hipEventRecord(e1, own);
+                    if (r == hipSuccess)
+                        r = hipEventSynchronize(e1);
+                    float ms = 0.0f;
+                    if (r == hipSuccess)
+                        r = hipEventElapsedTime(&ms, e0, e1);
+                    if (r != hipSuccess)
+                        return r;
+                    if (rep > 0)
+                        tsum += ms;
+                }
+                *ms_out = tsum;
+                return hipSuccess;
+            };
             float best = 1e30f;
+            if (ctx->force.line_angles == 0) {
+                for (const LineTile &t : kLineTiles) {
+                    float ms = 0.0f;
+                    TT_HIP(use_tile(t));
+                    TT_HIP(trial(best_w, best_b, &ms));
+                    if (ms < best) {
+                        best = ms;
+                        best_t = t;
+                    }
+                }
+                TT_HIP(use_tile(best_t));
+            }
+            best = 1e30f;
             const int waves[] = {1, 2, 4};
             const int per_cu[] = {2, 4, 8, 16};
             for (int w : waves)
                 for (int pc : per_cu) {
                     const int blocks = cus * pc * 4 / w;
-                    float tsum = 0.0f;
-                    for (int rep = 0; rep < 3; rep++) {
-                        TT_HIP(hipEventRecord(e0, own));
-                        TT_HIP(enqueue_batch(ctx, sp, cp, nimg, nmax, need_x, need_s,
-                                             do345, w, blocks, false, false, own));
-                        TT_HIP(hipEventRecord(e1, own));
-                        TT_HIP(hipEventSynchronize(e1));
-                        float ms = 0.0f;
-                        TT_HIP(hipEventElapsedTime(&ms, e0, e1));
-                        if (rep > 0)
-                            tsum += ms;
-                    }
-                    if (tsum < best) {
-                        best = tsum;
+                    float ms = 0.0f;
+                    TT_HIP(trial(w, blocks, &ms));
+                    if (ms < best) {
+                        best = ms;
                         best_w = w;
                         best_b = blocks;
                     }
@@ -1283,6 +1404,8 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         }
         B.waves_per_block = best_w;
         B.blocks = best_b;
+        B.tile_a = best_t.a;
+        B.tile_c = best_t.c;
         B.sp = sp;
         B.cp = cp;
         B.nimg = nimg;
@@ -1542,6 +1665,7 @@ int tile_count(int n, int tile)
     const int t = (n + tile - 1) / tile;
     return t * t;
 }
+
 
 tt_status image_vjp(tt_context *ctx, int n, int A, const int *slot,
                     const float *d_dsino, float *d_dimage)
@@ -2098,7 +2222,7 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
     }
     TT_TRY(ctx->set_device());
     std::vector<int64_t> key = {nimg, n_angles, (int64_t)(*(const uint32_t *)&angle_step_deg),
-                                nt, np, saved};
+                                nt, np, saved, ctx->autotune};
     for (int t = 0; t < nt; t++)
         key.push_back(tcodes[t]);
     for (int q = 0; q < np; q++)
@@ -2118,6 +2242,8 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
     const bool do12 = slot[1] >= 0 || slot[2] >= 0;
     if (!B.valid || B.key != key) {
         B.drop_graph();
+        // a replay of the previous plan may still read the descriptors
+        TT_HIP(hipStreamSynchronize(own));
         std::vector<int> order(nimg);
         for (int i = 0; i < nimg; i++)
             order[i] = i;
@@ -2154,19 +2280,19 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         TT_HIP(P.rec.ensure(sizeof(ttk::LineRecord) * ln));
         if (p3s.np3 > 0)
             TT_HIP(P.y.ensure(sizeof(float) * spec_tot));
-        int tiles32 = 0;
-        for (int i = 0; i < nimg; i++)
-            tiles32 += tile_count(images[i].n, 32);
-        const GatherLaunch gl = choose_gather(tiles32, n_angles, device_cus(ctx->device), nmax,
-                                              do345, do12, false, false);
-        P.gather_tile = gl.tile;
-        P.gather_waves = gl.waves;
+        // gather tile: forced, 32 pixels, or tuned below
+        int gtile = ctx->force.gather_tile > 0 ? ctx->force.gather_tile : 32;
+        int gwaves = ctx->force.gather_waves > 0 ? ctx->force.gather_waves
+                                                 : gather_waves_max(gtile, nmax, do345, do12);
+        TT_CHECK(gwaves >= 1 && ttk::gather_lds_bytes(gtile, gwaves, nmax, do345, do12,
+                                                      false) <= kLdsLimit,
+                 "gather tile %d with %d waves per workgroup exceeds the LDS", gtile, gwaves);
         std::vector<ttk::DftJob> fj, sj, bj;
         int ftiles = 0, btiles = 0;
         size_t so = 0;
         std::vector<ttk::BatchImage> d(nimg);
-        size_t po = 0, lo = 0, tiles = 0, cko = 0;
-        int items = 0, cols = 0;
+        size_t po = 0, lo = 0, cko = 0;
+        int cols = 0;
         for (int k = 0; k < nimg; k++) {
             const tt_image_grad &im = images[order[k]];
             ttk::BatchImage &b = d[k];
@@ -2174,8 +2300,6 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
             const size_t l = (size_t)im.n * n_angles;
             b.img = im.d_image;
             b.n = im.n;
-            b.nb = (im.n + ttk::kWave - 1) / ttk::kWave;
-            b.item0 = items;
             b.col0 = cols;
             b.circus = nullptr;
             b.dcircus = im.d_dcircus;
@@ -2184,7 +2308,6 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
             b.dimg = im.d_dimage;
             b.line0 = (int)lo;
             b.pix0 = (int)po;
-            b.tile0 = (int)tiles;
             if (saved) {
                 const SavedLayout L = saved_layout(im.n, n_angles, nt);
                 unsigned char *base = (unsigned char *)images[order[k]].d_saved;
@@ -2237,13 +2360,15 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
                 bj.push_back(jb);
             }
             so += (size_t)nt * n_angles * h2;
-            tiles += (size_t)tile_count(im.n, gl.tile);
-            items += b.nb * n_angles;
             cols += nt * n_angles;
             po += (size_t)im.n * im.n;
             lo += l;
         }
         TT_CHECK(po < (1ull << 31) && lo < (1ull << 31), "batch too large");
+        // the recompute's line tile (forced or the default); gather tiles
+        const LineTile lt = line_tile_of(ctx);
+        const int items = layout_line_items(d, n_angles, lt);
+        int tiles = layout_gather_tiles(d, gtile);
         TT_HIP(B.desc.ensure(sizeof(ttk::BatchImage) * nimg));
         TT_HIP(hipMemcpy(B.desc.ptr, d.data(), sizeof(ttk::BatchImage) * nimg,
                          hipMemcpyHostToDevice));
@@ -2282,6 +2407,8 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         sp.images = B.desc.as<ttk::BatchImage>();
         sp.nimg = nimg;
         sp.items = items;
+        sp.tile_a = lt.a;
+        sp.tile_c = lt.c;
         sp.trig = B.trig.as<float2>();
         sp.tab345 = B.tab.as<float2>();
         sp.tabn = nmax;
@@ -2306,7 +2433,7 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         ap.nmax = nmax;
         ap.lines = (int)lo;
         ap.pixels = (int)po;
-        ap.tiles = (int)tiles;
+        ap.tiles = tiles;
         ap.cols = cols;
         ap.trig = sp.trig;
         ap.tab345 = sp.tab345;
@@ -2331,6 +2458,52 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         P.saved = saved;
         P.p2src_on = p2src;
         P.cols = nt * n_angles;
+        B.tile_a = lt.a;
+        B.tile_c = lt.c;
+        P.gather_tile = gtile;
+        P.gather_waves = gwaves;
+        if (ctx->autotune && ctx->force.gather_tile == 0) {
+            // one eager backward writes the line records; then the gather
+            // alone with each candidate tile (the most waves that fit, and
+            // half as many), the mean of two runs after a first
+            TT_HIP(enqueue_backward(ctx, P, own, false));
+            hipEvent_t e0, e1;
+            TT_HIP(hipEventCreate(&e0));
+            TT_HIP(hipEventCreate(&e1));
+            float best = 1e30f;
+            for (int T : {32, 16}) {
+                const int wmax = gather_waves_max(T, nmax, do345, do12);
+                for (int w : {wmax, wmax / 2}) {
+                    if (w < 1 || (ctx->force.gather_waves > 0 && w != ctx->force.gather_waves))
+                        continue;
+                    P.ap.tiles = layout_gather_tiles(d, T);
+                    TT_HIP(hipMemcpy(B.desc.ptr, d.data(), sizeof(ttk::BatchImage) * nimg,
+                                     hipMemcpyHostToDevice));
+                    float tsum = 0.0f;
+                    for (int rep = 0; rep < 3; rep++) {
+                        TT_HIP(hipEventRecord(e0, own));
+                        TT_HIP(ttk::launch_gather_batched(P.ap, T, w, own));
+                        TT_HIP(hipEventRecord(e1, own));
+                        TT_HIP(hipEventSynchronize(e1));
+                        float ms = 0.0f;
+                        TT_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        if (rep > 0)
+                            tsum += ms;
+                    }
+                    if (tsum < best) {
+                        best = tsum;
+                        P.gather_tile = T;
+                        P.gather_waves = w;
+                    }
+                }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            tiles = layout_gather_tiles(d, P.gather_tile);
+            P.ap.tiles = tiles;
+            TT_HIP(hipMemcpy(B.desc.ptr, d.data(), sizeof(ttk::BatchImage) * nimg,
+                             hipMemcpyHostToDevice));
+        }
         TT_HIP(hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal));
         hipError_t ce = enqueue_backward(ctx, P, own, false);
         hipGraph_t g = nullptr;

@@ -68,7 +68,7 @@ struct BatchImage {
     const int *bbox;      // non-zero pixels' bounds {x0, x1, y0, y1} + flag, or null
     float2 *ckpt;         // [sino_ckpt_blocks(n)][A][n] walk prefixes (workspace)
     int n;
-    int nb;               // column blocks per angle
+    int nb;               // column tiles per angle tile (batched line kernel)
     int item0;            // first wave item of this image
     int col0;             // first sinogram column (over all T) of this image
     float *circus;        // [nt][np][A]
@@ -89,6 +89,8 @@ struct BatchSinoParams {
     const BatchImage *images;
     int nimg;
     int items;            // total wave items
+    int tile_a;           // angles of one wave item (a multiple of 64 / tile_c)
+    int tile_c;           // columns of one wave item: 64, 32 or 16
     const float2 *trig;   // [A]
     const float2 *tab345; // [3][tabn]
     int tabn;             // largest n of the batch
@@ -305,12 +307,29 @@ hipError_t launch_image_bbox(const float *img, int n, int *bbox, hipStream_t s);
 hipError_t launch_image_bbox_batched(const BatchImage *images, int nimg, int nmax,
                                      hipStream_t s);
 // Checkpoints of the median walks the T0..T5 line kernel keeps per line:
-// the prefixes at the starts of its 64-step blocks after the first (a wave's
-// first pass spans at most n + 63 steps with its row skew).
-inline int sino_ckpt_blocks(int n)
+// the prefixes at the starts of its 64-step blocks after the first, up to
+// this many (a wave of one angle and 64 columns spans at most n + 64 steps
+// with its row skew; a wave spanning more keeps the first ones).
+__host__ __device__ inline int sino_ckpt_blocks(int n)
 {
     const int b = (n + 62) / 64;
     return b > 1 ? b : 1;
+}
+
+// Line tiles of the batched T0..T5 kernel: tile_c columns of 64, 32 or 16
+// and tile_a angles, a multiple of 64 / tile_c up to 16.
+inline bool line_tile_ok(int tile_a, int tile_c)
+{
+    if (tile_c != 64 && tile_c != 32 && tile_c != 16)
+        return false;
+    return tile_a >= 1 && tile_a <= 16 && tile_a % (kWave / tile_c) == 0;
+}
+// Wave items of an image of side n at A angles, and the column tiles per
+// angle tile.
+inline int line_tile_items(int n, int A, int tile_a, int tile_c, int *nb)
+{
+    *nb = (n + tile_c - 1) / tile_c;
+    return *nb * ((A + tile_a - 1) / tile_a);
 }
 
 hipError_t launch_sino_t05(const SinoParams &p, bool need_x, bool need_s,

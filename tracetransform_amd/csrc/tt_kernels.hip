@@ -236,23 +236,29 @@ struct LineDst {
     Slots slot;
 };
 
-// Row schedule of a wave.  At step v lane l handles row v - d_l of its
-// line, d_l = rint(l t) - (its smallest value), t = -s/c with the row-major
-// copy (TR, |c| > |s|) and c/s with the column-major one, so |t| <= 1: the
-// 64 samples of one step then lie along one row of the copy the wave reads
-// (TR: y constant, else x), a few cache lines per load, instead of spanning
-// one image row per sample when the lanes all take the same row.  Each line
-// still sees its rows in increasing order, so every sum, prefix and walk is
-// the unskewed one.  Steps run from G, the smallest first row plus skew in
-// the wave, in groups of 8: slot j of the stride-8 accumulators takes the
-// rows of steps G + 8k + j, i.e. Eigen's accumulator (j + G - d_l) mod 8,
-// and the slots are rotated back after the pass.  Rows outside [0, n) (and
-// outside [0, a8) for the stride-8 accumulators) are masked; a group all of
-// whose lanes are inside skips the masks.
+// Row schedule of a wave.  At step v a lane handles row v - d of its line,
+// d = rint(col t) + 64 K, t = -s/c for a line with |c| > |s| and c/s
+// otherwise (|t| <= 1, from the line's own angle), K a wave-wide integer
+// that makes the smallest d of the wave fall in [0, 64).  Lanes of one
+// angle and consecutive columns then take samples along one row of the
+// image copy the wave reads (TR: y constant, else x), a few cache lines per
+// load, instead of spanning one image row per sample when the lanes all
+// take the same row.  Each line still sees its rows in increasing order, so
+// every sum, prefix and walk is the unskewed one, and d mod 64 depends on
+// the line alone, so the 64-step blocks of pass 2 (v in [64 k, 64 k + 64))
+// hold the same rows of a line whatever other lines share its wave: the
+// results of a line do not depend on the tile that computes it.  Steps run
+// from G, the smallest first row plus skew in the wave, in groups of 8: slot
+// j of the stride-8 accumulators takes the rows of steps G + 8k + j, i.e.
+// Eigen's accumulator (j + G - d) mod 8, and the slots are rotated back
+// after the pass.  Rows outside [0, n) (and outside [0, a8) for the stride-8
+// accumulators) are masked; a group all of whose lanes are inside skips the
+// masks.
 //
 // With CK, pass 1 also runs the median walks' sequential prefixes (the
 // same float additions in the same row order as the walk) and stores them
-// at the start of every 64-step block after the first.  When every sample
+// at the start of every 64-step block from G after the first (at most
+// sino_ckpt_blocks(n) of them).  When every sample
 // a line has before a block is non-negative its prefixes never decrease, so
 // a block whose starting prefix fails a lane's test 2 * prefix >= S holds
 // none of that lane's earlier passes: pass 2 starts at the last block start
@@ -277,9 +283,10 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
     constexpr int kNone = 1 << 30;
 
     // ---- schedule --------------------------------------------------------
-    int dl = (int)rintf((float)lane * (TR ? -s / c : c / s));
-    dl -= wave_min_i(dl);
-    const int dmax = wave_max_i(dl);
+    int dl = (int)rintf((float)col * (fabsf(c) > fabsf(s) ? -s / c : c / s));
+    dl -= wave_min_i(dl) & ~63;
+    const int dmin = wave_min_i(dl), dmax = wave_max_i(dl);
+    const int ncap = sino_ckpt_blocks(n);
     const int a8 = n & ~7;
     const int a4 = n & ~3;
     const int r0 = lo > 0 ? lo : 0;
@@ -326,9 +333,9 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
         }
     };
     for (int v = G; v < V1; v += 8) {
-        if (WALK && v > G && ((v - G) & 63) == 0 && active)
+        if (WALK && v > G && ((v - G) & 63) == 0 && active && (v - G) / 64 <= ncap)
             p.ckpt[(size_t)((v - G) / 64 - 1) * plane + line] = float2{cx, cs};
-        if (v >= dmax && v + 8 <= a8)
+        if (v >= dmax && v + 8 - dmin <= a8)
             group(std::false_type{}, v);
         else
             group(std::true_type{}, v);
@@ -406,7 +413,8 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
     if constexpr (WALK) {
         // NaN sums flag negative (square root NaN) or NaN samples
         const bool bad = (NEED_S && Ss != Ss) || Sx != Sx || vmin < 0.0f;
-        const int nck = V1 > G ? (V1 - G - 1) / 64 : 0;
+        int nck = V1 > G ? (V1 - G - 1) / 64 : 0;
+        nck = nck < ncap ? nck : ncap;
         int b0 = 0;
         float2 cb = {0.0f, 0.0f};
         if (nck > 0 && !__ballot(active && bad)) {
@@ -446,9 +454,10 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
     // values: they accumulate with float FMAs over blocks of 64 steps, each
     // block flushed into a double, so they land within ~1e-7 of the FP64
     // oracle instead of inheriting the reference's sequential float error.
-    // The blocks are the 64-step grid from G whatever step pass 2 starts at
-    // (every integral starts at or after it), so the integrals do not
-    // depend on the checkpoints.
+    // The blocks are the 64-step grid v in [64 k, 64 k + 64) whatever step
+    // pass 2 starts at (every integral starts at or after it), so the
+    // integrals depend neither on the checkpoints nor on the wave's other
+    // lines.
     double t1 = 0.0, t2 = 0.0;
     double r3 = 0.0, i3 = 0.0, r4 = 0.0, i4 = 0.0, r5 = 0.0, i5 = 0.0;
     int m = -1, sm = -1;
@@ -461,10 +470,10 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
             sm = 0;
     }
     for (int vb = start, ve; vb < V2; vb = ve) {
-        ve = G + ((vb - G) / 64 + 1) * 64;
+        ve = (vb & ~63) + 64;
         ve = ve < V2 ? ve : V2;
         // every lane's rows of the block inside [0, n)
-        const bool full = vb >= dmax && ve <= n;
+        const bool full = vb >= dmax && ve - dmin <= n;
         float f1 = 0.0f, f2 = 0.0f;
         float f3r = 0.0f, f3i = 0.0f, f4r = 0.0f, f4i = 0.0f, f5r = 0.0f, f5i = 0.0f;
         int r = vb - dl;
@@ -559,8 +568,8 @@ __device__ __forceinline__ void sino_line_any(const LineDst &dst, const float *i
                                               const float2 *s_tab, int a, int col,
                                               bool active, float2 cs, int lo, int hi)
 {
-    // Lanes step (dx, dy) = (c, -s) per column
-    const bool tr = fabsf(cs.x) > fabsf(cs.y);
+    // Lanes step (dx, dy) = (c, -s) per column; one copy for the whole wave
+    const bool tr = __builtin_amdgcn_readfirstlane(fabsf(cs.x) > fabsf(cs.y));
     const bool fs = bbox && bbox[4] == 0;
     if (tr && fs)
         sino_line<true, NEED_X, NEED_S, DO345, CK, true>(dst, imgT, s_tab, a, col, active,
@@ -616,12 +625,17 @@ __global__ __launch_bounds__(kBlock) void sino_t05_kernel(SinoParams p)
 }
 
 // ---------------------------------------------------------------------------
-// Batched T0..T5: one persistent launch over every (image, angle, column
-// block) wave item of a batch of padded images of uneven side.  Waves take
-// items from a device counter (dynamic load balance; items are ordered
-// largest image first) and find their image by binary search over the
-// descriptors' first-item offsets.  The counter is zeroed by
-// reset_counter_kernel ahead of each launch.
+// Batched T0..T5: one persistent launch over every (image, angle tile,
+// column tile) wave item of a batch of padded images of uneven side.  A
+// tile is tile_a consecutive angles by tile_c consecutive columns (64, 32
+// or 16): the wave takes it in tile_a tile_c / 64 passes, in each pass lane
+// l on column l % tile_c of angle l / tile_c of the pass.  Waves take items
+// from a device counter (dynamic load balance; items are ordered largest
+// image first) and find their image by binary search over the descriptors'
+// first-item offsets.  The counter is zeroed by reset_counter_kernel ahead
+// of each launch.  The image copy a pass reads (row- or column-major) is
+// the one that suits the angle of its first lane; the results of a line do
+// not depend on the tile.
 // ---------------------------------------------------------------------------
 template <bool NEED_X, bool NEED_S, bool DO345>
 __global__ __launch_bounds__(kBlock) void sino_t05_batched_kernel(BatchSinoParams p)
@@ -632,13 +646,22 @@ __global__ __launch_bounds__(kBlock) void sino_t05_batched_kernel(BatchSinoParam
         __syncthreads();
     }
     const int lane = threadIdx.x & (kWave - 1);
+    const int sub = kWave / p.tile_c;  // angles per pass
+    const int lc = lane & (p.tile_c - 1), la = lane / p.tile_c;
+    // one pass per iteration (an item's passes in a row): no loop nested
+    // inside the item loop, whose invariants would stay live across the
+    // line code
+    int item = 0, pass = 0, passes = 0;
     for (;;) {
-        unsigned int v = 0;
-        if (lane == 0)
-            v = atomicAdd(p.counter, 1u);
-        const int item = __builtin_amdgcn_readfirstlane(__shfl(v, 0, kWave));
-        if (item >= p.items)
-            break;
+        if (pass == passes) {
+            unsigned int v = 0;
+            if (lane == 0)
+                v = atomicAdd(p.counter, 1u);
+            item = __builtin_amdgcn_readfirstlane(__shfl(v, 0, kWave));
+            if (item >= p.items)
+                break;
+            pass = 0;
+        }
         int lo = 0, hi = p.nimg - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -649,12 +672,17 @@ __global__ __launch_bounds__(kBlock) void sino_t05_batched_kernel(BatchSinoParam
         }
         const BatchImage im = p.images[lo];
         const int local = item - im.item0;
-        const int a = local / im.nb;
-        const int cb = local - a * im.nb;
-        int col = cb * kWave + lane;
-        const bool active = col < im.n;
-        if (!active)
-            col = im.n - 1;
+        const int at = local / im.nb;
+        const int cb = local - at * im.nb;
+        const int a_first = at * p.tile_a;
+        const int a_end = a_first + p.tile_a < p.A ? a_first + p.tile_a : p.A;
+        passes = (a_end - a_first + sub - 1) / sub;
+        int a = a_first + pass * sub + la;
+        pass++;
+        int col = cb * p.tile_c + lc;
+        const bool active = col < im.n && a < a_end;
+        col = col < im.n ? col : im.n - 1;
+        a = a < a_end ? a : a_end - 1;
         const float2 cs = p.trig[a];
         const LineDst dst = {im.sino, im.med_x, im.med_s, im.z345, im.ckpt, im.n, p.A, p.tabn,
                              p.slot};
@@ -2503,10 +2531,9 @@ __global__ __launch_bounds__(kBlock) void gather_reduce_kernel(const float *part
     }
 }
 
-template <bool DO345, bool DO12>
+template <int T, bool DO345, bool DO12>
 __global__ __launch_bounds__(1024) void gather_batched_kernel(BatchAdjointParams p)
 {
-    constexpr int T = kGatherTile;
     extern __shared__ float4 smem4[];
     float *smem = reinterpret_cast<float *>(smem4);
     const int words = gather_wave_words(T, DO12, false);
@@ -3151,6 +3178,8 @@ hipError_t launch_sino_t05_batched(const BatchSinoParams &p, bool need_x,
                                    bool need_s, bool do345, int waves_per_block,
                                    int blocks, hipStream_t s)
 {
+    if (!line_tile_ok(p.tile_a, p.tile_c))
+        return hipErrorInvalidValue;
     const int threads = kWave * waves_per_block;
     const size_t shm = do345 ? sizeof(float2) * 3 * (size_t)p.tabn : 0;
 #define TT_LAUNCH(X, S, D)                                                     \
@@ -3335,23 +3364,28 @@ hipError_t launch_records_batched(const BatchAdjointParams &p, hipStream_t s)
 hipError_t launch_gather_batched(const BatchAdjointParams &p, int tile, int waves,
                                  hipStream_t s)
 {
-    if (tile != kGatherTile || waves < 1 || waves > 16)
+    if ((tile != 32 && tile != 16) || waves < 1 || waves > 16)
         return hipErrorInvalidValue;
     const bool do12 = p.slot.s[1] >= 0 || p.slot.s[2] >= 0;
     const size_t shm = gather_lds_bytes(tile, waves, p.tabn, p.do345, do12, false);
     const int blocks = (p.tiles + waves - 1) / waves;
     const dim3 block(kWave * waves);
-    if (p.do345) {
-        if (do12)
-            gather_batched_kernel<true, true><<<blocks, block, shm, s>>>(p);
-        else
-            gather_batched_kernel<true, false><<<blocks, block, shm, s>>>(p);
-    } else {
-        if (do12)
-            gather_batched_kernel<false, true><<<blocks, block, shm, s>>>(p);
-        else
-            gather_batched_kernel<false, false><<<blocks, block, shm, s>>>(p);
-    }
+#define TT_LAUNCH(T)                                                                   \
+    do {                                                                               \
+        if (p.do345 && do12)                                                           \
+            gather_batched_kernel<T, true, true><<<blocks, block, shm, s>>>(p);        \
+        else if (p.do345)                                                              \
+            gather_batched_kernel<T, true, false><<<blocks, block, shm, s>>>(p);       \
+        else if (do12)                                                                 \
+            gather_batched_kernel<T, false, true><<<blocks, block, shm, s>>>(p);       \
+        else                                                                           \
+            gather_batched_kernel<T, false, false><<<blocks, block, shm, s>>>(p);      \
+    } while (0)
+    if (tile == 32)
+        TT_LAUNCH(32);
+    else
+        TT_LAUNCH(16);
+#undef TT_LAUNCH
     return hipGetLastError();
 }
 
