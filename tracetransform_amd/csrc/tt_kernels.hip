@@ -1135,8 +1135,8 @@ __device__ __forceinline__ uint64_t shfl_xor_key(uint64_t v, int m)
 // with their directions known at compile time; for larger sizes the
 // direction depends on the lane only, strides of KPL and more exchange with
 // lane l ^ (stride / KPL) through shuffles.  Wave-synchronous: no barriers.
-// (DPP and ds_swizzle forms of the exchanges were measured slower here:
-// they raise the VGPR count and cost occupancy.)
+// (DPP forms of the exchanges were measured slower here: they raise the
+// VGPR count and cost occupancy.)
 template <int KPL, class Key>
 __device__ __forceinline__ void wave_bitonic_sort(Key (&k)[KPL], int lane)
 {
@@ -1503,24 +1503,26 @@ __device__ __forceinline__ uint64_t sorted_tail_median(float *xs, int L, int Le,
 }
 
 // The sorted tail with the smallest register network that holds its Le
-// sorted keys (at most KMAX per lane).
-template <int KMAX, bool T6, bool IDX7>
+// sorted keys (at least KMIN, at most KMAX per lane; the networks below
+// KMIN are not instantiated).
+template <int KMAX, int KMIN, bool T6, bool IDX7>
 __device__ __forceinline__ uint64_t sorted_tail_dispatch(float *xs, int L, int Le, int lane,
                                                          int *k, bool fs)
 {
     const int kneed = (Le + kWave - 1) / kWave;
-    if (kneed <= 1 || KMAX == 1)
-        return sorted_tail_median<1, T6, IDX7>(xs, L, Le, lane, k, fs);
-    if constexpr (KMAX >= 2)
+    if constexpr (KMIN <= 1)
+        if (kneed <= 1 || KMAX == 1)
+            return sorted_tail_median<1, T6, IDX7>(xs, L, Le, lane, k, fs);
+    if constexpr (KMAX >= 2 && KMIN <= 2)
         if (kneed <= 2 || KMAX == 2)
             return sorted_tail_median<2, T6, IDX7>(xs, L, Le, lane, k, fs);
-    if constexpr (KMAX >= 4)
+    if constexpr (KMAX >= 4 && KMIN <= 4)
         if (kneed <= 4 || KMAX == 4)
             return sorted_tail_median<4, T6, IDX7>(xs, L, Le, lane, k, fs);
-    if constexpr (KMAX >= 8)
+    if constexpr (KMAX >= 8 && KMIN <= 8)
         if (kneed <= 8 || KMAX == 8)
             return sorted_tail_median<8, T6, IDX7>(xs, L, Le, lane, k, fs);
-    if constexpr (KMAX >= 16)
+    if constexpr (KMAX >= 16 && KMIN <= 16)
         if (kneed <= 16 || KMAX == 16)
             return sorted_tail_median<16, T6, IDX7>(xs, L, Le, lane, k, fs);
     if constexpr (KMAX >= 32)
@@ -1528,36 +1530,37 @@ __device__ __forceinline__ uint64_t sorted_tail_dispatch(float *xs, int L, int L
     return 0;  // not reached: kneed <= KMAX
 }
 
-// Keys per lane of the network for Le keys: 1, 2, 4, ... up to KMAX.
-template <int KMAX>
+// Keys per lane of the network for Le keys: KMIN, 2 KMIN, ... up to KMAX.
+template <int KMAX, int KMIN = 1>
 __device__ __forceinline__ int keys_per_lane(int Le)
 {
-    int K = 1;
+    int K = KMIN;
     while (K < KMAX && K * kWave < Le)
         K <<= 1;
     return K;
 }
 
 // Sorts the first K of KMAX keys per lane (the rest hold the largest key)
-// with the K-per-lane network (K a power of two up to KMAX).
-template <int KMAX, class Key>
+// with the K-per-lane network (K a power of two from KMIN up to KMAX).
+template <int KMAX, int KMIN = 1, class Key>
 __device__ __forceinline__ void sort_prefix(Key (&k)[KMAX], int K, int lane)
 {
-    if (K <= 1 || KMAX == 1) {
-        wave_bitonic_sort<1>(reinterpret_cast<Key(&)[1]>(k), lane);
-        return;
-    }
-    if constexpr (KMAX >= 2)
+    if constexpr (KMIN <= 1)
+        if (K <= 1 || KMAX == 1) {
+            wave_bitonic_sort<1>(reinterpret_cast<Key(&)[1]>(k), lane);
+            return;
+        }
+    if constexpr (KMAX >= 2 && KMIN <= 2)
         if (K <= 2 || KMAX == 2) {
             wave_bitonic_sort<2>(reinterpret_cast<Key(&)[2]>(k), lane);
             return;
         }
-    if constexpr (KMAX >= 4)
+    if constexpr (KMAX >= 4 && KMIN <= 4)
         if (K <= 4 || KMAX == 4) {
             wave_bitonic_sort<4>(reinterpret_cast<Key(&)[4]>(k), lane);
             return;
         }
-    if constexpr (KMAX >= 8)
+    if constexpr (KMAX >= 8 && KMIN <= 8)
         if (K <= 8 || KMAX == 8) {
             wave_bitonic_sort<8>(reinterpret_cast<Key(&)[8]>(k), lane);
             return;
@@ -1782,7 +1785,7 @@ __device__ __forceinline__ void store_tail_result(const SortTParams &p, int line
 //     smallest register network that holds it (K = 1..KMAX keys per lane).
 // With DEFER, an entry whose sorted part needs more than KMAX keys per lane
 // is left for a wider pass: sort_line then returns true (nothing written).
-template <int KMAX, bool DEFER, bool IDX7>
+template <int KMAX, int KMIN, bool DEFER, bool IDX7>
 __device__ __forceinline__ bool sort_line(const SortTParams &p, int line, int which,
                                           float *xs, int lane)
 {
@@ -1843,8 +1846,8 @@ __device__ __forceinline__ bool sort_line(const SortTParams &p, int line, int wh
             return true;
         wave_lds_sync();
         const bool fs = p.bbox && p.bbox[4] == 0;
-        res = t6 ? sorted_tail_dispatch<KMAX, true, false>(xs, L, Le, lane, &k, fs)
-                 : sorted_tail_dispatch<KMAX, false, IDX7>(xs, L, Le, lane, &k, fs);
+        res = t6 ? sorted_tail_dispatch<KMAX, KMIN, true, false>(xs, L, Le, lane, &k, fs)
+                 : sorted_tail_dispatch<KMAX, KMIN, false, IDX7>(xs, L, Le, lane, &k, fs);
     }
     store_tail_result<IDX7>(p, line, which, res, k, base, lane);
     return false;
@@ -1860,7 +1863,7 @@ __device__ __forceinline__ bool sort_line(const SortTParams &p, int line, int wh
 // written for them).  A pair whose union exceeds the slice goes entry by
 // entry; so do the tails of a pair with a negative or NaN sample (largest
 // key), from the shared samples.
-template <int KMAX, bool DEFER, bool IDX7>
+template <int KMAX, int KMIN, bool DEFER, bool IDX7>
 __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *xs, float *xs2,
                                          int lane)
 {
@@ -1889,7 +1892,7 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
     if (deferred || Lu > M) {
         for (int which = 0; which < 2; which++) {
             if (!((deferred >> which) & 1) &&
-                sort_line<KMAX, DEFER, IDX7>(p, line, which, xs, lane))
+                sort_line<KMAX, KMIN, DEFER, IDX7>(p, line, which, xs, lane))
                 deferred |= 1 << which;
             wave_lds_sync();
         }
@@ -1929,15 +1932,15 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
             r7 = tail_max_key(xs + o7, Ls7, L7, false, lane);
         wave_lds_sync();
         if (!bad6)
-            r6 = sorted_tail_dispatch<KMAX, true, false>(xs + o6, L6, Le6, lane, &k6, fs);
+            r6 = sorted_tail_dispatch<KMAX, KMIN, true, false>(xs + o6, L6, Le6, lane, &k6, fs);
         if (!bad7)
-            r7 = sorted_tail_dispatch<KMAX, false, IDX7>(xs + o7, L7, Le7, lane, &k7, fs);
+            r7 = sorted_tail_dispatch<KMAX, KMIN, false, IDX7>(xs + o7, L7, Le7, lane, &k7, fs);
         store_tail_result<IDX7>(p, line, 0, r6, k6, sm, lane);
         store_tail_result<IDX7>(p, line, 1, r7, k7, m, lane);
         return 0;
     }
     using Key7 = TailKey<IDX7>;
-    const int K6 = keys_per_lane<KMAX>(Le6), K7 = keys_per_lane<KMAX>(Le7);
+    const int K6 = keys_per_lane<KMAX, KMIN>(Le6), K7 = keys_per_lane<KMAX, KMIN>(Le7);
     // T6 first: keys, sort, its samples gathered by index, roots into xs2
     // (the samples in xs stay for T7's keys, so the two key sets are never
     // both being sorted)
@@ -1952,7 +1955,7 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
             key6[r] = tail_key<true, false>(xs + o6, i, &z);
         ins6 += __popcll(__ballot(z));
     }
-    sort_prefix<KMAX>(key6, K6, lane);
+    sort_prefix<KMAX, KMIN>(key6, K6, lane);
     auto roots6 = [&](auto fsc) {
         constexpr bool FS = decltype(fsc)::value;
 #pragma unroll
@@ -1980,7 +1983,7 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
         if constexpr (IDX7)
             ins7 += __popcll(__ballot(z));
     }
-    sort_prefix<KMAX>(key7, K7, lane);
+    sort_prefix<KMAX, KMIN>(key7, K7, lane);
     wave_lds_sync();
     auto roots7 = [&](auto fsc) {
         constexpr bool FS = decltype(fsc)::value;
@@ -2024,9 +2027,12 @@ __device__ __forceinline__ int sort_pair(const SortTParams &p, int line, float *
 // VGPR budget would otherwise set the occupancy of every wave); a wave
 // collects a chunk's queued items in one register (lane i holds the i-th)
 // and queues them with one atomic.  Kernels of at most 16 keys per lane take
-// the two tails of a line together when both are requested.
-template <int KMAX, bool DEFER, bool IDX7>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KMAX <= 8 ? 5 : KMAX <= 16 ? 4 : 1)))
+// the two tails of a line together when both are requested.  The passes over
+// queued items (KMIN = KMAX) build only their widest network: a smaller code
+// and fewer spills than one network per width.  The occupancy hints are the
+// measured best (more waves beat the spills they cost).
+template <int KMAX, int KMIN, bool DEFER, bool IDX7>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KMAX <= 4 ? 5 : KMAX <= 8 ? 6 : KMAX <= 16 ? 4 : 1)))
 void sino_sort_kernel(SortTParams p)
 {
     constexpr int M = kWave * KMAX;
@@ -2057,12 +2063,12 @@ void sino_sort_kernel(SortTParams p)
             mask &= (p.slot6 >= 0 ? 1 : 0) | (p.slot7 >= 0 ? 2 : 0);
             int d = 0;
             if (PAIR && mask == 3) {
-                d = sort_pair<KMAX, DEFER, IDX7>(p, line, xs, xs + M, lane);
+                d = sort_pair<KMAX, KMIN, DEFER, IDX7>(p, line, xs, xs + M, lane);
                 wave_lds_sync();
             } else {
                 for (int which = 0; which < 2; which++) {
                     if (((mask >> which) & 1) &&
-                        sort_line<KMAX, DEFER, IDX7>(p, line, which, xs, lane))
+                        sort_line<KMAX, KMIN, DEFER, IDX7>(p, line, which, xs, lane))
                         d |= 1 << which;
                     wave_lds_sync();
                 }
@@ -3202,14 +3208,14 @@ hipError_t launch_sino_t05_batched(const BatchSinoParams &p, bool need_x,
 
 // Workgroups of one sort pass that fit on the device at once (the waves are
 // persistent, so more would only wait).
-template <int KMAX, bool DEFER, bool IDX7>
+template <int KMAX, int KMIN, bool DEFER, bool IDX7>
 static int sort_blocks(int cus, int items = 0x7fffffff)
 {
     static int per_cu = 0;
     if (per_cu == 0) {
         int b = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &b, sino_sort_kernel<KMAX, DEFER, IDX7>, kBlock, 0) != hipSuccess || b < 1)
+                &b, sino_sort_kernel<KMAX, KMIN, DEFER, IDX7>, kBlock, 0) != hipSuccess || b < 1)
             b = 2;
         per_cu = b;
     }
@@ -3235,7 +3241,7 @@ static hipError_t sino_sort_passes(const SortTParams &p_in, int32_t *queue_a,
     p.work = counts + 2;
     p.chunk = 8;
     if (p.n <= 512) {
-#define TT_SORT(K) sino_sort_kernel<K, false, IDX7><<<sort_blocks<K, false, IDX7>(cus, lines), kBlock, 0, s>>>(p)
+#define TT_SORT(K) sino_sort_kernel<K, 1, false, IDX7><<<sort_blocks<K, 1, false, IDX7>(cus, lines), kBlock, 0, s>>>(p)
         if (p.n <= 64)
             TT_SORT(1);
         else if (p.n <= 128)
@@ -3253,23 +3259,23 @@ static hipError_t sino_sort_passes(const SortTParams &p_in, int32_t *queue_a,
         return hipErrorInvalidValue;
     p.defer_out = queue_a;
     p.defer_out_count = counts;
-    sino_sort_kernel<8, true, IDX7><<<sort_blocks<8, true, IDX7>(cus, lines), kBlock, 0, s>>>(p);
+    sino_sort_kernel<8, 1, true, IDX7><<<sort_blocks<8, 1, true, IDX7>(cus, lines), kBlock, 0, s>>>(p);
     p.defer_in = queue_a;
     p.defer_in_count = counts;
     p.work = counts + 3;
     p.chunk = 2;
     if (p.n <= 1024) {
-        sino_sort_kernel<16, false, IDX7><<<sort_blocks<16, false, IDX7>(cus), kBlock, 0, s>>>(p);
+        sino_sort_kernel<16, 16, false, IDX7><<<sort_blocks<16, 16, false, IDX7>(cus), kBlock, 0, s>>>(p);
         return hipGetLastError();
     }
     p.defer_out = queue_b;
     p.defer_out_count = counts + 1;
-    sino_sort_kernel<16, true, IDX7><<<sort_blocks<16, true, IDX7>(cus), kBlock, 0, s>>>(p);
+    sino_sort_kernel<16, 16, true, IDX7><<<sort_blocks<16, 16, true, IDX7>(cus), kBlock, 0, s>>>(p);
     p.defer_in = queue_b;
     p.defer_in_count = counts + 1;
     p.work = counts + 4;
     p.chunk = 1;
-    sino_sort_kernel<32, false, IDX7><<<sort_blocks<32, false, IDX7>(cus), kBlock, 0, s>>>(p);
+    sino_sort_kernel<32, 32, false, IDX7><<<sort_blocks<32, 32, false, IDX7>(cus), kBlock, 0, s>>>(p);
     return hipGetLastError();
 }
 
