@@ -2214,6 +2214,7 @@ struct GatherShape {
     static constexpr int ACC = ((P * P + 3) / 4) * 4;  // floats, padded to float4
     static constexpr int KMAX = T == 16 ? 32 : 48;     // footprint columns per angle
     static constexpr int NPIX = T * T / kWave;         // interior pixels per lane
+    static_assert(T == 16 || T == 32, "tiles of 16 or 32 pixels");
 };
 
 // LDS words of one wave: accumulators, then the staged column data
@@ -2254,6 +2255,7 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
     // the tile grown by one pixel, plus a margin far above the float
     // rounding of any sample position (< 1e-3 for n <= 2730), relative to o
     const float dm = 0.01f;
+    const int X0m = X0 - 1, Y0m = Y0 - 1;  // accumulator cell 0 (the halo)
     const float xa = (float)(X0 - 1) - o - dm, xb = (float)(X0 + T) - o + dm;
     const float ya = (float)(Y0 - 1) - o - dm, yb = (float)(Y0 + T) - o + dm;
     const float4 z4 = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -2390,9 +2392,10 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                     float w00, w01, w10, w11;
                     bool use;
                 };
-                auto sample = [&](int row) {
+                // py = row - o, stepped by whole rows by the caller (integers
+                // or half-integers far below 2^23: exact)
+                auto sample = [&](int row, float py) {
                     Hit h;
-                    const float py = (float)row - o;
                     float x = pxc + py * s;
                     float y = pxns + py * c;
                     x = x + o;
@@ -2403,7 +2406,7 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                     const int Yb = inb ? (int)(uint32_t)y : -8;
                     const float fx = __builtin_amdgcn_fractf(x);
                     const float fy = __builtin_amdgcn_fractf(y);
-                    const int xi = Xb - X0 + 1, yi = Yb - Y0 + 1;  // accumulator cell
+                    const int xi = Xb - X0m, yi = Yb - Y0m;  // accumulator cell
                     h.use = (unsigned)xi <= (unsigned)T && (unsigned)yi <= (unsigned)T;
                     // G(row): g0 + [row >= m] (g1 d + g2 d^2)
                     //   + [row > sm] sum_t (alpha_t W_t.re + beta_t W_t.im)[row - sm]
@@ -2416,15 +2419,17 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                         G = row >= m ? hh : G;
                     }
                     if (DO345) {
-                        // table entry 0 is zero, so rows at or before sm add nothing
+                        // table entry 0 is zero, so rows at or before sm add
+                        // nothing; q <= n - 1 < tabn (row < n <= tabn)
                         int q = row - sm;
-                        q = q > 0 && q < tabn ? q : 0;
-                        const float2 w3 = s_tab[q], w4 = s_tab[tabn + q];
+                        q = q > 0 ? q : 0;
+                        // T3 and T4 entries of a row as one float4, T5 after them
+                        const float4 w34 = reinterpret_cast<const float4 *>(s_tab)[q];
                         const float2 w5 = s_tab[2 * tabn + q];
-                        G = fmaf(eb.x, w3.x, G);
-                        G = fmaf(eb.y, w3.y, G);
-                        G = fmaf(eb.z, w4.x, G);
-                        G = fmaf(eb.w, w4.y, G);
+                        G = fmaf(eb.x, w34.x, G);
+                        G = fmaf(eb.y, w34.y, G);
+                        G = fmaf(eb.z, w34.z, G);
+                        G = fmaf(eb.w, w34.w, G);
                         G = fmaf(ec.x, w5.x, G);
                         G = fmaf(ec.y, w5.y, G);
                     }
@@ -2434,7 +2439,12 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                     }
                     const float ofx = 1.0f - fx, ofy = 1.0f - fy;
                     const float ga = G * ofx, gb = G * fx;
-                    h.cell = h.use ? __mul24(xi, P) + yi : 0;
+                    // xi P + yi (read only where used, xi <= T): a 24-bit
+                    // multiply, which the compiler would otherwise widen to a
+                    // quarter-rate 32-bit one
+                    int xp;
+                    asm volatile("v_mul_u32_u24 %0, %1, %2" : "=v"(xp) : "n"(P), "v"(xi));
+                    h.cell = xp + yi;
                     h.w00 = ga * ofy;
                     h.w01 = ga * fy;
                     h.w10 = gb * ofy;
@@ -2447,8 +2457,10 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                     // are disjoint and every read-add-write of the class has
                     // its own four targets
                     const int roff = rc >= r3 ? rc - r3 : rc - r3 + 3;
-                    for (int row = rlo + roff + 3 * jl; row <= rhi; row += 12) {
-                        const Hit a = sample(row);
+                    const int row0 = rlo + roff + 3 * jl;
+                    float py = (float)row0 - o;
+                    for (int row = row0; row <= rhi; row += 12, py += 12.0f) {
+                        const Hit a = sample(row, py);
                         if (a.use) {
                             float *ca = acc + a.cell;
                             const float a00 = ca[0], a01 = ca[1], a10 = ca[P], a11 = ca[P + 1];
@@ -2484,17 +2496,15 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
     }
 }
 
-// Stages the T3..T5 tables into the workgroup's LDS (every wave helps),
+// Stages the T3..T5 tables into the workgroup's LDS (every wave helps; T3
+// and T4 of a row as one float4, T5 after them, as stage_tab345),
 // then returns this wave's slice of the dynamic LDS.
 __device__ __forceinline__ float *gather_lds(float *smem, const float2 *tab345, int tabn,
                                              bool do345, int wave_words)
 {
     const int tab_words = do345 ? 6 * tabn : 0;
-    if (do345) {
-        float2 *s_tab = reinterpret_cast<float2 *>(smem);
-        for (int i = threadIdx.x; i < 3 * tabn; i += blockDim.x)
-            s_tab[i] = tab345[i];
-    }
+    if (do345)
+        stage_tab345(reinterpret_cast<float2 *>(smem), tab345, tabn, blockDim.x);
     __syncthreads();
     return smem + ((tab_words + 3) & ~3) + wave_id() * wave_words;
 }
