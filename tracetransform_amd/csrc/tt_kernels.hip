@@ -2207,6 +2207,7 @@ __global__ __launch_bounds__(kBlock) void records_batched_kernel(BatchAdjointPar
 // ---------------------------------------------------------------------------
 constexpr int kGatherFlush = 4;
 constexpr int kGatherTile = 32;
+constexpr int kGatherTabWords = 6;  // LDS floats per row of the T3..T5 tables
 
 template <int T>
 struct GatherShape {
@@ -2388,13 +2389,13 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                 // One sample: its accumulator cell and the four weighted
                 // gradient values of its taps (use: its taps touch the tile).
                 struct Hit {
-                    int cell;
+                    int cell;  // byte offset of the sample's first accumulator
                     float w00, w01, w10, w11;
                     bool use;
                 };
                 // py = row - o, stepped by whole rows by the caller (integers
                 // or half-integers far below 2^23: exact)
-                auto sample = [&](int row, float py) {
+                auto sample = [&](int row, float py, int q) {
                     Hit h;
                     float x = pxc + py * s;
                     float y = pxns + py * c;
@@ -2419,13 +2420,12 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                         G = row >= m ? hh : G;
                     }
                     if (DO345) {
-                        // table entry 0 is zero, so rows at or before sm add
-                        // nothing; q <= n - 1 < tabn (row < n <= tabn)
-                        int q = row - sm;
-                        q = q > 0 ? q : 0;
-                        // T3 and T4 entries of a row as one float4, T5 after them
-                        const float4 w34 = reinterpret_cast<const float4 *>(s_tab)[q];
-                        const float2 w5 = s_tab[2 * tabn + q];
+                        // q = row - sm, stepped with the row; table entry 0 is
+                        // zero, so rows at or before sm add nothing; q <= n - 1
+                        // < tabn (row < n <= tabn)
+                        const int qc = q > 0 ? q : 0;
+                        const float4 w34 = reinterpret_cast<const float4 *>(s_tab)[qc];
+                        const float2 w5 = s_tab[2 * tabn + qc];
                         G = fmaf(eb.x, w34.x, G);
                         G = fmaf(eb.y, w34.y, G);
                         G = fmaf(eb.z, w34.z, G);
@@ -2439,12 +2439,12 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                     }
                     const float ofx = 1.0f - fx, ofy = 1.0f - fy;
                     const float ga = G * ofx, gb = G * fx;
-                    // xi P + yi (read only where used, xi <= T): a 24-bit
-                    // multiply, which the compiler would otherwise widen to a
-                    // quarter-rate 32-bit one
+                    // byte offset 4 (xi P + yi) (read only where used, xi <= T):
+                    // a 24-bit multiply, which the compiler would otherwise
+                    // widen to a quarter-rate 32-bit one
                     int xp;
-                    asm volatile("v_mul_u32_u24 %0, %1, %2" : "=v"(xp) : "n"(P), "v"(xi));
-                    h.cell = xp + yi;
+                    asm volatile("v_mul_u32_u24 %0, %1, %2" : "=v"(xp) : "n"(4 * P), "v"(xi));
+                    h.cell = xp + 4 * yi;
                     h.w00 = ga * ofy;
                     h.w01 = ga * fy;
                     h.w10 = gb * ofy;
@@ -2459,10 +2459,12 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                     const int roff = rc >= r3 ? rc - r3 : rc - r3 + 3;
                     const int row0 = rlo + roff + 3 * jl;
                     float py = (float)row0 - o;
-                    for (int row = row0; row <= rhi; row += 12, py += 12.0f) {
-                        const Hit a = sample(row, py);
+                    int q = row0 - sm;
+                    for (int row = row0; row <= rhi; row += 12, py += 12.0f, q += 12) {
+                        const Hit a = sample(row, py, q);
                         if (a.use) {
-                            float *ca = acc + a.cell;
+                            float *ca = reinterpret_cast<float *>(
+                                reinterpret_cast<char *>(acc) + a.cell);
                             const float a00 = ca[0], a01 = ca[1], a10 = ca[P], a11 = ca[P + 1];
                             ca[0] = a00 + a.w00;
                             ca[1] = a01 + a.w01;
@@ -2497,12 +2499,12 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
 }
 
 // Stages the T3..T5 tables into the workgroup's LDS (every wave helps; T3
-// and T4 of a row as one float4, T5 after them, as stage_tab345),
-// then returns this wave's slice of the dynamic LDS.
+// and T4 of a row as one float4, T5 after them, as stage_tab345), then
+// returns this wave's slice of the dynamic LDS.
 __device__ __forceinline__ float *gather_lds(float *smem, const float2 *tab345, int tabn,
                                              bool do345, int wave_words)
 {
-    const int tab_words = do345 ? 6 * tabn : 0;
+    const int tab_words = do345 ? kGatherTabWords * tabn : 0;
     if (do345)
         stage_tab345(reinterpret_cast<float2 *>(smem), tab345, tabn, blockDim.x);
     __syncthreads();
@@ -3312,7 +3314,7 @@ hipError_t launch_records(const RecordParams &p, hipStream_t s)
 
 size_t gather_lds_bytes(int tile, int waves, int tabn, bool do345, bool do12, bool do67)
 {
-    const size_t tab = do345 ? (((size_t)6 * tabn + 3) & ~(size_t)3) : 0;
+    const size_t tab = do345 ? (size_t)kGatherTabWords * tabn : 0;
     return sizeof(float) * (tab + (size_t)waves * gather_wave_words(tile, do12, do67));
 }
 
