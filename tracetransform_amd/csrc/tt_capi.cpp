@@ -1651,11 +1651,23 @@ GatherLaunch choose_gather(int tiles32, int A, int cus, int tabn, bool do345, bo
         }
     }
     if (split_angles) {
-        const int want = 16 * cus;
-        int groups = (want + tiles32 - 1) / tiles32;
-        groups = groups > 32 ? 32 : groups;
-        groups = groups > A ? A : groups;
-        g.groups = groups < 1 ? 1 : groups;
+        // Angle groups: the waves (tiles x groups) run in rounds of what the
+        // device holds at once (16 waves per CU: the LDS and 128 VGPRs), each
+        // wave over ceil(A / groups) angles; the group count minimises
+        // rounds x angles per wave, plus a little for each partial image the
+        // reduction reads (a count just above one round's worth would run a
+        // second round for a few waves).
+        const double cap = 16.0 * cus;
+        double best = 1e30;
+        const int gmax = A < 32 ? A : 32;
+        for (int groups = 1; groups <= gmax; groups++) {
+            const double rounds = ceil((double)tiles32 * groups / cap);
+            const double cost = rounds * ceil((double)A / groups) + groups / 8.0;
+            if (cost < best) {
+                best = cost;
+                g.groups = groups;
+            }
+        }
     }
     return g;
 }
