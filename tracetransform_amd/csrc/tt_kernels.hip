@@ -2257,6 +2257,7 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
     // rounding of any sample position (< 1e-3 for n <= 2730), relative to o
     const float dm = 0.01f;
     const int X0m = X0 - 1, Y0m = Y0 - 1;  // accumulator cell 0 (the halo)
+    const bool interior = X0 >= 2 && X0 + T <= n - 2 && Y0 >= 2 && Y0 + T <= n - 2;
     const float xa = (float)(X0 - 1) - o - dm, xb = (float)(X0 + T) - o + dm;
     const float ya = (float)(Y0 - 1) - o - dm, yb = (float)(Y0 + T) - o + dm;
     const float4 z4 = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -2351,7 +2352,13 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
         }
         const float c = f.c, s = f.s;
         const int c_lo = f.c_lo, K = f.K;
-        {
+        // CHK: test the sample positions against the image bounds.  A tile
+        // at least 2 pixels inside the image on every side can skip it: a
+        // position outside [0, n - 1) truncates (negative values to 0) to a
+        // cell outside the tile's grown range, so such a sample is dropped by
+        // the tile test alone, as the bounds test would drop it.
+        auto classes = [&](auto chk) {
+            constexpr bool CHK = decltype(chk)::value;
             const float pxc0 = (float)c_lo - o;
             const int clo3 = c_lo % 3;
             // one residue class (column, row mod 3) at a time: samples of one
@@ -2401,10 +2408,13 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                     float y = pxns + py * c;
                     x = x + o;
                     y = y + o;
-                    const bool inb = __float_as_uint(x) < __float_as_uint(lim) &&
-                                     __float_as_uint(y) < __float_as_uint(lim);
-                    const int Xb = inb ? (int)(uint32_t)x : -8;
-                    const int Yb = inb ? (int)(uint32_t)y : -8;
+                    int Xb = (int)(uint32_t)x, Yb = (int)(uint32_t)y;
+                    if constexpr (CHK) {
+                        const bool inb = __float_as_uint(x) < __float_as_uint(lim) &&
+                                         __float_as_uint(y) < __float_as_uint(lim);
+                        Xb = inb ? Xb : -8;
+                        Yb = inb ? Yb : -8;
+                    }
                     const float fx = __builtin_amdgcn_fractf(x);
                     const float fy = __builtin_amdgcn_fractf(y);
                     const int xi = Xb - X0m, yi = Yb - Y0m;  // accumulator cell
@@ -2474,7 +2484,11 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                     }
                 }
             }
-        }
+        };
+        if (interior)
+            classes(std::false_type{});
+        else
+            classes(std::true_type{});
         wave_lds_sync();  // staged columns are read; the next angle rewrites them
         if ((a - a0) % kGatherFlush == kGatherFlush - 1 || a == a1 - 1) {
             // flush the float accumulators of these angles into the doubles
