@@ -21,7 +21,7 @@ namespace {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kDftBM = 64;   // output rows per workgroup
+constexpr int kDftBM = 128;  // output rows per workgroup
 constexpr int kDftBN = 128;  // output columns per workgroup
 constexpr int kDftBK = 16;   // reduction chunk
 
@@ -73,11 +73,13 @@ __device__ __forceinline__ int find_job(const DftJob *jobs, int njobs, int tile)
     return lo;
 }
 
-// One 64 x 128 output tile per workgroup of four waves (2 x 2 wave tiles of
-// 32 x 64, two MFMA accumulators each), reduction in chunks of 16, double
-// buffered: while the matrix cores work on one chunk the waves generate the
-// next W chunk from the LDS twiddles (four consecutive entries per thread,
-// the phase stepped by u) and load the next input chunk into registers.
+// One 128 x 128 output tile per workgroup of four waves (2 x 2 wave tiles
+// of 64 x 64, four MFMA accumulators each), reduction in chunks of 16,
+// double buffered: while the matrix cores work on one chunk (32 MFMAs per
+// wave, long enough to cover the next chunk's global loads) the waves
+// generate the next W chunk from the LDS twiddles (eight consecutive entries
+// per thread, the phase stepped by u) and load the next input chunk into
+// registers.
 // Twiddles (cos, sin)(2 pi t / n) are rounded from double sincospi.
 template <bool FWD>
 __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
@@ -99,20 +101,20 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
     }
     const float inv_n = 1.0f / (float)n;
     const int lane = tid & (kWave - 1), wave = tid >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 64;
-    // W loader: row ar of the tile, reduction entries ak .. ak + 3
-    const int ar = tid & 63, ak = (tid >> 6) * 4;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    // W loader: row ar of the tile, reduction entries ak .. ak + 7
+    const int ar = tid & 127, ak = (tid >> 7) * 8;
     const int arow = m0 + ar;
     // input loader: columns bc and bc + 64 of the tile, entries bk .. bk + 3
     const int bc = tid & 63, bk = (tid >> 6) * 4;
     const int col0 = c0 + bc, col1 = c0 + 64 + bc;
     const float *src0 = jb.in + (size_t)(col0 < jb.cols ? col0 : 0) * jb.ld_in;
     const float *src1 = jb.in + (size_t)(col1 < jb.cols ? col1 : 0) * jb.ld_in;
-    float ra[4], rb0[4], rb1[4];
+    float ra[8], rb0[4], rb1[4];
     auto fetch = [&](int k0) {
         const int kk0 = k0 + ak;
-        if (arow < M && kk0 + 3 < Kd && (FWD || kk0 + 3 < h || kk0 >= h)) {
-            // four consecutive kk of one half: one modular product, then steps
+        if (arow < M && kk0 + 7 < Kd && (FWD || kk0 + 7 < h || kk0 >= h)) {
+            // eight consecutive kk of one half: one modular product, then steps
             int u, v;
             bool sn;
             if (FWD) {
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
             int t = phase_mod(u, v, n, inv_n);
             const int du = u >= n ? u - n : u;
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
+            for (int e = 0; e < 8; e++) {
                 const float2 w = s_tw[t];
                 ra[e] = sn ? -w.y : w.x;
                 t += du;
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
             }
         } else {
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
+            for (int e = 0; e < 8; e++) {
                 const int kk = kk0 + e;
                 ra[e] = arow < M && kk < Kd ? dft_weight<FWD>(s_tw, n, h, inv_n, arow, kk)
                                             : 0.0f;
@@ -150,13 +152,15 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
+        for (int e = 0; e < 8; e++)
             sA[buf][ak + e][ar] = ra[e];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
             sB[buf][bk + e][bc] = rb0[e];
             sB[buf][bk + e][64 + bc] = rb1[e];
         }
     };
-    f32x16 acc0 = {}, acc1 = {};
+    f32x16 acc00 = {}, acc01 = {}, acc10 = {}, acc11 = {};
     __syncthreads();  // twiddles
     fetch(0);
     store(0);
@@ -169,11 +173,14 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
 #pragma unroll
         for (int s = 0; s < kDftBK / 2; s++) {
             const int kr = 2 * s + (lane >> 5);
-            const float a = sA[buf][kr][wm + (lane & 31)];
+            const float a0 = sA[buf][kr][wm + (lane & 31)];
+            const float a1 = sA[buf][kr][wm + 32 + (lane & 31)];
             const float b0 = sB[buf][kr][wn + (lane & 31)];
             const float b1 = sB[buf][kr][wn + 32 + (lane & 31)];
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1, acc1, 0, 0, 0);
+            acc00 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc00, 0, 0, 0);
+            acc01 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc01, 0, 0, 0);
+            acc10 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc10, 0, 0, 0);
+            acc11 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc11, 0, 0, 0);
         }
         if (more)
             store(buf ^ 1);
@@ -181,18 +188,22 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
         buf ^= 1;
     }
     // accumulator register r of this lane: column wn (+ 32) + lane % 32, row
-    // wm + 8 (r / 4) + 4 (lane / 32) + r % 4
+    // wm (+ 32) + 8 (r / 4) + 4 (lane / 32) + r % 4
 #pragma unroll
     for (int half = 0; half < 2; half++) {
         const int oc = c0 + wn + 32 * half + (lane & 31);
         if (oc < jb.cols) {
             float *dst = jb.out + (size_t)oc * jb.ld_out;
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int om = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (om < M)
-                    dst[om] = half ? acc1[r] : acc0[r];
-            }
+            for (int mh = 0; mh < 2; mh++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int om =
+                        m0 + wm + 32 * mh + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    const f32x16 &acc = mh ? (half ? acc11 : acc10) : (half ? acc01 : acc00);
+                    if (om < M)
+                        dst[om] = acc[r];
+                }
         }
     }
 }
