@@ -898,7 +898,8 @@ TT_API tt_status tt_circus(tt_context *ctx, const float *d_sino, int n,
         dp.job.cols = (int)cols;
         dp.job.ld_in = n;
         dp.job.ld_out = 2 * h;
-        dp.tiles = ttk::dft_job_tiles(n, (int)cols, true);
+        dp.bm = ttk::dft_rows_per_tile(ttk::dft_job_tiles(n, (int)cols, true, 128));
+        dp.tiles = ttk::dft_job_tiles(n, (int)cols, true, dp.bm);
     }
     if (p3) {
         TT_HIP(ctx->timer.begin(TT_KERNEL_P3_DFT, st));
@@ -989,6 +990,22 @@ int gather_waves_max(int T, int tabn, bool do345, bool do12)
         if (ttk::gather_lds_bytes(T, waves, tabn, do345, do12, false) <= kLdsLimit)
             return waves;
     return 0;
+}
+
+// Workgroup offsets of a DFT job list for its launch's tile height (128
+// rows when the list is large enough, else 64); returns the workgroups.
+int layout_dft_jobs(std::vector<ttk::DftJob> &jobs, bool forward, int *bm)
+{
+    long long t128 = 0;
+    for (const ttk::DftJob &j : jobs)
+        t128 += ttk::dft_job_tiles(j.n, j.cols, forward, 128);
+    *bm = ttk::dft_rows_per_tile(t128);
+    int tiles = 0;
+    for (ttk::DftJob &j : jobs) {
+        j.tile0 = tiles;
+        tiles += ttk::dft_job_tiles(j.n, j.cols, forward, *bm);
+    }
+    return tiles;
 }
 
 // Enqueue the batched pass (used for capture and for autotuning).
@@ -1264,8 +1281,6 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
                 j.cols = nt * n_angles;
                 j.ld_in = im.n;
                 j.ld_out = h2;
-                j.tile0 = dft_tiles;
-                dft_tiles += ttk::dft_job_tiles(im.n, j.cols, true);
                 jobs.push_back(j);
             }
             b.img = im.d_image;
@@ -1286,6 +1301,8 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         }
         TT_HIP(B.desc.ensure(sizeof(ttk::BatchImage) * nimg));
         B.dft = {};
+        int dft_bm = 64;
+        dft_tiles = layout_dft_jobs(jobs, true, &dft_bm);
         if (p3) {
             TT_HIP(B.jobs.ensure(sizeof(ttk::DftJob) * jobs.size()));
             TT_HIP(hipMemcpy(B.jobs.ptr, jobs.data(), sizeof(ttk::DftJob) * jobs.size(),
@@ -1293,6 +1310,7 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
             B.dft.jobs = B.jobs.as<ttk::DftJob>();
             B.dft.njobs = (int)jobs.size();
             B.dft.tiles = dft_tiles;
+            B.dft.bm = dft_bm;
         }
         std::vector<float2> h;
         make_trig(n_angles, angle_step_deg, h);
@@ -1529,7 +1547,8 @@ tt_status circus_vjp(tt_context *ctx, const float *d_sino, int n, int n_cols,
         ttk::DftParams f = {};
         f.njobs = 1;
         f.job = j;
-        f.tiles = ttk::dft_job_tiles(n, cols, true);
+        f.bm = ttk::dft_rows_per_tile(ttk::dft_job_tiles(n, cols, true, 128));
+        f.tiles = ttk::dft_job_tiles(n, cols, true, f.bm);
         TT_HIP(ttk::launch_dft_gemm(f, true, n, st));
         TT_HIP(ttk::launch_p3_vjp_scale(sp, cols, st));
         ttk::DftParams b = {};
@@ -1540,7 +1559,8 @@ tt_status circus_vjp(tt_context *ctx, const float *d_sino, int n, int n_cols,
         b.job.cols = cols;
         b.job.ld_in = 2 * h;
         b.job.ld_out = n;
-        b.tiles = ttk::dft_job_tiles(n, cols, false);
+        b.bm = ttk::dft_rows_per_tile(ttk::dft_job_tiles(n, cols, false, 128));
+        b.tiles = ttk::dft_job_tiles(n, cols, false, b.bm);
         TT_HIP(ttk::launch_dft_gemm(b, false, n, st));
         TT_HIP(ctx->timer.end(st));
     }
@@ -2351,8 +2371,6 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
                 j.cols = nt * n_angles;
                 j.ld_in = im.n;
                 j.ld_out = h2;
-                j.tile0 = ftiles;
-                ftiles += ttk::dft_job_tiles(im.n, j.cols, true);
                 fj.push_back(j);
                 ttk::DftJob js = j;
                 js.in = b.spec;
@@ -2367,8 +2385,6 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
                 jb.out = b.dsino;
                 jb.ld_in = h2;
                 jb.ld_out = im.n;
-                jb.tile0 = btiles;
-                btiles += ttk::dft_job_tiles(im.n, j.cols, false);
                 bj.push_back(jb);
             }
             so += (size_t)nt * n_angles * h2;
@@ -2385,6 +2401,8 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         TT_HIP(hipMemcpy(B.desc.ptr, d.data(), sizeof(ttk::BatchImage) * nimg,
                          hipMemcpyHostToDevice));
         ttk::DftParams fdp = {}, bdp = {};
+        ftiles = layout_dft_jobs(fj, true, &fdp.bm);
+        btiles = layout_dft_jobs(bj, false, &bdp.bm);
         if (p3s.np3 > 0) {
             TT_HIP(P.fjobs.ensure(sizeof(ttk::DftJob) * fj.size()));
             TT_HIP(P.sjobs.ensure(sizeof(ttk::DftJob) * sj.size()));
@@ -2632,7 +2650,8 @@ TT_API tt_status tt_circus_noalloc(tt_context *ctx, const float *d_sino, int n,
         dp.job.cols = (int)cols;
         dp.job.ld_in = n;
         dp.job.ld_out = 2 * h;
-        dp.tiles = ttk::dft_job_tiles(n, (int)cols, true);
+        dp.bm = ttk::dft_rows_per_tile(ttk::dft_job_tiles(n, (int)cols, true, 128));
+        dp.tiles = ttk::dft_job_tiles(n, (int)cols, true, dp.bm);
         TT_HIP(ttk::launch_dft_gemm(dp, true, n, st));
     }
     TT_HIP(ttk::launch_circus(p, st));

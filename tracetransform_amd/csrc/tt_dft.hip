@@ -21,8 +21,7 @@ namespace {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kDftBM = 128;  // output rows per workgroup
-constexpr int kDftBN = 128;  // output columns per workgroup
+constexpr int kDftBN = 128;  // output columns per workgroup (rows: 64 or 128)
 constexpr int kDftBK = 16;   // reduction chunk
 
 // Twiddle index (u v) mod n of W(row, kk) for the first of four consecutive
@@ -73,26 +72,29 @@ __device__ __forceinline__ int find_job(const DftJob *jobs, int njobs, int tile)
     return lo;
 }
 
-// One 128 x 128 output tile per workgroup of four waves (2 x 2 wave tiles
-// of 64 x 64, four MFMA accumulators each), reduction in chunks of 16,
-// double buffered: while the matrix cores work on one chunk (32 MFMAs per
-// wave, long enough to cover the next chunk's global loads) the waves
-// generate the next W chunk from the LDS twiddles (eight consecutive entries
-// per thread, the phase stepped by u) and load the next input chunk into
-// registers.
+// One BM x 128 output tile per workgroup of four waves (2 x 2 wave tiles of
+// BM/2 x 64, BM/16 MFMA accumulators each; BM = 128 for large launches,
+// 64 where 128-row tiles would leave CUs idle), reduction in chunks of 16,
+// double buffered: while the matrix cores work on one chunk (16 or 32
+// MFMAs per wave, the latter long enough to cover the next chunk's global
+// loads) the waves generate the next W chunk from the LDS twiddles (BM/16
+// consecutive entries per thread, the phase stepped by u) and load the next
+// input chunk into registers.
 // Twiddles (cos, sin)(2 pi t / n) are rounded from double sincospi.
-template <bool FWD>
+template <bool FWD, int BM>
 __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
 {
-    __shared__ float sA[2][kDftBK][kDftBM];
+    constexpr int EA = BM / 16;  // W entries per thread and chunk
+    constexpr int MB = BM / 64;  // 32-row blocks per wave
+    __shared__ float sA[2][kDftBK][BM];
     __shared__ float sB[2][kDftBK][kDftBN];
     extern __shared__ float2 s_tw[];
     const DftJob &jb = p.jobs ? p.jobs[find_job(p.jobs, p.njobs, blockIdx.x)] : p.job;
     const int n = jb.n, h = n / 2 + 1;
     const int M = FWD ? 2 * h : n, Kd = FWD ? n : 2 * h;
     const int local = blockIdx.x - jb.tile0;
-    const int mtiles = (M + kDftBM - 1) / kDftBM;
-    const int m0 = (local % mtiles) * kDftBM, c0 = (local / mtiles) * kDftBN;
+    const int mtiles = (M + BM - 1) / BM;
+    const int m0 = (local % mtiles) * BM, c0 = (local / mtiles) * kDftBN;
     const int tid = threadIdx.x;
     for (int i = tid; i < n; i += kBlock) {
         double sn, cn;
@@ -101,20 +103,20 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
     }
     const float inv_n = 1.0f / (float)n;
     const int lane = tid & (kWave - 1), wave = tid >> 6;
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
-    // W loader: row ar of the tile, reduction entries ak .. ak + 7
-    const int ar = tid & 127, ak = (tid >> 7) * 8;
+    const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * 64;
+    // W loader: row ar of the tile, reduction entries ak .. ak + EA - 1
+    const int ar = tid % BM, ak = (tid / BM) * EA;
     const int arow = m0 + ar;
     // input loader: columns bc and bc + 64 of the tile, entries bk .. bk + 3
     const int bc = tid & 63, bk = (tid >> 6) * 4;
     const int col0 = c0 + bc, col1 = c0 + 64 + bc;
     const float *src0 = jb.in + (size_t)(col0 < jb.cols ? col0 : 0) * jb.ld_in;
     const float *src1 = jb.in + (size_t)(col1 < jb.cols ? col1 : 0) * jb.ld_in;
-    float ra[8], rb0[4], rb1[4];
+    float ra[EA], rb0[4], rb1[4];
     auto fetch = [&](int k0) {
         const int kk0 = k0 + ak;
-        if (arow < M && kk0 + 7 < Kd && (FWD || kk0 + 7 < h || kk0 >= h)) {
-            // eight consecutive kk of one half: one modular product, then steps
+        if (arow < M && kk0 + EA - 1 < Kd && (FWD || kk0 + EA - 1 < h || kk0 >= h)) {
+            // EA consecutive kk of one half: one modular product, then steps
             int u, v;
             bool sn;
             if (FWD) {
@@ -129,7 +131,7 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
             int t = phase_mod(u, v, n, inv_n);
             const int du = u >= n ? u - n : u;
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
+            for (int e = 0; e < EA; e++) {
                 const float2 w = s_tw[t];
                 ra[e] = sn ? -w.y : w.x;
                 t += du;
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
             }
         } else {
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
+            for (int e = 0; e < EA; e++) {
                 const int kk = kk0 + e;
                 ra[e] = arow < M && kk < Kd ? dft_weight<FWD>(s_tw, n, h, inv_n, arow, kk)
                                             : 0.0f;
@@ -152,7 +154,7 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int e = 0; e < 8; e++)
+        for (int e = 0; e < EA; e++)
             sA[buf][ak + e][ar] = ra[e];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
@@ -174,13 +176,15 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
         for (int s = 0; s < kDftBK / 2; s++) {
             const int kr = 2 * s + (lane >> 5);
             const float a0 = sA[buf][kr][wm + (lane & 31)];
-            const float a1 = sA[buf][kr][wm + 32 + (lane & 31)];
             const float b0 = sB[buf][kr][wn + (lane & 31)];
             const float b1 = sB[buf][kr][wn + 32 + (lane & 31)];
             acc00 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc00, 0, 0, 0);
             acc01 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc01, 0, 0, 0);
-            acc10 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc10, 0, 0, 0);
-            acc11 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc11, 0, 0, 0);
+            if constexpr (MB == 2) {
+                const float a1 = sA[buf][kr][wm + 32 + (lane & 31)];
+                acc10 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc10, 0, 0, 0);
+                acc11 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc11, 0, 0, 0);
+            }
         }
         if (more)
             store(buf ^ 1);
@@ -195,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
         if (oc < jb.cols) {
             float *dst = jb.out + (size_t)oc * jb.ld_out;
 #pragma unroll
-            for (int mh = 0; mh < 2; mh++)
+            for (int mh = 0; mh < MB; mh++)
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const int om =
@@ -252,6 +256,12 @@ __global__ __launch_bounds__(kBlock) void p3_vjp_scale_kernel(P3ScaleParams p)
 
 }  // namespace
 
+int dft_rows_per_tile(long long tiles128)
+{
+    // 128-row tiles once they give every CU several workgroups
+    return tiles128 >= 1024 ? 128 : 64;
+}
+
 size_t dft_lds_bytes(int nmax)
 {
     return sizeof(float2) * (size_t)nmax;
@@ -261,10 +271,17 @@ hipError_t launch_dft_gemm(const DftParams &p, bool forward, int nmax, hipStream
 {
     if (p.tiles <= 0)
         return hipSuccess;
-    if (forward)
-        dft_gemm_kernel<true><<<p.tiles, kBlock, dft_lds_bytes(nmax), s>>>(p);
+    if (p.bm != 64 && p.bm != 128)
+        return hipErrorInvalidValue;
+    const size_t shm = dft_lds_bytes(nmax);
+    if (forward && p.bm == 128)
+        dft_gemm_kernel<true, 128><<<p.tiles, kBlock, shm, s>>>(p);
+    else if (forward)
+        dft_gemm_kernel<true, 64><<<p.tiles, kBlock, shm, s>>>(p);
+    else if (p.bm == 128)
+        dft_gemm_kernel<false, 128><<<p.tiles, kBlock, shm, s>>>(p);
     else
-        dft_gemm_kernel<false><<<p.tiles, kBlock, dft_lds_bytes(nmax), s>>>(p);
+        dft_gemm_kernel<false, 64><<<p.tiles, kBlock, shm, s>>>(p);
     return hipGetLastError();
 }
 
@@ -278,11 +295,11 @@ hipError_t launch_p3_vjp_scale(const P3ScaleParams &p, int max_cols, hipStream_t
     return hipGetLastError();
 }
 
-int dft_job_tiles(int n, int cols, bool forward)
+int dft_job_tiles(int n, int cols, bool forward, int bm)
 {
     const int h = n / 2 + 1;
     const int M = forward ? 2 * h : n;
-    return ((M + kDftBM - 1) / kDftBM) * ((cols + kDftBN - 1) / kDftBN);
+    return ((M + bm - 1) / bm) * ((cols + kDftBN - 1) / kDftBN);
 }
 
 }  // namespace ttk

@@ -234,6 +234,7 @@ struct DftParams {
     const DftJob *jobs;   // device array of njobs, or null: job is the one
     int njobs;
     int tiles;            // total workgroups
+    int bm;               // output rows per workgroup: 64 or 128
     DftJob job;
 };
 struct P3ScaleParams {
@@ -243,7 +244,10 @@ struct P3ScaleParams {
     int np3;              // P3 entries of the P list, at positions q3
     int q3[kMaxP];
 };
-int dft_job_tiles(int n, int cols, bool forward);
+// Workgroups of one job with bm output rows per workgroup, and the bm for a
+// launch whose jobs need tiles128 workgroups of 128 rows.
+int dft_job_tiles(int n, int cols, bool forward, int bm);
+int dft_rows_per_tile(long long tiles128);
 size_t dft_lds_bytes(int nmax);
 hipError_t launch_dft_gemm(const DftParams &p, bool forward, int nmax, hipStream_t s);
 hipError_t launch_p3_vjp_scale(const P3ScaleParams &p, int max_cols, hipStream_t s);

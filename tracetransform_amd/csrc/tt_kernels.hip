@@ -20,11 +20,26 @@ namespace {
 // global memory on CDNA).
 typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
 
-// Load at a 32-bit byte offset from a wave-uniform base: matches the
-// scalar-base + vector-offset form of global loads (no 64-bit VALU math).
+// A pointer every lane of the wave holds, made visibly wave-uniform (its
+// two halves read from the first lane), so that loads from it can take the
+// scalar-base form.
+__device__ __forceinline__ const float *uniform_ptr(const float *p)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<const float *>(((uint64_t)hi << 32) | lo);
+}
+
+// Load at a 32-bit byte offset from a wave-uniform base in global memory:
+// the scalar-base + vector-offset form of global loads (no 64-bit VALU
+// math, no flat addressing).
 __device__ __forceinline__ f2u ld2b(const float *base, uint32_t byte_off)
 {
-    return *reinterpret_cast<const f2u *>(reinterpret_cast<const char *>(base) + byte_off);
+    typedef const char __attribute__((address_space(1))) *gbyte;
+    typedef const float __attribute__((address_space(1))) *gfloat;
+    const gfloat q = (gfloat)((gbyte)base + byte_off);
+    return f2u{q[0], q[1]};
 }
 
 __device__ __forceinline__ int wave_id()
@@ -272,6 +287,7 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
                                           bool active, float c, float s, int lo, int hi)
 {
     const int n = p.n;
+    src = uniform_ptr(src);
     const float o = (float)((n - 1) / 2.0);
     const float lim = (float)(n - 1);
     const float px = (float)col - o;
