@@ -21,6 +21,12 @@ namespace {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
+// Pointers into global memory (the job lists' pointers are generic): loads
+// and stores through them are global, not flat, instructions, so their
+// waits do not hold the LDS reads of the matrix operands.
+typedef const float __attribute__((address_space(1))) *gcfloat;
+typedef float __attribute__((address_space(1))) *gfloat;
+
 constexpr int kDftBN = 128;  // output columns per workgroup (rows: 64 or 128)
 constexpr int kDftBK = 16;   // reduction chunk
 
@@ -110,8 +116,8 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
     // input loader: columns bc and bc + 64 of the tile, entries bk .. bk + 3
     const int bc = tid & 63, bk = (tid >> 6) * 4;
     const int col0 = c0 + bc, col1 = c0 + 64 + bc;
-    const float *src0 = jb.in + (size_t)(col0 < jb.cols ? col0 : 0) * jb.ld_in;
-    const float *src1 = jb.in + (size_t)(col1 < jb.cols ? col1 : 0) * jb.ld_in;
+    const gcfloat src0 = (gcfloat)(jb.in + (size_t)(col0 < jb.cols ? col0 : 0) * jb.ld_in);
+    const gcfloat src1 = (gcfloat)(jb.in + (size_t)(col1 < jb.cols ? col1 : 0) * jb.ld_in);
     float ra[EA], rb0[4], rb1[4];
     auto fetch = [&](int k0) {
         const int kk0 = k0 + ak;
@@ -197,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
     for (int half = 0; half < 2; half++) {
         const int oc = c0 + wn + 32 * half + (lane & 31);
         if (oc < jb.cols) {
-            float *dst = jb.out + (size_t)oc * jb.ld_out;
+            const gfloat dst = (gfloat)(jb.out + (size_t)oc * jb.ld_out);
 #pragma unroll
             for (int mh = 0; mh < MB; mh++)
 #pragma unroll
@@ -231,8 +237,8 @@ __global__ __launch_bounds__(kBlock) void p3_vjp_scale_kernel(P3ScaleParams p)
         const size_t gb = (size_t)(c / jb.g_cols) * jb.g_stride + c % jb.g_cols;
         for (int i = 0; i < p.np3; i++)
             g += jb.g[gb + (size_t)p.q3[i] * jb.g_cols];
-        const float *x = jb.in + (size_t)c * jb.ld_in;
-        float *y = jb.out + (size_t)c * jb.ld_out;
+        const gcfloat x = (gcfloat)(jb.in + (size_t)c * jb.ld_in);
+        const gfloat y = (gfloat)(jb.out + (size_t)c * jb.ld_out);
         for (int k = threadIdx.x; k < h; k += kBlock) {
             double w = 0.0;
             for (int side = 0; side < 2; side++) {
