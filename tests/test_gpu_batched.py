@@ -60,6 +60,17 @@ def test_batched_matches_single_image(oracle, gpu):
     check_against_eager(b, 90, 4.0, T05, P123)
 
 
+def test_batched_p3_wide_dft_tiles_match_single_image(oracle, gpu):
+    # 80 images give the batch's P3 product enough work for 128-row DFT
+    # tiles while each single-image call uses 64-row tiles: the spectra,
+    # hence P1-P3, are the same bits either way
+    sides = inputs.batch_sides(80, 150, 250, 12)
+    imgs = make_batch(oracle, sides, 900)
+    b = core.Batch(imgs, 90, 2.0, T05, P123)
+    b.run(autotune=False)
+    check_against_eager(b, 90, 2.0, T05, P123)
+
+
 def test_batched_replays_follow_input_changes(oracle, gpu):
     sides = [64, 33, 100, 7, 64]
     imgs = make_batch(oracle, sides, 200)

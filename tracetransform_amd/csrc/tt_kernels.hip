@@ -239,16 +239,30 @@ __device__ __forceinline__ int2 line_rows(const int *bbox, int n, int col, float
     return int2{(int)ceil(lo), (int)floor(hi)};
 }
 
+// Outputs in global memory, typed so (pointers from a batch descriptor are
+// generic): their stores and loads are global, not flat, operations.
+typedef float __attribute__((address_space(1))) *gf32;
+typedef int32_t __attribute__((address_space(1))) *gi32;
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef f2v __attribute__((address_space(1))) *gf2;
+
 struct LineDst {
-    float *sino;
-    int32_t *med_x;
-    int32_t *med_s;
-    float2 *z345;
-    float2 *ckpt;  // walk checkpoints (with CK), see sino_line
+    gf32 sino;
+    gi32 med_x;
+    gi32 med_s;
+    gf2 z345;
+    gf2 ckpt;  // walk checkpoints (with CK), see sino_line
     int n;
     int A;
     int tabn;
     Slots slot;
+
+    __device__ LineDst(float *sino_, int32_t *med_x_, int32_t *med_s_, float2 *z345_,
+                       float2 *ckpt_, int n_, int A_, int tabn_, Slots slot_)
+        : sino((gf32)sino_), med_x((gi32)med_x_), med_s((gi32)med_s_), z345((gf2)z345_),
+          ckpt((gf2)ckpt_), n(n_), A(A_), tabn(tabn_), slot(slot_)
+    {
+    }
 };
 
 // Row schedule of a wave.  At step v a lane handles row v - d of its line,
@@ -350,7 +364,7 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
     };
     for (int v = G; v < V1; v += 8) {
         if (WALK && v > G && ((v - G) & 63) == 0 && active && (v - G) / 64 <= ncap)
-            p.ckpt[(size_t)((v - G) / 64 - 1) * plane + line] = float2{cx, cs};
+            p.ckpt[(size_t)((v - G) / 64 - 1) * plane + line] = f2v{cx, cs};
         if (v >= dmax && v + 8 - dmin <= a8)
             group(std::false_type{}, v);
         else
@@ -439,7 +453,10 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
 #pragma unroll
                 for (int u = 0; u < 4; u++)
                     if (b + u <= nck)
-                        ck[u] = p.ckpt[(size_t)(b + u - 1) * plane + line];
+                    {
+                        const f2v cv = p.ckpt[(size_t)(b + u - 1) * plane + line];
+                        ck[u] = float2{cv.x, cv.y};
+                    }
                 bool stop = false;
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
@@ -569,9 +586,9 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
             p.sino[p.slot.s[5] * tstride + line] = (float)hypot(r5, i5);
         if (p.z345) {
             const size_t plane = (size_t)p.A * n;
-            p.z345[line] = float2{(float)r3, (float)i3};
-            p.z345[plane + line] = float2{(float)r4, (float)i4};
-            p.z345[2 * plane + line] = float2{(float)r5, (float)i5};
+            p.z345[line] = f2v{(float)r3, (float)i3};
+            p.z345[plane + line] = f2v{(float)r4, (float)i4};
+            p.z345[2 * plane + line] = f2v{(float)r5, (float)i5};
         }
     }
 }
