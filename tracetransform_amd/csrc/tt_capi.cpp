@@ -1963,53 +1963,50 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
     const double one = 1.0, zero = 0.0;
     double *a_d = ctx->nos_a.as<double>(), *g = ctx->nos_v.as<double>();
     const rocblas_stride sA = (rocblas_stride)mA, sG = (rocblas_stride)AA;
-    // S = X^T X for the batch: one batched GEMM over the n_sino * kGramChunks
-    // row chunks (pointer tables: chunk c of matrix m of each X buffer, and
-    // the partial products), then the chunk sums in order
+    // S = X^T X for the batch: one batched product over the n_sino *
+    // kGramChunks row chunks (job table: chunk c of matrix m of each X
+    // buffer into its partial product), then the chunk sums in order.  The
+    // Gram matrices and every A x A product below are symmetric and run in
+    // the matrix-core kernel's symmetric mode.
     double *sp = ctx->nos_sp.as<double>();
     double *xbuf[3] = {a_d, ctx->nos_c.as<double>(), ctx->nos_u.as<double>()};
     const int nb = n_sino * kGramChunks;
-    std::vector<double *> tab((size_t)4 * nb);
+    std::vector<ttk::NosGemmJob> jobs((size_t)3 * nb);
     for (int b = 0; b < nb; b++) {
         const size_t off = mA * (size_t)(b / kGramChunks) +
                            (size_t)chunk_rows * (b % kGramChunks);
         for (int w = 0; w < 3; w++)
-            tab[(size_t)w * nb + b] = xbuf[w] + off;
-        tab[(size_t)3 * nb + b] = sp + AA * b;
+            jobs[(size_t)w * nb + b] = {xbuf[w] + off, xbuf[w] + off, sp + AA * b};
     }
     // The Gram-space steps after the first form M Q and Q Q in one batched
     // call of 2 n_sino products.  Their buffers: Q in nos_h's first quarter,
     // Q Q in its third, and M alternating between nos_q and the second
-    // quarter (see the loop below); one table of A, B and C pointers for
-    // each parity of the step.
+    // quarter (see the loop below); one job list for each parity of the step.
     double *hq = ctx->nos_h.as<double>(), *hm = hq + AA * n_sino, *hq2 = hq + 2 * AA * n_sino;
     double *qb = ctx->nos_q.as<double>();
-    const size_t mq0 = tab.size();
+    const size_t mq0 = jobs.size();
     for (int par = 0; par < 2; par++) {
         double *mcur = par == 0 ? qb : hm, *mnext = par == 0 ? hm : qb;
-        for (int w = 0; w < 3; w++)
-            for (int b = 0; b < 2 * n_sino; b++) {
-                const int i = b % n_sino;
-                const bool mq = b < n_sino;  // M Q, else Q Q
-                double *base = w == 0 ? (mq ? mcur : hq) : w == 1 ? hq : (mq ? mnext : hq2);
-                tab.push_back(base + AA * i);
-            }
+        for (int b = 0; b < 2 * n_sino; b++) {
+            const size_t i = (size_t)(b % n_sino);
+            const bool mq = b < n_sino;  // M Q, else Q Q
+            jobs.push_back({(mq ? mcur : hq) + AA * i, hq + AA * i,
+                            (mq ? mnext : hq2) + AA * i});
+        }
     }
-    TT_HIP(ctx->nos_ptrs.ensure(sizeof(double *) * tab.size()));
-    TT_HIP(hipMemcpyAsync(ctx->nos_ptrs.ptr, tab.data(), sizeof(double *) * tab.size(),
+    TT_HIP(ctx->nos_ptrs.ensure(sizeof(ttk::NosGemmJob) * jobs.size()));
+    TT_HIP(hipMemcpyAsync(ctx->nos_ptrs.ptr, jobs.data(), sizeof(ttk::NosGemmJob) * jobs.size(),
                           hipMemcpyHostToDevice, st));
-    double **tab_d = ctx->nos_ptrs.as<double *>();
+    const ttk::NosGemmJob *jobs_d = ctx->nos_ptrs.as<ttk::NosGemmJob>();
     auto gram = [&](const double *x, double *out) -> tt_status {
         int w = 0;
         while (w < 3 && xbuf[w] != x)
             w++;
         if (w == 3)
             return fail(TT_ERR_INVALID, "internal: unknown NOS buffer");
-        const double *const *xs = tab_d + (size_t)w * nb;
-        TT_BLAS(rocblas_dgemm_batched(ctx->blas, rocblas_operation_transpose,
-                                      rocblas_operation_none, A, A, chunk_rows, &one, xs,
-                                      rows_max, xs, rows_max, &zero, tab_d + (size_t)3 * nb,
-                                      A, nb));
+        const ttk::NosGemmBatch bt{jobs_d + (size_t)w * nb, nullptr, nullptr, nullptr, 0, 0, 0};
+        TT_HIP(ttk::launch_nos_gemm(bt, nb, true, true, A, A, chunk_rows, rows_max, rows_max, A,
+                                    1.0, 0.0, st));
         TT_HIP(ttk::launch_nos_reduce(sp, kGramChunks, A, n_sino, out, st));
         return TT_OK;
     };
@@ -2032,19 +2029,28 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
         const double *x = a_d;
         TT_TRY(gram(a_d, S));
         TT_HIP(ttk::launch_nos_scales(S, A, n_sino, part, scale, r, st));
-        // C = P Q for batches of A x A matrices
+        // C = P Q + beta C for batches of A x A matrices (polynomials of
+        // one Gram matrix, so the products are symmetric)
         auto mul = [&](const double *P, const double *Qm, double beta, double *C) {
+            const ttk::NosGemmBatch bt{nullptr, P, Qm, C, (long long)AA, (long long)AA,
+                                       (long long)AA};
+            return ttk::launch_nos_gemm(bt, n_sino, false, true, A, A, A, A, A, A, 1.0, beta,
+                                        st);
+        };
+        // Y = X P for the batch's rows_max x A matrices X
+        auto tall = [&](const double *X, const double *P, double *Y) {
             return rocblas_dgemm_strided_batched(ctx->blas, rocblas_operation_none,
-                                                 rocblas_operation_none, A, A, A, &one, P, A,
-                                                 sG, Qm, A, sG, &beta, C, A, sG, n_sino);
+                                                 rocblas_operation_none, rows_max, A, A, &one,
+                                                 X, rows_max, sA, P, A, sG, &zero, Y, rows_max,
+                                                 sA, n_sino);
         };
         // Q = q(S) of the step (the first step scales by s and r)
         auto poly = [&](int step) -> tt_status {
-            TT_BLAS(mul(S, S, 0.0, S2));
+            TT_HIP(mul(S, S, 0.0, S2));
             TT_HIP(ttk::launch_nos_poly(S, S2, A, n_sino, kPolarSteps[step],
                                         step == 0 ? scale : nullptr, step == 0 ? r : nullptr,
                                         T, Q, st));
-            TT_BLAS(mul(S2, T, 1.0, Q));
+            TT_HIP(mul(S2, T, 1.0, Q));
             return TT_OK;
         };
         // The first kPolarTallSteps steps update X itself; once X is well
@@ -2060,35 +2066,30 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
                 TT_TRY(gram(x, S));
             TT_TRY(poly(step));
             if (step < kPolarTallSteps) {
-                TT_BLAS(rocblas_dgemm_strided_batched(
-                    ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A,
-                    &one, x, rows_max, sA, Q, A, sG, &zero, xa, rows_max, sA, n_sino));
+                TT_BLAS(tall(x, Q, xa));
                 x = xa;
                 std::swap(xa, xb);
                 continue;
             }
             if (step == kPolarTallSteps) {
                 std::swap(M, Q);  // M = Q (Q gets M's buffer)
-                TT_BLAS(mul(M, M, 0.0, Q2));
+                TT_HIP(mul(M, M, 0.0, Q2));
             } else {
-                // M Q and Q Q together from the parity's pointer table
+                // M Q and Q Q together from the parity's job list
                 const int par = (step - kPolarTallSteps - 1) & 1;
-                double *const *t = tab_d + mq0 + (size_t)par * 6 * n_sino;
                 if (Q != hq || Q2 != hq2 || M != (par == 0 ? qb : hm) ||
                     Mn != (par == 0 ? hm : qb))
                     return fail(TT_ERR_INVALID, "internal: NOS buffer rotation");
-                TT_BLAS(rocblas_dgemm_batched(ctx->blas, rocblas_operation_none,
-                                              rocblas_operation_none, A, A, A, &one, t, A,
-                                              t + 2 * n_sino, A, &zero, t + 4 * n_sino, A,
-                                              2 * n_sino));
+                const ttk::NosGemmBatch bt{jobs_d + mq0 + (size_t)par * 2 * n_sino, nullptr,
+                                           nullptr, nullptr, 0, 0, 0};
+                TT_HIP(ttk::launch_nos_gemm(bt, 2 * n_sino, false, true, A, A, A, A, A, A, 1.0,
+                                            0.0, st));
                 std::swap(M, Mn);
             }
-            TT_BLAS(mul(S, Q2, 0.0, Sn));
+            TT_HIP(mul(S, Q2, 0.0, Sn));
             std::swap(S, Sn);
         }
-        TT_BLAS(rocblas_dgemm_strided_batched(
-            ctx->blas, rocblas_operation_none, rocblas_operation_none, rows_max, A, A, &one, x,
-            rows_max, sA, M, A, sG, &zero, xa, rows_max, sA, n_sino));
+        TT_BLAS(tall(x, M, xa));
         x = xa;
         // convergence: S (= X^T X up to rounding) against I
         TT_HIP(ttk::launch_nos_deviation(S, A, n_sino, part, st));

@@ -292,6 +292,26 @@ hipError_t launch_nos_deviation(const double *S, int k, int count, double *part,
 hipError_t launch_nos_poly(const double *S, const double *S2, int k, int count,
                            const PolarStep &c, const double *scale, const double *r, double *T,
                            double *Q, hipStream_t s);
+// Batched FP64 products C = alpha op(A) B + beta C (column major, op(A) =
+// A^T when trans_a) on the matrix cores (tt_nos.hip).  The batch is a
+// device table of pointer triples, or (table == nullptr) base pointers and
+// element strides.  sym: m == n and the product is known to be symmetric;
+// only tiles on and below the diagonal are computed and each value is
+// written to (i, j) and (j, i).  Without trans_a, B must be symmetric (it
+// is read transposed) when k is a multiple of 4.
+struct NosGemmJob {
+    const double *a, *b;
+    double *c;
+};
+struct NosGemmBatch {
+    const NosGemmJob *table;
+    const double *a, *b;
+    double *c;
+    long long sa, sb, sc;
+};
+hipError_t launch_nos_gemm(const NosGemmBatch &bt, int count, bool trans_a, bool sym, int m,
+                           int n, int k, int lda, int ldb, int ldc, double alpha, double beta,
+                           hipStream_t s);
 
 hipError_t launch_records(const RecordParams &p, hipStream_t s);
 hipError_t launch_gather(const GatherParams &p, hipStream_t s);

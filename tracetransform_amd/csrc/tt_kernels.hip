@@ -307,7 +307,6 @@ __device__ __forceinline__ void sino_line(const LineDst &p, const float *src,
     const float px = (float)col - o;
     const float pxc = px * c;
     const float pxns = px * (-s);
-    const int lane = threadIdx.x & (kWave - 1);
     const size_t line = (size_t)a * n + col;
     const size_t plane = (size_t)p.A * n;
     constexpr int kNone = 1 << 30;
