@@ -1935,7 +1935,7 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
     // unchanged and stay exactly zero in it.  rows_max is rounded up to
     // kGramChunks row chunks of a multiple of 8 rows, over which the Gram
     // GEMMs are split (more workgroups than the 360 x 360 outputs give).
-    constexpr int kGramChunks = 4;
+    constexpr int kGramChunks = 3;
     const int chunk_rows = ((rows_max + kGramChunks - 1) / kGramChunks + 7) / 8 * 8;
     rows_max = chunk_rows * kGramChunks;
     const size_t mA = (size_t)rows_max * A, AA = (size_t)A * A;
@@ -2051,10 +2051,17 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
         };
         // Q = q(S) of the step (the first step scales by s and r)
         auto poly = [&](int step) -> tt_status {
-            TT_HIP(mul(S, S, 0.0, S2));
-            TT_HIP(ttk::launch_nos_poly(S, S2, A, n_sino, kPolarSteps[step],
-                                        step == 0 ? scale : nullptr, step == 0 ? r : nullptr,
-                                        T, Q, st));
+            ttk::NosPolyOut po{T, Q, (long long)AA, step == 0 ? scale : nullptr,
+                               step == 0 ? r : nullptr, {}};
+            for (int j = 0; j < 5; j++)
+                po.c[j] = kPolarSteps[step].c[j];
+            if (A % 4 == 0) {
+                TT_HIP(ttk::launch_nos_square_poly(S, S2, A, n_sino, po, st));
+            } else {
+                TT_HIP(mul(S, S, 0.0, S2));
+                TT_HIP(ttk::launch_nos_poly(S, S2, A, n_sino, kPolarSteps[step], po.scale,
+                                            po.r, T, Q, st));
+            }
             TT_HIP(mul(S2, T, 1.0, Q));
             return TT_OK;
         };

@@ -312,6 +312,18 @@ struct NosGemmBatch {
 hipError_t launch_nos_gemm(const NosGemmBatch &bt, int count, bool trans_a, bool sym, int m,
                            int n, int k, int lda, int ldb, int ldc, double alpha, double beta,
                            hipStream_t s);
+// S2 = S S for a strided batch of k x k symmetric S (k a multiple of 4) with
+// a polynomial step's elementwise terms in the same kernel: with a = s S,
+// b = s^2 S2 (s = scale[m], r = r[m] on the first step, else 1),
+// T = s^2 r (c3 a + c4 b) and Q = r (c0 I + c1 a + c2 b), so Q + S2 T = q(S).
+struct NosPolyOut {
+    double *T, *Q;
+    long long stride;  // elements between matrices (k * k)
+    const double *scale, *r;
+    double c[5];
+};
+hipError_t launch_nos_square_poly(const double *S, double *S2, int k, int count,
+                                  const NosPolyOut &po, hipStream_t s);
 
 hipError_t launch_records(const RecordParams &p, hipStream_t s);
 hipError_t launch_gather(const GatherParams &p, hipStream_t s);

@@ -177,12 +177,17 @@ __global__ __launch_bounds__(kNgThreads) void nos_gemm_kernel(NosGemmBatch bt, i
 constexpr int kNsDepth = 4;
 constexpr int kNsWaves = kNgThreads / 64;
 
-template <bool SYM>
+// POLY: the product is S S of a polynomial step (A = B = S, strided batch);
+// the epilogue also forms the step's elementwise terms from S and S2 = S S:
+// with a = s S, b = s^2 S2 (s, r the first step's scales, else 1),
+// T = s^2 r (c3 a + c4 b) and Q = r (c0 I + c1 a + c2 b), so that
+// Q + S2 T = q(S).
+template <bool SYM, bool POLY>
 __global__ __launch_bounds__(kNgThreads) void nos_gemm_symb_kernel(NosGemmBatch bt, int count,
                                                                    int m, int n, int k,
                                                                    int lda, int ldb, int ldc,
                                                                    double alpha, double beta,
-                                                                   int bm)
+                                                                   int bm, NosPolyOut po)
 {
     __shared__ double s_part[kNsWaves - 1][16][64];
     const int job = blockIdx.x % count;
@@ -297,6 +302,22 @@ __global__ __launch_bounds__(kNgThreads) void nos_gemm_symb_kernel(NosGemmBatch 
                 C[o] = val;
                 if (SYM && row != col)
                     C[col + (size_t)row * ldc] = val;
+                if (POLY) {
+                    const double sc = po.scale ? po.scale[job] : 1.0;
+                    const double rr = po.r ? po.r[job] : 1.0;
+                    const double a = sc * A[row + (size_t)col * lda], b = sc * sc * val;
+                    const double d = row == col ? po.c[0] : 0.0;
+                    const double tv = sc * sc * rr * (po.c[3] * a + po.c[4] * b);
+                    const double qv = rr * (d + po.c[1] * a + po.c[2] * b);
+                    const size_t off = (size_t)job * po.stride;
+                    po.T[off + o] = tv;
+                    po.Q[off + o] = qv;
+                    if (row != col) {
+                        const size_t om = col + (size_t)row * ldc;
+                        po.T[off + om] = tv;
+                        po.Q[off + om] = qv;
+                    }
+                }
             }
 }
 
@@ -314,12 +335,13 @@ hipError_t launch_nos_gemm(const NosGemmBatch &bt, int count, bool trans_a, bool
         const int bm = (m + 31) / 32, bn = (n + 31) / 32;
         const int nblocks = sym ? bm * (bm + 1) / 2 : bm * bn;
         const dim3 grid((unsigned)(nblocks * count));
+        const NosPolyOut none{};
         if (sym)
-            nos_gemm_symb_kernel<true><<<grid, kNgThreads, 0, s>>>(bt, count, m, n, k, lda, ldb,
-                                                                   ldc, alpha, beta, bm);
+            nos_gemm_symb_kernel<true, false><<<grid, kNgThreads, 0, s>>>(
+                bt, count, m, n, k, lda, ldb, ldc, alpha, beta, bm, none);
         else
-            nos_gemm_symb_kernel<false><<<grid, kNgThreads, 0, s>>>(
-                bt, count, m, n, k, lda, ldb, ldc, alpha, beta, bm);
+            nos_gemm_symb_kernel<false, false><<<grid, kNgThreads, 0, s>>>(
+                bt, count, m, n, k, lda, ldb, ldc, alpha, beta, bm, none);
         return hipGetLastError();
     }
     const int tm = (m + kNgB - 1) / kNgB, tn = (n + kNgB - 1) / kNgB;
@@ -339,6 +361,18 @@ hipError_t launch_nos_gemm(const NosGemmBatch &bt, int count, bool trans_a, bool
             nos_gemm_kernel<false, false><<<grid, kNgThreads, 0, s>>>(bt, m, n, k, lda, ldb,
                                                                      ldc, alpha, beta, tm);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_nos_square_poly(const double *S, double *S2, int k, int count,
+                                  const NosPolyOut &po, hipStream_t s)
+{
+    if (count <= 0 || k <= 0 || k % 4 != 0 || po.stride != (long long)k * k)
+        return hipErrorInvalidValue;
+    const int bm = (k + 31) / 32, nblocks = bm * (bm + 1) / 2;
+    const NosGemmBatch bt{nullptr, S, S, S2, po.stride, po.stride, po.stride};
+    nos_gemm_symb_kernel<true, true><<<dim3((unsigned)(nblocks * count)), kNgThreads, 0, s>>>(
+        bt, count, k, k, k, k, k, k, 1.0, 0.0, bm, po);
     return hipGetLastError();
 }
 
