@@ -163,9 +163,9 @@ TT_API tt_status tt_zscore(tt_context *ctx, const float *d_in, int n, int count,
 /* Replaces nearest_orthonormal_sinogram (c++/src/circus.cpp:68-102): column
  * medians, alignment to the sinogram centre, then the polar factor
  * U[:, :A] V^T of the aligned (n + |max| + |min|) x A matrix, computed in
- * double (a polynomial polar iteration on rocBLAS FP64 GEMMs; rocSOLVER
- * dsyevd / dgesvdj for matrices beyond its reach: condition numbers above
- * about 1e8, rank deficiency) and stored as float.  d_out holds
+ * double (a polynomial polar iteration on FP64 matrix-core products;
+ * rocSOLVER dsyevd / dgesvdj for matrices beyond its reach: condition
+ * numbers above about 1e8, rank deficiency) and stored as float.  d_out holds
  * out_capacity_rows x A floats; *out_rows receives the aligned height and
  * *center the Hermite centre (c++/src/transform.cpp:78-84).  The aligned
  * height must be at least A (the reference's U block needs it). */
