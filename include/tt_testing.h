@@ -26,6 +26,22 @@ TT_API tt_status tt_testing_fail_allocations(long long allowed);
 TT_API tt_status tt_testing_short_sqrt(tt_context *ctx, uint32_t lo, uint64_t count,
                                        uint64_t *mismatches, uint32_t *first_bad);
 
+/* Runs the FP64 matrix-core products of the nearest-orthonormal iteration
+ * on a strided batch of `count` column-major matrices (device pointers,
+ * matrices stored back to back): C = A^T B (trans_a: A is k x m) or A B
+ * (A is m x k; when k is a multiple of 4, B must be symmetric: it is read
+ * transposed), B k x n, C m x n.  sym: m == n and the product is known to
+ * be symmetric; only the blocks on and below the diagonal are computed and
+ * each value is written to (i, j) and (j, i).  With coef (5 doubles,
+ * sym, A = B = S of k x k, k a multiple of 4) the product is the polynomial
+ * step's S S, whose kernel also writes T = c3 S + c4 S^2 to d_t and
+ * Q = c0 I + c1 S + c2 S^2 to d_q.  Returns after the context's stream has
+ * finished. */
+TT_API tt_status tt_testing_nos_product(tt_context *ctx, int trans_a, int sym, int count,
+                                        int m, int n, int k, const double *d_a,
+                                        const double *d_b, double *d_c, const double *coef,
+                                        double *d_t, double *d_q);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,7 @@ EXPORTED = (
     "tt_nearest_orthonormal_sinograms", "tt_backward_batched", "tt_saved_bytes",
     "tt_circus_noalloc", "tt_saved_offsets", "tt_context_set_tiles", "tt_batched_tiles",
     # test hooks (include/tt_testing.h)
-    "tt_testing_fail_allocations", "tt_testing_short_sqrt",
+    "tt_testing_fail_allocations", "tt_testing_short_sqrt", "tt_testing_nos_product",
 )
 
 
@@ -133,6 +133,9 @@ def load():
         L.tt_testing_fail_allocations.argtypes = [C.c_longlong]
         L.tt_testing_short_sqrt.argtypes = [vp, C.c_uint32, C.c_uint64,
                                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]
+        L.tt_testing_nos_product.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             C.c_int, vp, vp, vp, C.POINTER(C.c_double), vp,
+                                             vp]
         L.tt_profile_enable.argtypes = [vp, C.c_int]
         L.tt_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_int64)]

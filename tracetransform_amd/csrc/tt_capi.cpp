@@ -2678,6 +2678,34 @@ TT_API tt_status tt_testing_fail_allocations(long long allowed)
     return TT_OK;
 }
 
+TT_API tt_status tt_testing_nos_product(tt_context *ctx, int trans_a, int sym, int count,
+                                        int m, int n, int k, const double *d_a,
+                                        const double *d_b, double *d_c, const double *coef,
+                                        double *d_t, double *d_q)
+{
+    TT_CHECK(ctx && d_a && d_b && d_c, "NULL argument");
+    TT_CHECK(count >= 1 && m >= 1 && n >= 1 && k >= 1, "invalid sizes %d x %d by %d (x %d)",
+             m, n, k, count);
+    TT_CHECK(!sym || m == n, "a symmetric product is square (got %d x %d)", m, n);
+    TT_TRY(ctx->set_device());
+    hipStream_t st = ctx->stream;
+    if (coef) {
+        TT_CHECK(sym && !trans_a && m == k && d_a == d_b && d_t && d_q && k % 4 == 0,
+                 "the polynomial step squares one symmetric k x k matrix, k a multiple of 4");
+        ttk::NosPolyOut po{d_t, d_q, (long long)k * k, nullptr, nullptr, {}};
+        for (int j = 0; j < 5; j++)
+            po.c[j] = coef[j];
+        TT_HIP(ttk::launch_nos_square_poly(d_a, d_c, k, count, po, st));
+    } else {
+        const ttk::NosGemmBatch bt{nullptr, d_a, d_b, d_c, (long long)m * k, (long long)k * n,
+                                   (long long)m * n};
+        TT_HIP(ttk::launch_nos_gemm(bt, count, trans_a != 0, sym != 0, m, n, k,
+                                    trans_a ? k : m, k, m, 1.0, 0.0, st));
+    }
+    TT_HIP(hipStreamSynchronize(st));
+    return TT_OK;
+}
+
 TT_API tt_status tt_testing_short_sqrt(tt_context *ctx, uint32_t lo, uint64_t count,
                                        uint64_t *mismatches, uint32_t *first_bad)
 {
