@@ -1911,7 +1911,6 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
     // c++/src/circus.cpp:72-84: offsets to floor((n-1)/2), aligned height
     // n + |max| + |min| and the Hermite centre floor((n-1)/2) + max
     const int sc = (int)floor((n - 1) / 2.0);
-    std::vector<int> maxoff(n_sino);
     int rows_max = 0;
     for (int t = 0; t < n_sino; t++) {
         int mn = 1 << 30, mx = -(1 << 30);
@@ -1927,7 +1926,6 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
                  t, out_capacity_rows, rows);
         out_rows[t] = rows;
         centers[t] = sc + mx;
-        maxoff[t] = mx;
         rows_max = rows > rows_max ? rows : rows_max;
     }
     // Every aligned matrix is zero-padded to rows_max rows so one batched
@@ -1955,12 +1953,11 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
     TT_HIP(ctx->nos_r.ensure(sizeof(double) * 2 * n_sino));
     TT_HIP(ctx->nos_dev.ensure(sizeof(double) * ttk::kNosSlices * n_sino));
     TT_HIP(ctx->timer.begin(TT_KERNEL_NOS, st));
-    // row r of aligned column c reads row r - (maxoff - (med - sc)) of the
-    // sinogram column: the raw medians with maxoff + sc
-    for (int t = 0; t < n_sino; t++)
-        TT_HIP(ttk::launch_align(d_sino + (size_t)t * A * n, n, A,
-                                 ctx->nos_med.as<int32_t>() + (size_t)t * A, maxoff[t] + sc,
-                                 rows_max, ctx->nos_a.as<double>() + mA * t, st));
+    // row r of aligned column c reads row r - (max - med) of the sinogram
+    // column, max the sinogram's largest median (the kernels find it and the
+    // aligned heights from the medians themselves)
+    TT_HIP(ttk::launch_align_batched(d_sino, n, A, ctx->nos_med.as<int32_t>(), n_sino,
+                                     rows_max, mA, ctx->nos_a.as<double>(), st));
     if (!ctx->blas)
         TT_BLAS(rocblas_create_handle(&ctx->blas));
     TT_BLAS(rocblas_set_stream(ctx->blas, st));
@@ -2118,9 +2115,9 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
         }
         TT_HIP(hipMemcpyAsync(ctx->h_dev, part, sizeof(double) * ndev,
                               hipMemcpyDeviceToHost, st));
-        for (int t = 0; t < n_sino; t++)
-            TT_HIP(ttk::launch_to_float(x + mA * t, rows_max, out_rows[t], A,
-                                        d_out + (size_t)t * out_capacity_rows * A, st));
+        TT_HIP(ttk::launch_to_float_batched(x, rows_max, mA, A, ctx->nos_med.as<int32_t>(), n,
+                                            sc, n_sino, rows_max, d_out,
+                                            (size_t)out_capacity_rows * A, st));
         TT_HIP(ctx->timer.end(st));
         TT_HIP(hipStreamSynchronize(st));
         bool converged = true;
@@ -2194,9 +2191,9 @@ TT_API tt_status tt_nearest_orthonormal_sinograms(tt_context *ctx,
                                   hipMemcpyDeviceToHost, st));
         }
     }
-    for (int t = 0; t < n_sino; t++)
-        TT_HIP(ttk::launch_to_float(result + mA * t, rows_max, out_rows[t], A,
-                                    d_out + (size_t)t * out_capacity_rows * A, st));
+    TT_HIP(ttk::launch_to_float_batched(result, rows_max, mA, A, ctx->nos_med.as<int32_t>(), n,
+                                        sc, n_sino, rows_max, d_out,
+                                        (size_t)out_capacity_rows * A, st));
     TT_HIP(ctx->timer.end(st));
     TT_HIP(hipStreamSynchronize(st));
     if (!gram_ok)

@@ -265,10 +265,16 @@ hipError_t launch_zscore(const float *in, int n, int count, float *out,
                          hipStream_t s);
 hipError_t launch_column_medians(const float *in, int n, int cols, int32_t *med,
                                  hipStream_t s);
-hipError_t launch_align(const float *in, int n, int cols, const int32_t *offset,
-                        int maxoff, int rows_out, double *out, hipStream_t s);
-hipError_t launch_to_float(const double *in, int ld, int rows, int cols, float *out,
-                           hipStream_t s);
+// Batched forms over count sinograms (n x cols each, back to back) and their
+// column medians med[t * cols + c]: align into matrices mstride doubles
+// apart (offset origin: the sinogram's largest median); to_float of matrix
+// t's aligned height n + |max - sc| + |min - sc| rows into out + t * ostride.
+hipError_t launch_align_batched(const float *in, int n, int cols, const int32_t *med,
+                                int count, int rows_out, size_t mstride, double *out,
+                                hipStream_t s);
+hipError_t launch_to_float_batched(const double *in, int ld, size_t mstride, int cols,
+                                   const int32_t *med, int n, int sc, int count, int rows_max,
+                                   float *out, size_t ostride, hipStream_t s);
 hipError_t launch_scale_eigvec(const double *v, const double *lambda, int k, int count,
                                double *out, hipStream_t s);
 hipError_t launch_newton_schulz_factor(double *x, int k, int count, hipStream_t s);

@@ -2851,31 +2851,77 @@ __global__ __launch_bounds__(kBlock) void column_median_kernel(const float *in, 
         med[blockIdx.x] = weighted_median_walk(s_col, n, s_sum);
 }
 
-// aligned(maxoff + row - offset[col], col) = in(row, col), zero elsewhere
-// (c++/src/circus.cpp:85-93), in double for the SVD.
-__global__ __launch_bounds__(kBlock) void align_kernel(const float *in, int n, int cols,
-                                                       const int32_t *offset, int maxoff,
-                                                       int rows_out, double *out)
+// Smallest and largest of a sinogram's column medians med[0 .. cols), for
+// every thread of the block.
+__device__ __forceinline__ int2 block_min_max(const int32_t *med, int cols)
 {
+    __shared__ int s_mm[2][kBlock / kWave];
+    int mn = 1 << 30, mx = -(1 << 30);
+    for (int c = threadIdx.x; c < cols; c += kBlock) {
+        mn = min(mn, med[c]);
+        mx = max(mx, med[c]);
+    }
+    mn = wave_min_i(mn);
+    mx = wave_max_i(mx);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        s_mm[0][threadIdx.x / kWave] = mn;
+        s_mm[1][threadIdx.x / kWave] = mx;
+    }
+    __syncthreads();
+    mn = s_mm[0][0];
+    mx = s_mm[1][0];
+    for (int w = 1; w < kBlock / kWave; w++) {
+        mn = min(mn, s_mm[0][w]);
+        mx = max(mx, s_mm[1][w]);
+    }
+    return int2{mn, mx};
+}
+
+// aligned(max + row - med[col], col) = in(row, col), zero elsewhere
+// (c++/src/circus.cpp:85-93), in double for the SVD, for a batch: grid y =
+// sinogram t (sinograms of n x cols back to back, medians med[t * cols + c],
+// aligned matrices mstride doubles apart), max the largest median of
+// sinogram t.
+__global__ __launch_bounds__(kBlock) void align_batched_kernel(const float *in, int n, int cols,
+                                                               const int32_t *med,
+                                                               int rows_out, size_t mstride,
+                                                               double *out)
+{
+    const int t = blockIdx.y;
+    const int32_t *off = med + (size_t)t * cols;
+    const int maxoff = block_min_max(off, cols).y;
+    const float *src_t = in + (size_t)t * cols * n;
+    double *out_t = out + (size_t)t * mstride;
     const size_t total = (size_t)rows_out * cols;
     for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < total;
          i += (size_t)gridDim.x * kBlock) {
         const int r = (int)(i % rows_out), c = (int)(i / rows_out);
-        const int src = r - (maxoff - offset[c]);
-        out[i] = (src >= 0 && src < n) ? (double)in[(size_t)c * n + src] : 0.0;
+        const int src = r - (maxoff - off[c]);
+        out_t[i] = (src >= 0 && src < n) ? (double)src_t[(size_t)c * n + src] : 0.0;
     }
 }
 
-// out (rows x cols, leading dimension rows) = float of the first rows rows
-// of in (leading dimension ld).
-__global__ __launch_bounds__(kBlock) void to_float_kernel(const double *in, int ld,
-                                                          int rows, int cols, float *out)
+// Float copy of the aligned rows of a batch of matrices (grid y = matrix t,
+// column major with leading dimension ld, mstride doubles apart in, ostride
+// floats apart out, leading dimension the matrix's rows): matrix t keeps its
+// aligned height n + |max - sc| + |min - sc| of its medians' offsets from
+// sc.
+__global__ __launch_bounds__(kBlock) void to_float_batched_kernel(const double *in, int ld,
+                                                                  size_t mstride, int cols,
+                                                                  const int32_t *med, int n,
+                                                                  int sc, float *out,
+                                                                  size_t ostride)
 {
+    const int t = blockIdx.y;
+    const int2 mm = block_min_max(med + (size_t)t * cols, cols);
+    const int rows = n + abs(mm.y - sc) + abs(mm.x - sc);
+    const double *in_t = in + (size_t)t * mstride;
+    float *out_t = out + (size_t)t * ostride;
     const size_t total = (size_t)rows * cols;
     for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < total;
          i += (size_t)gridDim.x * kBlock) {
         const int r = (int)(i % rows), c = (int)(i / rows);
-        out[i] = (float)in[(size_t)c * ld + r];
+        out_t[i] = (float)in_t[(size_t)c * ld + r];
     }
 }
 
@@ -3497,14 +3543,6 @@ hipError_t launch_column_medians(const float *in, int n, int cols, int32_t *med,
     return hipGetLastError();
 }
 
-hipError_t launch_align(const float *in, int n, int cols, const int32_t *offset,
-                        int maxoff, int rows_out, double *out, hipStream_t s)
-{
-    align_kernel<<<grid_for((size_t)rows_out * cols), kBlock, 0, s>>>(
-        in, n, cols, offset, maxoff, rows_out, out);
-    return hipGetLastError();
-}
-
 hipError_t launch_image_bbox(const float *img, int n, int *bbox, hipStream_t s)
 {
     bbox_init_kernel<<<1, kWave, 0, s>>>(bbox, n);
@@ -3570,12 +3608,25 @@ hipError_t launch_nos_poly(const double *S, const double *S2, int k, int count,
 }
 
 
-hipError_t launch_to_float(const double *in, int ld, int rows, int cols, float *out,
-                           hipStream_t s)
+hipError_t launch_align_batched(const float *in, int n, int cols, const int32_t *med,
+                                int count, int rows_out, size_t mstride, double *out,
+                                hipStream_t s)
 {
-    to_float_kernel<<<grid_for((size_t)rows * cols), kBlock, 0, s>>>(in, ld, rows, cols,
-                                                                    out);
+    const int bx = (grid_for((size_t)rows_out * cols) + count - 1) / count;
+    align_batched_kernel<<<dim3(bx, count), kBlock, 0, s>>>(in, n, cols, med, rows_out,
+                                                             mstride, out);
     return hipGetLastError();
 }
+
+hipError_t launch_to_float_batched(const double *in, int ld, size_t mstride, int cols,
+                                   const int32_t *med, int n, int sc, int count, int rows_max,
+                                   float *out, size_t ostride, hipStream_t s)
+{
+    const int bx = (grid_for((size_t)rows_max * cols) + count - 1) / count;
+    to_float_batched_kernel<<<dim3(bx, count), kBlock, 0, s>>>(in, ld, mstride, cols, med, n,
+                                                                sc, out, ostride);
+    return hipGetLastError();
+}
+
 
 }  // namespace ttk
