@@ -345,23 +345,39 @@ hipError_t launch_nos_gemm(const NosGemmBatch &bt, int count, bool trans_a, bool
         return hipGetLastError();
     }
     const int tm = (m + kNgB - 1) / kNgB, tn = (n + kNgB - 1) / kNgB;
-    const dim3 grid(sym ? tm * (tm + 1) / 2 : tm * tn, count);
-    if (trans_a) {
-        if (sym)
-            nos_gemm_kernel<true, true><<<grid, kNgThreads, 0, s>>>(bt, m, n, k, lda, ldb, ldc,
-                                                                   alpha, beta, tm);
-        else
-            nos_gemm_kernel<true, false><<<grid, kNgThreads, 0, s>>>(bt, m, n, k, lda, ldb,
-                                                                    ldc, alpha, beta, tm);
-    } else {
-        if (sym)
-            nos_gemm_kernel<false, true><<<grid, kNgThreads, 0, s>>>(bt, m, n, k, lda, ldb,
-                                                                    ldc, alpha, beta, tm);
-        else
-            nos_gemm_kernel<false, false><<<grid, kNgThreads, 0, s>>>(bt, m, n, k, lda, ldb,
-                                                                     ldc, alpha, beta, tm);
+    constexpr int kMaxGridY = 65535;  // the batch runs along grid y
+    for (int c0 = 0; c0 < count; c0 += kMaxGridY) {
+        NosGemmBatch part = bt;
+        if (bt.table) {
+            part.table = bt.table + c0;
+        } else {
+            part.a = bt.a + (size_t)c0 * bt.sa;
+            part.b = bt.b + (size_t)c0 * bt.sb;
+            part.c = bt.c + (size_t)c0 * bt.sc;
+        }
+        const int cnt = count - c0 < kMaxGridY ? count - c0 : kMaxGridY;
+        const dim3 grid(sym ? tm * (tm + 1) / 2 : tm * tn, cnt);
+        if (trans_a) {
+            if (sym)
+                nos_gemm_kernel<true, true><<<grid, kNgThreads, 0, s>>>(part, m, n, k, lda, ldb,
+                                                                       ldc, alpha, beta, tm);
+            else
+                nos_gemm_kernel<true, false><<<grid, kNgThreads, 0, s>>>(part, m, n, k, lda, ldb,
+                                                                        ldc, alpha, beta, tm);
+        } else {
+            if (sym)
+                nos_gemm_kernel<false, true><<<grid, kNgThreads, 0, s>>>(part, m, n, k, lda, ldb,
+                                                                        ldc, alpha, beta, tm);
+            else
+                nos_gemm_kernel<false, false><<<grid, kNgThreads, 0, s>>>(part, m, n, k, lda,
+                                                                         ldb, ldc, alpha, beta,
+                                                                         tm);
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
     }
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 hipError_t launch_nos_square_poly(const double *S, double *S2, int k, int count,
