@@ -9,11 +9,14 @@
 // both (i, j) and (j, i), which also makes the results exactly symmetric.
 // The Gram matrices X^T X are products of that kind too (A transposed).
 //
-// One workgroup of four waves owns a 64 x 64 output tile; each wave a
-// 32 x 32 quarter as 2 x 2 MFMA blocks.  The reduction runs in chunks of 32
-// staged through LDS (double buffered, one barrier per chunk); every output
-// element is one FP64 multiply-add chain in reduction order.  Matrices are
-// column major, as in BLAS.
+// Two kernels.  nos_gemm_kernel (the Gram matrices, and any product whose
+// reduction length is not a multiple of 4): one workgroup of four waves
+// owns a 64 x 64 output tile, each wave a 32 x 32 quarter as 2 x 2 MFMA
+// blocks, the reduction in chunks of 32 staged through LDS (double
+// buffered, one barrier per chunk).  nos_gemm_symb_kernel (products with a
+// symmetric right operand): direct operand loads and the reduction split
+// over the workgroup's waves, described below.  Matrices are column major,
+// as in BLAS.
 #include "tt_internal.hpp"
 
 namespace ttk {
