@@ -2461,7 +2461,9 @@ __device__ __forceinline__ void gather_tile_wave(const GatherImg &im, const floa
                         const float hh = fmaf(d, fmaf(ed.y, d, ed.x), G);
                         G = row >= m ? hh : G;
                     }
-                    if (DO345) {
+                    // (a wave whose active lanes all sit at or before their
+                    // sm skips the term: a uniform branch)
+                    if (DO345 && __builtin_amdgcn_ballot_w64(q > 0) != 0) {
                         // q = row - sm, stepped with the row; table entry 0 is
                         // zero, so rows at or before sm add nothing; q <= n - 1
                         // < tabn (row < n <= tabn)
