@@ -898,8 +898,9 @@ TT_API tt_status tt_circus(tt_context *ctx, const float *d_sino, int n,
         dp.job.cols = (int)cols;
         dp.job.ld_in = n;
         dp.job.ld_out = 2 * h;
-        dp.bm = ttk::dft_rows_per_tile(ttk::dft_job_tiles(n, (int)cols, true, 128));
-        dp.tiles = ttk::dft_job_tiles(n, (int)cols, true, dp.bm);
+        ttk::dft_tile_shape(ttk::dft_job_tiles(n, (int)cols, true, 128, 128),
+                            ttk::dft_job_tiles(n, (int)cols, true, 64, 128), &dp.bm, &dp.bn);
+        dp.tiles = ttk::dft_job_tiles(n, (int)cols, true, dp.bm, dp.bn);
     }
     if (p3) {
         TT_HIP(ctx->timer.begin(TT_KERNEL_P3_DFT, st));
@@ -992,18 +993,20 @@ int gather_waves_max(int T, int tabn, bool do345, bool do12)
     return 0;
 }
 
-// Workgroup offsets of a DFT job list for its launch's tile height (128
-// rows when the list is large enough, else 64); returns the workgroups.
-int layout_dft_jobs(std::vector<ttk::DftJob> &jobs, bool forward, int *bm)
+// Workgroup offsets of a DFT job list for its launch's tile shape (larger
+// tiles when the list is large enough); returns the workgroups.
+int layout_dft_jobs(std::vector<ttk::DftJob> &jobs, bool forward, int *bm, int *bn)
 {
-    long long t128 = 0;
-    for (const ttk::DftJob &j : jobs)
-        t128 += ttk::dft_job_tiles(j.n, j.cols, forward, 128);
-    *bm = ttk::dft_rows_per_tile(t128);
+    long long t128 = 0, t64 = 0;
+    for (const ttk::DftJob &j : jobs) {
+        t128 += ttk::dft_job_tiles(j.n, j.cols, forward, 128, 128);
+        t64 += ttk::dft_job_tiles(j.n, j.cols, forward, 64, 128);
+    }
+    ttk::dft_tile_shape(t128, t64, bm, bn);
     int tiles = 0;
     for (ttk::DftJob &j : jobs) {
         j.tile0 = tiles;
-        tiles += ttk::dft_job_tiles(j.n, j.cols, forward, *bm);
+        tiles += ttk::dft_job_tiles(j.n, j.cols, forward, *bm, *bn);
     }
     return tiles;
 }
@@ -1301,8 +1304,8 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
         }
         TT_HIP(B.desc.ensure(sizeof(ttk::BatchImage) * nimg));
         B.dft = {};
-        int dft_bm = 64;
-        dft_tiles = layout_dft_jobs(jobs, true, &dft_bm);
+        int dft_bm = 64, dft_bn = 128;
+        dft_tiles = layout_dft_jobs(jobs, true, &dft_bm, &dft_bn);
         if (p3) {
             TT_HIP(B.jobs.ensure(sizeof(ttk::DftJob) * jobs.size()));
             TT_HIP(hipMemcpy(B.jobs.ptr, jobs.data(), sizeof(ttk::DftJob) * jobs.size(),
@@ -1311,6 +1314,7 @@ TT_API tt_status tt_forward_batched(tt_context *ctx, const tt_image *images,
             B.dft.njobs = (int)jobs.size();
             B.dft.tiles = dft_tiles;
             B.dft.bm = dft_bm;
+            B.dft.bn = dft_bn;
         }
         std::vector<float2> h;
         make_trig(n_angles, angle_step_deg, h);
@@ -1547,8 +1551,9 @@ tt_status circus_vjp(tt_context *ctx, const float *d_sino, int n, int n_cols,
         ttk::DftParams f = {};
         f.njobs = 1;
         f.job = j;
-        f.bm = ttk::dft_rows_per_tile(ttk::dft_job_tiles(n, cols, true, 128));
-        f.tiles = ttk::dft_job_tiles(n, cols, true, f.bm);
+        ttk::dft_tile_shape(ttk::dft_job_tiles(n, cols, true, 128, 128),
+                            ttk::dft_job_tiles(n, cols, true, 64, 128), &f.bm, &f.bn);
+        f.tiles = ttk::dft_job_tiles(n, cols, true, f.bm, f.bn);
         TT_HIP(ttk::launch_dft_gemm(f, true, n, st));
         TT_HIP(ttk::launch_p3_vjp_scale(sp, cols, st));
         ttk::DftParams b = {};
@@ -1559,8 +1564,9 @@ tt_status circus_vjp(tt_context *ctx, const float *d_sino, int n, int n_cols,
         b.job.cols = cols;
         b.job.ld_in = 2 * h;
         b.job.ld_out = n;
-        b.bm = ttk::dft_rows_per_tile(ttk::dft_job_tiles(n, cols, false, 128));
-        b.tiles = ttk::dft_job_tiles(n, cols, false, b.bm);
+        ttk::dft_tile_shape(ttk::dft_job_tiles(n, cols, false, 128, 128),
+                            ttk::dft_job_tiles(n, cols, false, 64, 128), &b.bm, &b.bn);
+        b.tiles = ttk::dft_job_tiles(n, cols, false, b.bm, b.bn);
         TT_HIP(ttk::launch_dft_gemm(b, false, n, st));
         TT_HIP(ctx->timer.end(st));
     }
@@ -2410,8 +2416,8 @@ TT_API tt_status tt_backward_batched(tt_context *ctx, const tt_image_grad *image
         TT_HIP(hipMemcpy(B.desc.ptr, d.data(), sizeof(ttk::BatchImage) * nimg,
                          hipMemcpyHostToDevice));
         ttk::DftParams fdp = {}, bdp = {};
-        ftiles = layout_dft_jobs(fj, true, &fdp.bm);
-        btiles = layout_dft_jobs(bj, false, &bdp.bm);
+        ftiles = layout_dft_jobs(fj, true, &fdp.bm, &fdp.bn);
+        btiles = layout_dft_jobs(bj, false, &bdp.bm, &bdp.bn);
         if (p3s.np3 > 0) {
             TT_HIP(P.fjobs.ensure(sizeof(ttk::DftJob) * fj.size()));
             TT_HIP(P.sjobs.ensure(sizeof(ttk::DftJob) * sj.size()));
@@ -2659,8 +2665,9 @@ TT_API tt_status tt_circus_noalloc(tt_context *ctx, const float *d_sino, int n,
         dp.job.cols = (int)cols;
         dp.job.ld_in = n;
         dp.job.ld_out = 2 * h;
-        dp.bm = ttk::dft_rows_per_tile(ttk::dft_job_tiles(n, (int)cols, true, 128));
-        dp.tiles = ttk::dft_job_tiles(n, (int)cols, true, dp.bm);
+        ttk::dft_tile_shape(ttk::dft_job_tiles(n, (int)cols, true, 128, 128),
+                            ttk::dft_job_tiles(n, (int)cols, true, 64, 128), &dp.bm, &dp.bn);
+        dp.tiles = ttk::dft_job_tiles(n, (int)cols, true, dp.bm, dp.bn);
         TT_HIP(ttk::launch_dft_gemm(dp, true, n, st));
     }
     TT_HIP(ttk::launch_circus(p, st));

@@ -27,7 +27,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef const float __attribute__((address_space(1))) *gcfloat;
 typedef float __attribute__((address_space(1))) *gfloat;
 
-constexpr int kDftBN = 128;  // output columns per workgroup (rows: 64 or 128)
+// output rows x columns per workgroup: 128 x 128, 64 x 128 or 64 x 64
 constexpr int kDftBK = 16;   // reduction chunk
 
 // Twiddle index (u v) mod n of W(row, kk) for the first of four consecutive
@@ -87,20 +87,21 @@ __device__ __forceinline__ int find_job(const DftJob *jobs, int njobs, int tile)
 // consecutive entries per thread, the phase stepped by u) and load the next
 // input chunk into registers.
 // Twiddles (cos, sin)(2 pi t / n) are rounded from double sincospi.
-template <bool FWD, int BM>
+template <bool FWD, int BM, int BN>
 __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
 {
     constexpr int EA = BM / 16;  // W entries per thread and chunk
     constexpr int MB = BM / 64;  // 32-row blocks per wave
+    constexpr int NB = BN / 64;  // 32-column blocks per wave
     __shared__ float sA[2][kDftBK][BM];
-    __shared__ float sB[2][kDftBK][kDftBN];
+    __shared__ float sB[2][kDftBK][BN];
     extern __shared__ float2 s_tw[];
     const DftJob &jb = p.jobs ? p.jobs[find_job(p.jobs, p.njobs, blockIdx.x)] : p.job;
     const int n = jb.n, h = n / 2 + 1;
     const int M = FWD ? 2 * h : n, Kd = FWD ? n : 2 * h;
     const int local = blockIdx.x - jb.tile0;
     const int mtiles = (M + BM - 1) / BM;
-    const int m0 = (local % mtiles) * BM, c0 = (local / mtiles) * kDftBN;
+    const int m0 = (local % mtiles) * BM, c0 = (local / mtiles) * BN;
     const int tid = threadIdx.x;
     for (int i = tid; i < n; i += kBlock) {
         double sn, cn;
@@ -109,11 +110,11 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
     }
     const float inv_n = 1.0f / (float)n;
     const int lane = tid & (kWave - 1), wave = tid >> 6;
-    const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * 64;
+    const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BN / 2);
     // W loader: row ar of the tile, reduction entries ak .. ak + EA - 1
     const int ar = tid % BM, ak = (tid / BM) * EA;
     const int arow = m0 + ar;
-    // input loader: columns bc and bc + 64 of the tile, entries bk .. bk + 3
+    // input loader: columns bc (and bc + 64) of the tile, entries bk .. bk + 3
     const int bc = tid & 63, bk = (tid >> 6) * 4;
     const int col0 = c0 + bc, col1 = c0 + 64 + bc;
     const gcfloat src0 = (gcfloat)(jb.in + (size_t)(col0 < jb.cols ? col0 : 0) * jb.ld_in);
@@ -155,7 +156,8 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
         for (int e = 0; e < 4; e++) {
             const int kk = k0 + bk + e;
             rb0[e] = col0 < jb.cols && kk < Kd ? src0[kk] : 0.0f;
-            rb1[e] = col1 < jb.cols && kk < Kd ? src1[kk] : 0.0f;
+            if constexpr (NB == 2)
+                rb1[e] = col1 < jb.cols && kk < Kd ? src1[kk] : 0.0f;
         }
     };
     auto store = [&](int buf) {
@@ -165,7 +167,8 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             sB[buf][bk + e][bc] = rb0[e];
-            sB[buf][bk + e][64 + bc] = rb1[e];
+            if constexpr (NB == 2)
+                sB[buf][bk + e][64 + bc] = rb1[e];
         }
     };
     f32x16 acc00 = {}, acc01 = {}, acc10 = {}, acc11 = {};
@@ -183,13 +186,17 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
             const int kr = 2 * s + (lane >> 5);
             const float a0 = sA[buf][kr][wm + (lane & 31)];
             const float b0 = sB[buf][kr][wn + (lane & 31)];
-            const float b1 = sB[buf][kr][wn + 32 + (lane & 31)];
             acc00 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc00, 0, 0, 0);
-            acc01 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc01, 0, 0, 0);
+            float b1 = 0.0f;
+            if constexpr (NB == 2) {
+                b1 = sB[buf][kr][wn + 32 + (lane & 31)];
+                acc01 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc01, 0, 0, 0);
+            }
             if constexpr (MB == 2) {
                 const float a1 = sA[buf][kr][wm + 32 + (lane & 31)];
                 acc10 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc10, 0, 0, 0);
-                acc11 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc11, 0, 0, 0);
+                if constexpr (NB == 2)
+                    acc11 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc11, 0, 0, 0);
             }
         }
         if (more)
@@ -200,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void dft_gemm_kernel(DftParams p)
     // accumulator register r of this lane: column wn (+ 32) + lane % 32, row
     // wm (+ 32) + 8 (r / 4) + 4 (lane / 32) + r % 4
 #pragma unroll
-    for (int half = 0; half < 2; half++) {
+    for (int half = 0; half < NB; half++) {
         const int oc = c0 + wn + 32 * half + (lane & 31);
         if (oc < jb.cols) {
             const gfloat dst = (gfloat)(jb.out + (size_t)oc * jb.ld_out);
@@ -262,10 +269,12 @@ __global__ __launch_bounds__(kBlock) void p3_vjp_scale_kernel(P3ScaleParams p)
 
 }  // namespace
 
-int dft_rows_per_tile(long long tiles128)
+void dft_tile_shape(long long tiles128, long long tiles64, int *bm, int *bn)
 {
-    // 128-row tiles once they give every CU several workgroups
-    return tiles128 >= 1024 ? 128 : 64;
+    // 128 x 128 tiles once they give every CU several workgroups, 64 x 128
+    // while those fill the chip, else 64 x 64 (twice the workgroups)
+    *bm = tiles128 >= 1024 ? 128 : 64;
+    *bn = *bm == 128 || tiles64 >= 256 ? 128 : 64;
 }
 
 size_t dft_lds_bytes(int nmax)
@@ -277,17 +286,27 @@ hipError_t launch_dft_gemm(const DftParams &p, bool forward, int nmax, hipStream
 {
     if (p.tiles <= 0)
         return hipSuccess;
-    if (p.bm != 64 && p.bm != 128)
+    const int shape = p.bm == 128 && p.bn == 128 ? 2 : p.bm == 64 && p.bn == 128 ? 1
+                                                     : p.bm == 64 && p.bn == 64   ? 0
+                                                                                  : -1;
+    if (shape < 0)
         return hipErrorInvalidValue;
     const size_t shm = dft_lds_bytes(nmax);
-    if (forward && p.bm == 128)
-        dft_gemm_kernel<true, 128><<<p.tiles, kBlock, shm, s>>>(p);
-    else if (forward)
-        dft_gemm_kernel<true, 64><<<p.tiles, kBlock, shm, s>>>(p);
-    else if (p.bm == 128)
-        dft_gemm_kernel<false, 128><<<p.tiles, kBlock, shm, s>>>(p);
-    else
-        dft_gemm_kernel<false, 64><<<p.tiles, kBlock, shm, s>>>(p);
+    if (forward) {
+        if (shape == 2)
+            dft_gemm_kernel<true, 128, 128><<<p.tiles, kBlock, shm, s>>>(p);
+        else if (shape == 1)
+            dft_gemm_kernel<true, 64, 128><<<p.tiles, kBlock, shm, s>>>(p);
+        else
+            dft_gemm_kernel<true, 64, 64><<<p.tiles, kBlock, shm, s>>>(p);
+    } else {
+        if (shape == 2)
+            dft_gemm_kernel<false, 128, 128><<<p.tiles, kBlock, shm, s>>>(p);
+        else if (shape == 1)
+            dft_gemm_kernel<false, 64, 128><<<p.tiles, kBlock, shm, s>>>(p);
+        else
+            dft_gemm_kernel<false, 64, 64><<<p.tiles, kBlock, shm, s>>>(p);
+    }
     return hipGetLastError();
 }
 
@@ -301,11 +320,11 @@ hipError_t launch_p3_vjp_scale(const P3ScaleParams &p, int max_cols, hipStream_t
     return hipGetLastError();
 }
 
-int dft_job_tiles(int n, int cols, bool forward, int bm)
+int dft_job_tiles(int n, int cols, bool forward, int bm, int bn)
 {
     const int h = n / 2 + 1;
     const int M = forward ? 2 * h : n;
-    return ((M + bm - 1) / bm) * ((cols + kDftBN - 1) / kDftBN);
+    return ((M + bm - 1) / bm) * ((cols + bn - 1) / bn);
 }
 
 }  // namespace ttk

@@ -235,6 +235,7 @@ struct DftParams {
     int njobs;
     int tiles;            // total workgroups
     int bm;               // output rows per workgroup: 64 or 128
+    int bn;               // output columns per workgroup: 64 or 128 (128 when bm is)
     DftJob job;
 };
 struct P3ScaleParams {
@@ -244,10 +245,12 @@ struct P3ScaleParams {
     int np3;              // P3 entries of the P list, at positions q3
     int q3[kMaxP];
 };
-// Workgroups of one job with bm output rows per workgroup, and the bm for a
-// launch whose jobs need tiles128 workgroups of 128 rows.
-int dft_job_tiles(int n, int cols, bool forward, int bm);
-int dft_rows_per_tile(long long tiles128);
+// Workgroups of one job with bm x bn output tiles, and the tile shape of a
+// launch whose jobs need tiles128 workgroups of 128 x 128 and tiles64 of
+// 64 x 128.  The shape only partitions the outputs: every output is the same
+// sum in the same order whatever the shape.
+int dft_job_tiles(int n, int cols, bool forward, int bm, int bn);
+void dft_tile_shape(long long tiles128, long long tiles64, int *bm, int *bn);
 size_t dft_lds_bytes(int nmax);
 hipError_t launch_dft_gemm(const DftParams &p, bool forward, int nmax, hipStream_t s);
 hipError_t launch_p3_vjp_scale(const P3ScaleParams &p, int max_cols, hipStream_t s);
